@@ -1,0 +1,161 @@
+/*
+ * radhip.h - C ABI of the MI355X progressive-refinement radiosity library
+ * (libradhip.so).  The entry points replace the hot path of
+ * david-sabata/Radiosity: Config, precomputeHemicubeFormFactors, the
+ * ModelContainer/Patch state, the GL item-buffer render, the OpenCL
+ * ProcessHemicube kernel and the host accumulate/shoot loops.  Each
+ * declaration cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every call returns RAD_OK (0) or a negative RAD_E* code; the message of
+ *    the last failure is rad_last_error(ctx).  No C++ exception crosses the ABI.
+ *  - Host pointers are caller-owned; arguments named *_or_null may be NULL to
+ *    skip a copy.  Device memory is owned by the context.
+ *  - One context = one HIP stream on the current device.  A context is not
+ *    thread-safe (the reference drives GL/CL from its UI thread); separate
+ *    contexts may be used from separate threads.
+ *  - Patch p is drawn into the item buffer as p + 1; 0 is background
+ *    (Colors.cpp:148, Kernel_ProcessHemicube.h:51).
+ *  - An item buffer holds K hemicubes stacked vertically, each TEX_W x TEX_H
+ *    texels in the cross layout of FormFactors.cpp:250-272, row-major with
+ *    row 0 at the bottom (GL window convention), exactly as the reference's
+ *    RGBA8 texture (Main.cpp:277).
+ */
+#ifndef RADHIP_H
+#define RADHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RAD_OK 0
+#define RAD_E_ARG -1      /* invalid argument or configuration */
+#define RAD_E_STATE -2    /* call out of order (no scene, no selection ...) */
+#define RAD_E_HIP -3      /* HIP runtime error (message names the call) */
+#define RAD_E_NOMEM -4    /* device allocation failed */
+#define RAD_E_LIMIT -5    /* size beyond a documented limit */
+
+/* Shooting update (Main.cpp:1274 vs. physically based) */
+#define RAD_MODE_REFERENCE 0 /* unshot_j += dB_i * F_ij * 0.3 * colour_i (Patch.h:13) */
+#define RAD_MODE_PHYSICAL 1  /* unshot_j += rho_j * dB_i * F_ij * A_i / A_j */
+
+/* Shooter selection */
+#define RAD_SELECT_REFERENCE 0 /* ModelContainer.cpp:259-299 list rule on |dB|^2 */
+#define RAD_SELECT_POWER 1     /* top-K by A_i*(r+g+b)(dB_i), ties to the lower index */
+
+#define RAD_MAX_K 256
+#define RAD_MAX_W 1024
+
+/* Mirrors Config (Config.cpp:5-10 defaults, CLI Main.cpp:769-785) */
+typedef struct rad_config {
+    uint32_t hemicube;        /* Config::hemicubeSide: W, even, 2..RAD_MAX_W (default 16) */
+    uint32_t hemicubes;       /* Config::hemicubesCount: K shooters per pass, 1..RAD_MAX_K (10) */
+    uint32_t shoots;          /* Config::shootsPerCycle: passes per rad_run chunk (500) */
+    uint32_t ocl_workitems_x; /* Config::oclWorkitemsX: spans per row for the run form (4) */
+    uint32_t mode;            /* RAD_MODE_* */
+    uint32_t select;          /* RAD_SELECT_* */
+    uint32_t small_tri_px;    /* raster tuning: bbox area (px) rasterized by one lane; 0 = auto */
+    uint32_t reserved;
+} rad_config;
+
+typedef struct rad_ctx rad_ctx;
+
+/* Config::freeze + InitCLObjects (Config.cpp:30-48, Main.cpp:405-611).
+ * Binds the current HIP device, creates the stream and uploads the delta
+ * form-factor table. */
+int rad_create(const rad_config *cfg, rad_ctx **out);
+/* CleanupGLObjects / cleanup of CL objects (Main.cpp:616-...) */
+void rad_destroy(rad_ctx *ctx);
+const char *rad_last_error(const rad_ctx *ctx);
+/* Config::PATCHVIEW_TEX_W/H/RES getters (Config.cpp:115-153) */
+int rad_geometry(const rad_ctx *ctx, uint32_t *tex_w, uint32_t *tex_h, uint32_t *tex_res);
+
+/* float* precomputeHemicubeFormFactors() (FormFactors.h:20, FormFactors.cpp:280-339).
+ * Writes ONE copy (TEX_RES floats) of the cross-layout table computed on the
+ * device; the reference's K replicas (FormFactors.cpp:316-323) are not kept. */
+int rad_precompute_dff(rad_ctx *ctx, float *out_or_null);
+
+/* ModelContainer::updateData output (ModelContainer.cpp:81-155) and the
+ * per-patch fields the hot path reads (Patch.h:39-55).
+ *  tri_xyz     ntri*9 floats, triangle t = (v0,v1,v2); a reference quad
+ *              patch is two triangles (0,1,2),(0,2,3) (ModelContainer.cpp:112-117)
+ *  tri_patch   ntri patch indices; must be non-decreasing in t (draw order)
+ *  patch_frame npatch*9 floats: eye, normal, up (Patch::getCenter/getNormal/getUp)
+ *  refl_rgb    npatch*3: REFERENCE: Patch::getColor(); PHYSICAL: rho_j
+ *  area        npatch patch areas (PHYSICAL area ratio and residual weights)
+ * Resets the state to zero. */
+int rad_scene_upload(rad_ctx *ctx, const float *tri_xyz, const uint32_t *tri_patch,
+                     uint32_t ntri, uint32_t npatch, const float *patch_frame,
+                     const float *refl_rgb, const float *area);
+
+/* Patch::radiosity (unshot) and Patch::illumination (shot), npatch*3 each
+ * (Patch.h:46-47).  emission_or_null sets the residual normalisation E
+ * (NULL: E = unshot as given). */
+int rad_set_state(rad_ctx *ctx, const float *unshot_rgb, const float *shot_rgb,
+                  const float *emission_or_null);
+int rad_get_state(rad_ctx *ctx, float *unshot_or_null, float *shot_or_null);
+
+/* ModelContainer::getHighestRadiosityPatchesId (ModelContainer.cpp:259-299)
+ * with the context's selection rule.  Writes k ids (unfilled slots 0, as the
+ * reference) and the number filled.  The selection stays on the device for
+ * the next rad_shoot_selected. */
+int rad_select(rad_ctx *ctx, uint32_t k, uint32_t *out_ids, uint32_t *out_n);
+
+/* Face loop + DrawPatchLook (Main.cpp:1145-1200, 689-725): renders the
+ * hemicubes of the k given shooters into the device item buffer and
+ * optionally copies ids (k*TEX_RES) and depths (k*TEX_RES) out. */
+int rad_render_itembuf(rad_ctx *ctx, const uint32_t *shooters, uint32_t k,
+                       uint32_t *out_ids_or_null, float *out_depth_or_null);
+
+/* ProcessHemicube kernel + host accumulation (Kernel_ProcessHemicube.h:9-70,
+ * Main.cpp:1207-1269) over the k hemicubes last rendered: per-patch hit
+ * counts and form-factor rows, k*npatch each. */
+int rad_process_hemicube(rad_ctx *ctx, uint32_t k, uint32_t *out_hits_or_null,
+                         float *out_ff_or_null);
+
+/* Same reduction over a caller-supplied item buffer (k*TEX_RES uint32). */
+int rad_process_itembuf(rad_ctx *ctx, const uint32_t *itembuf, uint32_t k,
+                        uint32_t *out_hits_or_null, float *out_ff_or_null);
+
+/* One pass of the OnIdle shoot loop (Main.cpp:1137-1295) for the k given
+ * shooters: snapshot, render, reduce, shoot, emitter update. */
+int rad_shoot(rad_ctx *ctx, const uint32_t *shooters, uint32_t k);
+
+/* passes x (select K with the context rule, then rad_shoot), all on the
+ * device without host synchronisation. */
+int rad_step(rad_ctx *ctx, uint32_t passes);
+
+/* Runs passes until the FP64 residual <= residual_target or max_passes;
+ * the residual is checked every check_every passes.  Mirrors the stop test of
+ * Main.cpp:1297-1303 plus the residual rule. */
+typedef struct rad_stats {
+    uint64_t passes;         /* passes executed */
+    uint64_t shooters;       /* shooters shot (<= passes*K) */
+    double residual;         /* final FP64 residual */
+    double seconds;          /* wall time of the loop */
+    float last_energy;       /* |dB| of the last shooter (Main.cpp:1292,1298) */
+    uint32_t reserved;
+} rad_stats;
+int rad_run(rad_ctx *ctx, uint64_t max_passes, double residual_target,
+            uint32_t check_every, rad_stats *out);
+
+/* sum_i A_i (r+g+b)(unshot_i) / sum_i A_i (r+g+b)(E_i), accumulated in FP64 */
+int rad_residual_fp64(rad_ctx *ctx, double *out);
+
+/* The five MVP matrices (column-major, 16 floats each, faces UP, DOWN, LEFT,
+ * RIGHT, FRONT) the device built for shooter patch p (Camera.cpp:19-52,
+ * Main.cpp:1171-1183). */
+int rad_face_mvps(rad_ctx *ctx, uint32_t patch, float *out80);
+
+/* Average device time (ms) per launch of each kernel of the last
+ * rad_step/rad_run, measured with HIP events on the context stream:
+ * out[0] select, [1] prepare+clear, [2] raster setup, [3] raster tiles,
+ * [4] reduce, [5] shoot; also the number of timed passes. */
+int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,7 @@
+"""MI355X-native progressive-refinement radiosity (hemicube shooting).
+
+The compute path is libradhip.so (hand-written HIP kernels for gfx950 behind
+the C ABI in include/radhip.h); this package only binds it with ctypes.
+"""
+from ._lib import (RAD_MODE_PHYSICAL, RAD_MODE_REFERENCE, RAD_SELECT_POWER,  # noqa: F401
+                   RAD_SELECT_REFERENCE, RadError, Scene, Solver, lib)

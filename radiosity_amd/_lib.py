@@ -1,0 +1,277 @@
+"""ctypes binding of libradhip.so (the C ABI in include/radhip.h).
+
+The library is loaded from the package directory only; a missing library is
+an error (there is no fallback implementation).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libradhip.so")
+
+RAD_OK = 0
+RAD_MODE_REFERENCE = 0
+RAD_MODE_PHYSICAL = 1
+RAD_SELECT_REFERENCE = 0
+RAD_SELECT_POWER = 1
+
+
+class RadError(RuntimeError):
+    pass
+
+
+class rad_config(C.Structure):
+    _fields_ = [("hemicube", C.c_uint32), ("hemicubes", C.c_uint32), ("shoots", C.c_uint32),
+                ("ocl_workitems_x", C.c_uint32), ("mode", C.c_uint32), ("select", C.c_uint32),
+                ("small_tri_px", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class rad_stats(C.Structure):
+    _fields_ = [("passes", C.c_uint64), ("shooters", C.c_uint64), ("residual", C.c_double),
+                ("seconds", C.c_double), ("last_energy", C.c_float), ("reserved", C.c_uint32)]
+
+
+class rad_scene(C.Structure):
+    _fields_ = [("ntri", C.c_uint32), ("npatch", C.c_uint32), ("nquad", C.c_uint32),
+                ("reserved", C.c_uint32),
+                ("tri", C.POINTER(C.c_float)), ("tri_patch", C.POINTER(C.c_uint32)),
+                ("frame", C.POINTER(C.c_float)), ("refl", C.POINTER(C.c_float)),
+                ("area", C.POINTER(C.c_float)), ("emission", C.POINTER(C.c_float)),
+                ("shot0", C.POINTER(C.c_float)), ("patch_quad", C.POINTER(C.c_uint32))]
+
+
+_lib = None
+
+_P = C.c_void_p
+_u32 = C.c_uint32
+_PF = C.POINTER(C.c_float)
+_PU = C.POINTER(C.c_uint32)
+
+# name -> (restype, argtypes); every symbol here is declared in include/*.h
+SIGNATURES = {
+    "rad_create": (C.c_int, [C.POINTER(rad_config), C.POINTER(_P)]),
+    "rad_destroy": (None, [_P]),
+    "rad_last_error": (C.c_char_p, [_P]),
+    "rad_geometry": (C.c_int, [_P, _PU, _PU, _PU]),
+    "rad_precompute_dff": (C.c_int, [_P, _PF]),
+    "rad_scene_upload": (C.c_int, [_P, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
+    "rad_set_state": (C.c_int, [_P, _PF, _PF, _PF]),
+    "rad_get_state": (C.c_int, [_P, _PF, _PF]),
+    "rad_select": (C.c_int, [_P, _u32, _PU, _PU]),
+    "rad_render_itembuf": (C.c_int, [_P, _PU, _u32, _PU, _PF]),
+    "rad_process_hemicube": (C.c_int, [_P, _u32, _PU, _PF]),
+    "rad_process_itembuf": (C.c_int, [_P, _PU, _u32, _PU, _PF]),
+    "rad_shoot": (C.c_int, [_P, _PU, _u32]),
+    "rad_step": (C.c_int, [_P, _u32]),
+    "rad_run": (C.c_int, [_P, C.c_uint64, C.c_double, _u32, C.POINTER(rad_stats)]),
+    "rad_residual_fp64": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rad_face_mvps": (C.c_int, [_P, _u32, _PF]),
+    "rad_kernel_times": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "rad_scene_cornell": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_uint64,
+                                    C.POINTER(C.POINTER(rad_scene))]),
+    "rad_scene_boxes": (C.c_int, [_u32, C.c_float, C.c_float, C.c_float, _u32, _u32,
+                                  C.c_uint64, C.POINTER(C.POINTER(rad_scene))]),
+    "rad_scene_free": (None, [C.POINTER(rad_scene)]),
+}
+
+
+def lib():
+    """Loads libradhip.so from the package directory (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RadError(f"{LIBPATH} is missing: run radiosity_amd.build.build() first")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def fptr(a):
+    return a.ctypes.data_as(_PF) if a is not None else None
+
+
+def uptr(a):
+    return a.ctypes.data_as(_PU) if a is not None else None
+
+
+def _arr(ptr, n, dtype, width=1):
+    if n == 0:
+        return np.zeros((0, width) if width > 1 else 0, dtype=dtype)
+    a = np.ctypeslib.as_array(ptr, shape=(n * width,)).astype(dtype, copy=True)
+    return a.reshape(n, width) if width > 1 else a
+
+
+class Scene:
+    """Host copy of a generated scene (numpy arrays)."""
+
+    def __init__(self, tri, tri_patch, frame, refl, area, emission, shot0, patch_quad=None):
+        self.tri = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 9)
+        self.tri_patch = np.ascontiguousarray(tri_patch, dtype=np.uint32)
+        self.frame = np.ascontiguousarray(frame, dtype=np.float32).reshape(-1, 9)
+        self.refl = np.ascontiguousarray(refl, dtype=np.float32).reshape(-1, 3)
+        self.area = np.ascontiguousarray(area, dtype=np.float32)
+        self.emission = np.ascontiguousarray(emission, dtype=np.float32).reshape(-1, 3)
+        self.shot0 = np.ascontiguousarray(shot0, dtype=np.float32).reshape(-1, 3)
+        self.patch_quad = patch_quad
+
+    @property
+    def ntri(self):
+        return self.tri.shape[0]
+
+    @property
+    def npatch(self):
+        return self.frame.shape[0]
+
+    @staticmethod
+    def _from_c(p):
+        s = p.contents
+        out = Scene(_arr(s.tri, s.ntri, np.float32, 9), _arr(s.tri_patch, s.ntri, np.uint32),
+                    _arr(s.frame, s.npatch, np.float32, 9), _arr(s.refl, s.npatch, np.float32, 3),
+                    _arr(s.area, s.npatch, np.float32), _arr(s.emission, s.npatch, np.float32, 3),
+                    _arr(s.shot0, s.npatch, np.float32, 3), _arr(s.patch_quad, s.npatch, np.uint32))
+        out.nquad = s.nquad
+        lib().rad_scene_free(p)
+        return out
+
+    @staticmethod
+    def cornell(max_area, triangle_patches=True, physical=True, seed=1):
+        p = C.POINTER(rad_scene)()
+        rc = lib().rad_scene_cornell(float(max_area), int(triangle_patches), int(physical),
+                                     int(seed), C.byref(p))
+        if rc != RAD_OK:
+            raise RadError(f"rad_scene_cornell failed ({rc})")
+        return Scene._from_c(p)
+
+    @staticmethod
+    def boxes(nboxes=400, room=(20.0, 5.0, 20.0), target_tris=200000, nlights=16, seed=3):
+        p = C.POINTER(rad_scene)()
+        rc = lib().rad_scene_boxes(int(nboxes), float(room[0]), float(room[1]), float(room[2]),
+                                   int(target_tris), int(nlights), int(seed), C.byref(p))
+        if rc != RAD_OK:
+            raise RadError(f"rad_scene_boxes failed ({rc})")
+        return Scene._from_c(p)
+
+
+class Solver:
+    """One rad_ctx: scene, state and the shooting passes on the current GPU."""
+
+    def __init__(self, hemicube=16, hemicubes=1, mode=RAD_MODE_PHYSICAL, select=RAD_SELECT_POWER,
+                 shoots=500, small_tri_px=0):
+        L = lib()
+        cfg = rad_config(hemicube, hemicubes, shoots, 4, mode, select, small_tri_px, 0)
+        self._ctx = C.c_void_p()
+        rc = L.rad_create(C.byref(cfg), C.byref(self._ctx))
+        if rc != RAD_OK:
+            raise RadError(f"rad_create failed ({rc})")
+        self.W, self.K = hemicube, hemicubes
+        tw, th, res = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        L.rad_geometry(self._ctx, C.byref(tw), C.byref(th), C.byref(res))
+        self.tex_w, self.tex_h, self.tex_res = tw.value, th.value, res.value
+        self.npatch = 0
+
+    def _chk(self, rc, what):
+        if rc != RAD_OK:
+            msg = lib().rad_last_error(self._ctx)
+            raise RadError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._ctx:
+            lib().rad_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dff(self):
+        out = np.zeros(self.tex_res, np.float32)
+        self._chk(lib().rad_precompute_dff(self._ctx, fptr(out)), "rad_precompute_dff")
+        return out
+
+    def upload(self, sc, state=True):
+        self._chk(lib().rad_scene_upload(self._ctx, fptr(sc.tri), uptr(sc.tri_patch), sc.ntri,
+                                         sc.npatch, fptr(sc.frame), fptr(sc.refl), fptr(sc.area)),
+                  "rad_scene_upload")
+        self.npatch = sc.npatch
+        if state:
+            self.set_state(sc.emission, sc.shot0, sc.emission)
+
+    def set_state(self, unshot, shot, emission=None):
+        u = np.ascontiguousarray(unshot, np.float32)
+        s = np.ascontiguousarray(shot, np.float32)
+        e = None if emission is None else np.ascontiguousarray(emission, np.float32)
+        self._chk(lib().rad_set_state(self._ctx, fptr(u), fptr(s), fptr(e)), "rad_set_state")
+
+    def state(self):
+        u = np.zeros((self.npatch, 3), np.float32)
+        s = np.zeros((self.npatch, 3), np.float32)
+        self._chk(lib().rad_get_state(self._ctx, fptr(u), fptr(s)), "rad_get_state")
+        return u, s
+
+    def select(self, k):
+        ids = np.zeros(k, np.uint32)
+        n = C.c_uint32()
+        self._chk(lib().rad_select(self._ctx, k, uptr(ids), C.byref(n)), "rad_select")
+        return ids, n.value
+
+    def render(self, shooters, depth=False):
+        sh = np.ascontiguousarray(shooters, np.uint32)
+        k = len(sh)
+        ids = np.zeros(k * self.tex_res, np.uint32)
+        z = np.zeros(k * self.tex_res, np.float32) if depth else None
+        self._chk(lib().rad_render_itembuf(self._ctx, uptr(sh), k, uptr(ids), fptr(z)),
+                  "rad_render_itembuf")
+        return (ids, z) if depth else ids
+
+    def process(self, k):
+        hits = np.zeros(k * self.npatch, np.uint32)
+        ff = np.zeros(k * self.npatch, np.float32)
+        self._chk(lib().rad_process_hemicube(self._ctx, k, uptr(hits), fptr(ff)),
+                  "rad_process_hemicube")
+        return hits.reshape(k, self.npatch), ff.reshape(k, self.npatch)
+
+    def process_itembuf(self, item, k):
+        it = np.ascontiguousarray(item, np.uint32)
+        hits = np.zeros(k * self.npatch, np.uint32)
+        ff = np.zeros(k * self.npatch, np.float32)
+        self._chk(lib().rad_process_itembuf(self._ctx, uptr(it), k, uptr(hits), fptr(ff)),
+                  "rad_process_itembuf")
+        return hits.reshape(k, self.npatch), ff.reshape(k, self.npatch)
+
+    def shoot(self, shooters):
+        sh = np.ascontiguousarray(shooters, np.uint32)
+        self._chk(lib().rad_shoot(self._ctx, uptr(sh), len(sh)), "rad_shoot")
+
+    def step(self, passes=1):
+        self._chk(lib().rad_step(self._ctx, passes), "rad_step")
+
+    def run(self, max_passes, target, check_every=0):
+        st = rad_stats()
+        self._chk(lib().rad_run(self._ctx, max_passes, target, check_every, C.byref(st)), "rad_run")
+        return st
+
+    def residual(self):
+        r = C.c_double()
+        self._chk(lib().rad_residual_fp64(self._ctx, C.byref(r)), "rad_residual_fp64")
+        return r.value
+
+    def face_mvps(self, patch):
+        out = np.zeros(80, np.float32)
+        self._chk(lib().rad_face_mvps(self._ctx, patch, fptr(out)), "rad_face_mvps")
+        return out.reshape(5, 16)
+
+    def kernel_times(self, enable=-1):
+        out = (C.c_double * 6)()
+        n = C.c_uint64()
+        self._chk(lib().rad_kernel_times(self._ctx, enable, out, C.byref(n)), "rad_kernel_times")
+        return list(out), n.value

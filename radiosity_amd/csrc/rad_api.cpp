@@ -1,0 +1,677 @@
+// rad_api.cpp - C ABI of libradhip: context, device memory, pass driver.
+//
+// The pass driver restates the OnIdle shoot loop (Main.cpp:1137-1309) as a
+// sequence of device kernels on one stream with no host synchronisation
+// between passes: select -> prepare -> clear -> raster setup -> raster tiles
+// -> reduce -> shoot.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rad_internal.h"
+
+using namespace rad;
+
+struct rad_ctx {
+    rad_config cfg{};
+    Geometry g{};
+    uint32_t K = 1;
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+
+    // scene
+    uint32_t ntri = 0, npatch = 0;
+    float *tri = nullptr;          // 9 planes of ntri
+    uint32_t *tri_patch = nullptr; // ntri
+    float *frames = nullptr;       // npatch*9
+    float *refl = nullptr;         // npatch*3
+    float *area = nullptr;         // npatch
+    float *emis = nullptr;         // npatch*3
+    float *unshot = nullptr;       // npatch*3
+    float *shot = nullptr;         // npatch*3
+    float *ff = nullptr;           // K*npatch
+    uint32_t *hits = nullptr;      // K*npatch
+
+    // per pass
+    float *dff = nullptr;          // RES
+    uint64_t *keys = nullptr;      // K*RES
+    float *mvp = nullptr;          // K*5*16
+    float *snap = nullptr;         // K*3
+    uint32_t *shooters = nullptr;  // K
+    uint32_t *nshoot = nullptr;    // 1
+    uint32_t *counters = nullptr;  // kCntN
+    uint64_t *sel = nullptr;       // 1 (+ pad)
+    double *resid = nullptr;       // 2 (+ scratch)
+    RasterRec *recs = nullptr;
+    TileItem *items = nullptr;
+    uint64_t *cand = nullptr;      // top-K candidates (two halves)
+    uint32_t rec_cap = 0, item_cap = 0;
+    int sel_blocks = 1024, tile_blocks = 4096;
+    bool rendered = false;        // keys hold the item buffer of the last render
+    uint32_t rendered_k = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double t_acc[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t t_passes = 0;
+};
+
+namespace {
+
+int fail(rad_ctx *c, int code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), RAD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T> int dalloc(rad_ctx *c, T **p, size_t n)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (n == 0)
+        n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, RAD_E_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                        " bytes: " + hipGetErrorString(e));
+    }
+    return RAD_OK;
+}
+
+void dfree(void *p)
+{
+    if (p)
+        (void)hipFree(p);
+}
+
+int check_launch(rad_ctx *c, const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return RAD_OK;
+}
+
+void record(rad_ctx *c, size_t idx)
+{
+    if (c->timing && idx < c->ev.size())
+        (void)hipEventRecord(c->ev[idx], c->st);
+}
+
+// Renders the hemicubes of the shooters currently in c->shooters/nshoot
+// (k slots) into c->keys.
+int render_pass(rad_ctx *c, uint32_t k, size_t evbase)
+{
+    launch_prepare(c->g, k, c->shooters, c->nshoot, c->frames, c->unshot, c->snap, c->mvp, c->st);
+    launch_clear(c->g, k, c->keys, c->ff, nullptr, c->npatch, c->counters, c->st);
+    record(c, evbase + 2);
+    launch_raster_setup(c->g, k, c->tri, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->keys, c->recs,
+                        c->rec_cap, c->items, c->item_cap, c->counters,
+                        c->cfg.small_tri_px ? c->cfg.small_tri_px : 64u, c->st);
+    record(c, evbase + 3);
+    launch_raster_tiles(c->recs, c->items, c->counters, c->rec_cap, c->item_cap,
+                        (int32_t)c->g.TW, c->keys, c->tile_blocks, c->st);
+    record(c, evbase + 4);
+    return check_launch(c, "render kernels");
+}
+
+int select_dev(rad_ctx *c, uint32_t k)
+{
+    if (k == 1) {
+        uint32_t rule = c->cfg.select;
+        launch_select_top1(rule, c->unshot, c->area, c->emis, c->npatch, c->shooters, c->nshoot,
+                           nullptr, c->sel, nullptr, c->resid, c->sel_blocks, c->st);
+        return check_launch(c, "select");
+    }
+    if (c->cfg.select == RAD_SELECT_REFERENCE)
+        launch_select_reference(c->unshot, c->npatch, k, c->shooters, c->nshoot, c->st);
+    else
+        launch_select_topk(c->cfg.select, c->unshot, c->area, c->emis, c->npatch, k, c->shooters,
+                           c->nshoot, c->cand, nullptr, nullptr, c->resid, c->st);
+    return check_launch(c, "select");
+}
+
+int one_pass(rad_ctx *c, bool do_select, size_t evbase)
+{
+    int rc;
+    record(c, evbase + 0);
+    if (do_select && (rc = select_dev(c, c->K)) != RAD_OK)
+        return rc;
+    record(c, evbase + 1);
+    if ((rc = render_pass(c, c->K, evbase)) != RAD_OK)
+        return rc;
+    launch_reduce(c->g, c->K, c->nshoot, c->keys, c->dff, c->npatch, c->ff, nullptr, c->counters,
+                  c->st);
+    record(c, evbase + 5);
+    launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl, c->area,
+                 c->npatch, c->unshot, c->shot, c->st);
+    record(c, evbase + 6);
+    c->rendered = true;
+    c->rendered_k = c->K;
+    return check_launch(c, "reduce/shoot");
+}
+
+int need_scene(rad_ctx *c)
+{
+    if (c->npatch == 0)
+        return fail(c, RAD_E_STATE, "no scene uploaded (rad_scene_upload)");
+    return RAD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rad_create(const rad_config *cfg, rad_ctx **out)
+{
+    if (!cfg || !out)
+        return RAD_E_ARG;
+    *out = nullptr;
+    if (cfg->hemicube < 2 || cfg->hemicube > RAD_MAX_W || (cfg->hemicube & 1))
+        return RAD_E_ARG;
+    if (cfg->hemicubes < 1 || cfg->hemicubes > RAD_MAX_K)
+        return RAD_E_ARG;
+    if (cfg->mode > RAD_MODE_PHYSICAL || cfg->select > RAD_SELECT_POWER)
+        return RAD_E_ARG;
+    rad_ctx *c = new (std::nothrow) rad_ctx();
+    if (!c)
+        return RAD_E_NOMEM;
+    c->cfg = *cfg;
+    if (c->cfg.ocl_workitems_x == 0)
+        c->cfg.ocl_workitems_x = 4;
+    c->K = cfg->hemicubes;
+    // Config.cpp:34-45
+    c->g.W = cfg->hemicube;
+    c->g.TW = c->g.W * 2;
+    c->g.TH = (uint32_t)((double)c->g.W * 1.5);
+    c->g.RES = c->g.TW * c->g.TH;
+    int rc = RAD_OK;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess)
+        e = hipStreamCreate(&c->st);
+    if (e != hipSuccess) {
+        delete c;
+        return RAD_E_HIP;
+    }
+    if ((rc = dalloc(c, &c->dff, c->g.RES)) || (rc = dalloc(c, &c->keys, (size_t)c->K * c->g.RES)) ||
+        (rc = dalloc(c, &c->mvp, (size_t)c->K * 80)) || (rc = dalloc(c, &c->snap, (size_t)c->K * 3)) ||
+        (rc = dalloc(c, &c->shooters, c->K)) || (rc = dalloc(c, &c->nshoot, 1)) ||
+        (rc = dalloc(c, &c->counters, kCntN)) || (rc = dalloc(c, &c->sel, 4)) ||
+        (rc = dalloc(c, &c->resid, 8))) {
+        rad_destroy(c);
+        return rc;
+    }
+    if (hipMemsetAsync(c->counters, 0, kCntN * sizeof(uint32_t), c->st) != hipSuccess ||
+        hipMemsetAsync(c->nshoot, 0, sizeof(uint32_t), c->st) != hipSuccess ||
+        hipMemsetAsync(c->shooters, 0, c->K * sizeof(uint32_t), c->st) != hipSuccess) {
+        rad_destroy(c);
+        return RAD_E_HIP;
+    }
+    launch_dff(c->g, c->dff, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+        rad_destroy(c);
+        return RAD_E_HIP;
+    }
+    int ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        ncu = prop.multiProcessorCount;
+    c->sel_blocks = ncu * 2;
+    c->tile_blocks = ncu * 8;
+    *out = c;
+    return RAD_OK;
+}
+
+void rad_destroy(rad_ctx *c)
+{
+    if (!c)
+        return;
+    if (c->st)
+        (void)hipStreamSynchronize(c->st);
+    void *ptrs[] = {c->tri,  c->tri_patch, c->frames, c->refl,     c->area,   c->emis,
+                    c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
+                    c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
+                    c->resid, c->recs,    c->items,  c->cand};
+    for (void *p : ptrs)
+        dfree(p);
+    for (hipEvent_t e : c->ev)
+        (void)hipEventDestroy(e);
+    if (c->st)
+        (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char *rad_last_error(const rad_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int rad_geometry(const rad_ctx *c, uint32_t *tw, uint32_t *th, uint32_t *res)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (tw)
+        *tw = c->g.TW;
+    if (th)
+        *th = c->g.TH;
+    if (res)
+        *res = c->g.RES;
+    return RAD_OK;
+}
+
+int rad_precompute_dff(rad_ctx *c, float *out)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (out) {
+        HIPCHK(c, hipMemcpyAsync(out, c->dff, c->g.RES * sizeof(float), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    return RAD_OK;
+}
+
+int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                     uint32_t npatch, const float *frames, const float *refl, const float *area)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (!tri_xyz || !tri_patch || !frames || !refl || !area || npatch == 0)
+        return fail(c, RAD_E_ARG, "scene: null array or zero patches");
+    for (uint32_t t = 0; t < ntri; ++t) {
+        if (tri_patch[t] >= npatch)
+            return fail(c, RAD_E_ARG, "scene: tri_patch[" + std::to_string(t) + "] >= npatch");
+        if (t && tri_patch[t] < tri_patch[t - 1])
+            return fail(c, RAD_E_ARG, "scene: tri_patch must be non-decreasing (draw order)");
+    }
+    for (uint64_t i = 0; i < (uint64_t)ntri * 9; ++i)
+        if (!std::isfinite(tri_xyz[i]) || std::fabs(tri_xyz[i]) > 1.0e4f)
+            return fail(c, RAD_E_ARG, "scene: vertex coordinate not finite or beyond 1e4");
+    for (uint32_t i = 0; i < npatch; ++i)
+        if (!(area[i] > 0.0f))
+            return fail(c, RAD_E_ARG, "scene: patch area must be > 0");
+    (void)hipStreamSynchronize(c->st);
+    int rc;
+    size_t K = c->K;
+    if ((rc = dalloc(c, &c->tri, (size_t)ntri * 9)) || (rc = dalloc(c, &c->tri_patch, ntri)) ||
+        (rc = dalloc(c, &c->frames, (size_t)npatch * 9)) || (rc = dalloc(c, &c->refl, (size_t)npatch * 3)) ||
+        (rc = dalloc(c, &c->area, npatch)) || (rc = dalloc(c, &c->emis, (size_t)npatch * 3)) ||
+        (rc = dalloc(c, &c->unshot, (size_t)npatch * 3)) || (rc = dalloc(c, &c->shot, (size_t)npatch * 3)) ||
+        (rc = dalloc(c, &c->ff, K * npatch)) || (rc = dalloc(c, &c->hits, K * npatch)))
+        return rc;
+    uint64_t want = (uint64_t)K * 5 * (uint64_t)ntri / 4 + 65536;
+    c->rec_cap = (uint32_t)(want > (1u << 23) ? (1u << 23) : want);
+    c->item_cap = c->rec_cap * 4;
+    if ((rc = dalloc(c, &c->recs, c->rec_cap)) || (rc = dalloc(c, &c->items, c->item_cap)) ||
+        (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
+        return rc;
+    // triangles -> 9 planes
+    std::vector<float> planes((size_t)ntri * 9);
+    for (uint32_t t = 0; t < ntri; ++t)
+        for (int k = 0; k < 9; ++k)
+            planes[(size_t)k * ntri + t] = tri_xyz[(size_t)t * 9 + k];
+    HIPCHK(c, hipMemcpy(c->tri, planes.data(), planes.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->tri_patch, tri_patch, (size_t)ntri * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->frames, frames, (size_t)npatch * 36, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->refl, refl, (size_t)npatch * 12, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->area, area, (size_t)npatch * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->emis, 0, (size_t)npatch * 12));
+    HIPCHK(c, hipMemset(c->unshot, 0, (size_t)npatch * 12));
+    HIPCHK(c, hipMemset(c->shot, 0, (size_t)npatch * 12));
+    HIPCHK(c, hipMemset(c->ff, 0, K * npatch * 4));
+    HIPCHK(c, hipMemset(c->hits, 0, K * npatch * 4));
+    c->ntri = ntri;
+    c->npatch = npatch;
+    c->rendered = false;
+    return RAD_OK;
+}
+
+int rad_set_state(rad_ctx *c, const float *unshot, const float *shot, const float *emission)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    size_t n = (size_t)c->npatch * 12;
+    if (unshot)
+        HIPCHK(c, hipMemcpyAsync(c->unshot, unshot, n, hipMemcpyHostToDevice, c->st));
+    if (shot)
+        HIPCHK(c, hipMemcpyAsync(c->shot, shot, n, hipMemcpyHostToDevice, c->st));
+    if (emission)
+        HIPCHK(c, hipMemcpyAsync(c->emis, emission, n, hipMemcpyHostToDevice, c->st));
+    else if (unshot)
+        HIPCHK(c, hipMemcpyAsync(c->emis, unshot, n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+int rad_get_state(rad_ctx *c, float *unshot, float *shot)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    size_t n = (size_t)c->npatch * 12;
+    if (unshot)
+        HIPCHK(c, hipMemcpyAsync(unshot, c->unshot, n, hipMemcpyDeviceToHost, c->st));
+    if (shot)
+        HIPCHK(c, hipMemcpyAsync(shot, c->shot, n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+int rad_select(rad_ctx *c, uint32_t k, uint32_t *out_ids, uint32_t *out_n)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (k < 1 || k > c->K)
+        return fail(c, RAD_E_ARG, "select: k must be in [1, hemicubes]");
+    if ((rc = select_dev(c, k)))
+        return rc;
+    uint32_t n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, c->nshoot, 4, hipMemcpyDeviceToHost, c->st));
+    if (out_ids)
+        HIPCHK(c, hipMemcpyAsync(out_ids, c->shooters, (size_t)k * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (out_n)
+        *out_n = n;
+    return RAD_OK;
+}
+
+static int upload_shooters(rad_ctx *c, const uint32_t *shooters, uint32_t k)
+{
+    if (!shooters || k < 1 || k > c->K)
+        return fail(c, RAD_E_ARG, "shooters: k must be in [1, hemicubes]");
+    for (uint32_t s = 0; s < k; ++s) {
+        if (shooters[s] >= c->npatch)
+            return fail(c, RAD_E_ARG, "shooters: patch index out of range");
+        for (uint32_t r = 0; r < s; ++r)
+            if (shooters[r] == shooters[s])
+                return fail(c, RAD_E_ARG, "shooters: duplicate patch index");
+    }
+    HIPCHK(c, hipMemcpyAsync(c->shooters, shooters, (size_t)k * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->nshoot, &k, 4, hipMemcpyHostToDevice, c->st));
+    // the host value k must outlive the async copy
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_t *out_ids,
+                       float *out_depth)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)) || (rc = upload_shooters(c, shooters, k)))
+        return rc;
+    if ((rc = render_pass(c, k, 0)))
+        return rc;
+    c->rendered = true;
+    c->rendered_k = k;
+    size_t n = (size_t)k * c->g.RES;
+    if (out_ids || out_depth) {
+        uint32_t *dids = nullptr;
+        float *dz = nullptr;
+        if (out_ids && (rc = dalloc(c, &dids, n)))
+            return rc;
+        if (out_depth && (rc = dalloc(c, &dz, n))) {
+            dfree(dids);
+            return rc;
+        }
+        launch_split_keys(c->keys, dids, dz, n, c->st);
+        if (out_ids)
+            (void)hipMemcpyAsync(out_ids, dids, n * 4, hipMemcpyDeviceToHost, c->st);
+        if (out_depth)
+            (void)hipMemcpyAsync(out_depth, dz, n * 4, hipMemcpyDeviceToHost, c->st);
+        hipError_t e = hipStreamSynchronize(c->st);
+        dfree(dids);
+        dfree(dz);
+        if (e != hipSuccess)
+            return fail(c, RAD_E_HIP, std::string("render copy-out: ") + hipGetErrorString(e));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
+{
+    size_t nf = (size_t)k * c->npatch;
+    HIPCHK(c, hipMemsetAsync(c->ff, 0, nf * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(c->hits, 0, nf * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(c->counters, 0, kCntN * 4, c->st));
+    launch_reduce(c->g, k, c->nshoot, c->keys, c->dff, c->npatch, c->ff, c->hits, c->counters, c->st);
+    int rc;
+    if ((rc = check_launch(c, "reduce")))
+        return rc;
+    if (out_hits)
+        HIPCHK(c, hipMemcpyAsync(out_hits, c->hits, nf * 4, hipMemcpyDeviceToHost, c->st));
+    if (out_ff)
+        HIPCHK(c, hipMemcpyAsync(out_ff, c->ff, nf * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+int rad_process_hemicube(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (!c->rendered || k != c->rendered_k)
+        return fail(c, RAD_E_STATE, "process_hemicube: k must match the last render");
+    return reduce_out(c, k, out_hits, out_ff);
+}
+
+int rad_process_itembuf(rad_ctx *c, const uint32_t *itembuf, uint32_t k, uint32_t *out_hits,
+                        float *out_ff)
+{
+    if (!c || !itembuf)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (k < 1 || k > c->K)
+        return fail(c, RAD_E_ARG, "process_itembuf: k must be in [1, hemicubes]");
+    size_t n = (size_t)k * c->g.RES;
+    uint32_t *dids = nullptr;
+    if ((rc = dalloc(c, &dids, n)))
+        return rc;
+    (void)hipMemcpyAsync(dids, itembuf, n * 4, hipMemcpyHostToDevice, c->st);
+    launch_ids_to_keys(dids, c->keys, n, c->st);
+    HIPCHK(c, hipMemcpyAsync(c->nshoot, &k, 4, hipMemcpyHostToDevice, c->st));
+    hipError_t e = hipStreamSynchronize(c->st);
+    dfree(dids);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("process_itembuf upload: ") + hipGetErrorString(e));
+    c->rendered = true;
+    c->rendered_k = k;
+    return reduce_out(c, k, out_hits, out_ff);
+}
+
+int rad_shoot(rad_ctx *c, const uint32_t *shooters, uint32_t k)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)) || (rc = upload_shooters(c, shooters, k)))
+        return rc;
+    uint32_t saveK = c->K;
+    c->K = k;
+    rc = one_pass(c, false, 0);
+    c->K = saveK;
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+static int ensure_events(rad_ctx *c, size_t n)
+{
+    while (c->ev.size() < n) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    return RAD_OK;
+}
+
+static int step_impl(rad_ctx *c, uint32_t passes)
+{
+    int rc;
+    if (c->timing && (rc = ensure_events(c, (size_t)passes * 7)))
+        return rc;
+    for (uint32_t p = 0; p < passes; ++p)
+        if ((rc = one_pass(c, true, (size_t)p * 7)))
+            return rc;
+    if (c->timing) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (uint32_t p = 0; p < passes; ++p) {
+            float ms[6];
+            for (int k = 0; k < 6; ++k) {
+                ms[k] = 0.f;
+                (void)hipEventElapsedTime(&ms[k], c->ev[(size_t)p * 7 + k],
+                                          c->ev[(size_t)p * 7 + k + 1]);
+            }
+            // stages: select, prepare+clear, setup, tiles, reduce, shoot
+            for (int k = 0; k < 6; ++k)
+                c->t_acc[k] += ms[k];
+        }
+        c->t_passes += passes;
+    }
+    return RAD_OK;
+}
+
+int rad_step(rad_ctx *c, uint32_t passes)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if ((rc = step_impl(c, passes)))
+        return rc;
+    return RAD_OK;
+}
+
+int rad_residual_fp64(rad_ctx *c, double *out)
+{
+    if (!c || !out)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    launch_residual(c->unshot, c->area, c->emis, c->npatch, nullptr, nullptr, c->resid + 4,
+                    c->sel_blocks, c->st);
+    if ((rc = check_launch(c, "residual")))
+        return rc;
+    double r[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(r, c->resid + 4, 16, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    *out = r[1] > 0 ? r[0] / r[1] : 0.0;
+    return RAD_OK;
+}
+
+int rad_run(rad_ctx *c, uint64_t max_passes, double target, uint32_t check_every, rad_stats *out)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (check_every == 0)
+        check_every = c->cfg.shoots ? c->cfg.shoots : 500;
+    auto t0 = std::chrono::steady_clock::now();
+    uint64_t done = 0;
+    double r = 0.0;
+    if ((rc = rad_residual_fp64(c, &r)))
+        return rc;
+    while (done < max_passes && !(r <= target)) {
+        uint64_t n = max_passes - done;
+        if (n > check_every)
+            n = check_every;
+        if ((rc = step_impl(c, (uint32_t)n)))
+            return rc;
+        done += n;
+        if ((rc = rad_residual_fp64(c, &r)))
+            return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    auto t1 = std::chrono::steady_clock::now();
+    if (out) {
+        out->passes = done;
+        out->shooters = done * c->K;
+        out->residual = r;
+        out->seconds = std::chrono::duration<double>(t1 - t0).count();
+        float sn[3] = {0, 0, 0};
+        if (done) {
+            uint32_t n = 0;
+            (void)hipMemcpy(&n, c->nshoot, 4, hipMemcpyDeviceToHost);
+            if (n)
+                (void)hipMemcpy(sn, c->snap + 3 * (n - 1), 12, hipMemcpyDeviceToHost);
+        }
+        out->last_energy = std::sqrt(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]);
+        out->reserved = 0;
+    }
+    return RAD_OK;
+}
+
+int rad_face_mvps(rad_ctx *c, uint32_t patch, float *out80)
+{
+    if (!c || !out80)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (patch >= c->npatch)
+        return fail(c, RAD_E_ARG, "face_mvps: patch out of range");
+    uint32_t one = 1;
+    HIPCHK(c, hipMemcpyAsync(c->shooters, &patch, 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->nshoot, &one, 4, hipMemcpyHostToDevice, c->st));
+    launch_prepare(c->g, 1, c->shooters, c->nshoot, c->frames, c->unshot, c->snap, c->mvp, c->st);
+    if ((rc = check_launch(c, "prepare")))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(out80, c->mvp, 80 * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+int rad_kernel_times(rad_ctx *c, int enable, double *out6, uint64_t *npasses)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (out6)
+        for (int k = 0; k < 6; ++k)
+            out6[k] = c->t_passes ? c->t_acc[k] / (double)c->t_passes : 0.0;
+    if (npasses)
+        *npasses = c->t_passes;
+    if (enable >= 0) {
+        c->timing = enable != 0;
+        for (int k = 0; k < 6; ++k)
+            c->t_acc[k] = 0;
+        c->t_passes = 0;
+    }
+    return RAD_OK;
+}
+
+} // extern "C"

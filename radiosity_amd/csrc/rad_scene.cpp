@@ -1,0 +1,444 @@
+// rad_scene.cpp - scene sources (host code): the reference's Cornell data
+// with its quad subdivision, and seeded random-box rooms.
+//
+// Compiled with -ffp-contract=off so the float subdivision arithmetic is the
+// written sequence (Patch.cpp:47-125).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "../../include/radhip.h"
+#include "../../include/radhip_scene.h"
+
+namespace {
+
+struct V3 {
+    float x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, float t) { return {a.x * t, a.y * t, a.z * t}; }
+// Vector.h:448-452: division by a scalar multiplies by its reciprocal
+inline V3 divs(V3 a, float t)
+{
+    t = 1 / t;
+    return {a.x * t, a.y * t, a.z * t};
+}
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float length(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Vector.h:534-537: a.v_Cross(b) is b x a
+inline V3 vcross(V3 a, V3 b)
+{
+    return {b.y * a.z - b.z * a.y, b.z * a.x - b.x * a.z, b.x * a.y - b.y * a.x};
+}
+inline V3 cross(V3 a, V3 b) { return vcross(b, a); }
+
+struct Quad {
+    V3 v[4];
+    V3 color, illum, rad;
+    int lightish; // 1 for an emitter source quad
+    int group;    // source surface (used to pick ceiling lights)
+};
+
+// Patch::divide (Patch.cpp:47-125): returns false when the quad is already
+// small enough (S <= area * 1.01).
+bool divide(const Quad &q, double area, std::vector<Quad> &out)
+{
+    V3 A = q.v[0], B = q.v[1], C = q.v[2], D = q.v[3];
+    V3 u1 = A - C, u2 = B - D;
+    double phi = (double)acosf(dot(u1, u2) / (length(u1) * length(u2)));
+    double S = (double)(0.5f * length(u1) * length(u2)) * sin(phi);
+    if (S <= (area * 1.01))
+        return false;
+    double a = sqrt(area);
+    unsigned kx = (unsigned)ceil((double)length(B - A) / a);
+    unsigned ky = (unsigned)ceil((double)length(C - B) / a);
+    V3 pCD = divs(C - D, (float)kx);
+    V3 pAB = divs(B - A, (float)kx);
+    for (unsigned i = 0; i < kx * ky; ++i) {
+        unsigned col = i % kx, row = i / kx;
+        V3 bCD = pCD * (float)col + D;
+        V3 bAB = pAB * (float)col + A;
+        V3 nA = bAB + divs(bCD - bAB, (float)ky) * (float)row;
+        V3 bCD1 = pCD * (float)(col + 1) + D;
+        V3 bAB1 = pAB * (float)(col + 1) + A;
+        V3 nB = bAB1 + divs(bCD1 - bAB1, (float)ky) * (float)row;
+        V3 nC = bAB1 + divs(bCD1 - bAB1, (float)ky) * (float)(row + 1);
+        V3 nD = bAB + divs(bCD - bAB, (float)ky) * (float)(row + 1);
+        Quad c = q;
+        c.v[0] = nA;
+        c.v[1] = nB;
+        c.v[2] = nC;
+        c.v[3] = nD;
+        out.push_back(c);
+    }
+    return true;
+}
+
+// Model::subdivide (Model.cpp:27-60): one pass, children replace parents in order.
+std::vector<Quad> subdivide(const std::vector<Quad> &in, double area, bool keep_lights)
+{
+    std::vector<Quad> out;
+    for (const Quad &q : in) {
+        if (area > 0 && !(keep_lights && q.lightish) && divide(q, area, out))
+            continue;
+        out.push_back(q);
+    }
+    return out;
+}
+
+uint64_t count_tris(const std::vector<Quad> &in, double area)
+{
+    uint64_t n = 0;
+    std::vector<Quad> tmp;
+    for (const Quad &q : in) {
+        tmp.clear();
+        if (divide(q, area, tmp))
+            n += 2 * tmp.size();
+        else
+            n += 2;
+    }
+    return n;
+}
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull) {}
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float uni(float lo, float hi) { return lo + (hi - lo) * (float)((next() >> 40) * (1.0 / 16777216.0)); }
+};
+
+rad_scene *alloc_scene(uint32_t ntri, uint32_t npatch)
+{
+    rad_scene *s = (rad_scene *)calloc(1, sizeof(rad_scene));
+    if (!s)
+        return nullptr;
+    s->ntri = ntri;
+    s->npatch = npatch;
+    s->tri = (float *)calloc((size_t)ntri * 9 + 1, sizeof(float));
+    s->tri_patch = (uint32_t *)calloc((size_t)ntri + 1, sizeof(uint32_t));
+    s->frame = (float *)calloc((size_t)npatch * 9 + 1, sizeof(float));
+    s->refl = (float *)calloc((size_t)npatch * 3 + 1, sizeof(float));
+    s->area = (float *)calloc((size_t)npatch + 1, sizeof(float));
+    s->emission = (float *)calloc((size_t)npatch * 3 + 1, sizeof(float));
+    s->shot0 = (float *)calloc((size_t)npatch * 3 + 1, sizeof(float));
+    s->patch_quad = (uint32_t *)calloc((size_t)npatch + 1, sizeof(uint32_t));
+    if (!s->tri || !s->tri_patch || !s->frame || !s->refl || !s->area || !s->emission ||
+        !s->shot0 || !s->patch_quad) {
+        rad_scene_free(s);
+        return nullptr;
+    }
+    return s;
+}
+
+float tri_area(V3 a, V3 b, V3 c)
+{
+    V3 n = cross(b - a, c - a);
+    return (float)(0.5 * sqrt((double)n.x * n.x + (double)n.y * n.y + (double)n.z * n.z));
+}
+
+// Flattens quads like ModelContainer::updateData (ModelContainer.cpp:81-155):
+// triangles (0,1,2), (0,2,3) per quad.  Per-patch values come from `fill`.
+template <typename Fill>
+rad_scene *flatten(const std::vector<Quad> &quads, bool tri_patches, Fill fill)
+{
+    uint32_t nq = (uint32_t)quads.size();
+    uint32_t np = tri_patches ? 2 * nq : nq;
+    rad_scene *s = alloc_scene(2 * nq, np);
+    if (!s)
+        return nullptr;
+    s->nquad = nq;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const Quad &Q = quads[q];
+        const V3 *v = Q.v;
+        V3 t0[3] = {v[0], v[1], v[2]}, t1[3] = {v[0], v[2], v[3]};
+        for (int k = 0; k < 3; ++k) {
+            memcpy(&s->tri[(size_t)(2 * q) * 9 + 3 * k], &t0[k], 12);
+            memcpy(&s->tri[(size_t)(2 * q + 1) * 9 + 3 * k], &t1[k], 12);
+        }
+        if (tri_patches) {
+            s->tri_patch[2 * q] = 2 * q;
+            s->tri_patch[2 * q + 1] = 2 * q + 1;
+            for (int h = 0; h < 2; ++h) {
+                const V3 *t = h ? t1 : t0;
+                uint32_t p = 2 * q + h;
+                // triangle frame: centroid, (c-a) x (b-a), up c-a - the quad
+                // rule of Patch.cpp:253-276 applied to the triangle's edges
+                V3 eye = {(t[0].x + t[1].x + t[2].x) / 3.0f, (t[0].y + t[1].y + t[2].y) / 3.0f,
+                          (t[0].z + t[1].z + t[2].z) / 3.0f};
+                V3 n = vcross(t[1] - t[0], t[2] - t[0]);
+                V3 up = t[2] - t[0];
+                float fr[9] = {eye.x, eye.y, eye.z, n.x, n.y, n.z, up.x, up.y, up.z};
+                memcpy(&s->frame[(size_t)p * 9], fr, sizeof(fr));
+                s->area[p] = tri_area(t[0], t[1], t[2]);
+                s->patch_quad[p] = q;
+                fill(s, p, Q, h);
+            }
+        } else {
+            s->tri_patch[2 * q] = q;
+            s->tri_patch[2 * q + 1] = q;
+            // Patch.cpp:253-276
+            V3 eye = {(v[0].x + v[1].x + v[2].x + v[3].x) / 4.0f,
+                      (v[0].y + v[1].y + v[2].y + v[3].y) / 4.0f,
+                      (v[0].z + v[1].z + v[2].z + v[3].z) / 4.0f};
+            V3 n = vcross(v[1] - v[0], v[3] - v[0]);
+            V3 up = v[3] - v[0];
+            float fr[9] = {eye.x, eye.y, eye.z, n.x, n.y, n.z, up.x, up.y, up.z};
+            memcpy(&s->frame[(size_t)q * 9], fr, sizeof(fr));
+            s->area[q] = tri_area(v[0], v[1], v[2]) + tri_area(v[0], v[2], v[3]);
+            s->patch_quad[q] = q;
+            fill(s, q, Q, 0);
+        }
+    }
+    return s;
+}
+
+// PrimitiveModel.cpp:92-213 data (vertices x,y,z per corner, 4 corners per quad)
+const float kRoom[60] = {
+    0.0f,   0.0f,   5.592f, 5.496f, 0.0f,   5.592f, 5.560f, 5.488f, 5.592f, 0.0f,   5.488f, 5.592f,
+    0.0f,   5.488f, 0.0f,   0.0f,   5.488f, 5.592f, 5.560f, 5.488f, 5.592f, 5.560f, 5.488f, 0.0f,
+    0.0f,   0.0f,   0.0f,   5.528f, 0.0f,   0.0f,   5.496f, 0.0f,   5.592f, 0.0f,   0.0f,   5.592f,
+    0.0f,   0.0f,   0.0f,   0.0f,   0.0f,   5.592f, 0.0f,   5.488f, 5.593f, 0.0f,   5.488f, 0.0f,
+    5.496f, 0.0f,   5.592f, 5.528f, 0.0f,   0.0f,   5.560f, 5.488f, 0.0f,   5.560f, 5.488f, 5.592f};
+const float kRoomColors[15] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 1, 0};
+const float kRoomClosure[12] = {5.528f, 0.0f, 0.0f, 0.0f,   0.0f,   0.0f,
+                                0.0f,   5.488f, 0.0f, 5.560f, 5.488f, 0.0f};
+const float kCube[60] = {
+    1.3f,  1.65f, 0.65f, 2.9f,  1.65f, 1.14f, 2.4f,  1.65f, 2.72f, 0.82f, 1.65f, 2.25f,
+    2.9f,  0.0f,  1.14f, 2.4f,  0.0f,  2.72f, 2.4f,  1.65f, 2.72f, 2.9f,  1.65f, 1.14f,
+    1.3f,  0.0f,  0.65f, 2.9f,  0.0f,  1.14f, 2.9f,  1.65f, 1.14f, 1.3f,  1.65f, 0.65f,
+    0.82f, 0.0f,  2.25f, 1.3f,  0.0f,  0.65f, 1.3f,  1.65f, 0.65f, 0.82f, 1.65f, 2.25f,
+    2.4f,  0.0f,  2.72f, 0.82f, 0.0f,  2.25f, 0.82f, 1.65f, 2.25f, 2.4f,  1.65f, 2.72f};
+const float kBlock[60] = {
+    4.23f, 3.3f, 2.47f, 4.72f, 3.3f, 4.06f, 3.14f, 3.3f, 4.56f, 2.65f, 3.3f, 2.96f,
+    4.23f, 0.0f, 2.47f, 4.72f, 0.0f, 4.06f, 4.72f, 3.3f, 4.06f, 4.23f, 3.3f, 2.47f,
+    4.72f, 0.0f, 4.06f, 3.14f, 0.0f, 4.56f, 3.14f, 3.3f, 4.56f, 4.72f, 3.3f, 4.06f,
+    3.14f, 0.0f, 4.56f, 2.65f, 0.0f, 2.96f, 2.65f, 3.3f, 2.96f, 3.14f, 3.3f, 4.56f,
+    2.65f, 0.0f, 2.96f, 4.23f, 0.0f, 2.47f, 4.23f, 3.3f, 2.47f, 2.65f, 3.3f, 2.96f};
+
+Quad quad_from(const float *p, V3 color)
+{
+    Quad q;
+    for (int k = 0; k < 4; ++k)
+        q.v[k] = {p[3 * k], p[3 * k + 1], p[3 * k + 2]};
+    q.color = color;
+    q.illum = {0, 0, 0};
+    q.rad = {0, 0, 0};
+    q.lightish = 0;
+    q.group = 0;
+    return q;
+}
+
+// One model's quads subdivided like Model::getPatches(area) (PrimitiveModel.cpp:79-84).
+std::vector<Quad> cornell_quads(double max_area, bool keep_light)
+{
+    std::vector<Quad> all;
+    // ModelContainer.cpp:38-47: ROOM, ROOMCLOSURE, CUBE, BLOCK
+    std::vector<Quad> room;
+    for (int i = 0; i < 5; ++i)
+        room.push_back(quad_from(kRoom + 12 * i, {kRoomColors[3 * i], kRoomColors[3 * i + 1],
+                                                   kRoomColors[3 * i + 2]}));
+    {
+        // PrimitiveModel.cpp:22-28: the light, radiosity 100 and illumination 1
+        const float l[12] = {3.430f, 5.485f, 2.270f, 2.130f, 5.485f, 2.270f,
+                             2.130f, 5.485f, 3.320f, 3.430f, 5.485f, 3.320f};
+        Quad q = quad_from(l, {1.0f, 1.0f, 1.0f});
+        q.illum = {1.0f, 1.0f, 1.0f};
+        q.rad = {100.0f, 100.0f, 100.0f};
+        q.lightish = 1;
+        room.push_back(q);
+    }
+    std::vector<Quad> closure = {quad_from(kRoomClosure, {1, 1, 1})};
+    std::vector<Quad> cube, block;
+    for (int i = 0; i < 5; ++i) {
+        cube.push_back(quad_from(kCube + 12 * i, {1, 1, 1}));
+        block.push_back(quad_from(kBlock + 12 * i, {1, 1, 1}));
+    }
+    for (auto *m : {&room, &closure, &cube, &block}) {
+        std::vector<Quad> d = subdivide(*m, max_area, keep_light);
+        all.insert(all.end(), d.begin(), d.end());
+    }
+    return all;
+}
+
+// Axis-aligned face as a quad with normal = (v4-v1) x (v2-v1) along n.
+Quad face_quad(V3 o, V3 e1, V3 e2, V3 n, int group)
+{
+    V3 c = cross(e2, e1);
+    if (dot(c, n) < 0) {
+        V3 t = e1;
+        e1 = e2;
+        e2 = t;
+    }
+    Quad q;
+    q.v[0] = o;
+    q.v[1] = o + e1;
+    q.v[2] = o + e1 + e2;
+    q.v[3] = o + e2;
+    q.color = {1, 1, 1};
+    q.illum = {0, 0, 0};
+    q.rad = {0, 0, 0};
+    q.lightish = 0;
+    q.group = group;
+    return q;
+}
+
+} // namespace
+
+extern "C" {
+
+void rad_scene_free(rad_scene *s)
+{
+    if (!s)
+        return;
+    free(s->tri);
+    free(s->tri_patch);
+    free(s->frame);
+    free(s->refl);
+    free(s->area);
+    free(s->emission);
+    free(s->shot0);
+    free(s->patch_quad);
+    free(s);
+}
+
+int rad_scene_cornell(double max_area, int triangle_patches, int physical, uint64_t seed,
+                      rad_scene **out)
+{
+    if (!out || !(max_area >= 0.0))
+        return RAD_E_ARG;
+    *out = nullptr;
+    std::vector<Quad> quads = cornell_quads(max_area, physical != 0);
+    Rng rng(seed);
+    rad_scene *s = flatten(quads, triangle_patches != 0, [&](rad_scene *sc, uint32_t p, const Quad &q, int) {
+        if (physical) {
+            sc->refl[3 * p + 0] = rng.uni(0.1f, 0.8f);
+            sc->refl[3 * p + 1] = rng.uni(0.1f, 0.8f);
+            sc->refl[3 * p + 2] = rng.uni(0.1f, 0.8f);
+            if (q.lightish) {
+                sc->emission[3 * p + 0] = 17.0f;
+                sc->emission[3 * p + 1] = 12.0f;
+                sc->emission[3 * p + 2] = 4.0f;
+            }
+        } else {
+            sc->refl[3 * p + 0] = q.color.x;
+            sc->refl[3 * p + 1] = q.color.y;
+            sc->refl[3 * p + 2] = q.color.z;
+            sc->emission[3 * p + 0] = q.rad.x;
+            sc->emission[3 * p + 1] = q.rad.y;
+            sc->emission[3 * p + 2] = q.rad.z;
+            sc->shot0[3 * p + 0] = q.illum.x;
+            sc->shot0[3 * p + 1] = q.illum.y;
+            sc->shot0[3 * p + 2] = q.illum.z;
+        }
+    });
+    if (!s)
+        return RAD_E_NOMEM;
+    *out = s;
+    return RAD_OK;
+}
+
+int rad_scene_boxes(uint32_t nboxes, float rx, float ry, float rz, uint32_t target_tris,
+                    uint32_t nlights, uint64_t seed, rad_scene **out)
+{
+    if (!out || !(rx > 0 && ry > 0 && rz > 0) || nboxes > 100000)
+        return RAD_E_ARG;
+    *out = nullptr;
+    Rng rng(seed);
+    std::vector<Quad> base;
+    // room, inward normals; group 1 = ceiling
+    base.push_back(face_quad({0, 0, 0}, {rx, 0, 0}, {0, 0, rz}, {0, 1, 0}, 0));   // floor
+    base.push_back(face_quad({0, ry, 0}, {rx, 0, 0}, {0, 0, rz}, {0, -1, 0}, 1)); // ceiling
+    base.push_back(face_quad({0, 0, 0}, {rx, 0, 0}, {0, ry, 0}, {0, 0, 1}, 0));
+    base.push_back(face_quad({0, 0, rz}, {rx, 0, 0}, {0, ry, 0}, {0, 0, -1}, 0));
+    base.push_back(face_quad({0, 0, 0}, {0, 0, rz}, {0, ry, 0}, {1, 0, 0}, 0));
+    base.push_back(face_quad({rx, 0, 0}, {0, 0, rz}, {0, ry, 0}, {-1, 0, 0}, 0));
+    struct Box {
+        float x0, y0, z0, x1, y1, z1;
+    };
+    std::vector<Box> boxes;
+    for (uint32_t b = 0; b < nboxes; ++b) {
+        Box bx{};
+        bool placed = false;
+        for (int attempt = 0; attempt < 200 && !placed; ++attempt) {
+            float sx = rng.uni(0.2f, 2.0f), sy = rng.uni(0.2f, 2.0f), sz = rng.uni(0.2f, 2.0f);
+            sx = std::min(sx, rx * 0.9f);
+            sy = std::min(sy, ry * 0.9f);
+            sz = std::min(sz, rz * 0.9f);
+            float x0 = rng.uni(0.0f, rx - sx), y0 = rng.uni(0.0f, ry - sy), z0 = rng.uni(0.0f, rz - sz);
+            bx = {x0, y0, z0, x0 + sx, y0 + sy, z0 + sz};
+            placed = true;
+            for (const Box &o : boxes)
+                if (bx.x0 < o.x1 + 0.01f && o.x0 < bx.x1 + 0.01f && bx.y0 < o.y1 + 0.01f &&
+                    o.y0 < bx.y1 + 0.01f && bx.z0 < o.z1 + 0.01f && o.z0 < bx.z1 + 0.01f) {
+                    placed = false;
+                    break;
+                }
+            if (attempt == 199)
+                placed = true; // crowded room: accept an overlap
+        }
+        boxes.push_back(bx);
+        V3 o = {bx.x0, bx.y0, bx.z0};
+        float sx = bx.x1 - bx.x0, sy = bx.y1 - bx.y0, sz = bx.z1 - bx.z0;
+        base.push_back(face_quad(o, {sx, 0, 0}, {0, 0, sz}, {0, -1, 0}, 2));
+        base.push_back(face_quad(o + V3{0, sy, 0}, {sx, 0, 0}, {0, 0, sz}, {0, 1, 0}, 2));
+        base.push_back(face_quad(o, {sx, 0, 0}, {0, sy, 0}, {0, 0, -1}, 2));
+        base.push_back(face_quad(o + V3{0, 0, sz}, {sx, 0, 0}, {0, sy, 0}, {0, 0, 1}, 2));
+        base.push_back(face_quad(o, {0, 0, sz}, {0, sy, 0}, {-1, 0, 0}, 2));
+        base.push_back(face_quad(o + V3{sx, 0, 0}, {0, 0, sz}, {0, sy, 0}, {1, 0, 0}, 2));
+    }
+    // max patch area giving close to target_tris (count decreases with area)
+    double lo = 1e-6, hi = 1e3;
+    if (target_tris > 2 * base.size()) {
+        for (int it = 0; it < 60; ++it) {
+            double mid = sqrt(lo * hi);
+            if (count_tris(base, mid) > target_tris)
+                lo = mid;
+            else
+                hi = mid;
+        }
+    } else {
+        hi = 0;
+    }
+    std::vector<Quad> quads = subdivide(base, hi, false);
+    // lights: nlights distinct ceiling quads, both triangles of the first
+    // half of picks ... one triangle per pick
+    std::vector<uint32_t> ceiling;
+    for (uint32_t q = 0; q < quads.size(); ++q)
+        if (quads[q].group == 1)
+            ceiling.push_back(q);
+    std::vector<uint32_t> picks;
+    Rng lr(seed ^ 0x5eed1167ull);
+    for (uint32_t l = 0; l < nlights && !ceiling.empty(); ++l) {
+        uint32_t i = (uint32_t)(lr.next() % ceiling.size());
+        picks.push_back(2 * ceiling[i] + (uint32_t)(lr.next() & 1));
+        ceiling.erase(ceiling.begin() + i);
+    }
+    std::sort(picks.begin(), picks.end());
+    rad_scene *s = flatten(quads, true, [&](rad_scene *sc, uint32_t p, const Quad &, int) {
+        sc->refl[3 * p + 0] = rng.uni(0.1f, 0.8f);
+        sc->refl[3 * p + 1] = rng.uni(0.1f, 0.8f);
+        sc->refl[3 * p + 2] = rng.uni(0.1f, 0.8f);
+        if (std::binary_search(picks.begin(), picks.end(), p)) {
+            sc->emission[3 * p + 0] = lr.uni(5.0f, 20.0f);
+            sc->emission[3 * p + 1] = lr.uni(5.0f, 20.0f);
+            sc->emission[3 * p + 2] = lr.uni(5.0f, 20.0f);
+        }
+    });
+    if (!s)
+        return RAD_E_NOMEM;
+    *out = s;
+    return RAD_OK;
+}
+
+} // extern "C"
