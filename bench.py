@@ -1,0 +1,193 @@
+"""Benchmark of the hemicube shooting path on one MI355X.
+
+Workload (default): the 20 x 5 x 20 room with 400 seeded boxes, ~200k
+triangle patches, 16 emitters, hemicube 256.  A step is one pass of K=64
+shooters (select, render, reduce, shoot).  Reported:
+  value          batched shooters/s (K=64, FP32)
+  plain          the single-shooter drop-in (K=1): shooters/s
+  converge       wall ms until the FP64 residual unshot energy <= 1e-3,
+                 batched and plain (plain is capped; see its fields)
+  roofline       the dominant kernel, timed with HIP events on the library's
+                 stream, against the MI355X HBM peak
+  cpu_baseline   the oracle's single-thread CPU pass on the same scene
+
+    python bench.py [--steps 50] [--warmup 5] [--no-converge] [--cpu-sample-s 10]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--K", type=int, default=64)
+    ap.add_argument("--W", type=int, default=256)
+    ap.add_argument("--plain-passes", type=int, default=2000)
+    ap.add_argument("--target", type=float, default=1e-3)
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--plain-cap-s", type=float, default=90.0)
+    ap.add_argument("--cpu-sample-s", type=float, default=10.0)
+    ap.add_argument("--small-tri-px", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import radiosity_amd as R
+
+    log("scene: room 20x5x20, 400 boxes, seed 3")
+    sc = R.Scene.boxes(nboxes=400, room=(20.0, 5.0, 20.0), target_tris=200000, nlights=16, seed=3)
+    W, K = args.W, args.K
+    names = ["select", "prepare_clear", "raster_setup", "raster_tiles", "reduce", "shoot"]
+
+    # ---------------------------------------------------------- batched K
+    s = R.Solver(hemicube=W, hemicubes=K, small_tri_px=args.small_tri_px)
+    s.upload(sc)
+    s.step(args.warmup)
+    s.finish()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.finish()
+    dt = time.perf_counter() - t0
+    batched_sps = args.steps * K / dt
+    ms_per_step = dt * 1e3 / args.steps
+    log(f"batched K={K}: {ms_per_step:.3f} ms/pass, {batched_sps:.0f} shooters/s")
+    # per-kernel device times over the same number of passes (HIP events on
+    # the library stream)
+    s.kernel_times(1)
+    s.step(args.steps)
+    s.finish()
+    kt, npass = s.kernel_times(0)
+    kt = dict(zip(names, kt))
+    log("kernel ms/pass: " + json.dumps({k: round(v, 4) for k, v in kt.items()}))
+
+    # ----------------------------------------------------------- plain K=1
+    p = R.Solver(hemicube=W, hemicubes=1, small_tri_px=args.small_tri_px)
+    p.upload(sc)
+    p.step(20)
+    p.finish()
+    t0 = time.perf_counter()
+    p.step(args.plain_passes)
+    p.finish()
+    pdt = time.perf_counter() - t0
+    plain_sps = args.plain_passes / pdt
+    log(f"plain K=1: {pdt * 1e3 / args.plain_passes:.4f} ms/pass, {plain_sps:.0f} shooters/s")
+    p.kernel_times(1)
+    p.step(200)
+    p.finish()
+    pkt, _ = p.kernel_times(0)
+    pkt = dict(zip(names, pkt))
+
+    # --------------------------------------------------------- convergence
+    conv = None
+    if not args.no_converge:
+        s.upload(sc)
+        st = s.run(10 ** 9, args.target, check_every=100)
+        conv = {"batched": {"ms": st.seconds * 1e3, "passes": st.passes, "shooters": st.shooters,
+                            "residual": st.residual}}
+        log(f"batched to {args.target}: {st.seconds:.2f} s, {st.shooters} shooters, "
+            f"residual {st.residual:.3e}")
+        p.upload(sc)
+        cap = int(max(1000, plain_sps * args.plain_cap_s))
+        st = p.run(cap, args.target, check_every=2000)
+        reached = st.residual <= args.target
+        conv["plain"] = {"ms": st.seconds * 1e3, "passes": st.passes, "shooters": st.shooters,
+                         "residual": st.residual, "reached": bool(reached),
+                         "cap_passes": cap}
+        log(f"plain to {args.target}: {st.seconds:.2f} s, {st.shooters} shooters, residual "
+            f"{st.residual:.3e} ({'reached' if reached else 'capped'})")
+        if reached:
+            conv["speedup_batched_vs_plain"] = conv["plain"]["ms"] / conv["batched"]["ms"]
+
+    # ------------------------------------------------------------ roofline
+    # Dominant kernel of the batched pass.  Algorithmic bytes per launch:
+    #  raster_setup: 36 B per triangle (its vertices) read per shooter plus
+    #                8 B per item-buffer texel written, K shooters per launch
+    #  reduce:       8 B per texel read + 8 B per visible run (F row update)
+    RES = 3 * W * W
+    dom = max(("raster_setup", "reduce", "raster_tiles", "shoot", "select"), key=lambda k: kt[k])
+    if dom == "raster_setup":
+        alg = K * (36.0 * sc.ntri + 8.0 * RES)
+    elif dom == "reduce":
+        alg = K * (8.0 * RES)
+    elif dom == "raster_tiles":
+        alg = K * 8.0 * RES
+    elif dom == "shoot":
+        alg = sc.npatch * (4.0 * K + 40.0)
+    else:
+        alg = sc.npatch * 16.0
+    achieved = alg / (kt[dom] * 1e-3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "alg_bytes_per_launch": alg, "avg_launch_ms": round(kt[dom], 5), "traffic": None}
+
+    # --------------------------------------------------------- CPU baseline
+    cpu = None
+    if args.cpu_sample_s > 0:
+        from oracle import oracle as O
+        dff = O.dff_cross(W)
+        u = sc.emission.copy()
+        sh = sc.shot0.copy()
+        item = np.zeros(RES, np.uint32)
+        ff = np.zeros(sc.npatch, np.float32)
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            O.plain_pass(W, 1, 1, sc, dff, u, sh, item, ff)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_sample_s and n >= 3:
+                break
+        cpu = {"value": n / el, "unit": "shooters/s", "cores": 1, "kind": "port",
+               "sample": f"{n} plain passes (K=1, W={W}) of the oracle on the same 200k-patch "
+                         f"scene from the initial state, {el:.1f} s"}
+        log(f"cpu oracle: {n / el:.2f} shooters/s")
+
+    out = {
+        "metric": "shooters/sec + ms to 1e-3 unshot energy at 200k patches, plain FP32 vs "
+                  "batched/BF16",
+        "value": round(batched_sps, 2),
+        "unit": "shooters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-box room)",
+        "config": {"workload": f"room200k: 20x5x20 room, 400 boxes (seed 3), {sc.npatch} "
+                               f"triangle patches, 16 emitters, hemicube {W}, K={K} shooters "
+                               f"per pass", "patches": sc.npatch, "hemicube": W,
+                   "hemicubes_per_pass": K, "parallelism": "single GPU"},
+        "plain": {"shooters_per_s": round(plain_sps, 2),
+                  "ms_per_pass": round(pdt * 1e3 / args.plain_passes, 5),
+                  "kernel_ms": {k: round(v, 5) for k, v in pkt.items()}},
+        "batched": {"shooters_per_s": round(batched_sps, 2),
+                    "kernel_ms": {k: round(v, 5) for k, v in kt.items()}},
+        "converge": conv,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -141,6 +141,10 @@ typedef struct rad_stats {
 int rad_run(rad_ctx *ctx, uint64_t max_passes, double residual_target,
             uint32_t check_every, rad_stats *out);
 
+/* glFinish/clFinish (Main.cpp:1202, 1228): waits for all work queued on the
+ * context's stream. */
+int rad_finish(rad_ctx *ctx);
+
 /* sum_i A_i (r+g+b)(unshot_i) / sum_i A_i (r+g+b)(E_i), accumulated in FP64 */
 int rad_residual_fp64(rad_ctx *ctx, double *out);
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "rad_step": (C.c_int, [_P, _u32]),
     "rad_run": (C.c_int, [_P, C.c_uint64, C.c_double, _u32, C.POINTER(rad_stats)]),
     "rad_residual_fp64": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "rad_finish": (C.c_int, [_P]),
     "rad_face_mvps": (C.c_int, [_P, _u32, _PF]),
     "rad_kernel_times": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "rad_scene_cornell": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_uint64,
@@ -254,6 +255,9 @@ class Solver:
 
     def step(self, passes=1):
         self._chk(lib().rad_step(self._ctx, passes), "rad_step")
+
+    def finish(self):
+        self._chk(lib().rad_finish(self._ctx), "rad_finish")
 
     def run(self, max_passes, target, check_every=0):
         st = rad_stats()

@@ -48,7 +48,8 @@ struct rad_ctx {
     uint32_t *nshoot = nullptr;    // 1
     uint32_t *counters = nullptr;  // kCntN
     uint64_t *sel = nullptr;       // 1 (+ pad)
-    double *resid = nullptr;       // 2 (+ scratch)
+    double *resid = nullptr;       // [0,1] selection residual, [4,5] rad_residual_fp64
+    double *parts = nullptr;       // per-block partials of the selection (3 words per block)
     RasterRec *recs = nullptr;
     TileItem *items = nullptr;
     uint64_t *cand = nullptr;      // top-K candidates (two halves)
@@ -139,7 +140,8 @@ int select_dev(rad_ctx *c, uint32_t k)
     if (k == 1) {
         uint32_t rule = c->cfg.select;
         launch_select_top1(rule, c->unshot, c->area, c->emis, c->npatch, c->shooters, c->nshoot,
-                           nullptr, c->sel, nullptr, c->resid, c->sel_blocks, c->st);
+                           c->parts, (uint64_t *)(c->parts + 2 * c->sel_blocks), nullptr, c->resid,
+                           c->sel_blocks, c->st);
         return check_launch(c, "select");
     }
     if (c->cfg.select == RAD_SELECT_REFERENCE)
@@ -237,6 +239,10 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         ncu = prop.multiProcessorCount;
     c->sel_blocks = ncu * 2;
     c->tile_blocks = ncu * 8;
+    if ((rc = dalloc(c, &c->parts, (size_t)c->sel_blocks * 3))) {
+        rad_destroy(c);
+        return rc;
+    }
     *out = c;
     return RAD_OK;
 }
@@ -250,7 +256,7 @@ void rad_destroy(rad_ctx *c)
     void *ptrs[] = {c->tri,  c->tri_patch, c->frames, c->refl,     c->area,   c->emis,
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
-                    c->resid, c->recs,    c->items,  c->cand};
+                    c->resid, c->recs,    c->items,  c->cand,   c->parts};
     for (void *p : ptrs)
         dfree(p);
     for (hipEvent_t e : c->ev)
@@ -574,6 +580,14 @@ int rad_step(rad_ctx *c, uint32_t passes)
     return RAD_OK;
 }
 
+int rad_finish(rad_ctx *c)
+{
+    if (!c)
+        return RAD_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
 int rad_residual_fp64(rad_ctx *c, double *out)
 {
     if (!c || !out)
@@ -581,7 +595,7 @@ int rad_residual_fp64(rad_ctx *c, double *out)
     int rc;
     if ((rc = need_scene(c)))
         return rc;
-    launch_residual(c->unshot, c->area, c->emis, c->npatch, nullptr, nullptr, c->resid + 4,
+    launch_residual(c->unshot, c->area, c->emis, c->npatch, c->parts, nullptr, c->resid + 4,
                     c->sel_blocks, c->st);
     if ((rc = check_launch(c, "residual")))
         return rc;

@@ -785,15 +785,6 @@ __device__ static inline uint32_t sel_decode(uint32_t rule, uint64_t key)
     return 0xFFFFFFFFu - (uint32_t)key;
 }
 
-__global__ void k_sel_reset(uint64_t *sel, double *resid)
-{
-    if (threadIdx.x == 0) {
-        sel[0] = 0;
-        resid[0] = 0.0;
-        resid[1] = 0.0;
-    }
-}
-
 template <typename T> __device__ static inline T wave_sum(T v)
 {
 #pragma unroll
@@ -812,11 +803,37 @@ __device__ static inline uint64_t wave_max_u64(uint64_t v)
     return v;
 }
 
+// Block-level reduction of (max key, num, den) for 256 threads; the result
+// is valid in thread 0.
+__device__ static inline void block_reduce3(uint64_t &best, double &num, double &den)
+{
+    __shared__ uint64_t sb[4];
+    __shared__ double sn[4], sd[4];
+    best = wave_max_u64(best);
+    num = wave_sum(num);
+    den = wave_sum(den);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sb[w] = best;
+        sn[w] = num;
+        sd[w] = den;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            best = sb[q] > best ? sb[q] : best;
+            num += sn[q];
+            den += sd[q];
+        }
+    }
+}
+
+// Stage 1: one partial (max key, residual numerator, denominator) per block.
 __global__ __launch_bounds__(256) void k_select_top1(uint32_t rule, const float *__restrict__ unshot,
                                                      const float *__restrict__ area,
                                                      const float *__restrict__ emis,
-                                                     uint32_t npatch, uint64_t *sel,
-                                                     double *resid)
+                                                     uint32_t npatch, uint64_t *__restrict__ pkey,
+                                                     double *__restrict__ pres)
 {
     uint64_t best = 0;
     double num = 0.0, den = 0.0;
@@ -830,23 +847,37 @@ __global__ __launch_bounds__(256) void k_select_top1(uint32_t rule, const float 
         den += a * ((double)emis[3 * (size_t)j] + (double)emis[3 * (size_t)j + 1] +
                     (double)emis[3 * (size_t)j + 2]);
     }
-    best = wave_max_u64(best);
-    num = wave_sum(num);
-    den = wave_sum(den);
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax((unsigned long long *)sel, (unsigned long long)best);
-        atomicAdd(&resid[0], num);
-        atomicAdd(&resid[1], den);
+    block_reduce3(best, num, den);
+    if (threadIdx.x == 0) {
+        pkey[blockIdx.x] = best;
+        pres[2 * blockIdx.x] = num;
+        pres[2 * blockIdx.x + 1] = den;
     }
 }
 
-// Writes the decoded single shooter for the following kernels.
-__global__ void k_sel_decode(uint32_t rule, const uint64_t *sel, uint32_t *shooters,
-                             uint32_t *nshoot)
+// Stage 2 (one block): reduce the partials, decode the shooter.
+__global__ __launch_bounds__(256) void k_select_final(uint32_t rule, uint32_t nparts,
+                                                      const uint64_t *__restrict__ pkey,
+                                                      const double *__restrict__ pres,
+                                                      uint32_t *__restrict__ shooters,
+                                                      uint32_t *__restrict__ nshoot,
+                                                      double *__restrict__ resid)
 {
+    uint64_t best = 0;
+    double num = 0.0, den = 0.0;
+    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+        best = pkey[b] > best ? pkey[b] : best;
+        num += pres[2 * b];
+        den += pres[2 * b + 1];
+    }
+    block_reduce3(best, num, den);
     if (threadIdx.x == 0) {
-        shooters[0] = sel_decode(rule, sel[0]);
-        nshoot[0] = 1;
+        if (shooters) {
+            shooters[0] = sel_decode(rule, best);
+            nshoot[0] = 1;
+        }
+        resid[0] = num;
+        resid[1] = den;
     }
 }
 
@@ -855,25 +886,22 @@ void launch_select_top1(uint32_t rule, const float *unshot, const float *area, c
                         uint64_t *pkeys, uint32_t *counter, double *resid, int nblocks,
                         hipStream_t st)
 {
-    (void)partials;
     (void)counter;
-    k_sel_reset<<<1, 64, 0, st>>>(pkeys, resid);
-    k_select_top1<<<nblocks, 256, 0, st>>>(rule, unshot, area, emis, npatch, pkeys, resid);
-    k_sel_decode<<<1, 64, 0, st>>>(rule, pkeys, shooters, nshoot);
+    k_select_top1<<<nblocks, 256, 0, st>>>(rule, unshot, area, emis, npatch, pkeys, partials);
+    k_select_final<<<1, 256, 0, st>>>(rule, (uint32_t)nblocks, pkeys, partials, shooters, nshoot,
+                                      resid);
 }
 
 void launch_residual(const float *unshot, const float *area, const float *emis, uint32_t npatch,
                      double *partials, uint32_t *counter, double *resid, int nblocks,
                      hipStream_t st)
 {
-    (void)partials;
     (void)counter;
-    // The selection kernel accumulates the residual; its key output goes to
-    // a scratch word that nothing reads.
-    uint64_t *scratch = (uint64_t *)(resid + 2);
-    k_sel_reset<<<1, 64, 0, st>>>(scratch, resid);
-    k_select_top1<<<nblocks, 256, 0, st>>>(RAD_SELECT_POWER, unshot, area, emis, npatch, scratch,
-                                           resid);
+    uint64_t *pkeys = (uint64_t *)(partials + 2 * nblocks);
+    k_select_top1<<<nblocks, 256, 0, st>>>(RAD_SELECT_POWER, unshot, area, emis, npatch, pkeys,
+                                           partials);
+    k_select_final<<<1, 256, 0, st>>>(RAD_SELECT_POWER, (uint32_t)nblocks, pkeys, partials,
+                                      nullptr, nullptr, resid);
 }
 
 // ------------------------------------------------------------------ helpers
