@@ -54,7 +54,7 @@ def main():
     log("scene: room 20x5x20, 400 boxes, seed 3")
     sc = R.Scene.boxes(nboxes=400, room=(20.0, 5.0, 20.0), target_tris=200000, nlights=16, seed=3)
     W, K = args.W, args.K
-    names = ["select", "prepare_clear", "raster_setup", "raster_tiles", "reduce", "shoot"]
+    names = ["select", "prepare_clear", "raster_setup", "bin", "tile_raster_reduce", "shoot"]
 
     # ---------------------------------------------------------- batched K
     s = R.Solver(hemicube=W, hemicubes=K, small_tri_px=args.small_tri_px)
@@ -116,22 +116,25 @@ def main():
             conv["speedup_batched_vs_plain"] = conv["plain"]["ms"] / conv["batched"]["ms"]
 
     # ------------------------------------------------------------ roofline
-    # Dominant kernel of the batched pass.  Algorithmic bytes per launch:
-    #  raster_setup: 36 B per triangle (its vertices) read per shooter plus
-    #                8 B per item-buffer texel written, K shooters per launch
-    #  reduce:       8 B per texel read + 8 B per visible run (F row update)
+    # Dominant kernel of the batched pass and its algorithmic bytes per launch
+    # (K shooters):
+    #  raster_setup        36 B of vertices + 4 B patch id read per triangle
+    #                      per shooter
+    #  tile_raster_reduce  4 B of item-buffer id + 4 B of ΔFF per texel (the
+    #                      ProcessHemicube input, here resolved in LDS) plus a
+    #                      4 B F-row entry per patch
+    #  bin                 8 B per bin entry, shoot 4 B x K + 40 B per patch
     RES = 3 * W * W
-    dom = max(("raster_setup", "reduce", "raster_tiles", "shoot", "select"), key=lambda k: kt[k])
-    if dom == "raster_setup":
-        alg = K * (36.0 * sc.ntri + 8.0 * RES)
-    elif dom == "reduce":
-        alg = K * (8.0 * RES)
-    elif dom == "raster_tiles":
-        alg = K * 8.0 * RES
-    elif dom == "shoot":
-        alg = sc.npatch * (4.0 * K + 40.0)
-    else:
-        alg = sc.npatch * 16.0
+    per_launch = {
+        "raster_setup": K * 40.0 * sc.ntri,
+        "tile_raster_reduce": K * (8.0 * RES + 4.0 * sc.npatch),
+        "shoot": sc.npatch * (4.0 * K + 40.0),
+        "select": sc.npatch * 16.0,
+        "bin": K * 3.0 * RES,
+        "prepare_clear": K * 4.0 * sc.npatch,
+    }
+    dom = max(per_launch, key=lambda k: kt[k])
+    alg = per_launch[dom]
     achieved = alg / (kt[dom] * 1e-3) / 1e9
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),

@@ -56,8 +56,8 @@ typedef struct rad_config {
     uint32_t ocl_workitems_x; /* Config::oclWorkitemsX: spans per row for the run form (4) */
     uint32_t mode;            /* RAD_MODE_* */
     uint32_t select;          /* RAD_SELECT_* */
-    uint32_t small_tri_px;    /* raster tuning: bbox area (px) rasterized by one lane; 0 = auto */
-    uint32_t reserved;
+    uint32_t small_tri_px;    /* raster tuning: triangle pixels in a tile drawn by one lane (0: 16) */
+    uint32_t tile_px;         /* raster tuning: screen tile side 8, 16 or 32 (0: auto) */
 } rad_config;
 
 typedef struct rad_ctx rad_ctx;
@@ -155,8 +155,9 @@ int rad_face_mvps(rad_ctx *ctx, uint32_t patch, float *out80);
 
 /* Average device time (ms) per launch of each kernel of the last
  * rad_step/rad_run, measured with HIP events on the context stream:
- * out[0] select, [1] prepare+clear, [2] raster setup, [3] raster tiles,
- * [4] reduce, [5] shoot; also the number of timed passes. */
+ * out[0] select, [1] prepare+clear, [2] raster setup, [3] binning (scan +
+ * fill), [4] tile raster with the fused ProcessHemicube, [5] shoot; also the
+ * number of timed passes. */
 int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
 
 #ifdef __cplusplus

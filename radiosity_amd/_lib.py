@@ -25,7 +25,7 @@ class RadError(RuntimeError):
 class rad_config(C.Structure):
     _fields_ = [("hemicube", C.c_uint32), ("hemicubes", C.c_uint32), ("shoots", C.c_uint32),
                 ("ocl_workitems_x", C.c_uint32), ("mode", C.c_uint32), ("select", C.c_uint32),
-                ("small_tri_px", C.c_uint32), ("reserved", C.c_uint32)]
+                ("small_tri_px", C.c_uint32), ("tile_px", C.c_uint32)]
 
 
 class rad_stats(C.Structure):
@@ -165,9 +165,9 @@ class Solver:
     """One rad_ctx: scene, state and the shooting passes on the current GPU."""
 
     def __init__(self, hemicube=16, hemicubes=1, mode=RAD_MODE_PHYSICAL, select=RAD_SELECT_POWER,
-                 shoots=500, small_tri_px=0):
+                 shoots=500, small_tri_px=0, tile_px=0):
         L = lib()
-        cfg = rad_config(hemicube, hemicubes, shoots, 4, mode, select, small_tri_px, 0)
+        cfg = rad_config(hemicube, hemicubes, shoots, 4, mode, select, small_tri_px, tile_px)
         self._ctx = C.c_void_p()
         rc = L.rad_create(C.byref(cfg), C.byref(self._ctx))
         if rc != RAD_OK:

@@ -50,11 +50,10 @@ struct rad_ctx {
     uint64_t *sel = nullptr;       // 1 (+ pad)
     double *resid = nullptr;       // [0,1] selection residual, [4,5] rad_residual_fp64
     double *parts = nullptr;       // per-block partials of the selection (3 words per block)
-    RasterRec *recs = nullptr;
-    TileItem *items = nullptr;
+    BinBuffers bb{};               // sort-middle rasterizer bins
     uint64_t *cand = nullptr;      // top-K candidates (two halves)
-    uint32_t rec_cap = 0, item_cap = 0;
-    int sel_blocks = 1024, tile_blocks = 4096;
+    uint32_t small_px = 16;        // in-lane rasterization limit (pixels of a tile)
+    int sel_blocks = 1024;
     bool rendered = false;        // keys hold the item buffer of the last render
     uint32_t rendered_k = 0;
 
@@ -62,6 +61,7 @@ struct rad_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev;
     double t_acc[6] = {0, 0, 0, 0, 0, 0};
+    std::string pending_error;
     uint64_t t_passes = 0;
 };
 
@@ -119,20 +119,39 @@ void record(rad_ctx *c, size_t idx)
 }
 
 // Renders the hemicubes of the shooters currently in c->shooters/nshoot
-// (k slots) into c->keys.
-int render_pass(rad_ctx *c, uint32_t k, size_t evbase)
+// (k slots): keys != null writes the item buffer, otherwise the tiles are
+// reduced into F rows (and hit counts when hits != null).
+int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits)
 {
     launch_prepare(c->g, k, c->shooters, c->nshoot, c->frames, c->unshot, c->snap, c->mvp, c->st);
-    launch_clear(c->g, k, c->keys, c->ff, nullptr, c->npatch, c->counters, c->st);
+    launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->npatch, c->counters, c->st);
     record(c, evbase + 2);
-    launch_raster_setup(c->g, k, c->tri, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->keys, c->recs,
-                        c->rec_cap, c->items, c->item_cap, c->counters,
-                        c->cfg.small_tri_px ? c->cfg.small_tri_px : 64u, c->st);
+    launch_raster_setup(c->g, k, c->tri, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
+                        c->counters, c->st);
     record(c, evbase + 3);
-    launch_raster_tiles(c->recs, c->items, c->counters, c->rec_cap, c->item_cap,
-                        (int32_t)c->g.TW, c->keys, c->tile_blocks, c->st);
+    launch_bin(k, c->nshoot, c->bb, c->counters, c->st);
     record(c, evbase + 4);
+    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, keys, c->dff, c->npatch,
+                keys ? nullptr : c->ff, hits, c->counters, c->st);
+    record(c, evbase + 5);
     return check_launch(c, "render kernels");
+}
+
+// Reads the pass counters after a synchronisation: a record or bin-entry
+// overflow means the last pass lost triangles.
+int check_overflow(rad_ctx *c)
+{
+    uint32_t cnt[kCntN];
+    hipError_t e = hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("counters: ") + hipGetErrorString(e));
+    if (cnt[kCntOverflow] || cnt[kCntEntries] > c->bb.ent_cap)
+        return fail(c, RAD_E_LIMIT, "rasterizer bins overflowed (records " +
+                                        std::to_string(cnt[kCntSpill]) + " spilled of " +
+                                        std::to_string(c->bb.spill_cap) + ", entries " +
+                                        std::to_string(cnt[kCntEntries]) + " of " +
+                                        std::to_string(c->bb.ent_cap) + ")");
+    return RAD_OK;
 }
 
 int select_dev(rad_ctx *c, uint32_t k)
@@ -159,17 +178,13 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
     if (do_select && (rc = select_dev(c, c->K)) != RAD_OK)
         return rc;
     record(c, evbase + 1);
-    if ((rc = render_pass(c, c->K, evbase)) != RAD_OK)
+    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr)) != RAD_OK)
         return rc;
-    launch_reduce(c->g, c->K, c->nshoot, c->keys, c->dff, c->npatch, c->ff, nullptr, c->counters,
-                  c->st);
-    record(c, evbase + 5);
     launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl, c->area,
                  c->npatch, c->unshot, c->shot, c->st);
     record(c, evbase + 6);
-    c->rendered = true;
-    c->rendered_k = c->K;
-    return check_launch(c, "reduce/shoot");
+    c->rendered = false;
+    return check_launch(c, "shoot");
 }
 
 int need_scene(rad_ctx *c)
@@ -215,7 +230,7 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         return RAD_E_HIP;
     }
     if ((rc = dalloc(c, &c->dff, c->g.RES)) || (rc = dalloc(c, &c->keys, (size_t)c->K * c->g.RES)) ||
-        (rc = dalloc(c, &c->mvp, (size_t)c->K * 80)) || (rc = dalloc(c, &c->snap, (size_t)c->K * 3)) ||
+        (rc = dalloc(c, &c->mvp, (size_t)c->K * kShooterStride)) || (rc = dalloc(c, &c->snap, (size_t)c->K * 3)) ||
         (rc = dalloc(c, &c->shooters, c->K)) || (rc = dalloc(c, &c->nshoot, 1)) ||
         (rc = dalloc(c, &c->counters, kCntN)) || (rc = dalloc(c, &c->sel, 4)) ||
         (rc = dalloc(c, &c->resid, 8))) {
@@ -238,7 +253,27 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         ncu = prop.multiProcessorCount;
     c->sel_blocks = ncu * 2;
-    c->tile_blocks = ncu * 8;
+    // screen tiles: 32x32 when enough hemicubes fill the GPU, else 16x16
+    uint32_t T = cfg->tile_px ? cfg->tile_px : (c->K >= 4 ? 32u : 16u);
+    if (T != 8 && T != 16 && T != 32) {
+        rad_destroy(c);
+        return RAD_E_ARG;
+    }
+    c->bb.tshift = T == 8 ? 3 : (T == 16 ? 4 : 5);
+    c->bb.ntx = (c->g.TW + T - 1) / T;
+    c->bb.nty = (c->g.TH + T - 1) / T;
+    c->bb.ntiles = c->bb.ntx * c->bb.nty;
+    if (c->bb.ntiles > kMaxTilesPerSlab) {
+        rad_destroy(c);
+        return RAD_E_ARG;
+    }
+    c->small_px = cfg->small_tri_px ? cfg->small_tri_px : 16u;
+    size_t nbins = (size_t)c->K * c->bb.ntiles;
+    if ((rc = dalloc(c, &c->bb.count, nbins)) || (rc = dalloc(c, &c->bb.start, nbins)) ||
+        (rc = dalloc(c, &c->bb.fill, nbins))) {
+        rad_destroy(c);
+        return rc;
+    }
     if ((rc = dalloc(c, &c->parts, (size_t)c->sel_blocks * 3))) {
         rad_destroy(c);
         return rc;
@@ -256,7 +291,8 @@ void rad_destroy(rad_ctx *c)
     void *ptrs[] = {c->tri,  c->tri_patch, c->frames, c->refl,     c->area,   c->emis,
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
-                    c->resid, c->recs,    c->items,  c->cand,   c->parts};
+                    c->resid, c->bb.recs, c->bb.count, c->bb.start, c->bb.fill,
+                    c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts};
     for (void *p : ptrs)
         dfree(p);
     for (hipEvent_t e : c->ev)
@@ -320,10 +356,26 @@ int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch
         (rc = dalloc(c, &c->unshot, (size_t)npatch * 3)) || (rc = dalloc(c, &c->shot, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->ff, K * npatch)) || (rc = dalloc(c, &c->hits, K * npatch)))
         return rc;
-    uint64_t want = (uint64_t)K * 5 * (uint64_t)ntri / 4 + 65536;
-    c->rec_cap = (uint32_t)(want > (1u << 23) ? (1u << 23) : want);
-    c->item_cap = c->rec_cap * 4;
-    if ((rc = dalloc(c, &c->recs, c->rec_cap)) || (rc = dalloc(c, &c->items, c->item_cap)) ||
+    // records: one region of kRecsPerBlock per (setup block, shooter) plus a
+    // spill region; bin entries: three per record slot
+    c->bb.nblk = (ntri + 255) / 256;
+    if (c->bb.nblk == 0)
+        c->bb.nblk = 1;
+    c->bb.nz = K <= 4 ? 5u : 1u;
+    uint64_t regions = (uint64_t)c->bb.nblk * K * c->bb.nz * kRecsPerBlock;
+    uint64_t spill = regions / 8 + 65536;
+    if (regions + spill > (1ull << 31))
+        return fail(c, RAD_E_LIMIT, "scene too large for the rasterizer record regions");
+    c->bb.spill_base = (uint32_t)regions;
+    c->bb.spill_cap = (uint32_t)spill;
+    c->bb.clip_cap = (uint32_t)(spill / 2);
+    c->bb.clip_blocks = 512;
+    uint64_t went = 2 * (regions + spill);
+    c->bb.ent_cap = (uint32_t)(went > 0xF0000000ull ? 0xF0000000ull : went);
+    if ((rc = dalloc(c, &c->bb.recs, regions + spill)) || (rc = dalloc(c, &c->bb.bbox, regions + spill)) ||
+        (rc = dalloc(c, &c->bb.list, c->bb.ent_cap)) ||
+        (rc = dalloc(c, &c->bb.blk_nrec, (size_t)c->bb.nblk * K * c->bb.nz)) ||
+        (rc = dalloc(c, &c->bb.clipq, c->bb.clip_cap)) ||
         (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
         return rc;
     // triangles -> 9 planes
@@ -430,7 +482,7 @@ int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_
     int rc;
     if ((rc = need_scene(c)) || (rc = upload_shooters(c, shooters, k)))
         return rc;
-    if ((rc = render_pass(c, k, 0)))
+    if ((rc = render_pass(c, k, 0, c->keys, nullptr)))
         return rc;
     c->rendered = true;
     c->rendered_k = k;
@@ -456,7 +508,7 @@ int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_
             return fail(c, RAD_E_HIP, std::string("render copy-out: ") + hipGetErrorString(e));
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
-    return RAD_OK;
+    return check_overflow(c);
 }
 
 static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
@@ -529,7 +581,7 @@ int rad_shoot(rad_ctx *c, const uint32_t *shooters, uint32_t k)
     if (rc)
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
-    return RAD_OK;
+    return check_overflow(c);
 }
 
 static int ensure_events(rad_ctx *c, size_t n)
@@ -627,7 +679,7 @@ int rad_run(rad_ctx *c, uint64_t max_passes, double target, uint32_t check_every
         if ((rc = step_impl(c, (uint32_t)n)))
             return rc;
         done += n;
-        if ((rc = rad_residual_fp64(c, &r)))
+        if ((rc = rad_residual_fp64(c, &r)) || (rc = check_overflow(c)))
             return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));

@@ -17,26 +17,51 @@ constexpr int kSub = 256;
 constexpr float kGuard = 4096.0f;
 // Depth 1.0 (the glClear value, Main.cpp:1148) with id 0 (black background).
 constexpr uint64_t kClearKey = (uint64_t)0x3F800000u << 32;
-// Big triangles are split into 8x8 pixel tiles, one wavefront per tile.
-constexpr int kTile = 8;
+// Largest screen tile of the binned rasterizer (texels per side), the most
+// tiles per hemicube slab (LDS histograms), and the records each setup
+// workgroup (256 triangles of one shooter) may keep in its own region before
+// spilling to the shared spill region.
+constexpr int kMaxTile = 32;
+constexpr uint32_t kMaxTilesPerSlab = 4096;
+constexpr uint32_t kRecsPerBlock = 768;
+// Floats per shooter in the MVP buffer: 5 face MVPs + the culling frame.
+constexpr int kShooterStride = 96;
 
-// Setup record of a triangle too big for one lane (48 bytes).
+// Setup record of one snapped (sub-)triangle of one face of one shooter.
 struct RasterRec {
     int32_t x[3], y[3];   // snapped face-local window coordinates (1/256 px)
     float dzdx, dzdy, z0; // depth plane anchored at vertex 0
     uint32_t id;          // patch + 1
-    int32_t base;         // texel of face-local pixel (0,0) in the key buffer
-    int16_t bx0, by0, bx1, by1; // inclusive pixel bbox, clamped to the scissor
+    uint32_t sf;          // shooter slot * 8 + face
+    uint16_t bx0, by0, bx1, by1; // inclusive texel bbox inside the hemicube slab
 };
 static_assert(sizeof(RasterRec) == 52, "RasterRec layout");
 
-struct TileItem {
-    uint32_t rec;
-    uint16_t tx, ty;
+// Counters used by one pass (reset by the clear kernel).
+enum {
+    kCntRec = 0,      // records emitted
+    kCntOverflow = 2, // a record did not fit
+    kCntEntries = 3,  // bin entries (scan total)
+    kCntUnknown = 4,  // item-buffer ids beyond npatch
+    kCntSpill = 5,    // records in the spill region
+    kCntClip = 6,     // triangle-face jobs queued for clipping
+    kCntN = 8
 };
 
-// Counters used by one pass (reset by the clear kernel).
-enum { kCntRec = 0, kCntItem = 1, kCntOverflow = 2, kCntDone = 3, kCntUnknown = 4, kCntN = 8 };
+// Bins of the sort-middle rasterizer: one per (shooter slot, tile).
+struct BinBuffers {
+    RasterRec *recs;       // nblk*K regions of kRecsPerBlock, then spill_cap
+    uint64_t *bbox;        // texel bbox per record slot, read by the bin fill
+    uint32_t *blk_nrec;    // records per (shooter, setup block)
+    uint32_t *count, *start, *fill, *list;
+    uint32_t nblk;         // setup blocks per shooter and face group (256 triangles each)
+    uint32_t nz;           // face groups: 5 (one face per block, small K) or 1
+    uint32_t spill_base, spill_cap, ent_cap;
+    uint2 *clipq;          // clipping jobs of a pass
+    uint32_t clip_cap;
+    int clip_blocks;
+    uint32_t ntx, nty, ntiles, tshift;
+};
 
 struct Geometry {
     uint32_t W, TW, TH, RES;
@@ -58,16 +83,20 @@ void launch_select_reference(const float *unshot, uint32_t npatch, uint32_t k,
 void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
                     const uint32_t *nshoot, const float *frames, const float *unshot,
                     float *snap, float *mvp, hipStream_t st);
-void launch_clear(const Geometry &g, uint32_t K, uint64_t *keys, float *ff,
-                  uint32_t *hits, uint32_t npatch, uint32_t *counters, hipStream_t st);
+void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
+                  uint32_t npatch, uint32_t *counters, hipStream_t st);
+// Binned render of the nshoot (<= K) hemicubes in three launches:
+// setup (records + bin counts), bin (scan + fill), tile (LDS depth test; keys
+// != null writes the item buffer, ff != null reduces into F rows and hits).
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
-                         const uint32_t *nshoot, uint64_t *keys, RasterRec *recs,
-                         uint32_t rec_cap, TileItem *items, uint32_t item_cap,
-                         uint32_t *counters, uint32_t small_px, hipStream_t st);
-void launch_raster_tiles(const RasterRec *recs, const TileItem *items,
-                         const uint32_t *counters, uint32_t rec_cap, uint32_t item_cap,
-                         int32_t TW, uint64_t *keys, int nblocks, hipStream_t st);
+                         const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
+                         hipStream_t st);
+void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
+                hipStream_t st);
+void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
+                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
+                 uint32_t *hits, uint32_t *counters, hipStream_t st);
 void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot,
                    const uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
                    uint32_t *hits, uint32_t *counters, hipStream_t st);

@@ -9,13 +9,14 @@
 // Kernels (one pass over K shooters):
 //   k_select_*     shooter selection (+ FP64 residual partial sums)
 //   k_prepare      dB snapshot and the five face MVPs per shooter
-//   k_clear        item/depth keys to the clear value, F rows to 0, counters
-//   k_raster_setup one lane per (triangle, face, shooter): transform, clip,
-//                  snap, cull, scissor; small triangles rasterized in-lane,
-//                  big ones emitted as 8x8 pixel tiles
-//   k_raster_tiles one wavefront per 8x8 tile of a big triangle
-//   k_reduce       ProcessHemicube: wave-level run-length reduction of the
-//                  item buffer into per-patch form factors and hit counts
+//   k_clear        F rows to 0, bin counts and pass counters reset
+//   k_raster_setup one lane per (triangle, shooter) over the five faces:
+//                  cull, transform, clip, snap; records counted into tile bins
+//   k_bin_scan / k_bin_fill   bin offsets and record lists
+//   k_tile_raster  one workgroup per (shooter, tile): depth test in LDS, then
+//                  the tile's item buffer out or ProcessHemicube fused
+//   k_reduce       ProcessHemicube over a whole item buffer (wave-level
+//                  run-length reduction into form factors and hit counts)
 //   k_shoot        energy shooting into unshot RGB, emitter update
 //
 // Depth test: each texel holds a 64-bit key (float depth bits << 32 | id).
@@ -168,6 +169,9 @@ __device__ static void face_mvp(const float *fr, int face, float *mvp)
 }
 
 // One thread per (slot, face): snapshot dB (Main.cpp:1161) and the MVP.
+// Per shooter the mvp buffer holds kShooterStride floats: five column-major
+// MVPs, then the culling frame used by k_raster_setup: eye, unit normal n,
+// unit up u (made orthogonal to n) and n x u.
 __global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
                           const uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
                           const float *__restrict__ unshot, float *__restrict__ snap,
@@ -180,18 +184,27 @@ __global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
     if (s >= *nshoot)
         return;
     uint32_t p = shooters[s];
+    float fr[9];
+    for (int k = 0; k < 9; ++k)
+        fr[k] = frames[9 * (size_t)p + k];
+    float *out = mvp + (size_t)s * kShooterStride;
     if (face == 0) {
         snap[3 * s + 0] = unshot[3 * (size_t)p + 0];
         snap[3 * s + 1] = unshot[3 * (size_t)p + 1];
         snap[3 * s + 2] = unshot[3 * (size_t)p + 2];
+        f3 n = normalize3({fr[3], fr[4], fr[5]});
+        f3 u = {fr[6], fr[7], fr[8]};
+        float un = u.x * n.x + u.y * n.y + u.z * n.z;
+        u = normalize3({u.x - un * n.x, u.y - un * n.y, u.z - un * n.z});
+        f3 v = {n.y * u.z - n.z * u.y, n.z * u.x - n.x * u.z, n.x * u.y - n.y * u.x};
+        float cf[12] = {fr[0], fr[1], fr[2], n.x, n.y, n.z, u.x, u.y, u.z, v.x, v.y, v.z};
+        for (int k = 0; k < 12; ++k)
+            out[80 + k] = cf[k];
     }
-    float fr[9];
-    for (int k = 0; k < 9; ++k)
-        fr[k] = frames[9 * (size_t)p + k];
     float m[16];
     face_mvp(fr, (int)face, m);
     for (int k = 0; k < 16; ++k)
-        mvp[(size_t)t * 16 + k] = m[k];
+        out[face * 16 + k] = m[k];
 }
 
 void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
@@ -204,15 +217,17 @@ void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
 }
 
 // --------------------------------------------------------------------- clear
-__global__ void k_clear(uint64_t *__restrict__ keys, size_t nkeys, float *__restrict__ ff,
+// Per pass: F rows (and hit counts) to zero, bin counts and pass counters
+// reset.  A kernel, not hipMemsetAsync, so a pass can be captured in a graph.
+__global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *__restrict__ ff,
                         uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
-    if (i < kCntN && i != kCntDone)
+    if (i < kCntN)
         counters[i] = 0;
-    for (size_t k = i; k < nkeys; k += stride)
-        keys[k] = kClearKey;
+    for (size_t k = i; k < nbins; k += stride)
+        bin_count[k] = 0u;
     for (size_t k = i; k < nff; k += stride) {
         ff[k] = 0.0f;
         if (hits)
@@ -220,18 +235,30 @@ __global__ void k_clear(uint64_t *__restrict__ keys, size_t nkeys, float *__rest
     }
 }
 
-void launch_clear(const Geometry &g, uint32_t K, uint64_t *keys, float *ff, uint32_t *hits,
+void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
                   uint32_t npatch, uint32_t *counters, hipStream_t st)
 {
-    size_t nkeys = (size_t)K * g.RES, nff = (size_t)K * npatch;
-    size_t n = nkeys > nff ? nkeys : nff;
+    size_t nbins = (size_t)K * ntiles, nff = (size_t)K * npatch;
+    size_t n = nbins > nff ? nbins : nff;
     size_t blocks = (n + 255) / 256;
     if (blocks > 8192)
         blocks = 8192;
-    k_clear<<<(unsigned)blocks, 256, 0, st>>>(keys, nkeys, ff, hits, nff, counters);
+    if (blocks == 0)
+        blocks = 1;
+    k_clear<<<(unsigned)blocks, 256, 0, st>>>(bin_count, nbins, ff, hits, nff, counters);
 }
 
 // ---------------------------------------------------------------- rasterizer
+// Sort-middle rasterizer.
+//  k_raster_setup  one lane per (triangle, shooter), looping over the five
+//                  faces: cull, transform, clip, snap; every surviving
+//                  (sub-)triangle becomes a RasterRec and is counted in the
+//                  bins of the screen tiles its bbox touches
+//  k_bin_scan      exclusive scan of the bin counts
+//  k_bin_fill      record indices into their bins
+//  k_tile_raster   one workgroup per tile: depth test in LDS with 64-bit
+//                  atomicMin, then either writes the tile's keys (item buffer
+//                  out) or reduces it into form factors (ProcessHemicube fused)
 struct CV4 {
     float c[4];
 };
@@ -242,7 +269,7 @@ struct SV {
 };
 
 struct FaceView {
-    int32_t vpx, vpy;       // viewport origin inside the hemicube slab (Main.cpp:316-347)
+    int32_t vpx, vpy;           // viewport origin inside the hemicube slab (Main.cpp:316-347)
     int32_t sx0, sy0, sx1, sy1; // scissor in face-local pixels, [x0,x1) x [y0,y1)
 };
 
@@ -283,139 +310,137 @@ __device__ static inline SV project(const CV4 &v, float halfW)
     return s;
 }
 
-struct TriSetup {
+// Edge functions of a CCW triangle: E_e(p) = A_e p.x + B_e p.y + C_e, inside
+// when E_e + bias_e > 0 for all three (top-left rule, y up).
+struct Edges {
     int32_t A[3], B[3];
     int64_t C[3];
     int32_t bias[3];
-    float dzdx, dzdy;
-    int32_t ix0, iy0, ix1, iy1;
 };
 
-// Edge functions, top-left bias, bbox clamped to the scissor and the depth
-// plane.  Returns false when culled (area <= 0) or the bbox is empty.
-__device__ static inline bool tri_setup(const SV &a, const SV &b, const SV &c,
-                                        const FaceView &fv, TriSetup &ts)
+__device__ static inline void make_edges(const int32_t *x, const int32_t *y, Edges &e)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int q = k == 2 ? 0 : k + 1;
+        int32_t A = y[k] - y[q], B = x[q] - x[k];
+        e.A[k] = A;
+        e.B[k] = B;
+        e.C[k] = -((int64_t)A * x[k] + (int64_t)B * y[k]);
+        e.bias[k] = (A > 0 || (A == 0 && B < 0)) ? 1 : 0;
+    }
+}
+
+// Depth key of face-local pixel (ix, iy) or 0 when the centre is outside
+// the triangle or fails the depth range.
+__device__ static inline uint64_t frag_key(const Edges &e, const RasterRec &r, int32_t ix,
+                                           int32_t iy)
+{
+    int32_t px = ix * kSub + kSub / 2, py = iy * kSub + kSub / 2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int64_t E = (int64_t)e.A[k] * px + (int64_t)e.B[k] * py + e.C[k];
+        if (E + e.bias[k] <= 0)
+            return 0;
+    }
+    float z = fmaf(r.dzdx, (float)(px - r.x[0]), fmaf(r.dzdy, (float)(py - r.y[0]), r.z0));
+    if (!(z > 0.0f))
+        z = 0.0f;
+    if (!(z < 1.0f))
+        return 0;
+    return ((uint64_t)__float_as_uint(z) << 32) | r.id;
+}
+
+struct BinCtx {
+    RasterRec *recs;       // block regions of kRecsPerBlock, then the spill region
+    uint64_t *bbox;        // texel bbox of each record slot (x0 | y0 << 16 | x1 << 32 | y1 << 48)
+    uint32_t *bin_count;   // K * ntiles
+    uint32_t *counters;
+    uint32_t *lnrec;       // LDS: records in this block's region
+    uint32_t *lcnt;        // LDS: per-tile record counts of this block
+    uint32_t region;       // first record of this block's region
+    uint32_t spill_base, spill_cap;
+    uint2 *clipq;          // (triangle, shooter * 8 + face) jobs that need clipping
+    uint32_t clip_cap;
+    uint32_t ntx, ntiles, tshift; // tiles per row, per hemicube, log2(tile side)
+};
+
+// Emits a snapped triangle (face-local coordinates) as a record into the
+// block's region (spill region when full) and counts it in the LDS
+// histogram of the tiles its texel bbox touches.
+__device__ static void emit_tri(const SV &a, const SV &b, const SV &c, uint32_t id,
+                                const FaceView &fv, uint32_t s, int face, const BinCtx &bc)
 {
     int64_t area2 = (int64_t)(b.x - a.x) * (c.y - a.y) - (int64_t)(c.x - a.x) * (b.y - a.y);
     if (area2 <= 0)
-        return false;
+        return;
     int32_t xmin = min(a.x, min(b.x, c.x)), xmax = max(a.x, max(b.x, c.x));
     int32_t ymin = min(a.y, min(b.y, c.y)), ymax = max(a.y, max(b.y, c.y));
-    ts.ix0 = max((xmin - 128 + 255) >> 8, fv.sx0);
-    ts.ix1 = min((xmax - 128) >> 8, fv.sx1 - 1);
-    ts.iy0 = max((ymin - 128 + 255) >> 8, fv.sy0);
-    ts.iy1 = min((ymax - 128) >> 8, fv.sy1 - 1);
-    if (ts.ix0 > ts.ix1 || ts.iy0 > ts.iy1)
-        return false;
-    const SV *v[3] = {&a, &b, &c};
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        const SV *p = v[e];
-        const SV *q = v[e == 2 ? 0 : e + 1];
-        int32_t A = p->y - q->y, B = q->x - p->x;
-        ts.A[e] = A;
-        ts.B[e] = B;
-        ts.C[e] = -((int64_t)A * p->x + (int64_t)B * p->y);
-        ts.bias[e] = (A > 0 || (A == 0 && B < 0)) ? 1 : 0;
+    int32_t ix0 = max((xmin - 128 + 255) >> 8, fv.sx0);
+    int32_t ix1 = min((xmax - 128) >> 8, fv.sx1 - 1);
+    int32_t iy0 = max((ymin - 128 + 255) >> 8, fv.sy0);
+    int32_t iy1 = min((ymax - 128) >> 8, fv.sy1 - 1);
+    if (ix0 > ix1 || iy0 > iy1)
+        return;
+    uint32_t slot = bc.lnrec ? atomicAdd(bc.lnrec, 1u) : kRecsPerBlock;
+    uint32_t r;
+    bool local = slot < kRecsPerBlock;
+    if (local) {
+        r = bc.region + slot;
+    } else {
+        if (bc.lnrec)
+            atomicSub(bc.lnrec, 1u);
+        uint32_t q = atomicAdd(&bc.counters[kCntSpill], 1u);
+        if (q >= bc.spill_cap) {
+            atomicOr(&bc.counters[kCntOverflow], 1u);
+            return;
+        }
+        r = bc.spill_base + q;
     }
     double dx1 = (double)(b.x - a.x), dy1 = (double)(b.y - a.y);
     double dx2 = (double)(c.x - a.x), dy2 = (double)(c.y - a.y);
     double dz1 = (double)b.z - (double)a.z, dz2 = (double)c.z - (double)a.z;
     double ar = (double)area2;
-    ts.dzdx = (float)((dz1 * dy2 - dz2 * dy1) / ar);
-    ts.dzdy = (float)((dx1 * dz2 - dx2 * dz1) / ar);
-    return true;
-}
-
-__device__ static inline void shade_pixel(int32_t ix, int32_t iy, const int32_t *A,
-                                          const int32_t *B, const int64_t *C,
-                                          const int32_t *bias, float dzdx, float dzdy,
-                                          int32_t x0, int32_t y0, float z0, uint32_t id,
-                                          int32_t base, int32_t TW, uint64_t *keys)
-{
-    int32_t px = ix * kSub + kSub / 2, py = iy * kSub + kSub / 2;
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        int64_t E = (int64_t)A[e] * px + (int64_t)B[e] * py + C[e];
-        if (E + bias[e] <= 0)
-            return;
+    RasterRec rr;
+    rr.x[0] = a.x; rr.x[1] = b.x; rr.x[2] = c.x;
+    rr.y[0] = a.y; rr.y[1] = b.y; rr.y[2] = c.y;
+    rr.dzdx = (float)((dz1 * dy2 - dz2 * dy1) / ar);
+    rr.dzdy = (float)((dx1 * dz2 - dx2 * dz1) / ar);
+    rr.z0 = a.z;
+    rr.id = id;
+    rr.sf = s * 8u + (uint32_t)face;
+    // bbox in texel coordinates of the hemicube slab
+    int32_t tx0 = fv.vpx + ix0, tx1 = fv.vpx + ix1, ty0 = fv.vpy + iy0, ty1 = fv.vpy + iy1;
+    rr.bx0 = (uint16_t)tx0;
+    rr.by0 = (uint16_t)ty0;
+    rr.bx1 = (uint16_t)tx1;
+    rr.by1 = (uint16_t)ty1;
+    bc.recs[r] = rr;
+    bc.bbox[r] = (uint64_t)(uint16_t)tx0 | ((uint64_t)(uint16_t)ty0 << 16) |
+                 ((uint64_t)(uint16_t)tx1 << 32) | ((uint64_t)(uint16_t)ty1 << 48);
+    uint32_t g0 = (uint32_t)tx0 >> bc.tshift, g1 = (uint32_t)tx1 >> bc.tshift;
+    uint32_t h0 = (uint32_t)ty0 >> bc.tshift, h1 = (uint32_t)ty1 >> bc.tshift;
+    if (local) {
+        for (uint32_t gy = h0; gy <= h1; ++gy)
+            for (uint32_t gx = g0; gx <= g1; ++gx)
+                atomicAdd(&bc.lcnt[gy * bc.ntx + gx], 1u);
+    } else {
+        uint32_t *cnt = bc.bin_count + (size_t)s * bc.ntiles;
+        for (uint32_t gy = h0; gy <= h1; ++gy)
+            for (uint32_t gx = g0; gx <= g1; ++gx)
+                atomicAdd(&cnt[gy * bc.ntx + gx], 1u);
     }
-    float z = fmaf(dzdx, (float)(px - x0), fmaf(dzdy, (float)(py - y0), z0));
-    if (!(z > 0.0f))
-        z = 0.0f;
-    if (!(z < 1.0f))
-        return;
-    uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | id;
-    uint64_t *dst = keys + (base + iy * TW + ix);
-    if (key < *dst)
-        atomicMin((unsigned long long *)dst, (unsigned long long)key);
-}
-
-// Rasterize one snapped triangle: in-lane when its bbox is small, otherwise
-// as 8x8 tiles for k_raster_tiles.
-__device__ static void raster_tri(const SV &a, const SV &b, const SV &c, uint32_t id,
-                                  const FaceView &fv, int32_t base, int32_t TW,
-                                  uint64_t *keys, RasterRec *recs, uint32_t rec_cap,
-                                  TileItem *items, uint32_t item_cap, uint32_t *counters,
-                                  uint32_t small_px)
-{
-    TriSetup ts;
-    if (!tri_setup(a, b, c, fv, ts))
-        return;
-    uint32_t w = (uint32_t)(ts.ix1 - ts.ix0 + 1), h = (uint32_t)(ts.iy1 - ts.iy0 + 1);
-    if (w * h > small_px) {
-        uint32_t ntx = (w + kTile - 1) / kTile, nty = (h + kTile - 1) / kTile;
-        uint32_t nt = ntx * nty;
-        // Items are reserved first and a record only when they fit; every
-        // item slot below item_cap that is reserved gets written, either
-        // pointing at a written record or marked invalid, so the tile kernel
-        // never reads an unwritten record.
-        uint32_t it = atomicAdd(&counters[kCntItem], nt);
-        bool items_ok = it <= item_cap && nt <= item_cap - it;
-        uint32_t r = items_ok ? atomicAdd(&counters[kCntRec], 1u) : 0xFFFFFFFFu;
-        bool ok = items_ok && r < rec_cap;
-        if (ok) {
-            RasterRec rr;
-            rr.x[0] = a.x; rr.x[1] = b.x; rr.x[2] = c.x;
-            rr.y[0] = a.y; rr.y[1] = b.y; rr.y[2] = c.y;
-            rr.dzdx = ts.dzdx;
-            rr.dzdy = ts.dzdy;
-            rr.z0 = a.z;
-            rr.id = id;
-            rr.base = base;
-            rr.bx0 = (int16_t)ts.ix0; rr.by0 = (int16_t)ts.iy0;
-            rr.bx1 = (int16_t)ts.ix1; rr.by1 = (int16_t)ts.iy1;
-            recs[r] = rr;
-        }
-        uint32_t lim = it >= item_cap ? 0u : min(nt, item_cap - it);
-        for (uint32_t k = 0; k < lim; ++k) {
-            TileItem ti;
-            ti.rec = ok ? r : 0xFFFFFFFFu;
-            ti.tx = (uint16_t)(k % ntx);
-            ti.ty = (uint16_t)(k / ntx);
-            items[it + k] = ti;
-        }
-        if (ok)
-            return;
-        // Queue full: rasterize this triangle in-lane instead.
-        atomicOr(&counters[kCntOverflow], 1u);
-    }
-    for (int32_t iy = ts.iy0; iy <= ts.iy1; ++iy)
-        for (int32_t ix = ts.ix0; ix <= ts.ix1; ++ix)
-            shade_pixel(ix, iy, ts.A, ts.B, ts.C, ts.bias, ts.dzdx, ts.dzdy, a.x, a.y, a.z,
-                        id, base, TW, keys);
 }
 
 // Sutherland-Hodgman against the near plane and the guard band; an
 // intersection is interpolated from the inside vertex towards the outside.
-__device__ static __noinline__ void raster_clipped(CV4 *poly, float halfW, uint32_t id,
-                                                   const FaceView &fv, int32_t base,
-                                                   int32_t TW, uint64_t *keys,
-                                                   RasterRec *recs, uint32_t rec_cap,
-                                                   TileItem *items, uint32_t item_cap,
-                                                   uint32_t *counters, uint32_t small_px)
+__device__ static void emit_clipped(const CV4 *in3, float halfW, uint32_t id, const FaceView &fv,
+                                   uint32_t s, int face, const BinCtx &bc)
 {
-    CV4 tmp[12];
+    CV4 poly[12], tmp[12];
+    poly[0] = in3[0];
+    poly[1] = in3[1];
+    poly[2] = in3[2];
     int n = 3;
     for (int plane = 0; plane < 5; ++plane) {
         float d[12];
@@ -451,50 +476,36 @@ __device__ static __noinline__ void raster_clipped(CV4 *poly, float halfW, uint3
     }
     if (n < 3)
         return;
-    SV s[12];
+    SV sv[12];
     for (int k = 0; k < n; ++k)
-        s[k] = project(poly[k], halfW);
+        sv[k] = project(poly[k], halfW);
     for (int k = 1; k + 1 < n; ++k)
-        raster_tri(s[0], s[k], s[k + 1], id, fv, base, TW, keys, recs, rec_cap, items,
-                   item_cap, counters, small_px);
+        emit_tri(sv[0], sv[k], sv[k + 1], id, fv, s, face, bc);
 }
 
-// Triangles are stored as 9 planes of ntri floats (v0.x, v0.y, ..., v2.z) so
-// consecutive lanes load consecutive addresses.
-__global__ __launch_bounds__(256) void k_raster_setup(
-    uint32_t W, uint32_t TW, uint32_t RES, const float *__restrict__ tri,
-    const uint32_t *__restrict__ tri_patch, uint32_t ntri, const float *__restrict__ mvp,
-    const uint32_t *__restrict__ nshoot, uint64_t *__restrict__ keys, RasterRec *recs,
-    uint32_t rec_cap, TileItem *items, uint32_t item_cap, uint32_t *counters,
-    uint32_t small_px)
+// Clip-space transform of a triangle: MVP (column-major) times (x, y, z, 1),
+// each row summed left to right.
+__device__ static inline void transform3(const float *mm, const float (&vx)[3][3], CV4 *poly)
 {
-    uint32_t sf = blockIdx.y;
-    uint32_t s = sf / 5, face = sf % 5;
-    if (s >= *nshoot)
-        return;
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntri)
-        return;
-    const float *m = mvp + (size_t)sf * 16;
-    float mm[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-        mm[k] = m[k];
-    CV4 poly[12];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float x = tri[(size_t)(3 * k + 0) * ntri + t];
-        float y = tri[(size_t)(3 * k + 1) * ntri + t];
-        float z = tri[(size_t)(3 * k + 2) * ntri + t];
+        float x = vx[k][0], y = vx[k][1], z = vx[k][2];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             poly[k].c[r] = mm[0 * 4 + r] * x + mm[1 * 4 + r] * y + mm[2 * 4 + r] * z +
                            mm[3 * 4 + r];
     }
-    int32_t Wi = (int32_t)W;
-    FaceView fv = face_view(Wi, (int)face);
-    // Trivial rejects; each is conservative by at least one pixel so it never
-    // removes a fragment the exact test would keep.
+}
+
+// Exact per-face pipeline of one triangle up to the record.  Triangles that
+// need clipping (near plane or guard band) are queued for k_setup_clipped.
+__device__ static inline void setup_face(const float *mm, const float (&vx)[3][3], uint32_t W,
+                                         uint32_t s, int face, uint32_t id, uint32_t t,
+                                         const BinCtx &bc)
+{
+    CV4 poly[3];
+    transform3(mm, vx, poly);
+    FaceView fv = face_view((int32_t)W, face);
     uint32_t all_out = 0x1f, any_out = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -507,95 +518,591 @@ __global__ __launch_bounds__(256) void k_raster_setup(
     }
     if (all_out)
         return;
+    if (any_out) {
+        uint32_t q = atomicAdd(&bc.counters[kCntClip], 1u);
+        if (q < bc.clip_cap)
+            bc.clipq[q] = make_uint2(t, s * 8u + (uint32_t)face);
+        else
+            atomicOr(&bc.counters[kCntOverflow], 1u);
+        return;
+    }
     float halfW = 0.5f * (float)W;
-    int32_t base = (int32_t)(s * RES) + fv.vpy * (int32_t)TW + fv.vpx;
-    uint32_t id = tri_patch[t] + 1u;
-    if (any_out == 0) {
-        // All vertices in front of the near plane (w > 0): reject against the
-        // scissor rectangle in clip space with a one-pixel margin.
-        float lo_x = 2.0f * (float)(fv.sx0 - 1) / (float)W - 1.0f;
-        float hi_x = 2.0f * (float)(fv.sx1 + 1) / (float)W - 1.0f;
-        float lo_y = 2.0f * (float)(fv.sy0 - 1) / (float)W - 1.0f;
-        float hi_y = 2.0f * (float)(fv.sy1 + 1) / (float)W - 1.0f;
-        bool lx = true, hx = true, ly = true, hy = true;
+    // All vertices in front of the near plane (w > 0): reject against the
+    // scissor rectangle in clip space with a one-pixel margin.
+    float lo_x = 2.0f * (float)(fv.sx0 - 1) / (float)W - 1.0f;
+    float hi_x = 2.0f * (float)(fv.sx1 + 1) / (float)W - 1.0f;
+    float lo_y = 2.0f * (float)(fv.sy0 - 1) / (float)W - 1.0f;
+    float hi_y = 2.0f * (float)(fv.sy1 + 1) / (float)W - 1.0f;
+    bool lx = true, hx = true, ly = true, hy = true;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float w = poly[k].c[3];
-            lx = lx && (poly[k].c[0] < lo_x * w);
-            hx = hx && (poly[k].c[0] > hi_x * w);
-            ly = ly && (poly[k].c[1] < lo_y * w);
-            hy = hy && (poly[k].c[1] > hi_y * w);
+    for (int k = 0; k < 3; ++k) {
+        float w = poly[k].c[3];
+        lx = lx && (poly[k].c[0] < lo_x * w);
+        hx = hx && (poly[k].c[0] > hi_x * w);
+        ly = ly && (poly[k].c[1] < lo_y * w);
+        hy = hy && (poly[k].c[1] > hi_y * w);
+    }
+    if (lx || hx || ly || hy)
+        return;
+    SV a = project(poly[0], halfW), b = project(poly[1], halfW), c = project(poly[2], halfW);
+    emit_tri(a, b, c, id, fv, s, face, bc);
+}
+
+// Queued (triangle, shooter, face) jobs that cross the near plane or the
+// guard band: same transform, then Sutherland-Hodgman and a fan; records go
+// to the spill region.
+__global__ __launch_bounds__(256) void k_setup_clipped(uint32_t W, const float *__restrict__ tri,
+                                                       const uint32_t *__restrict__ tri_patch,
+                                                       uint32_t ntri, const float *__restrict__ mvp,
+                                                       BinCtx bc)
+{
+    uint32_t n = min(bc.counters[kCntClip], bc.clip_cap);
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        uint2 job = bc.clipq[q];
+        uint32_t t = job.x, s = job.y >> 3;
+        int face = (int)(job.y & 7u);
+        float vx[3][3];
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c)
+                vx[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
+        CV4 poly[3];
+        transform3(mvp + (size_t)s * kShooterStride + face * 16, vx, poly);
+        FaceView fv = face_view((int32_t)W, face);
+        emit_clipped(poly, 0.5f * (float)W, tri_patch[t] + 1u, fv, s, face, bc);
+    }
+}
+
+__device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
+                                      int f0, int f1);
+
+// One workgroup per (256 triangles, shooter[, face]).  Phase 1: one lane
+// per triangle runs the conservative culls (cull_faces) and queues the
+// surviving (triangle, face) pairs in LDS; phase 2: the lanes take the queue
+// so the exact pipeline runs with every lane busy.  Records go to the block's
+// own region (LDS slot counter) and are counted in an LDS tile histogram,
+// flushed with one global atomic per touched tile.  Triangles are stored as 9
+// planes of ntri floats (v0.x, v0.y, ..., v2.z) so consecutive lanes load
+// consecutive addresses.
+__device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
+                                      int f0, int f1);
+
+// One lane per (triangle, shooter), looping over the five faces.  Two
+// conservative culls skip work the exact per-face test would reject anyway:
+//  - a triangle strictly below the shooter's patch plane (beyond a float
+//    rounding margin) projects into the scissored-away half of every side
+//    face and behind the near plane of the front face;
+//  - a face whose frustum (90 degree pyramid around its axis, near 0.01)
+//    misses the triangle's bounding sphere by more than a margin.
+// Triangles are stored as 9 planes of ntri floats (v0.x, v0.y, ..., v2.z) so
+// consecutive lanes load consecutive addresses.
+__global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *__restrict__ tri,
+                                                      const uint32_t *__restrict__ tri_patch,
+                                                      uint32_t ntri, const float *__restrict__ mvp,
+                                                      const uint32_t *__restrict__ nshoot,
+                                                      BinCtx bc, uint32_t *__restrict__ blk_nrec)
+{
+    __shared__ uint32_t lcnt[kMaxTilesPerSlab];
+    __shared__ float lv[9][256];
+    __shared__ uint32_t lid[256];
+    __shared__ uint16_t lq[256 * 5];
+    __shared__ uint32_t lnrec, lqn;
+    uint32_t s = blockIdx.y;
+    if (s >= *nshoot)
+        return;
+    for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
+        lcnt[i] = 0u;
+    if (threadIdx.x == 0) {
+        lnrec = 0u;
+        lqn = 0u;
+    }
+    __syncthreads();
+    bc.lcnt = lcnt;
+    bc.lnrec = &lnrec;
+    // gridDim.z == 5 splits the faces over blocks (small K), else one block
+    // walks all five faces
+    const uint32_t rb = (s * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x;
+    bc.region = rb * kRecsPerBlock;
+    int f0 = gridDim.z == 1 ? 0 : (int)blockIdx.z, f1 = gridDim.z == 1 ? 5 : f0 + 1;
+    const float *sm = mvp + (size_t)s * kShooterStride;
+    // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntri) {
+        float vx[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                vx[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
+                lv[3 * k + c][threadIdx.x] = vx[k][c];
+            }
+        lid[threadIdx.x] = tri_patch[t] + 1u;
+        uint32_t fm = cull_faces(sm, vx, f0, f1);
+        while (fm) {
+            int f = __ffs(fm) - 1;
+            fm &= fm - 1;
+            lq[atomicAdd(&lqn, 1u)] = (uint16_t)((threadIdx.x << 3) | (uint32_t)f);
         }
-        if (lx || hx || ly || hy)
+    }
+    __syncthreads();
+    // phase 2: the exact per-face pipeline over the compacted pairs
+    const uint32_t nq = lqn;
+    for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
+        uint32_t e = lq[q];
+        uint32_t tl = e >> 3;
+        int face = (int)(e & 7u);
+        float vx[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                vx[k][c] = lv[3 * k + c][tl];
+        setup_face(sm + face * 16, vx, W, s, face, lid[tl], blockIdx.x * blockDim.x + tl, bc);
+    }
+    __syncthreads();
+    // flush: one global atomic per touched tile, and the block's record count
+    uint32_t *cnt = bc.bin_count + (size_t)s * bc.ntiles;
+    for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
+        if (lcnt[i])
+            atomicAdd(&cnt[i], lcnt[i]);
+    if (threadIdx.x == 0)
+        blk_nrec[rb] = lnrec;
+}
+
+// Conservative culls of one triangle for one shooter; returns the mask of
+// faces in [f0, f1) the exact pipeline must look at.
+//  - a triangle strictly below the shooter's patch plane (beyond a float
+//    rounding margin) projects into the scissored-away half of every side
+//    face and behind the near plane of the front face;
+//  - a face whose frustum (90 degree pyramid around its axis, near 0.01)
+//    misses the triangle's bounding sphere by more than a margin.
+__device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
+                                      int f0, int f1)
+{
+    const float ex = sm[80], ey = sm[81], ez = sm[82];
+    const float nx = sm[83], ny = sm[84], nz = sm[85];
+    const float ux = sm[86], uy = sm[87], uz = sm[88];
+    const float wx = sm[89], wy = sm[90], wz = sm[91];
+    float d[3][3];
+    bool below = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d[k][0] = vx[k][0] - ex;
+        d[k][1] = vx[k][1] - ey;
+        d[k][2] = vx[k][2] - ez;
+        float h = d[k][0] * nx + d[k][1] * ny + d[k][2] * nz;
+        float tol = 1e-6f * (fabsf(d[k][0]) + fabsf(d[k][1]) + fabsf(d[k][2]));
+        below = below && (h < -tol);
+    }
+    if (below)
+        return 0u;
+    float cx = (d[0][0] + d[1][0] + d[2][0]) * (1.0f / 3.0f);
+    float cy = (d[0][1] + d[1][1] + d[2][1]) * (1.0f / 3.0f);
+    float cz = (d[0][2] + d[1][2] + d[2][2]) * (1.0f / 3.0f);
+    float r2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float qx = d[k][0] - cx, qy = d[k][1] - cy, qz = d[k][2] - cz;
+        r2 = fmaxf(r2, qx * qx + qy * qy + qz * qz);
+    }
+    float rho = sqrtf(r2);
+    float pa = cx * nx + cy * ny + cz * nz;
+    float pb = cx * ux + cy * uy + cz * uz;
+    float pc = cx * wx + cy * wy + cz * wz;
+    float m = rho * 1.4143f + 1e-4f * (fabsf(pa) + fabsf(pb) + fabsf(pc) + rho) + 1e-6f;
+    uint32_t mask = 0u;
+    for (int face = f0; face < f1; ++face) {
+        float q, s1, s2;
+        switch (face) {
+        case 0: q = pb; s1 = pa; s2 = pc; break;   // UP    (axis  u)
+        case 1: q = -pb; s1 = pa; s2 = pc; break;  // DOWN  (axis -u)
+        case 2: q = pc; s1 = pa; s2 = pb; break;   // LEFT  (axis n x u)
+        case 3: q = -pc; s1 = pa; s2 = pb; break;  // RIGHT (axis -(n x u))
+        default: q = pa; s1 = pb; s2 = pc; break;  // FRONT (axis n)
+        }
+        if (q + m < fabsf(s1) || q + m < fabsf(s2) || q + rho + m < 0.0099f)
+            continue;
+        mask |= 1u << face;
+    }
+    return mask;
+}
+
+// Exclusive scan of n bin counts by one block of 1024 threads (each thread
+// scans a contiguous chunk); writes start offsets, resets the fill counters and
+// stores the total in counters[kCntEntries].
+__global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ cnt,
+                                                   uint32_t *__restrict__ start,
+                                                   uint32_t *__restrict__ fill, uint32_t n,
+                                                   uint32_t *__restrict__ counters)
+{
+    __shared__ uint32_t part[1024];
+    uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i)
+        sum += cnt[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t i = b0; i < b1; ++i) {
+        start[i] = run;
+        fill[i] = 0u;
+        run += cnt[i];
+    }
+    if (threadIdx.x == blockDim.x - 1)
+        counters[kCntEntries] = part[threadIdx.x];
+}
+
+// Same grid as k_raster_setup: block (blk, s) puts the records of its region
+// into their bins, reserving each bin's range with one global atomic per
+// touched tile.  The extra column blk == nblk handles the spill region.
+__device__ static inline void unpack_bbox(uint64_t v, uint32_t tshift, uint32_t &g0,
+                                          uint32_t &g1, uint32_t &h0, uint32_t &h1)
+{
+    g0 = (uint32_t)(v & 0xFFFFu) >> tshift;
+    h0 = (uint32_t)((v >> 16) & 0xFFFFu) >> tshift;
+    g1 = (uint32_t)((v >> 32) & 0xFFFFu) >> tshift;
+    h1 = (uint32_t)(v >> 48) >> tshift;
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ recs,
+                                                  const uint64_t *__restrict__ bbox,
+                                                  const uint32_t *__restrict__ blk_nrec,
+                                                  const uint32_t *__restrict__ counters,
+                                                  uint32_t nblk, uint32_t spill_base,
+                                                  uint32_t spill_cap,
+                                                  const uint32_t *__restrict__ start,
+                                                  uint32_t *__restrict__ fill,
+                                                  uint32_t *__restrict__ list, uint32_t ent_cap,
+                                                  uint32_t ntx, uint32_t ntiles, uint32_t tshift,
+                                                  const uint32_t *__restrict__ nshoot)
+{
+    __shared__ uint32_t lcnt[kMaxTilesPerSlab];
+    __shared__ uint32_t lbase[kMaxTilesPerSlab];
+    const uint32_t s = blockIdx.y;
+    if (s >= *nshoot)
+        return;
+    uint32_t *fl = fill + (size_t)s * ntiles;
+    const uint32_t *stt = start + (size_t)s * ntiles;
+    if (blockIdx.x == nblk) {
+        if (blockIdx.z != 0)
             return;
-        SV a = project(poly[0], halfW), b = project(poly[1], halfW), c = project(poly[2], halfW);
-        raster_tri(a, b, c, id, fv, base, (int32_t)TW, keys, recs, rec_cap, items, item_cap,
-                   counters, small_px);
-    } else {
-        raster_clipped(poly, halfW, id, fv, base, (int32_t)TW, keys, recs, rec_cap, items,
-                       item_cap, counters, small_px);
+        // spill records of this shooter: one global atomic each (rare)
+        uint32_t ns = min(counters[kCntSpill], spill_cap);
+        for (uint32_t q = threadIdx.x; q < ns; q += blockDim.x) {
+            uint32_t r = spill_base + q;
+            if ((recs[r].sf >> 3) != s)
+                continue;
+            uint32_t g0, g1, h0, h1;
+            unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
+            for (uint32_t gy = h0; gy <= h1; ++gy)
+                for (uint32_t gx = g0; gx <= g1; ++gx) {
+                    uint32_t b = gy * ntx + gx;
+                    uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
+                    if (pos < ent_cap)
+                        list[pos] = r;
+                }
+        }
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x)
+        lcnt[i] = 0u;
+    __syncthreads();
+    const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
+    const uint32_t region = blk * kRecsPerBlock, nrec = blk_nrec[blk];
+    for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
+        uint32_t g0, g1, h0, h1;
+        unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
+        for (uint32_t gy = h0; gy <= h1; ++gy)
+            for (uint32_t gx = g0; gx <= g1; ++gx)
+                atomicAdd(&lcnt[gy * ntx + gx], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
+        uint32_t c = lcnt[i];
+        lbase[i] = c ? stt[i] + atomicAdd(&fl[i], c) : 0u;
+        lcnt[i] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
+        uint32_t g0, g1, h0, h1;
+        unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
+        for (uint32_t gy = h0; gy <= h1; ++gy)
+            for (uint32_t gx = g0; gx <= g1; ++gx) {
+                uint32_t b = gy * ntx + gx;
+                uint32_t pos = lbase[b] + atomicAdd(&lcnt[b], 1u);
+                if (pos < ent_cap)
+                    list[pos] = region + q;
+            }
+    }
+}
+
+// Wave-level ProcessHemicube on 64 consecutive texels (any grouping of equal
+// ids gives the same sums; consecutive texels make runs long).  Every lane
+// of the wavefront must call it.
+__device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t npatch,
+                                          float *__restrict__ ff, uint32_t *__restrict__ hits,
+                                          uint32_t *__restrict__ counters)
+{
+    int lane = threadIdx.x & 63;
+    uint32_t c = id ? 1u : 0u;
+    uint32_t pid = __shfl_up(id, 1), phi = __shfl_up(hi, 1);
+    bool head = lane == 0 || pid != id || phi != hi;
+    int ss = head ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(ss, d);
+        if (lane >= d && o > ss)
+            ss = o;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        float fo = __shfl_up(f, d);
+        uint32_t co = __shfl_up(c, d);
+        if (lane - d >= ss) {
+            f += fo;
+            c += co;
+        }
+    }
+    uint32_t nid = __shfl_down(id, 1), nhi = __shfl_down(hi, 1);
+    bool tail = lane == 63 || nid != id || nhi != hi;
+    if (tail && id != 0) {
+        uint32_t p = id - 1;
+        if (p < npatch) {
+            size_t o = (size_t)hi * npatch + p;
+            atomicAdd(&ff[o], f);
+            if (hits)
+                atomicAdd(&hits[o], c);
+        } else {
+            atomicAdd(&counters[kCntUnknown], 1u);
+        }
+    }
+}
+
+__device__ static inline void draw_span(const RasterRec &r, const Edges &e, int32_t vpx,
+                                        int32_t vpy, int32_t ax0, int32_t ay0, uint32_t w,
+                                        uint32_t area, uint32_t p0, uint32_t step, int32_t ox,
+                                        int32_t oy, uint32_t tshift, uint64_t *tk)
+{
+    for (uint32_t p = p0; p < area; p += step) {
+        int32_t tx = ax0 + (int32_t)(p % w), ty = ay0 + (int32_t)(p / w);
+        uint64_t key = frag_key(e, r, tx - vpx, ty - vpy);
+        uint32_t pp = (uint32_t)((ty - oy) << tshift) + (uint32_t)(tx - ox);
+        if (key && key < tk[pp])
+            atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
+    }
+}
+
+// One lane draws a record's pixels [ax0,ax1] x [ay0,ay1] (texels).  When the
+// snapped vertices span less than 64 pixels every edge value inside the bbox
+// fits in int32, so the edge functions are stepped with 32-bit adds; the
+// values equal the 64-bit evaluation exactly.  Depth is the same
+// fmaf(dzdx, px-x0, fmaf(dzdy, py-y0, z0)) with the inner term per row.
+__device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t vpy,
+                                        int32_t ax0, int32_t ax1, int32_t ay0, int32_t ay1,
+                                        int32_t ox, int32_t oy, uint32_t tshift, uint64_t *tk)
+{
+    int32_t xmin = min(r.x[0], min(r.x[1], r.x[2])), xmax = max(r.x[0], max(r.x[1], r.x[2]));
+    int32_t ymin = min(r.y[0], min(r.y[1], r.y[2])), ymax = max(r.y[0], max(r.y[1], r.y[2]));
+    if (xmax - xmin >= (64 << 8) || ymax - ymin >= (64 << 8)) {
+        Edges e;
+        make_edges(r.x, r.y, e);
+        uint32_t w = (uint32_t)(ax1 - ax0 + 1), area = w * (uint32_t)(ay1 - ay0 + 1);
+        draw_span(r, e, vpx, vpy, ax0, ay0, w, area, 0, 1, ox, oy, tshift, tk);
+        return;
+    }
+    const int32_t px0 = (ax0 - vpx) * kSub + kSub / 2, py0 = (ay0 - vpy) * kSub + kSub / 2;
+    int32_t v[3], dA[3], dB[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int q = k == 2 ? 0 : k + 1;
+        int32_t A = r.y[k] - r.y[q], B = r.x[q] - r.x[k];
+        int32_t bias = (A > 0 || (A == 0 && B < 0)) ? 1 : 0;
+        // E(p) = A (p.x - x_k) + B (p.y - y_k); inside iff E + bias - 1 >= 0
+        v[k] = A * (px0 - r.x[k]) + B * (py0 - r.y[k]) + bias - 1;
+        dA[k] = A * kSub;
+        dB[k] = B * kSub;
+    }
+    int32_t py = py0;
+    for (int32_t ty = ay0; ty <= ay1; ++ty, py += kSub) {
+        float zr = fmaf(r.dzdy, (float)(py - r.y[0]), r.z0);
+        int32_t e0 = v[0], e1 = v[1], e2 = v[2];
+        int32_t px = px0;
+        uint32_t prow = (uint32_t)((ty - oy) << tshift) - (uint32_t)ox;
+        for (int32_t tx = ax0; tx <= ax1; ++tx, px += kSub) {
+            if ((e0 | e1 | e2) >= 0) {
+                float z = fmaf(r.dzdx, (float)(px - r.x[0]), zr);
+                if (!(z > 0.0f))
+                    z = 0.0f;
+                if (z < 1.0f) {
+                    uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | r.id;
+                    uint32_t pp = prow + (uint32_t)tx;
+                    if (key < tk[pp])
+                        atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
+                }
+            }
+            e0 += dA[0];
+            e1 += dA[1];
+            e2 += dA[2];
+        }
+        v[0] += dB[0];
+        v[1] += dB[1];
+        v[2] += dB[2];
+    }
+}
+
+// One workgroup per bin = (shooter, tile).  The bin's records are taken 256
+// at a time and counting-sorted in LDS by their pixel count inside the tile
+// (power-of-two classes), so each wavefront holds records of about the same
+// size: records of at most 64 pixels are then drawn one per lane with
+// incremental edge functions, larger ones by the whole wavefront, one pixel
+// per lane.  Depth is resolved in LDS with 64-bit atomicMin.  Then the tile
+// either goes to the key buffer (keys != null) or is reduced into F rows and
+// hit counts (ff != null).
+__global__ __launch_bounds__(256) void k_tile_raster(
+    uint32_t W, uint32_t TW, uint32_t TH, uint32_t RES, const uint32_t *__restrict__ nshoot,
+    const RasterRec *__restrict__ recs, const uint64_t *__restrict__ bbox,
+    const uint32_t *__restrict__ bin_count, const uint32_t *__restrict__ bin_start,
+    const uint32_t *__restrict__ list, uint32_t ent_cap, uint32_t ntx, uint32_t ntiles,
+    uint32_t tshift, uint32_t small_px, uint64_t *__restrict__ keys, const float *__restrict__ dff,
+    uint32_t npatch, float *__restrict__ ff, uint32_t *__restrict__ hits,
+    uint32_t *__restrict__ counters)
+{
+    __shared__ uint64_t tk[kMaxTile * kMaxTile];
+    __shared__ uint32_t perm[256];
+    __shared__ uint32_t cls_n[9], cls_off[9];
+    const uint32_t s = blockIdx.y;
+    if (s >= *nshoot)
+        return;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t T = 1u << tshift, TT = T * T;
+    const int32_t ox = (int32_t)((tile % ntx) << tshift), oy = (int32_t)((tile / ntx) << tshift);
+    for (uint32_t p = threadIdx.x; p < TT; p += blockDim.x)
+        tk[p] = kClearKey;
+    const size_t b = (size_t)s * ntiles + tile;
+    const uint32_t n = bin_count[b], st0 = bin_start[b];
+    const int32_t Wi = (int32_t)W;
+    const uint32_t lane = threadIdx.x & 63, me = threadIdx.x;
+    for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
+        if (me < 9)
+            cls_n[me] = 0u;
+        __syncthreads();
+        // class of this lane's record: 0..6 for <= 1, 2, 4 ... 64 pixels,
+        // 7 for larger, 8 for no record
+        uint32_t i = c0 + me, cls = 8, ri = 0;
+        if (i < n && st0 + i < ent_cap) {
+            ri = list[st0 + i];
+            uint64_t bv = bbox[ri];
+            int32_t ax0 = max((int32_t)(bv & 0xFFFFu), ox);
+            int32_t ay0 = max((int32_t)((bv >> 16) & 0xFFFFu), oy);
+            int32_t ax1 = min((int32_t)((bv >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
+            int32_t ay1 = min((int32_t)(bv >> 48), oy + (int32_t)T - 1);
+            uint32_t area = (uint32_t)(ax1 - ax0 + 1) * (uint32_t)(ay1 - ay0 + 1);
+            cls = area <= 1 ? 0u : (area > small_px ? 7u : min(6u, 32u - __clz(area - 1)));
+        }
+        uint32_t rank = atomicAdd(&cls_n[cls], 1u);
+        __syncthreads();
+        if (me == 0) {
+            uint32_t acc = 0;
+            for (int k = 0; k < 9; ++k) {
+                cls_off[k] = acc;
+                acc += cls_n[k];
+            }
+        }
+        __syncthreads();
+        if (cls < 8)
+            perm[cls_off[cls] + rank] = ri;
+        __syncthreads();
+        const uint32_t nsmall = cls_off[7], nall = cls_off[8];
+        if (me < nsmall) {
+            RasterRec r = recs[perm[me]];
+            int32_t ax0 = max((int32_t)r.bx0, ox), ax1 = min((int32_t)r.bx1, ox + (int32_t)T - 1);
+            int32_t ay0 = max((int32_t)r.by0, oy), ay1 = min((int32_t)r.by1, oy + (int32_t)T - 1);
+            FaceView fv = face_view(Wi, (int)(r.sf & 7u));
+            draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
+        }
+        // large records: the whole wavefront per record, one pixel per lane;
+        // wave w takes records nsmall + w, nsmall + w + 4, ...
+        for (uint32_t q = nsmall + (me >> 6); q < nall; q += blockDim.x >> 6) {
+            RasterRec r = recs[perm[q]];
+            int32_t ax0 = max((int32_t)r.bx0, ox), ax1 = min((int32_t)r.bx1, ox + (int32_t)T - 1);
+            int32_t ay0 = max((int32_t)r.by0, oy), ay1 = min((int32_t)r.by1, oy + (int32_t)T - 1);
+            uint32_t w = (uint32_t)(ax1 - ax0 + 1), area = w * (uint32_t)(ay1 - ay0 + 1);
+            FaceView fv = face_view(Wi, (int)(r.sf & 7u));
+            Edges e;
+            make_edges(r.x, r.y, e);
+            draw_span(r, e, fv.vpx, fv.vpy, ax0, ay0, w, area, lane, 64, ox, oy, tshift, tk);
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    // resolve: whole waves walk the tile's texels in row-major order
+    for (uint32_t p0 = 0; p0 < TT; p0 += blockDim.x) {
+        uint32_t p = p0 + threadIdx.x;
+        int32_t tx = ox + (int32_t)(p & (T - 1)), ty = oy + (int32_t)(p >> tshift);
+        bool inside = p < TT && tx < (int32_t)TW && ty < (int32_t)TH;
+        uint64_t key = inside ? tk[p] : kClearKey;
+        uint32_t texel = (uint32_t)ty * TW + (uint32_t)tx;
+        if (keys && inside)
+            keys[(size_t)s * RES + texel] = key;
+        if (ff) {
+            uint32_t id = (uint32_t)key;
+            float f = id ? dff[texel] : 0.0f;
+            reduce_wave(id, f, s, npatch, ff, hits, counters);
+        }
     }
 }
 
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
-                         const uint32_t *nshoot, uint64_t *keys, RasterRec *recs,
-                         uint32_t rec_cap, TileItem *items, uint32_t item_cap,
-                         uint32_t *counters, uint32_t small_px, hipStream_t st)
+                         const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
+                         hipStream_t st)
 {
-    if (ntri == 0)
-        return;
-    dim3 grid((ntri + 255) / 256, K * 5);
-    k_raster_setup<<<grid, 256, 0, st>>>(g.W, g.TW, g.RES, tri, tri_patch, ntri, mvp, nshoot,
-                                         keys, recs, rec_cap, items, item_cap, counters,
-                                         small_px);
+    BinCtx bc;
+    bc.recs = bb.recs;
+    bc.bbox = bb.bbox;
+    bc.bin_count = bb.count;
+    bc.counters = counters;
+    bc.lnrec = nullptr;
+    bc.lcnt = nullptr;
+    bc.region = 0;
+    bc.spill_base = bb.spill_base;
+    bc.spill_cap = bb.spill_cap;
+    bc.clipq = bb.clipq;
+    bc.clip_cap = bb.clip_cap;
+    bc.ntx = bb.ntx;
+    bc.ntiles = bb.ntiles;
+    bc.tshift = bb.tshift;
+    dim3 grid(bb.nblk, K, bb.nz);
+    k_raster_setup<<<grid, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec);
+    bc.lnrec = nullptr;
+    k_setup_clipped<<<bb.clip_blocks, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
 }
 
-// One wavefront per 8x8 tile item; lane l covers pixel (l & 7, l >> 3).
-__global__ __launch_bounds__(256) void k_raster_tiles(const RasterRec *__restrict__ recs,
-                                                      const TileItem *__restrict__ items,
-                                                      const uint32_t *__restrict__ counters,
-                                                      uint32_t rec_cap, uint32_t item_cap,
-                                                      int32_t TW, uint64_t *__restrict__ keys)
+void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
+                hipStream_t st)
 {
-    uint32_t nitem = counters[kCntItem];
-    if (nitem > item_cap)
-        nitem = item_cap;
-    uint32_t lane = threadIdx.x & 63;
-    uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
-    for (uint32_t it = wave; it < nitem; it += nwaves) {
-        TileItem ti = items[it];
-        if (ti.rec >= rec_cap)
-            continue;
-        const RasterRec &r = recs[ti.rec];
-        int32_t ix = r.bx0 + ti.tx * kTile + (int32_t)(lane & 7);
-        int32_t iy = r.by0 + ti.ty * kTile + (int32_t)(lane >> 3);
-        if (ix > r.bx1 || iy > r.by1)
-            continue;
-        SV a = {r.x[0], r.y[0], r.z0}, b = {r.x[1], r.y[1], 0.f}, c = {r.x[2], r.y[2], 0.f};
-        int32_t A[3], B[3], bias[3];
-        int64_t C[3];
-        const SV *v[3] = {&a, &b, &c};
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            const SV *p = v[e];
-            const SV *q = v[e == 2 ? 0 : e + 1];
-            A[e] = p->y - q->y;
-            B[e] = q->x - p->x;
-            C[e] = -((int64_t)A[e] * p->x + (int64_t)B[e] * p->y);
-            bias[e] = (A[e] > 0 || (A[e] == 0 && B[e] < 0)) ? 1 : 0;
-        }
-        shade_pixel(ix, iy, A, B, C, bias, r.dzdx, r.dzdy, r.x[0], r.y[0], r.z0, r.id, r.base,
-                    TW, keys);
-    }
+    uint32_t nb = K * bb.ntiles;
+    k_bin_scan<<<1, 1024, 0, st>>>(bb.count, bb.start, bb.fill, nb, counters);
+    dim3 grid(bb.nblk + 1, K, bb.nz);
+    k_bin_fill<<<grid, 256, 0, st>>>(bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base,
+                                     bb.spill_cap, bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx,
+                                     bb.ntiles, bb.tshift, nshoot);
 }
 
-void launch_raster_tiles(const RasterRec *recs, const TileItem *items, const uint32_t *counters,
-                         uint32_t rec_cap, uint32_t item_cap, int32_t TW, uint64_t *keys,
-                         int nblocks, hipStream_t st)
+void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
+                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
+                 uint32_t *hits, uint32_t *counters, hipStream_t st)
 {
-    k_raster_tiles<<<nblocks, 256, 0, st>>>(recs, items, counters, rec_cap, item_cap, TW, keys);
+    dim3 tg(bb.ntiles, K);
+    k_tile_raster<<<tg, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start,
+                                      bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px,
+                                      keys, dff, npatch, ff, hits, counters);
 }
 
 // ----------------------------------------------------------- ProcessHemicube

@@ -95,16 +95,42 @@ def test_itembuf_bit_exact_boxes(rad, boxes_small):
     _render_case(rad, boxes_small, 128, sh)
 
 
-def test_itembuf_small_triangle_threshold_invariant(rad, boxes_small):
-    # the in-lane / tiled split is a tuning knob: results must not depend on it
-    say("itembuf small_tri_px sweep")
+def test_itembuf_tuning_knobs_invariant(rad, boxes_small):
+    # tile size and the in-lane pixel limit are tuning knobs: the item buffer
+    # must not depend on them
+    say("itembuf tile_px / small_tri_px sweep")
     outs = []
-    for px in (1, 16, 64, 100000):
-        s = rad.Solver(hemicube=64, hemicubes=2, small_tri_px=px)
+    for tile, px in ((8, 1), (16, 16), (32, 16), (32, 1024), (16, 4)):
+        s = rad.Solver(hemicube=64, hemicubes=2, small_tri_px=px, tile_px=tile)
         s.upload(boxes_small)
         outs.append(s.render([5, 4000]))
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+    it_o = O.render_hemicube(64, boxes_small.tri, boxes_small.tri_patch, boxes_small.frame[5])
+    assert np.array_equal(outs[0][:len(it_o)], it_o)
+
+
+def test_fused_reduce_matches_item_buffer_reduce(rad, cornell_t):
+    # the shooting pass reduces tiles in the raster kernel; its F rows must
+    # equal the reduction of the written item buffer
+    say("fused vs item-buffer reduction")
+    W, K = 64, 3
+    sc = cornell_t
+    s = _solver(rad, sc, W, K=K)
+    sh = [int(np.argmax(sc.emission.sum(1))), 100, 900]
+    item = s.render(sh)
+    h_o, ff_o = O.histogram(W, K, item, O.dff_cross(W), sc.npatch)
+    u0, s0 = s.state()
+    s.shoot(sh)
+    u1, _ = s.state()
+    snap = u0[sh].astype(np.float64)
+    d = np.zeros_like(u0, dtype=np.float64)
+    for k, i in enumerate(sh):
+        d += sc.refl * snap[k] * ff_o[k][:, None] * (sc.area[i] / sc.area)[:, None]
+    d[sh] -= snap
+    want = u0.astype(np.float64) + d
+    scale = float(np.abs(want).max())
+    assert _close(u1, want, scale)
 
 
 def test_process_hemicube_hits_and_ff(rad, cornell_q):
