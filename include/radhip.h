@@ -56,7 +56,7 @@ typedef struct rad_config {
     uint32_t ocl_workitems_x; /* Config::oclWorkitemsX: spans per row for the run form (4) */
     uint32_t mode;            /* RAD_MODE_* */
     uint32_t select;          /* RAD_SELECT_* */
-    uint32_t small_tri_px;    /* raster tuning: triangle pixels in a tile drawn by one lane (0: 16) */
+    uint32_t small_tri_px;    /* raster tuning: triangle pixels in a tile drawn by one lane (0: 64) */
     uint32_t tile_px;         /* raster tuning: screen tile side 8, 16 or 32 (0: auto) */
 } rad_config;
 
@@ -159,6 +159,11 @@ int rad_face_mvps(rad_ctx *ctx, uint32_t patch, float *out80);
  * fill), [4] tile raster with the fused ProcessHemicube, [5] shoot; also the
  * number of timed passes. */
 int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
+
+/* Diagnostics (no counterpart in the reference): copies the per-tile record
+ * counts of the last pass (hemicubes * tiles entries, row-major tiles of
+ * tile_px) into out and returns the tile count per hemicube in *ntiles. */
+int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32_t *tile_px);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,7 @@ SIGNATURES = {
     "rad_scene_boxes": (C.c_int, [_u32, C.c_float, C.c_float, C.c_float, _u32, _u32,
                                   C.c_uint64, C.POINTER(C.POINTER(rad_scene))]),
     "rad_scene_free": (None, [C.POINTER(rad_scene)]),
+    "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
 }
 
 
@@ -273,6 +274,14 @@ class Solver:
         out = np.zeros(80, np.float32)
         self._chk(lib().rad_face_mvps(self._ctx, patch, fptr(out)), "rad_face_mvps")
         return out.reshape(5, 16)
+
+    def debug_bins(self):
+        nt, tp = C.c_uint32(), C.c_uint32()
+        self._chk(lib().rad_debug_bins(self._ctx, None, C.byref(nt), C.byref(tp)), "rad_debug_bins")
+        out = np.zeros(self.K * nt.value, np.uint32)
+        self._chk(lib().rad_debug_bins(self._ctx, uptr(out), C.byref(nt), C.byref(tp)),
+                  "rad_debug_bins")
+        return out.reshape(self.K, nt.value), tp.value
 
     def kernel_times(self, enable=-1):
         out = (C.c_double * 6)()

@@ -267,7 +267,7 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         rad_destroy(c);
         return RAD_E_ARG;
     }
-    c->small_px = cfg->small_tri_px ? cfg->small_tri_px : 16u;
+    c->small_px = cfg->small_tri_px ? cfg->small_tri_px : 64u;
     size_t nbins = (size_t)c->K * c->bb.ntiles;
     if ((rc = dalloc(c, &c->bb.count, nbins)) || (rc = dalloc(c, &c->bb.start, nbins)) ||
         (rc = dalloc(c, &c->bb.fill, nbins))) {
@@ -737,6 +737,19 @@ int rad_kernel_times(rad_ctx *c, int enable, double *out6, uint64_t *npasses)
             c->t_acc[k] = 0;
         c->t_passes = 0;
     }
+    return RAD_OK;
+}
+
+int rad_debug_bins(rad_ctx *c, uint32_t *out, uint32_t *ntiles, uint32_t *tile_px)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (ntiles)
+        *ntiles = c->bb.ntiles;
+    if (tile_px)
+        *tile_px = 1u << c->bb.tshift;
+    if (out)
+        HIPCHK(c, hipMemcpy(out, c->bb.count, (size_t)c->K * c->bb.ntiles * 4, hipMemcpyDeviceToHost));
     return RAD_OK;
 }
 

@@ -670,9 +670,13 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
 
 // Conservative culls of one triangle for one shooter; returns the mask of
 // faces in [f0, f1) the exact pipeline must look at.
-//  - a triangle strictly below the shooter's patch plane (beyond a float
-//    rounding margin) projects into the scissored-away half of every side
-//    face and behind the near plane of the front face;
+//  - a triangle with no vertex above the shooter's patch plane by more than
+//    tol = min(1e-6 |v - eye|_1, 0.0099) (which also bounds the float error
+//    of the height): in the side faces it lies in the scissored-away half or
+//    within tol/|v - eye| * W/2 << 0.5 pixel of the horizon, where the
+//    nearest kept pixel centre is half a pixel away; in the front face it is
+//    nearer than the near plane (0.01).  This removes the shooter's own
+//    plane (floor under a floor patch) before any per-face work;
 //  - a face whose frustum (90 degree pyramid around its axis, near 0.01)
 //    misses the triangle's bounding sphere by more than a margin.
 __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
@@ -690,8 +694,8 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
         d[k][1] = vx[k][1] - ey;
         d[k][2] = vx[k][2] - ez;
         float h = d[k][0] * nx + d[k][1] * ny + d[k][2] * nz;
-        float tol = 1e-6f * (fabsf(d[k][0]) + fabsf(d[k][1]) + fabsf(d[k][2]));
-        below = below && (h < -tol);
+        float tol = fminf(1e-6f * (fabsf(d[k][0]) + fabsf(d[k][1]) + fabsf(d[k][2])), 0.0099f);
+        below = below && (h < tol);
     }
     if (below)
         return 0u;

@@ -251,3 +251,36 @@ def test_run_converges_and_conserves(rad, cornell_t):
     assert np.all(np.isfinite(u)) and np.all(u >= -1e-6) and np.all(sh >= 0)
     # emitted power ends up shot or still unshot, both weighted by area
     assert (sc.area[:, None] * (sh + u)).sum() >= (sc.area[:, None] * sc.emission).sum() * 0.999
+
+
+def test_itembuf_dense_bins_bit_exact(rad):
+    # a dense scene on a small hemicube with 8-pixel tiles: hundreds of
+    # records per bin, many of them in several bins
+    say("itembuf dense bins: 40k triangles, W=16, 8-pixel tiles")
+    sc = rad.Scene.boxes(nboxes=60, room=(6.0, 3.0, 6.0), target_tris=40000, nlights=2, seed=7)
+    s = rad.Solver(hemicube=16, hemicubes=3, tile_px=8)
+    s.upload(sc)
+    sh = [10, 20000, 39000]
+    item = s.render(sh)
+    bins, _ = s.debug_bins()
+    say(f"  largest bin {int(bins.max())} records")
+    assert bins.max() > 256
+    res = s.tex_res
+    for k, p in enumerate(sh):
+        want = O.render_hemicube(16, sc.tri, sc.tri_patch, sc.frame[p])
+        assert np.array_equal(item[k * res:(k + 1) * res], want)
+    hits, ff = s.process(3)
+    h_o, ff_o = O.histogram(16, 3, item, O.dff_cross(16), sc.npatch)
+    assert np.array_equal(hits, h_o)
+    s2 = rad.Solver(hemicube=16, hemicubes=3, tile_px=8)
+    s2.upload(sc)
+    u0, _ = s2.state()
+    s2.shoot(sh)
+    u1, _ = s2.state()
+    snap = u0[sh].astype(np.float64)
+    d = np.zeros_like(u0, dtype=np.float64)
+    for k, i in enumerate(sh):
+        d += sc.refl * snap[k] * ff_o[k][:, None] * (sc.area[i] / sc.area)[:, None]
+    d[sh] -= snap
+    want = u0.astype(np.float64) + d
+    assert _close(u1, want, float(np.abs(want).max()))
