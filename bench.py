@@ -29,6 +29,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(W, K):
+    """HBM bytes per launch from the committed counter summary of the default
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
+    if (W, K) != (256, 64):
+        return {}
+    path = os.path.join(ROOT, "profiles", "room200k_W256_K64_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return json.loads(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return {}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +57,10 @@ def main():
     ap.add_argument("--plain-cap-s", type=float, default=90.0)
     ap.add_argument("--cpu-sample-s", type=float, default=10.0)
     ap.add_argument("--small-tri-px", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true", help="skip the K=1 leg (profiling)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB, so that a "
+                         "counter run can scale FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
@@ -50,6 +69,16 @@ def main():
 
     import numpy as np
     import radiosity_amd as R
+
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n} bytes read, {n} bytes written")
+        del src, dst
 
     log("scene: room 20x5x20, 400 boxes, seed 3")
     sc = R.Scene.boxes(nboxes=400, room=(20.0, 5.0, 20.0), target_tris=200000, nlights=16, seed=3)
@@ -78,21 +107,23 @@ def main():
     log("kernel ms/pass: " + json.dumps({k: round(v, 4) for k, v in kt.items()}))
 
     # ----------------------------------------------------------- plain K=1
-    p = R.Solver(hemicube=W, hemicubes=1, small_tri_px=args.small_tri_px)
-    p.upload(sc)
-    p.step(20)
-    p.finish()
-    t0 = time.perf_counter()
-    p.step(args.plain_passes)
-    p.finish()
-    pdt = time.perf_counter() - t0
-    plain_sps = args.plain_passes / pdt
-    log(f"plain K=1: {pdt * 1e3 / args.plain_passes:.4f} ms/pass, {plain_sps:.0f} shooters/s")
-    p.kernel_times(1)
-    p.step(200)
-    p.finish()
-    pkt, _ = p.kernel_times(0)
-    pkt = dict(zip(names, pkt))
+    p = plain_sps = pdt = pkt = None
+    if not args.no_plain:
+        p = R.Solver(hemicube=W, hemicubes=1, small_tri_px=args.small_tri_px)
+        p.upload(sc)
+        p.step(20)
+        p.finish()
+        t0 = time.perf_counter()
+        p.step(args.plain_passes)
+        p.finish()
+        pdt = time.perf_counter() - t0
+        plain_sps = args.plain_passes / pdt
+        log(f"plain K=1: {pdt * 1e3 / args.plain_passes:.4f} ms/pass, {plain_sps:.0f} shooters/s")
+        p.kernel_times(1)
+        p.step(200)
+        p.finish()
+        pkt, _ = p.kernel_times(0)
+        pkt = dict(zip(names, pkt))
 
     # --------------------------------------------------------- convergence
     conv = None
@@ -103,31 +134,34 @@ def main():
                             "residual": st.residual}}
         log(f"batched to {args.target}: {st.seconds:.2f} s, {st.shooters} shooters, "
             f"residual {st.residual:.3e}")
-        p.upload(sc)
-        cap = int(max(1000, plain_sps * args.plain_cap_s))
-        st = p.run(cap, args.target, check_every=2000)
-        reached = st.residual <= args.target
-        conv["plain"] = {"ms": st.seconds * 1e3, "passes": st.passes, "shooters": st.shooters,
-                         "residual": st.residual, "reached": bool(reached),
-                         "cap_passes": cap}
-        log(f"plain to {args.target}: {st.seconds:.2f} s, {st.shooters} shooters, residual "
-            f"{st.residual:.3e} ({'reached' if reached else 'capped'})")
-        if reached:
-            conv["speedup_batched_vs_plain"] = conv["plain"]["ms"] / conv["batched"]["ms"]
+        if p is not None:
+            p.upload(sc)
+            cap = int(max(1000, plain_sps * args.plain_cap_s))
+            st = p.run(cap, args.target, check_every=2000)
+            reached = st.residual <= args.target
+            conv["plain"] = {"ms": st.seconds * 1e3, "passes": st.passes, "shooters": st.shooters,
+                             "residual": st.residual, "reached": bool(reached),
+                             "cap_passes": cap}
+            log(f"plain to {args.target}: {st.seconds:.2f} s, {st.shooters} shooters, residual "
+                f"{st.residual:.3e} ({'reached' if reached else 'capped'})")
+            if reached:
+                conv["speedup_batched_vs_plain"] = conv["plain"]["ms"] / conv["batched"]["ms"]
 
     # ------------------------------------------------------------ roofline
     # Dominant kernel of the batched pass and its algorithmic bytes per launch
     # (K shooters):
-    #  raster_setup        36 B of vertices + 4 B patch id read per triangle
-    #                      per shooter
-    #  tile_raster_reduce  4 B of item-buffer id + 4 B of ΔFF per texel (the
-    #                      ProcessHemicube input, here resolved in LDS) plus a
-    #                      4 B F-row entry per patch
-    #  bin                 8 B per bin entry, shoot 4 B x K + 40 B per patch
+    #  tile_raster_reduce  ProcessHemicube: a 4 B item-buffer id per texel and
+    #                      shooter (resolved in LDS here, never written), a
+    #                      4 B F-row entry per patch and shooter, and the 4 B
+    #                      per texel ΔFF table once per launch
+    #  raster_setup        36 B of vertices + 4 B patch id per triangle and
+    #                      shooter
+    #  shoot               4 B of F per shooter and patch + 40 B of patch state
+    #  bin, select, prepare_clear: list entries, 16 B / 4 B per patch
     RES = 3 * W * W
     per_launch = {
         "raster_setup": K * 40.0 * sc.ntri,
-        "tile_raster_reduce": K * (8.0 * RES + 4.0 * sc.npatch),
+        "tile_raster_reduce": K * (4.0 * RES + 4.0 * sc.npatch) + 4.0 * RES,
         "shoot": sc.npatch * (4.0 * K + 40.0),
         "select": sc.npatch * 16.0,
         "bin": K * 3.0 * RES,
@@ -138,7 +172,8 @@ def main():
     achieved = alg / (kt[dom] * 1e-3) / 1e9
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "alg_bytes_per_launch": alg, "avg_launch_ms": round(kt[dom], 5), "traffic": None}
+            "alg_bytes_per_launch": alg, "avg_launch_ms": round(kt[dom], 5),
+            "traffic": measured_traffic(W, K).get(dom)}
 
     # --------------------------------------------------------- CPU baseline
     cpu = None
@@ -180,9 +215,10 @@ def main():
                                f"triangle patches, 16 emitters, hemicube {W}, K={K} shooters "
                                f"per pass", "patches": sc.npatch, "hemicube": W,
                    "hemicubes_per_pass": K, "parallelism": "single GPU"},
-        "plain": {"shooters_per_s": round(plain_sps, 2),
-                  "ms_per_pass": round(pdt * 1e3 / args.plain_passes, 5),
-                  "kernel_ms": {k: round(v, 5) for k, v in pkt.items()}},
+        "plain": None if p is None else {
+            "shooters_per_s": round(plain_sps, 2),
+            "ms_per_pass": round(pdt * 1e3 / args.plain_passes, 5),
+            "kernel_ms": {k: round(v, 5) for k, v in pkt.items()}},
         "batched": {"shooters_per_s": round(batched_sps, 2),
                     "kernel_ms": {k: round(v, 5) for k, v in kt.items()}},
         "converge": conv,
