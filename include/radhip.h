@@ -165,6 +165,67 @@ int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
  * tile_px) into out and returns the tile count per hemicube in *ntiles. */
 int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32_t *tile_px);
 
+/* ---------------------------------------------------------------------------
+ * Dense form factors and the Jacobi gather (the precision path).
+ *
+ * The reference shoots one hemicube row at a time (Main.cpp:1251-1279).  The
+ * rows of all N patches form the dense matrix F (F_ij: form factor from
+ * patch i to patch j, one hemicube of patch i per row), stored on the device
+ * as BF16 plus the BF16 remainder F - bf16(F) (4 B per entry, N rounded up to
+ * 512).  The gather solves
+ *     B = E + diag(rho) M B          (column r uses the reflectance channel r % 3)
+ * with M = F (RAD_DENSE_ROWS, the textbook gathering form) or
+ * M_ji = F_ij A_i / A_j (RAD_DENSE_SHOOT: the factor PHYSICAL shooting applies,
+ * so the fixed point is the one rad_run converges to).  Jacobi sweeps run on
+ * v_mfma_f32_16x16x32_bf16 with FP32 accumulation; each outer step computes the
+ * residual r = E + diag(rho) M B - B from both BF16 halves of M and B (about
+ * 16 significant bits) in FP64 arithmetic, then solves the correction with
+ * BF16 sweeps (iterative refinement).  Residual = sum_i A_i sum_r |r_ir| /
+ * sum_i A_i sum_r |E_ir|, the measure rad_run uses for unshot energy.
+ * No counterpart in the reference beyond the shooting step it replaces.
+ * ------------------------------------------------------------------------- */
+#define RAD_DENSE_ROWS 0u
+#define RAD_DENSE_SHOOT 1u
+#define RAD_GATHER_MAX_R 192u
+
+/* Renders the hemicube of every patch (K per pass, the context's hemicube
+ * resolution) and stores the matrix.  Needs 4 * npad^2 bytes of free device
+ * memory (RAD_E_NOMEM otherwise).  Replaces rows of Main.cpp:1207-1249. */
+int rad_dense_build(rad_ctx *ctx, uint32_t layout);
+/* Stores a caller-given matrix M (npatch x npatch f32, row-major) instead. */
+int rad_dense_upload(rad_ctx *ctx, const float *m);
+/* Rows row0 .. row0+nrows-1 of the stored matrix (hi + lo) as f32. */
+int rad_dense_download(rad_ctx *ctx, uint32_t row0, uint32_t nrows, float *out);
+/* y = M x for R <= 192 columns (x, y: R x npatch, column r contiguous).
+ * split = 0: BF16 M and x (the sweep product); 1: both halves (the residual
+ * product). */
+int rad_dense_apply(rad_ctx *ctx, const float *x, uint32_t R, int split, float *y);
+/* Frees the matrix. */
+int rad_dense_release(rad_ctx *ctx);
+
+typedef struct rad_gather_opts {
+    uint32_t max_outer;   /* refinement steps (0: 50) */
+    uint32_t inner;       /* BF16 Jacobi sweeps per step (0: 8) */
+    uint32_t reserved[2];
+    double target;        /* stop when the residual <= target (e.g. 1e-3) */
+} rad_gather_opts;
+
+typedef struct rad_gather_stats {
+    uint32_t outer;       /* refinement steps taken */
+    uint32_t sweeps;      /* BF16 sweeps taken */
+    uint32_t reserved[2];
+    double residual;      /* final residual (FP64 sums) */
+    double seconds;       /* wall time of the solve */
+    double history[16];   /* residual after each of the first 16 steps */
+} rad_gather_stats;
+
+/* Solves for R <= 192 right-hand sides.  rhs: R x npatch (column r
+ * contiguous), or NULL with R = 3 for the scene's emission.  out (R x npatch)
+ * receives B + r, i.e. E + diag(rho) M B, the radiosity including the energy
+ * not yet gathered (as shot + unshot after shooting).  Blocks until done. */
+int rad_gather_solve(rad_ctx *ctx, const float *rhs, uint32_t R, const rad_gather_opts *opts,
+                     float *out, rad_gather_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

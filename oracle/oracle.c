@@ -671,3 +671,42 @@ uint32_t orc_plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule,
     }
     return i;
 }
+
+uint32_t orc_gather_jacobi(uint32_t n, uint32_t R, const float *M, const float *refl,
+                           const float *area, const double *E, double tol, uint32_t max_it,
+                           double *B)
+{
+    size_t nn = (size_t)R * n;
+    double *Y = (double *)malloc(nn * sizeof(double));
+    double den = 0.0;
+    for (uint32_t r = 0; r < R; ++r)
+        for (uint32_t i = 0; i < n; ++i)
+            den += (double)area[i] * fabs(E[(size_t)r * n + i]);
+    for (size_t e = 0; e < nn; ++e)
+        B[e] = E[e];
+    uint32_t it = 0;
+    for (; it < max_it; ++it) {
+        /* Y = E + diag(rho) M B */
+        double num = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const float *row = M + (size_t)i * n;
+            for (uint32_t r = 0; r < R; ++r) {
+                const double *b = B + (size_t)r * n;
+                double s = 0.0;
+                for (uint32_t j = 0; j < n; ++j)
+                    s += (double)row[j] * b[j];
+                double y = E[(size_t)r * n + i] + (double)refl[(size_t)i * 3 + r % 3] * s;
+                Y[(size_t)r * n + i] = y;
+                num += (double)area[i] * fabs(y - b[i]);
+            }
+        }
+        for (size_t e = 0; e < nn; ++e)
+            B[e] = Y[e];
+        if (den <= 0.0 || num / den <= tol) {
+            ++it;
+            break;
+        }
+    }
+    free(Y);
+    return it;
+}

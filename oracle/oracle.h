@@ -104,6 +104,16 @@ uint32_t orc_plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule,
                         float *unshot, float *shot,
                         uint32_t *item_scratch, float *ff_scratch);
 
+/* FP64 Jacobi for the gather B = E + diag(rho) M B (no reference
+ * counterpart: the fixed point the shooting loop of Main.cpp:1251-1295
+ * converges to, with M the shooting factors).  M: n x n row-major float;
+ * column r of E and B (R x n, column r contiguous) uses reflectance channel
+ * r % 3.  Iterates until sum A|r| / sum A|E| <= tol or max_it; returns the
+ * iterations taken.  B is the last iterate. */
+uint32_t orc_gather_jacobi(uint32_t n, uint32_t R, const float *M, const float *refl,
+                           const float *area, const double *E, double tol, uint32_t max_it,
+                           double *B);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,7 @@ _SIG = {
     "orc_residual": (C.c_double, [_PD, _PF, _PF, _u32]),
     "orc_plain_pass": (_u32, [_u32, _u32, _u32, _PF, _PU, _u32, _PF, _PF, _PF, _u32, _PF,
                               _PF, _PF, _PU, _PF]),
+    "orc_gather_jacobi": (_u32, [_u32, _u32, _PF, _PF, _PF, _PD, C.c_double, _u32, _PD]),
 }
 
 _lib = None
@@ -178,3 +179,15 @@ def plain_pass(W, mode, rule, sc, dff, unshot, shot, item_scratch=None, ff_scrat
     return lib().orc_plain_pass(W, mode, rule, _f(sc.tri), _u(sc.tri_patch), sc.ntri,
                                 _f(sc.frame), _f(sc.refl), _f(sc.area), sc.npatch, _f(dff),
                                 _f(unshot), _f(shot), _u(item_scratch), _f(ff_scratch))
+
+
+def gather_jacobi(M, refl, area, E, tol=1e-13, max_it=100000):
+    """FP64 Jacobi fixed point of B = E + diag(rho_{r%3}) M B; E, B: R x n."""
+    M = np.ascontiguousarray(M, np.float32)
+    n = M.shape[0]
+    E = np.ascontiguousarray(np.atleast_2d(E), np.float64)
+    B = np.zeros_like(E)
+    it = lib().orc_gather_jacobi(n, E.shape[0], _f(M), _f(np.ascontiguousarray(refl, np.float32)),
+                                 _f(np.ascontiguousarray(area, np.float32)), _d(E), tol, max_it,
+                                 _d(B))
+    return B, it

@@ -16,6 +16,8 @@ RAD_MODE_REFERENCE = 0
 RAD_MODE_PHYSICAL = 1
 RAD_SELECT_REFERENCE = 0
 RAD_SELECT_POWER = 1
+RAD_DENSE_ROWS = 0
+RAD_DENSE_SHOOT = 1
 
 
 class RadError(RuntimeError):
@@ -40,6 +42,17 @@ class rad_scene(C.Structure):
                 ("frame", C.POINTER(C.c_float)), ("refl", C.POINTER(C.c_float)),
                 ("area", C.POINTER(C.c_float)), ("emission", C.POINTER(C.c_float)),
                 ("shot0", C.POINTER(C.c_float)), ("patch_quad", C.POINTER(C.c_uint32))]
+
+
+class rad_gather_opts(C.Structure):
+    _fields_ = [("max_outer", C.c_uint32), ("inner", C.c_uint32),
+                ("reserved", C.c_uint32 * 2), ("target", C.c_double)]
+
+
+class rad_gather_stats(C.Structure):
+    _fields_ = [("outer", C.c_uint32), ("sweeps", C.c_uint32), ("reserved", C.c_uint32 * 2),
+                ("residual", C.c_double), ("seconds", C.c_double),
+                ("history", C.c_double * 16)]
 
 
 _lib = None
@@ -76,6 +89,13 @@ SIGNATURES = {
                                   C.c_uint64, C.POINTER(C.POINTER(rad_scene))]),
     "rad_scene_free": (None, [C.POINTER(rad_scene)]),
     "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
+    "rad_dense_build": (C.c_int, [_P, _u32]),
+    "rad_dense_upload": (C.c_int, [_P, _PF]),
+    "rad_dense_download": (C.c_int, [_P, _u32, _u32, _PF]),
+    "rad_dense_apply": (C.c_int, [_P, _PF, _u32, C.c_int, _PF]),
+    "rad_dense_release": (C.c_int, [_P]),
+    "rad_gather_solve": (C.c_int, [_P, _PF, _u32, C.POINTER(rad_gather_opts), _PF,
+                                   C.POINTER(rad_gather_stats)]),
 }
 
 
@@ -288,3 +308,44 @@ class Solver:
         n = C.c_uint64()
         self._chk(lib().rad_kernel_times(self._ctx, enable, out, C.byref(n)), "rad_kernel_times")
         return list(out), n.value
+
+    # ------------------------------------------------------ dense / gather
+    def dense_build(self, layout=RAD_DENSE_SHOOT):
+        self._chk(lib().rad_dense_build(self._ctx, layout), "rad_dense_build")
+
+    def dense_upload(self, m):
+        m = np.ascontiguousarray(m, np.float32)
+        assert m.shape == (self.npatch, self.npatch)
+        self._chk(lib().rad_dense_upload(self._ctx, fptr(m)), "rad_dense_upload")
+
+    def dense_download(self, row0=0, nrows=None):
+        if nrows is None:
+            nrows = self.npatch - row0
+        out = np.zeros((nrows, self.npatch), np.float32)
+        self._chk(lib().rad_dense_download(self._ctx, row0, nrows, fptr(out)), "rad_dense_download")
+        return out
+
+    def dense_apply(self, x, split=False):
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float32)
+        y = np.zeros_like(x)
+        self._chk(lib().rad_dense_apply(self._ctx, fptr(x), x.shape[0], int(split), fptr(y)),
+                  "rad_dense_apply")
+        return y
+
+    def dense_release(self):
+        self._chk(lib().rad_dense_release(self._ctx), "rad_dense_release")
+
+    def gather(self, rhs=None, target=1e-3, max_outer=0, inner=0):
+        """Returns (B, stats): B is R x npatch (R = 3 and the scene's emission
+        when rhs is None)."""
+        if rhs is None:
+            R, ptr = 3, None
+        else:
+            rhs = np.ascontiguousarray(np.atleast_2d(rhs), np.float32)
+            R, ptr = rhs.shape[0], fptr(rhs)
+        out = np.zeros((R, self.npatch), np.float32)
+        o = rad_gather_opts(max_outer, inner, (C.c_uint32 * 2)(), target)
+        st = rad_gather_stats()
+        self._chk(lib().rad_gather_solve(self._ctx, ptr, R, C.byref(o), fptr(out), C.byref(st)),
+                  "rad_gather_solve")
+        return out, st

@@ -57,6 +57,11 @@ struct rad_ctx {
     bool rendered = false;        // keys hold the item buffer of the last render
     uint32_t rendered_k = 0;
 
+    // dense matrix of the gather: npad x npad bf16 halves (hi, lo)
+    uint16_t *fhi = nullptr, *flo = nullptr;
+    uint32_t npad = 0;
+    bool dense = false;
+
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -124,7 +129,8 @@ void record(rad_ctx *c, size_t idx)
 int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits)
 {
     launch_prepare(c->g, k, c->shooters, c->nshoot, c->frames, c->unshot, c->snap, c->mvp, c->st);
-    launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->npatch, c->counters, c->st);
+    launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->npatch, c->counters, c->bb.ent_cap,
+                 c->st);
     record(c, evbase + 2);
     launch_raster_setup(c->g, k, c->tri, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
                         c->counters, c->st);
@@ -138,19 +144,23 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t 
 }
 
 // Reads the pass counters after a synchronisation: a record or bin-entry
-// overflow means the last pass lost triangles.
+// overflow in the last pass, or the sticky flag of an earlier one, means a
+// pass lost triangles.
 int check_overflow(rad_ctx *c)
 {
     uint32_t cnt[kCntN];
     hipError_t e = hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost);
     if (e != hipSuccess)
         return fail(c, RAD_E_HIP, std::string("counters: ") + hipGetErrorString(e));
-    if (cnt[kCntOverflow] || cnt[kCntEntries] > c->bb.ent_cap)
+    if (cnt[kCntOverflow] || cnt[kCntEntries] > c->bb.ent_cap || cnt[kCntSticky]) {
+        uint32_t zero = 0;
+        (void)hipMemcpy(c->counters + kCntSticky, &zero, 4, hipMemcpyHostToDevice);
         return fail(c, RAD_E_LIMIT, "rasterizer bins overflowed (records " +
                                         std::to_string(cnt[kCntSpill]) + " spilled of " +
                                         std::to_string(c->bb.spill_cap) + ", entries " +
                                         std::to_string(cnt[kCntEntries]) + " of " +
                                         std::to_string(c->bb.ent_cap) + ")");
+    }
     return RAD_OK;
 }
 
@@ -292,7 +302,8 @@ void rad_destroy(rad_ctx *c)
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
                     c->resid, c->bb.recs, c->bb.count, c->bb.start, c->bb.fill,
-                    c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts};
+                    c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts,
+                    c->fhi,  c->flo};
     for (void *p : ptrs)
         dfree(p);
     for (hipEvent_t e : c->ev)
@@ -637,7 +648,7 @@ int rad_finish(rad_ctx *c)
     if (!c)
         return RAD_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->st));
-    return RAD_OK;
+    return c->npatch ? check_overflow(c) : RAD_OK;
 }
 
 int rad_residual_fp64(rad_ctx *c, double *out)
@@ -750,6 +761,302 @@ int rad_debug_bins(rad_ctx *c, uint32_t *out, uint32_t *ntiles, uint32_t *tile_p
         *tile_px = 1u << c->bb.tshift;
     if (out)
         HIPCHK(c, hipMemcpy(out, c->bb.count, (size_t)c->K * c->bb.ntiles * 4, hipMemcpyDeviceToHost));
+    return RAD_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Dense matrix and Jacobi gather (rad_gather.hip kernels).
+namespace {
+
+uint32_t dense_pad(uint32_t n) { return (n + 511u) & ~511u; }
+
+// Split-K factor of the gather product: enough workgroups (128 rows each)
+// to give every CU two, npad / S stays a multiple of 64.
+uint32_t gather_split(uint32_t npad)
+{
+    uint32_t rows = npad / 128, S = 1;
+    while (S < 8 && rows * S < 512)
+        S *= 2;
+    return S;
+}
+
+int dense_alloc(rad_ctx *c)
+{
+    uint32_t npad = dense_pad(c->npatch);
+    if (c->fhi && c->npad == npad)
+        return RAD_OK;
+    dfree(c->fhi);
+    dfree(c->flo);
+    c->fhi = c->flo = nullptr;
+    c->dense = false;
+    size_t n = (size_t)npad * npad;
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    if (fr && 4 * n + ((size_t)512 << 20) > fr)
+        return fail(c, RAD_E_NOMEM, "dense matrix needs " + std::to_string(4 * n) +
+                                        " bytes, " + std::to_string(fr) + " free");
+    int rc;
+    if ((rc = dalloc(c, &c->fhi, n)) || (rc = dalloc(c, &c->flo, n)))
+        return rc;
+    c->npad = npad;
+    HIPCHK(c, hipMemsetAsync(c->fhi, 0, n * 2, c->st));
+    HIPCHK(c, hipMemsetAsync(c->flo, 0, n * 2, c->st));
+    return RAD_OK;
+}
+
+int need_dense(rad_ctx *c)
+{
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (!c->dense || !c->fhi)
+        return fail(c, RAD_E_STATE, "no dense matrix (rad_dense_build / rad_dense_upload)");
+    return RAD_OK;
+}
+
+// Work buffers of one gather call (R columns, rp = 16 * tiles).
+struct GatherBufs {
+    uint32_t rt = 1, rp = 16, S = 1, nblk = 1024;
+    float *yp = nullptr, *E = nullptr, *B = nullptr, *rv = nullptr, *out = nullptr;
+    uint16_t *xhi = nullptr, *xlo = nullptr;
+    double *parts = nullptr;
+    ~GatherBufs()
+    {
+        void *p[] = {yp, E, B, rv, out, xhi, xlo, parts};
+        for (void *q : p)
+            dfree(q);
+    }
+};
+
+int gather_alloc(rad_ctx *c, GatherBufs &g, uint32_t R)
+{
+    g.rt = gemm_tiles(R);
+    g.rp = g.rt * 16;
+    g.S = gather_split(c->npad);
+    size_t plane = (size_t)g.rp * c->npad;
+    int rc;
+    if ((rc = dalloc(c, &g.yp, plane * g.S)) || (rc = dalloc(c, &g.E, plane)) ||
+        (rc = dalloc(c, &g.B, plane)) || (rc = dalloc(c, &g.rv, plane)) ||
+        (rc = dalloc(c, &g.out, (size_t)R * c->npatch)) || (rc = dalloc(c, &g.xhi, plane)) ||
+        (rc = dalloc(c, &g.xlo, plane)) || (rc = dalloc(c, &g.parts, g.nblk)))
+        return rc;
+    HIPCHK(c, hipMemsetAsync(g.E, 0, plane * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(g.B, 0, plane * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(g.rv, 0, plane * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(g.xhi, 0, plane * 2, c->st));
+    HIPCHK(c, hipMemsetAsync(g.xlo, 0, plane * 2, c->st));
+    return RAD_OK;
+}
+
+int sum_parts(rad_ctx *c, GatherBufs &g, double *out)
+{
+    std::vector<double> h(g.nblk);
+    HIPCHK(c, hipMemcpyAsync(h.data(), g.parts, g.nblk * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double s = 0.0;
+    for (double v : h)
+        s += v;
+    *out = s;
+    return RAD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rad_dense_build(rad_ctx *c, uint32_t layout)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (layout > RAD_DENSE_SHOOT)
+        return fail(c, RAD_E_ARG, "dense_build: unknown layout");
+    int rc;
+    if ((rc = need_scene(c)) || (rc = dense_alloc(c)))
+        return rc;
+    bool timing = c->timing;
+    c->timing = false;
+    for (uint32_t base = 0; base < c->npatch; base += c->K) {
+        uint32_t k = c->npatch - base < c->K ? c->npatch - base : c->K;
+        launch_iota_shooters(base, k, c->shooters, c->nshoot, c->st);
+        if ((rc = render_pass(c, k, 0, nullptr, nullptr)))
+            break;
+        if (layout == RAD_DENSE_ROWS)
+            launch_dense_store(c->ff, k, base, c->npatch, c->npad, c->fhi, c->flo, c->st);
+        else
+            launch_dense_store_t(c->ff, k, base, c->npatch, c->npad, c->area, c->fhi, c->flo, c->st);
+        if ((rc = check_launch(c, "dense store")))
+            break;
+    }
+    c->timing = timing;
+    c->rendered = false;
+    if (rc)
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if ((rc = check_overflow(c)))
+        return rc;
+    c->dense = true;
+    return RAD_OK;
+}
+
+int rad_dense_upload(rad_ctx *c, const float *m)
+{
+    if (!c || !m)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)) || (rc = dense_alloc(c)))
+        return rc;
+    size_t n = (size_t)c->npatch * c->npatch;
+    float *tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, n)))
+        return rc;
+    hipError_t e = hipMemcpy(tmp, m, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_dense_from_f32(tmp, c->npatch, 0, c->npatch, c->npad, c->fhi, c->flo, c->st);
+        e = hipStreamSynchronize(c->st);
+    }
+    dfree(tmp);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("dense_upload: ") + hipGetErrorString(e));
+    c->dense = true;
+    return RAD_OK;
+}
+
+int rad_dense_download(rad_ctx *c, uint32_t row0, uint32_t nrows, float *out)
+{
+    if (!c || !out)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_dense(c)))
+        return rc;
+    if (row0 > c->npatch || nrows > c->npatch - row0)
+        return fail(c, RAD_E_ARG, "dense_download: rows out of range");
+    size_t n = (size_t)nrows * c->npatch;
+    float *tmp = nullptr;
+    if ((rc = dalloc(c, &tmp, n)))
+        return rc;
+    launch_dense_to_f32(c->fhi, c->flo, nrows, row0, c->npatch, c->npad, tmp, c->st);
+    hipError_t e = hipMemcpyAsync(out, tmp, n * 4, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->st);
+    dfree(tmp);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("dense_download: ") + hipGetErrorString(e));
+    return RAD_OK;
+}
+
+int rad_dense_release(rad_ctx *c)
+{
+    if (!c)
+        return RAD_E_ARG;
+    (void)hipStreamSynchronize(c->st);
+    dfree(c->fhi);
+    dfree(c->flo);
+    c->fhi = c->flo = nullptr;
+    c->npad = 0;
+    c->dense = false;
+    return RAD_OK;
+}
+
+int rad_dense_apply(rad_ctx *c, const float *x, uint32_t R, int split, float *y)
+{
+    if (!c || !x || !y)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_dense(c)))
+        return rc;
+    if (R == 0 || R > RAD_GATHER_MAX_R)
+        return fail(c, RAD_E_ARG, "dense_apply: R must be 1..192");
+    GatherBufs g;
+    if ((rc = gather_alloc(c, g, R)))
+        return rc;
+    size_t n = (size_t)R * c->npatch;
+    HIPCHK(c, hipMemcpyAsync(g.out, x, n * 4, hipMemcpyHostToDevice, c->st));
+    launch_pad_columns(R, c->npad, c->npatch, g.out, nullptr, g.xhi, split ? g.xlo : nullptr, c->st);
+    launch_gemm(g.rt, split != 0, c->fhi, c->flo, g.xhi, g.xlo, c->npad, g.S, g.yp, c->st);
+    launch_sum_partials(R, c->npad, c->npatch, g.S, g.rp, g.yp, g.out, c->st);
+    if ((rc = check_launch(c, "dense_apply")))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(y, g.out, n * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
+}
+
+int rad_gather_solve(rad_ctx *c, const float *rhs, uint32_t R, const rad_gather_opts *opts,
+                     float *out, rad_gather_stats *stats)
+{
+    if (!c || !out)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_dense(c)))
+        return rc;
+    if (R == 0 || R > RAD_GATHER_MAX_R || (!rhs && R != 3))
+        return fail(c, RAD_E_ARG, "gather: R must be 1..192 (3 without rhs)");
+    uint32_t max_outer = opts && opts->max_outer ? opts->max_outer : 50;
+    uint32_t inner = opts && opts->inner ? opts->inner : 8;
+    double target = opts ? opts->target : 1e-3;
+    auto t0 = std::chrono::steady_clock::now();
+    GatherBufs g;
+    if ((rc = gather_alloc(c, g, R)))
+        return rc;
+    const uint32_t N = c->npatch, P = c->npad;
+    size_t n = (size_t)R * N;
+    if (rhs) {
+        HIPCHK(c, hipMemcpyAsync(g.out, rhs, n * 4, hipMemcpyHostToDevice, c->st));
+    } else {
+        std::vector<float> em((size_t)N * 3), col(n);
+        HIPCHK(c, hipMemcpyAsync(em.data(), c->emis, em.size() * 4, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (uint32_t i = 0; i < N; ++i)
+            for (int ch = 0; ch < 3; ++ch)
+                col[(size_t)ch * N + i] = em[(size_t)i * 3 + ch];
+        HIPCHK(c, hipMemcpy(g.out, col.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    launch_pad_columns(R, P, N, g.out, g.E, nullptr, nullptr, c->st);
+    // B = 0, r = E; denominator sum_i A_i sum_r |E_ir|
+    launch_gather_update(0, g.yp, g.S, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
+                         g.xlo, g.parts, g.nblk, c->st);
+    if ((rc = check_launch(c, "gather init")))
+        return rc;
+    double den = 0.0, num = 0.0;
+    if ((rc = sum_parts(c, g, &den)))
+        return rc;
+    double res = den > 0 ? 1.0 : 0.0;
+    uint32_t outer = 0, sweeps = 0;
+    rad_gather_stats st{};
+    while (res > target && outer < max_outer) {
+        // correction d = r + diag(rho) M_bf16 d, `inner` sweeps from d = r,
+        // the last one adds d to B and splits B into its two bf16 halves
+        for (uint32_t s = 0; s < inner; ++s) {
+            launch_gemm(g.rt, false, c->fhi, nullptr, g.xhi, nullptr, P, g.S, g.yp, c->st);
+            launch_gather_update(s + 1 == inner ? 2 : 1, g.yp, g.S, g.rp, R, P, N, g.E, c->refl,
+                                 c->area, g.rv, g.B, g.xhi, g.xlo, g.parts, g.nblk, c->st);
+        }
+        sweeps += inner;
+        // residual r = E + diag(rho) M B - B from both halves, FP64 sums
+        launch_gemm(g.rt, true, c->fhi, c->flo, g.xhi, g.xlo, P, g.S, g.yp, c->st);
+        launch_gather_update(3, g.yp, g.S, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
+                             g.xlo, g.parts, g.nblk, c->st);
+        if ((rc = check_launch(c, "gather step")) || (rc = sum_parts(c, g, &num)))
+            return rc;
+        res = num / den;
+        if (outer < 16)
+            st.history[outer] = res;
+        ++outer;
+    }
+    launch_gather_out(R, P, N, g.B, g.rv, g.out, c->st);
+    if ((rc = check_launch(c, "gather out")))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(out, g.out, n * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (stats) {
+        st.outer = outer;
+        st.sweeps = sweeps;
+        st.residual = res;
+        st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *stats = st;
+    }
     return RAD_OK;
 }
 

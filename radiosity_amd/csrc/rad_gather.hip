@@ -1,0 +1,510 @@
+// rad_gather.hip - dense form-factor matrix and the Jacobi gather on MFMA.
+//
+// The shooting update of the reference (Main.cpp:1251-1279) distributes one
+// shooter's unshot radiosity along its hemicube row F_i.  Collecting every
+// row gives the dense matrix F (F_ij = form factor from patch i to patch j),
+// and the same fixed point is reached by gathering:
+//     B = E + diag(rho) F B                      (per colour channel)
+// The kernels here keep F in HBM as BF16 (round to nearest even) plus,
+// optionally, the BF16 remainder F - bf16(F), and apply it to up to 192
+// right-hand-side columns with v_mfma_f32_16x16x32_bf16 (FP32 accumulation).
+//
+// Layouts (npad = N rounded up to 512, padding is zero):
+//   F      npad x npad u16 (bf16 bits), row i = hemicube row of patch i
+//   X      rp x npad u16: column r of the right-hand side, contiguous over j
+//   vectors (E, B, r) rp x npad f32, same column-major order
+//   YP     S x rp x npad f32: split-K partial products F X
+#include "rad_internal.h"
+
+namespace rad {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- F build
+
+// Slots [0,k) of a pass render the hemicubes of patches row0 .. row0+k-1.
+__global__ void k_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, uint32_t *nshoot)
+{
+    uint32_t s = threadIdx.x + blockIdx.x * blockDim.x;
+    if (s < k)
+        shooters[s] = row0 + s;
+    if (s == 0)
+        *nshoot = k;
+}
+
+// Rows row0 .. row0+k-1 of F from the F rows of one pass (k x npatch f32).
+// One lane writes 8 consecutive columns (16 B of hi, 16 B of lo).
+__global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t row0,
+                              uint32_t npatch, uint32_t npad, uint16_t *__restrict__ fhi,
+                              uint16_t *__restrict__ flo)
+{
+    uint32_t s = blockIdx.y;
+    if (s >= k)
+        return;
+    const float *src = ff + (size_t)s * npatch;
+    size_t row = (size_t)(row0 + s) * npad;
+    for (uint32_t j0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8; j0 < npad;
+         j0 += gridDim.x * blockDim.x * 8) {
+        uint16_t h[8], l[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            uint32_t j = j0 + q;
+            float f = j < npatch ? src[j] : 0.0f;
+            h[q] = bf16_rne(f);
+            l[q] = bf16_rne(f - bf16_val(h[q]));
+        }
+        uint4 vh, vl;
+        vh.x = h[0] | ((uint32_t)h[1] << 16);
+        vh.y = h[2] | ((uint32_t)h[3] << 16);
+        vh.z = h[4] | ((uint32_t)h[5] << 16);
+        vh.w = h[6] | ((uint32_t)h[7] << 16);
+        vl.x = l[0] | ((uint32_t)l[1] << 16);
+        vl.y = l[2] | ((uint32_t)l[3] << 16);
+        vl.z = l[4] | ((uint32_t)l[5] << 16);
+        vl.w = l[6] | ((uint32_t)l[7] << 16);
+        *(uint4 *)(fhi + row + j0) = vh;
+        if (flo)
+            *(uint4 *)(flo + row + j0) = vl;
+    }
+}
+
+// Shooting-consistent layout: row j of G holds G_ji = F_ij A_i / A_j, the
+// factor PHYSICAL shooting applies when shooter i sends to receiver j
+// (k_shoot), so B = E + diag(rho) G B is the fixed point of the shooting
+// iteration.  Pass rows row0 .. row0+k-1 of F become columns of G: one lane
+// writes 8 consecutive columns (16 B) of one row j, never a column outside
+// [row0, row0+k).
+__global__ void k_dense_store_t(const float *__restrict__ ff, uint32_t k, uint32_t row0,
+                                uint32_t npatch, uint32_t npad, const float *__restrict__ area,
+                                uint16_t *__restrict__ fhi, uint16_t *__restrict__ flo)
+{
+    uint32_t groups = (k + 7) / 8;
+    size_t total = (size_t)npatch * groups;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        uint32_t g = (uint32_t)(e / npatch), j = (uint32_t)(e - (size_t)g * npatch);
+        float inv_aj = 1.0f / area[j];
+        uint16_t h[8], l[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            uint32_t s = g * 8 + q;
+            float f = 0.0f;
+            if (s < k)
+                f = ff[(size_t)s * npatch + j] * area[row0 + s] * inv_aj;
+            h[q] = bf16_rne(f);
+            l[q] = bf16_rne(f - bf16_val(h[q]));
+        }
+        size_t o = (size_t)j * npad + row0 + g * 8;
+        if ((row0 & 7u) == 0 && g * 8 + 8 <= k) {
+            uint4 vh, vl;
+            vh.x = h[0] | ((uint32_t)h[1] << 16);
+            vh.y = h[2] | ((uint32_t)h[3] << 16);
+            vh.z = h[4] | ((uint32_t)h[5] << 16);
+            vh.w = h[6] | ((uint32_t)h[7] << 16);
+            vl.x = l[0] | ((uint32_t)l[1] << 16);
+            vl.y = l[2] | ((uint32_t)l[3] << 16);
+            vl.z = l[4] | ((uint32_t)l[5] << 16);
+            vl.w = l[6] | ((uint32_t)l[7] << 16);
+            *(uint4 *)(fhi + o) = vh;
+            *(uint4 *)(flo + o) = vl;
+        } else {
+            // ragged group: only the k columns of this pass
+            for (uint32_t q = 0; q < 8 && g * 8 + q < k; ++q) {
+                fhi[o + q] = h[q];
+                flo[o + q] = l[q];
+            }
+        }
+    }
+}
+
+// Row-major f32 matrix (rows x npatch, host order) into rows of F.
+__global__ void k_dense_from_f32(const float *__restrict__ src, uint32_t rows, uint32_t row0,
+                                 uint32_t npatch, uint32_t npad, uint16_t *__restrict__ fhi,
+                                 uint16_t *__restrict__ flo)
+{
+    size_t n = (size_t)rows * npatch;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        size_t i = e / npatch, j = e - i * npatch;
+        float f = src[e];
+        uint16_t h = bf16_rne(f);
+        size_t o = (row0 + i) * (size_t)npad + j;
+        fhi[o] = h;
+        if (flo)
+            flo[o] = bf16_rne(f - bf16_val(h));
+    }
+}
+
+__global__ void k_dense_to_f32(const uint16_t *__restrict__ fhi, const uint16_t *__restrict__ flo,
+                               uint32_t rows, uint32_t row0, uint32_t npatch, uint32_t npad,
+                               float *__restrict__ dst)
+{
+    size_t n = (size_t)rows * npatch;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        size_t i = e / npatch, j = e - i * npatch;
+        size_t o = (row0 + i) * (size_t)npad + j;
+        float f = bf16_val(fhi[o]);
+        if (flo)
+            f += bf16_val(flo[o]);
+        dst[e] = f;
+    }
+}
+
+// ------------------------------------------------------------------ GEMM
+
+// YP[s] = F[:, kspan slice s] X[slice s]  for 128 rows per workgroup.
+// Wave w owns rows i0 .. i0+31 (two 16-row A fragments) and all RT column
+// tiles of 16.  Per 64-wide k chunk the X slice (RT*16 rows x 64 bf16) is
+// staged in LDS and shared by the four waves; the A fragments go straight
+// from HBM to registers (each F element is used by one wave only), with the
+// next chunk's loads issued before the current chunk's MFMAs.
+// SPLIT accumulates Fhi Xhi + Fhi Xlo + Flo Xhi (about 16 significant bits
+// of F and X) for the refinement residual.
+template <int RT, bool SPLIT>
+__global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
+                                              const uint16_t *__restrict__ flo,
+                                              const uint16_t *__restrict__ xhi,
+                                              const uint16_t *__restrict__ xlo, uint32_t npad,
+                                              uint32_t kspan, float *__restrict__ yp)
+{
+    constexpr int NB = SPLIT ? 2 : 1;
+    constexpr int LROW = 72;               // bf16 per LDS row: 64 + 8 pad (144 B)
+    constexpr int UNITS = RT * 16 * 8;     // 16-byte units of one X chunk
+    constexpr int UPT = (UNITS + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NB][RT * 16][LROW];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const uint32_t i0 = blockIdx.x * 128 + wave * 32;
+    const uint32_t kbeg = blockIdx.y * kspan, kend = kbeg + kspan;
+
+    f32x4 acc[2][RT];
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const size_t arow0 = (size_t)(i0 + m) * npad + 8 * kg;
+    const size_t arow1 = arow0 + (size_t)16 * npad;
+
+    uint4 xr[NB][UPT];
+    uint4 ar[NB][2][2];
+
+    auto load_x = [&](uint32_t kc) {
+#pragma unroll
+        for (int u = 0; u < UPT; ++u) {
+            int idx = tid + u * 256;
+            if (idx < UNITS) {
+                size_t o = (size_t)(idx >> 3) * npad + kc + (idx & 7) * 8;
+                xr[0][u] = *(const uint4 *)(xhi + o);
+                if (SPLIT)
+                    xr[NB - 1][u] = *(const uint4 *)(xlo + o);
+            }
+        }
+    };
+    auto load_a = [&](uint32_t kc) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            ar[0][0][ks] = *(const uint4 *)(fhi + arow0 + kc + ks * 32);
+            ar[0][1][ks] = *(const uint4 *)(fhi + arow1 + kc + ks * 32);
+            if (SPLIT) {
+                ar[NB - 1][0][ks] = *(const uint4 *)(flo + arow0 + kc + ks * 32);
+                ar[NB - 1][1][ks] = *(const uint4 *)(flo + arow1 + kc + ks * 32);
+            }
+        }
+    };
+
+    load_x(kbeg);
+    load_a(kbeg);
+    for (uint32_t kc = kbeg; kc < kend; kc += 64) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < UPT; ++u) {
+            int idx = tid + u * 256;
+            if (idx < UNITS) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    *(uint4 *)&lds[b][idx >> 3][(idx & 7) * 8] = xr[b][u];
+            }
+        }
+        __syncthreads();
+        uint4 ac[NB][2][2];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int f = 0; f < 2; ++f)
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+                    ac[b][f][ks] = ar[b][f][ks];
+        if (kc + 64 < kend) {
+            load_x(kc + 64);
+            load_a(kc + 64);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                bf16x8 bh = *(const bf16x8 *)&lds[0][t * 16 + m][ks * 32 + 8 * kg];
+#pragma unroll
+                for (int f = 0; f < 2; ++f) {
+                    bf16x8 ah = __builtin_bit_cast(bf16x8, ac[0][f][ks]);
+                    acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[f][t], 0, 0, 0);
+                    if (SPLIT) {
+                        bf16x8 bl = *(const bf16x8 *)&lds[NB - 1][t * 16 + m][ks * 32 + 8 * kg];
+                        bf16x8 al = __builtin_bit_cast(bf16x8, ac[NB - 1][f][ks]);
+                        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[f][t], 0, 0, 0);
+                        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[f][t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    // lane holds rows 4*kg .. 4*kg+3 of column m of each 16x16 tile
+    float *out = yp + (size_t)blockIdx.y * (RT * 16) * npad;
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            *(f32x4 *)(out + (size_t)(t * 16 + m) * npad + i0 + f * 16 + 4 * kg) = acc[f][t];
+}
+
+// ------------------------------------------------------- vector updates
+
+// Block partial sums in FP64 (one double per block).
+__device__ __forceinline__ void block_sum_store(double v, double *parts)
+{
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        parts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Update kinds of k_gather_update (one lane per (column, patch) element):
+//  kInit    r = E, B = 0, X = bf16(r); partial sums of A|E|
+//  kInner   d = r + rho y, X = bf16(d)
+//  kLast    d = r + rho y, B += d, (Xhi, Xlo) = split(B)
+//  kResid   r = E + rho y - B (FP64), X = bf16(r); partial sums of A|r|
+//  kOut     out = B + r
+enum { kInit = 0, kInner = 1, kLast = 2, kResid = 3 };
+
+__global__ __launch_bounds__(256) void k_gather_update(
+    int kind, const float *__restrict__ yp, uint32_t S, uint32_t rp, uint32_t R, uint32_t npad,
+    uint32_t n, const float *__restrict__ E, const float *__restrict__ refl,
+    const float *__restrict__ area, float *__restrict__ rv, float *__restrict__ B,
+    uint16_t *__restrict__ xhi, uint16_t *__restrict__ xlo, double *__restrict__ parts)
+{
+    double acc = 0.0;
+    size_t total = (size_t)R * npad;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t plane = (size_t)rp * npad;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += stride) {
+        uint32_t r = (uint32_t)(e / npad), i = (uint32_t)(e - (size_t)r * npad);
+        if (i >= n)
+            continue;
+        if (kind == kInit) {
+            float v = E[e];
+            rv[e] = v;
+            B[e] = 0.0f;
+            xhi[e] = bf16_rne(v);
+            acc += (double)area[i] * fabs((double)v);
+            continue;
+        }
+        float y = 0.0f;
+        for (uint32_t s = 0; s < S; ++s)
+            y += yp[s * plane + e];
+        float rho = refl[(size_t)i * 3 + r % 3];
+        if (kind == kResid) {
+            double res = (double)E[e] + (double)rho * (double)y - (double)B[e];
+            float rf = (float)res;
+            rv[e] = rf;
+            xhi[e] = bf16_rne(rf);
+            acc += (double)area[i] * fabs(res);
+        } else {
+            float d = rv[e] + rho * y;
+            if (kind == kInner) {
+                xhi[e] = bf16_rne(d);
+            } else {
+                float b = B[e] + d;
+                B[e] = b;
+                uint16_t h = bf16_rne(b);
+                xhi[e] = h;
+                xlo[e] = bf16_rne(b - bf16_val(h));
+            }
+        }
+    }
+    if (kind == kInit || kind == kResid)
+        block_sum_store(acc, parts);
+}
+
+__global__ void k_gather_out(uint32_t R, uint32_t npad, uint32_t n, const float *__restrict__ B,
+                             const float *__restrict__ rv, float *__restrict__ out)
+{
+    size_t total = (size_t)R * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        size_t r = e / n, i = e - r * n;
+        size_t o = r * npad + i;
+        out[e] = (float)((double)B[o] + (double)rv[o]);
+    }
+}
+
+// Columns in from host order (R x n) into the padded device layout.
+__global__ void k_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *__restrict__ src,
+                              float *__restrict__ dst, uint16_t *__restrict__ xhi,
+                              uint16_t *__restrict__ xlo)
+{
+    size_t total = (size_t)R * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        size_t r = e / n, i = e - r * n;
+        size_t o = r * npad + i;
+        float v = src[e];
+        if (dst)
+            dst[o] = v;
+        if (xhi) {
+            uint16_t h = bf16_rne(v);
+            xhi[o] = h;
+            if (xlo)
+                xlo[o] = bf16_rne(v - bf16_val(h));
+        }
+    }
+}
+
+// Sum of the S partial products into host column order (R x n).
+__global__ void k_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
+                               const float *__restrict__ yp, float *__restrict__ out)
+{
+    size_t total = (size_t)R * n;
+    size_t plane = (size_t)rp * npad;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        size_t r = e / n, i = e - r * n;
+        float y = 0.0f;
+        for (uint32_t s = 0; s < S; ++s)
+            y += yp[s * plane + r * npad + i];
+        out[e] = y;
+    }
+}
+
+// --------------------------------------------------------------- launchers
+
+void launch_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, uint32_t *nshoot,
+                          hipStream_t st)
+{
+    k_iota_shooters<<<(k + 255) / 256, 256, 0, st>>>(row0, k, shooters, nshoot);
+}
+
+void launch_dense_store(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch, uint32_t npad,
+                        uint16_t *fhi, uint16_t *flo, hipStream_t st)
+{
+    uint32_t bx = (npad / 8 + 255) / 256;
+    if (bx > 64)
+        bx = 64;
+    k_dense_store<<<dim3(bx, k), 256, 0, st>>>(ff, k, row0, npatch, npad, fhi, flo);
+}
+
+void launch_dense_store_t(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
+                          uint32_t npad, const float *area, uint16_t *fhi, uint16_t *flo,
+                          hipStream_t st)
+{
+    size_t total = (size_t)npatch * ((k + 7) / 8);
+    size_t b = (total + 255) / 256;
+    k_dense_store_t<<<(unsigned)(b > 4096 ? 4096 : b), 256, 0, st>>>(ff, k, row0, npatch, npad,
+                                                                    area, fhi, flo);
+}
+
+void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint32_t npatch,
+                           uint32_t npad, uint16_t *fhi, uint16_t *flo, hipStream_t st)
+{
+    k_dense_from_f32<<<2048, 256, 0, st>>>(src, rows, row0, npatch, npad, fhi, flo);
+}
+
+void launch_dense_to_f32(const uint16_t *fhi, const uint16_t *flo, uint32_t rows, uint32_t row0,
+                         uint32_t npatch, uint32_t npad, float *dst, hipStream_t st)
+{
+    k_dense_to_f32<<<2048, 256, 0, st>>>(fhi, flo, rows, row0, npatch, npad, dst);
+}
+
+uint32_t gemm_tiles(uint32_t R)
+{
+    uint32_t t = (R + 15) / 16;
+    if (t <= 1)
+        return 1;
+    if (t <= 2)
+        return 2;
+    if (t <= 4)
+        return 4;
+    if (t <= 8)
+        return 8;
+    return 12;
+}
+
+template <int RT>
+static void gemm_rt(bool split, const uint16_t *fhi, const uint16_t *flo, const uint16_t *xhi,
+                    const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp, hipStream_t st)
+{
+    dim3 grid(npad / 128, S);
+    uint32_t kspan = npad / S;
+    if (split)
+        k_gemm<RT, true><<<grid, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, yp);
+    else
+        k_gemm<RT, false><<<grid, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, yp);
+}
+
+void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
+                 const uint16_t *xhi, const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp,
+                 hipStream_t st)
+{
+    switch (rt) {
+    case 1: gemm_rt<1>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
+    case 2: gemm_rt<2>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
+    case 4: gemm_rt<4>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
+    case 8: gemm_rt<8>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
+    default: gemm_rt<12>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
+    }
+}
+
+void launch_gather_update(int kind, const float *yp, uint32_t S, uint32_t rp, uint32_t R,
+                          uint32_t npad, uint32_t n, const float *E, const float *refl,
+                          const float *area, float *rv, float *B, uint16_t *xhi, uint16_t *xlo,
+                          double *parts, uint32_t nblocks, hipStream_t st)
+{
+    k_gather_update<<<nblocks, 256, 0, st>>>(kind, yp, S, rp, R, npad, n, E, refl, area, rv, B,
+                                             xhi, xlo, parts);
+}
+
+void launch_gather_out(uint32_t R, uint32_t npad, uint32_t n, const float *B, const float *rv,
+                       float *out, hipStream_t st)
+{
+    k_gather_out<<<1024, 256, 0, st>>>(R, npad, n, B, rv, out);
+}
+
+void launch_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *src, float *dst,
+                        uint16_t *xhi, uint16_t *xlo, hipStream_t st)
+{
+    k_pad_columns<<<1024, 256, 0, st>>>(R, npad, n, src, dst, xhi, xlo);
+}
+
+void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
+                         const float *yp, float *out, hipStream_t st)
+{
+    k_sum_partials<<<1024, 256, 0, st>>>(R, npad, n, S, rp, yp, out);
+}
+
+} // namespace rad

@@ -45,6 +45,7 @@ enum {
     kCntUnknown = 4,  // item-buffer ids beyond npatch
     kCntSpill = 5,    // records in the spill region
     kCntClip = 6,     // triangle-face jobs queued for clipping
+    kCntSticky = 7,   // some earlier pass overflowed (cleared by the host only)
     kCntN = 8
 };
 
@@ -84,7 +85,7 @@ void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
                     const uint32_t *nshoot, const float *frames, const float *unshot,
                     float *snap, float *mvp, hipStream_t st);
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
-                  uint32_t npatch, uint32_t *counters, hipStream_t st);
+                  uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st);
 // Binned render of the nshoot (<= K) hemicubes in three launches:
 // setup (records + bin counts), bin (scan + fill), tile (LDS depth test; keys
 // != null writes the item buffer, ff != null reduces into F rows and hits).
@@ -110,5 +111,32 @@ void launch_split_keys(const uint64_t *keys, uint32_t *ids, float *depth, size_t
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);
+
+// Launchers implemented in rad_gather.hip.
+void launch_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, uint32_t *nshoot,
+                          hipStream_t st);
+void launch_dense_store(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch, uint32_t npad,
+                        uint16_t *fhi, uint16_t *flo, hipStream_t st);
+void launch_dense_store_t(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
+                          uint32_t npad, const float *area, uint16_t *fhi, uint16_t *flo,
+                          hipStream_t st);
+void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint32_t npatch,
+                           uint32_t npad, uint16_t *fhi, uint16_t *flo, hipStream_t st);
+void launch_dense_to_f32(const uint16_t *fhi, const uint16_t *flo, uint32_t rows, uint32_t row0,
+                         uint32_t npatch, uint32_t npad, float *dst, hipStream_t st);
+uint32_t gemm_tiles(uint32_t R);
+void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
+                 const uint16_t *xhi, const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp,
+                 hipStream_t st);
+void launch_gather_update(int kind, const float *yp, uint32_t S, uint32_t rp, uint32_t R,
+                          uint32_t npad, uint32_t n, const float *E, const float *refl,
+                          const float *area, float *rv, float *B, uint16_t *xhi, uint16_t *xlo,
+                          double *parts, uint32_t nblocks, hipStream_t st);
+void launch_gather_out(uint32_t R, uint32_t npad, uint32_t n, const float *B, const float *rv,
+                       float *out, hipStream_t st);
+void launch_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *src, float *dst,
+                        uint16_t *xhi, uint16_t *xlo, hipStream_t st);
+void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
+                         const float *yp, float *out, hipStream_t st);
 
 } // namespace rad

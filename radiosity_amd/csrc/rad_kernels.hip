@@ -219,13 +219,22 @@ void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
 // --------------------------------------------------------------------- clear
 // Per pass: F rows (and hit counts) to zero, bin counts and pass counters
 // reset.  A kernel, not hipMemsetAsync, so a pass can be captured in a graph.
+// The previous pass's overflow state is folded into the sticky counter,
+// which only the host clears, so a multi-pass call reports any lost pass.
 __global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *__restrict__ ff,
-                        uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters)
+                        uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters,
+                        uint32_t ent_cap)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
-    if (i < kCntN)
-        counters[i] = 0;
+    if (i == 0) {
+        bool bad = counters[kCntOverflow] != 0 || counters[kCntEntries] > ent_cap;
+        for (int k = 0; k < kCntN; ++k)
+            if (k != kCntSticky)
+                counters[k] = 0;
+        if (bad)
+            counters[kCntSticky] = 1;
+    }
     for (size_t k = i; k < nbins; k += stride)
         bin_count[k] = 0u;
     for (size_t k = i; k < nff; k += stride) {
@@ -236,7 +245,7 @@ __global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *_
 }
 
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
-                  uint32_t npatch, uint32_t *counters, hipStream_t st)
+                  uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st)
 {
     size_t nbins = (size_t)K * ntiles, nff = (size_t)K * npatch;
     size_t n = nbins > nff ? nbins : nff;
@@ -245,7 +254,7 @@ void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, u
         blocks = 8192;
     if (blocks == 0)
         blocks = 1;
-    k_clear<<<(unsigned)blocks, 256, 0, st>>>(bin_count, nbins, ff, hits, nff, counters);
+    k_clear<<<(unsigned)blocks, 256, 0, st>>>(bin_count, nbins, ff, hits, nff, counters, ent_cap);
 }
 
 // ---------------------------------------------------------------- rasterizer

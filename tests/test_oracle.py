@@ -462,3 +462,41 @@ def test_residual():
     e = np.array([[2, 2, 2], [1, 1, 1]], np.float32)
     a = np.array([1.0, 2.0], np.float32)
     assert O.residual(u, e, a) == pytest.approx(3.0 / (6.0 + 6.0))
+
+
+# ------------------------------------------------------------------ gather
+
+def _random_system(seed, n, R):
+    rng = np.random.default_rng(seed)
+    M = rng.random((n, n)).astype(np.float32)
+    np.fill_diagonal(M, 0.0)
+    M /= (M.sum(1, keepdims=True) * 1.02).astype(np.float32)
+    refl = rng.uniform(0.1, 0.8, (n, 3)).astype(np.float32)
+    area = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    E = rng.random((R, n)) * (rng.random((R, n)) < 0.2)
+    return M, refl, area, E
+
+
+def test_gather_jacobi_equals_direct_solve(orc):
+    """The FP64 Jacobi fixed point equals LAPACK's solve of
+    (I - diag(rho_c) M) B = E per column (channel c = r % 3)."""
+    M, refl, area, E = _random_system(5, 60, 5)
+    B, it = orc.gather_jacobi(M, refl, area, E, tol=1e-14)
+    assert 5 < it < 2000
+    for r in range(E.shape[0]):
+        A = np.eye(60) - refl[:, r % 3, None].astype(np.float64) * M.astype(np.float64)
+        assert np.allclose(B[r], np.linalg.solve(A, E[r]), rtol=1e-11, atol=1e-13)
+
+
+def test_gather_jacobi_residual_measure(orc):
+    """tol is the area-weighted residual sum_A|B_k+1 - B_k| / sum_A|E|: a
+    loose tolerance stops early, and the stopped iterate's true residual is
+    within the bound the contraction gives."""
+    M, refl, area, E = _random_system(6, 40, 3)
+    B1, it1 = orc.gather_jacobi(M, refl, area, E, tol=1e-3)
+    B2, it2 = orc.gather_jacobi(M, refl, area, E, tol=1e-14)
+    assert it1 < it2
+    den = float((area[None, :] * np.abs(E)).sum())
+    err = float((area[None, :] * np.abs(B1 - B2)).sum()) / den
+    # ||B* - B_k|| <= q/(1-q) ||B_k - B_k-1||, q = max rho * max row sum < 0.8
+    assert err <= 1e-3 * 0.8 / 0.2 * 1.5
