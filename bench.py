@@ -44,6 +44,39 @@ def measured_traffic(W, K):
     return {}
 
 
+def gather_leg(R, s, sc, args):
+    """Dense matrix from the K=64 hemicube passes (shooting layout, bf16 hi +
+    lo) and the BF16-MFMA Jacobi gather with FP64-residual refinement to the
+    same residual target.  The context `s` has just converged by batched
+    shooting; its radiosity (shot + unshot) is compared with the gather's."""
+    import numpy as np
+    u, shot = s.state()
+    shoot_B = (u + shot).T.astype(np.float64)
+    t0 = time.perf_counter()
+    s.dense_build(R.RAD_DENSE_SHOOT)
+    t1 = time.perf_counter()
+    B, st = s.gather(target=args.target, inner=args.gather_inner)
+    t2 = time.perf_counter()
+    s.dense_release()
+    a = sc.area.astype(np.float64)[None, :]
+    diff = float((a * np.abs(B - shoot_B)).sum() / (a * np.abs(shoot_B)).sum())
+    npad = (sc.npatch + 511) // 512 * 512
+    sweep_gbs = 2.0 * npad * npad / (st.sweep_ms * 1e-3) / 1e9 if st.sweep_ms else None
+    resid_gbs = 4.0 * npad * npad / (st.resid_ms * 1e-3) / 1e9 if st.resid_ms else None
+    out = {"ms": (t2 - t0) * 1e3, "build_ms": (t1 - t0) * 1e3, "solve_ms": (t2 - t1) * 1e3,
+           "outer": st.outer, "sweeps": st.sweeps, "residual": st.residual,
+           "history": [st.history[k] for k in range(min(st.outer, 16))],
+           "sweep_kernel_ms": st.sweep_ms, "sweep_hbm_gbs": sweep_gbs,
+           "resid_kernel_ms": st.resid_ms, "resid_hbm_gbs": resid_gbs,
+           "matrix_bytes": 4 * npad * npad,
+           "rel_l1_vs_batched_shooting": diff}
+    log(f"gather bf16 to {args.target}: {out['ms'] / 1e3:.2f} s (build {out['build_ms'] / 1e3:.2f}"
+        f" s, solve {out['solve_ms']:.0f} ms, {st.outer} steps, {st.sweeps} sweeps), residual "
+        f"{st.residual:.2e}; sweep {st.sweep_ms:.2f} ms ({sweep_gbs:.0f} GB/s), residual product "
+        f"{st.resid_ms:.2f} ms ({resid_gbs:.0f} GB/s); vs shooting {diff:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +91,8 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=10.0)
     ap.add_argument("--small-tri-px", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the K=1 leg (profiling)")
+    ap.add_argument("--no-gather", action="store_true", help="skip the dense BF16 gather leg")
+    ap.add_argument("--gather-inner", type=int, default=4)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, so that a "
                          "counter run can scale FETCH_SIZE / WRITE_SIZE to bytes")
@@ -134,6 +169,8 @@ def main():
                             "residual": st.residual}}
         log(f"batched to {args.target}: {st.seconds:.2f} s, {st.shooters} shooters, "
             f"residual {st.residual:.3e}")
+        if not args.no_gather:
+            conv["gather_bf16"] = gather_leg(R, s, sc, args)
         if p is not None:
             p.upload(sc)
             cap = int(max(1000, plain_sps * args.plain_cap_s))
@@ -146,6 +183,9 @@ def main():
                 f"{st.residual:.3e} ({'reached' if reached else 'capped'})")
             if reached:
                 conv["speedup_batched_vs_plain"] = conv["plain"]["ms"] / conv["batched"]["ms"]
+                if "gather_bf16" in conv:
+                    conv["speedup_gather_vs_plain"] = (conv["plain"]["ms"]
+                                                       / conv["gather_bf16"]["ms"])
 
     # ------------------------------------------------------------ roofline
     # Dominant kernel of the batched pass and its algorithmic bytes per launch

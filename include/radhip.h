@@ -142,7 +142,8 @@ int rad_run(rad_ctx *ctx, uint64_t max_passes, double residual_target,
             uint32_t check_every, rad_stats *out);
 
 /* glFinish/clFinish (Main.cpp:1202, 1228): waits for all work queued on the
- * context's stream. */
+ * context's stream; RAD_E_LIMIT if a pass since the last check lost
+ * triangles to a full rasterizer bin (capacities are set at upload). */
 int rad_finish(rad_ctx *ctx);
 
 /* sum_i A_i (r+g+b)(unshot_i) / sum_i A_i (r+g+b)(E_i), accumulated in FP64 */
@@ -217,6 +218,8 @@ typedef struct rad_gather_stats {
     double residual;      /* final residual (FP64 sums) */
     double seconds;       /* wall time of the solve */
     double history[16];   /* residual after each of the first 16 steps */
+    double sweep_ms;      /* device time of one BF16 product (HIP events, mean) */
+    double resid_ms;      /* device time of one two-half residual product (mean) */
 } rad_gather_stats;
 
 /* Solves for R <= 192 right-hand sides.  rhs: R x npatch (column r

@@ -52,7 +52,8 @@ class rad_gather_opts(C.Structure):
 class rad_gather_stats(C.Structure):
     _fields_ = [("outer", C.c_uint32), ("sweeps", C.c_uint32), ("reserved", C.c_uint32 * 2),
                 ("residual", C.c_double), ("seconds", C.c_double),
-                ("history", C.c_double * 16)]
+                ("history", C.c_double * 16), ("sweep_ms", C.c_double),
+                ("resid_ms", C.c_double)]
 
 
 _lib = None

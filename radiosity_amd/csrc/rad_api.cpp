@@ -1025,26 +1025,55 @@ int rad_gather_solve(rad_ctx *c, const float *rhs, uint32_t R, const rad_gather_
     double res = den > 0 ? 1.0 : 0.0;
     uint32_t outer = 0, sweeps = 0;
     rad_gather_stats st{};
+    // events around the first sweep product and the residual product of
+    // each step (device time of the two product kernels)
+    hipEvent_t ev[4] = {};
+    bool have_ev = true;
+    for (hipEvent_t &e : ev)
+        have_ev = have_ev && hipEventCreate(&e) == hipSuccess;
+    double t_sweep = 0.0, t_res = 0.0;
     while (res > target && outer < max_outer) {
         // correction d = r + diag(rho) M_bf16 d, `inner` sweeps from d = r,
         // the last one adds d to B and splits B into its two bf16 halves
         for (uint32_t s = 0; s < inner; ++s) {
+            if (s == 0 && have_ev)
+                (void)hipEventRecord(ev[0], c->st);
             launch_gemm(g.rt, false, c->fhi, nullptr, g.xhi, nullptr, P, g.S, g.yp, c->st);
+            if (s == 0 && have_ev)
+                (void)hipEventRecord(ev[1], c->st);
             launch_gather_update(s + 1 == inner ? 2 : 1, g.yp, g.S, g.rp, R, P, N, g.E, c->refl,
                                  c->area, g.rv, g.B, g.xhi, g.xlo, g.parts, g.nblk, c->st);
         }
         sweeps += inner;
         // residual r = E + diag(rho) M B - B from both halves, FP64 sums
+        if (have_ev)
+            (void)hipEventRecord(ev[2], c->st);
         launch_gemm(g.rt, true, c->fhi, c->flo, g.xhi, g.xlo, P, g.S, g.yp, c->st);
+        if (have_ev)
+            (void)hipEventRecord(ev[3], c->st);
         launch_gather_update(3, g.yp, g.S, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
                              g.xlo, g.parts, g.nblk, c->st);
         if ((rc = check_launch(c, "gather step")) || (rc = sum_parts(c, g, &num)))
-            return rc;
+            break;
+        if (have_ev) {
+            float a = 0.f, b = 0.f;
+            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&b, ev[2], ev[3]);
+            t_sweep += a;
+            t_res += b;
+        }
         res = num / den;
         if (outer < 16)
             st.history[outer] = res;
         ++outer;
     }
+    for (hipEvent_t e : ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (rc)
+        return rc;
+    st.sweep_ms = outer ? t_sweep / outer : 0.0;
+    st.resid_ms = outer ? t_res / outer : 0.0;
     launch_gather_out(R, P, N, g.B, g.rv, g.out, c->st);
     if ((rc = check_launch(c, "gather out")))
         return rc;
