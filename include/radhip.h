@@ -58,6 +58,8 @@ typedef struct rad_config {
     uint32_t select;          /* RAD_SELECT_* */
     uint32_t small_tri_px;    /* raster tuning: triangle pixels in a tile drawn by one lane (0: 64) */
     uint32_t tile_px;         /* raster tuning: screen tile side 8, 16 or 32 (0: auto) */
+    uint32_t graph_passes;    /* rad_step replays this many passes as one captured
+                                 HIP graph (0: plain launches) */
 } rad_config;
 
 typedef struct rad_ctx rad_ctx;
@@ -165,6 +167,21 @@ int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
  * counts of the last pass (hemicubes * tiles entries, row-major tiles of
  * tile_px) into out and returns the tile count per hemicube in *ntiles. */
 int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32_t *tile_px);
+
+/* Autotuner of the rasterizer: times `passes` render passes (select once,
+ * then render + reduce, no shooting, state unchanged) for each screen tile
+ * side {16, 32} (and 8 for K <= 2) and in-lane triangle limit
+ * {16, 32, 64, 128, 256}, and keeps the fastest pair.  No counterpart in the
+ * reference (its GL driver rasterizes). */
+#define RAD_TUNE_MAX 16
+typedef struct rad_tune_result {
+    uint32_t tile_px, small_tri_px; /* chosen */
+    uint32_t ncand, reserved;
+    double best_ms;                 /* device ms per render pass, chosen pair */
+    uint32_t cand_tile[RAD_TUNE_MAX], cand_small[RAD_TUNE_MAX];
+    double cand_ms[RAD_TUNE_MAX];   /* < 0: the candidate overflowed a bin */
+} rad_tune_result;
+int rad_autotune(rad_ctx *ctx, uint32_t passes, rad_tune_result *out_or_null);
 
 /* ---------------------------------------------------------------------------
  * Dense form factors and the Jacobi gather (the precision path).

@@ -27,7 +27,8 @@ class RadError(RuntimeError):
 class rad_config(C.Structure):
     _fields_ = [("hemicube", C.c_uint32), ("hemicubes", C.c_uint32), ("shoots", C.c_uint32),
                 ("ocl_workitems_x", C.c_uint32), ("mode", C.c_uint32), ("select", C.c_uint32),
-                ("small_tri_px", C.c_uint32), ("tile_px", C.c_uint32)]
+                ("small_tri_px", C.c_uint32), ("tile_px", C.c_uint32),
+                ("graph_passes", C.c_uint32)]
 
 
 class rad_stats(C.Structure):
@@ -54,6 +55,13 @@ class rad_gather_stats(C.Structure):
                 ("residual", C.c_double), ("seconds", C.c_double),
                 ("history", C.c_double * 16), ("sweep_ms", C.c_double),
                 ("resid_ms", C.c_double)]
+
+
+class rad_tune_result(C.Structure):
+    _fields_ = [("tile_px", C.c_uint32), ("small_tri_px", C.c_uint32), ("ncand", C.c_uint32),
+                ("reserved", C.c_uint32), ("best_ms", C.c_double),
+                ("cand_tile", C.c_uint32 * 16), ("cand_small", C.c_uint32 * 16),
+                ("cand_ms", C.c_double * 16)]
 
 
 _lib = None
@@ -90,6 +98,7 @@ SIGNATURES = {
                                   C.c_uint64, C.POINTER(C.POINTER(rad_scene))]),
     "rad_scene_free": (None, [C.POINTER(rad_scene)]),
     "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
+    "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
     "rad_dense_build": (C.c_int, [_P, _u32]),
     "rad_dense_upload": (C.c_int, [_P, _PF]),
     "rad_dense_download": (C.c_int, [_P, _u32, _u32, _PF]),
@@ -187,9 +196,10 @@ class Solver:
     """One rad_ctx: scene, state and the shooting passes on the current GPU."""
 
     def __init__(self, hemicube=16, hemicubes=1, mode=RAD_MODE_PHYSICAL, select=RAD_SELECT_POWER,
-                 shoots=500, small_tri_px=0, tile_px=0):
+                 shoots=500, small_tri_px=0, tile_px=0, graph_passes=0):
         L = lib()
-        cfg = rad_config(hemicube, hemicubes, shoots, 4, mode, select, small_tri_px, tile_px)
+        cfg = rad_config(hemicube, hemicubes, shoots, 4, mode, select, small_tri_px, tile_px,
+                         graph_passes)
         self._ctx = C.c_void_p()
         rc = L.rad_create(C.byref(cfg), C.byref(self._ctx))
         if rc != RAD_OK:
@@ -309,6 +319,14 @@ class Solver:
         n = C.c_uint64()
         self._chk(lib().rad_kernel_times(self._ctx, enable, out, C.byref(n)), "rad_kernel_times")
         return list(out), n.value
+
+    def autotune(self, passes=3):
+        """Picks the fastest (tile_px, small_tri_px); returns (tile, small,
+        ms per render pass, {(tile, small): ms})."""
+        r = rad_tune_result()
+        self._chk(lib().rad_autotune(self._ctx, passes, C.byref(r)), "rad_autotune")
+        cands = {(r.cand_tile[k], r.cand_small[k]): r.cand_ms[k] for k in range(r.ncand)}
+        return r.tile_px, r.small_tri_px, r.best_ms, cands
 
     # ------------------------------------------------------ dense / gather
     def dense_build(self, layout=RAD_DENSE_SHOOT):

@@ -57,6 +57,10 @@ struct rad_ctx {
     bool rendered = false;        // keys hold the item buffer of the last render
     uint32_t rendered_k = 0;
 
+    // captured passes (rad_config.graph_passes of them) replayed by rad_step
+    hipGraphExec_t gexec = nullptr;
+    uint32_t gpasses = 0;
+
     // dense matrix of the gather: npad x npad bf16 halves (hi, lo)
     uint16_t *fhi = nullptr, *flo = nullptr;
     uint32_t npad = 0;
@@ -77,6 +81,19 @@ int fail(rad_ctx *c, int code, const std::string &msg)
     if (c)
         c->err = msg;
     return code;
+}
+
+// Buffers a captured pass refers to change (scene upload, retiling): the
+// next rad_step captures again.
+void drop_graph(rad_ctx *c)
+{
+    if (c->gexec) {
+        if (c->st)
+            (void)hipStreamSynchronize(c->st);
+        (void)hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    c->gpasses = 0;
 }
 
 #define HIPCHK(c, call)                                                                   \
@@ -197,6 +214,35 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
     return check_launch(c, "shoot");
 }
 
+// Screen tile side T (8, 16, 32) and the in-lane triangle limit; (re)sizes
+// the per-(shooter, tile) bin arrays.
+int set_tiles(rad_ctx *c, uint32_t T, uint32_t small_px)
+{
+    if (T != 8 && T != 16 && T != 32)
+        return fail(c, RAD_E_ARG, "tile_px must be 8, 16 or 32");
+    if (small_px == 0 || small_px > 4096)
+        return fail(c, RAD_E_ARG, "small_tri_px must be 1..4096");
+    uint32_t ntx = (c->g.TW + T - 1) / T, nty = (c->g.TH + T - 1) / T;
+    if (ntx * nty > kMaxTilesPerSlab)
+        return fail(c, RAD_E_ARG, "too many screen tiles per hemicube (use a larger tile_px)");
+    if (c->st)
+        (void)hipStreamSynchronize(c->st);
+    c->bb.tshift = T == 8 ? 3 : (T == 16 ? 4 : 5);
+    c->bb.ntx = ntx;
+    c->bb.nty = nty;
+    c->bb.ntiles = ntx * nty;
+    c->small_px = small_px;
+    c->cfg.tile_px = T;
+    c->cfg.small_tri_px = small_px;
+    drop_graph(c);
+    size_t nbins = (size_t)c->K * c->bb.ntiles;
+    int rc;
+    if ((rc = dalloc(c, &c->bb.count, nbins)) || (rc = dalloc(c, &c->bb.start, nbins)) ||
+        (rc = dalloc(c, &c->bb.fill, nbins)))
+        return rc;
+    return RAD_OK;
+}
+
 int need_scene(rad_ctx *c)
 {
     if (c->npatch == 0)
@@ -265,22 +311,7 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
     c->sel_blocks = ncu * 2;
     // screen tiles: 32x32 when enough hemicubes fill the GPU, else 16x16
     uint32_t T = cfg->tile_px ? cfg->tile_px : (c->K >= 4 ? 32u : 16u);
-    if (T != 8 && T != 16 && T != 32) {
-        rad_destroy(c);
-        return RAD_E_ARG;
-    }
-    c->bb.tshift = T == 8 ? 3 : (T == 16 ? 4 : 5);
-    c->bb.ntx = (c->g.TW + T - 1) / T;
-    c->bb.nty = (c->g.TH + T - 1) / T;
-    c->bb.ntiles = c->bb.ntx * c->bb.nty;
-    if (c->bb.ntiles > kMaxTilesPerSlab) {
-        rad_destroy(c);
-        return RAD_E_ARG;
-    }
-    c->small_px = cfg->small_tri_px ? cfg->small_tri_px : 64u;
-    size_t nbins = (size_t)c->K * c->bb.ntiles;
-    if ((rc = dalloc(c, &c->bb.count, nbins)) || (rc = dalloc(c, &c->bb.start, nbins)) ||
-        (rc = dalloc(c, &c->bb.fill, nbins))) {
+    if ((rc = set_tiles(c, T, cfg->small_tri_px ? cfg->small_tri_px : 64u))) {
         rad_destroy(c);
         return rc;
     }
@@ -298,6 +329,7 @@ void rad_destroy(rad_ctx *c)
         return;
     if (c->st)
         (void)hipStreamSynchronize(c->st);
+    drop_graph(c);
     void *ptrs[] = {c->tri,  c->tri_patch, c->frames, c->refl,     c->area,   c->emis,
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
@@ -359,6 +391,7 @@ int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch
         if (!(area[i] > 0.0f))
             return fail(c, RAD_E_ARG, "scene: patch area must be > 0");
     (void)hipStreamSynchronize(c->st);
+    drop_graph(c);
     int rc;
     size_t K = c->K;
     if ((rc = dalloc(c, &c->tri, (size_t)ntri * 9)) || (rc = dalloc(c, &c->tri_patch, ntri)) ||
@@ -605,11 +638,45 @@ static int ensure_events(rad_ctx *c, size_t n)
     return RAD_OK;
 }
 
+// Captures graph_passes passes (kernels only: every clear in a pass is a
+// kernel, no memset or copy node) into one executable graph.
+static int capture_graph(rad_ctx *c, uint32_t n)
+{
+    drop_graph(c);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int rc = RAD_OK;
+    for (uint32_t p = 0; p < n && rc == RAD_OK; ++p)
+        rc = one_pass(c, true, 0);
+    hipError_t e = hipStreamEndCapture(c->st, &graph);
+    if (rc)
+        return rc;
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, RAD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c->gpasses = n;
+    return RAD_OK;
+}
+
 static int step_impl(rad_ctx *c, uint32_t passes)
 {
     int rc;
     if (c->timing && (rc = ensure_events(c, (size_t)passes * 7)))
         return rc;
+    uint32_t G = c->cfg.graph_passes;
+    if (!c->timing && G && passes >= G) {
+        if (!c->gexec || c->gpasses != G)
+            if ((rc = capture_graph(c, G)))
+                return rc;
+        for (; passes >= G; passes -= G)
+            HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+        c->rendered = false;
+    }
     for (uint32_t p = 0; p < passes; ++p)
         if ((rc = one_pass(c, true, (size_t)p * 7)))
             return rc;
@@ -748,6 +815,76 @@ int rad_kernel_times(rad_ctx *c, int enable, double *out6, uint64_t *npasses)
             c->t_acc[k] = 0;
         c->t_passes = 0;
     }
+    return RAD_OK;
+}
+
+int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (passes == 0)
+        passes = 3;
+    rad_tune_result r{};
+    const uint32_t tiles[3] = {8, 16, 32}, smalls[5] = {16, 32, 64, 128, 256};
+    const uint32_t T0 = c->cfg.tile_px, S0 = c->small_px;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(c, RAD_E_HIP, "hipEventCreate");
+    }
+    bool timing = c->timing;
+    c->timing = false;
+    if ((rc = select_dev(c, c->K)) == RAD_OK) {
+        for (uint32_t ti = 0; ti < 3 && rc == RAD_OK; ++ti) {
+            if (tiles[ti] == 8 && c->K > 2)
+                continue;
+            for (uint32_t si = 0; si < 5 && rc == RAD_OK && r.ncand < RAD_TUNE_MAX; ++si) {
+                if ((rc = set_tiles(c, tiles[ti], smalls[si])))
+                    break;
+                rc = render_pass(c, c->K, 0, nullptr, nullptr); // warm
+                (void)hipEventRecord(e0, c->st);
+                for (uint32_t p = 0; p < passes && rc == RAD_OK; ++p)
+                    rc = render_pass(c, c->K, 0, nullptr, nullptr);
+                (void)hipEventRecord(e1, c->st);
+                if (rc)
+                    break;
+                float ms = 0.f;
+                if (hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = fail(c, RAD_E_HIP, "autotune timing");
+                    break;
+                }
+                uint32_t k = r.ncand++;
+                r.cand_tile[k] = tiles[ti];
+                r.cand_small[k] = smalls[si];
+                r.cand_ms[k] = check_overflow(c) == RAD_OK ? ms / passes : -1.0;
+                if (r.cand_ms[k] >= 0 && (r.best_ms == 0.0 || r.cand_ms[k] < r.best_ms)) {
+                    r.best_ms = r.cand_ms[k];
+                    r.tile_px = tiles[ti];
+                    r.small_tri_px = smalls[si];
+                }
+            }
+        }
+    }
+    c->timing = timing;
+    c->rendered = false;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == RAD_OK)
+        c->err.clear(); // overflowing candidates are reported in cand_ms only
+    if (rc == RAD_OK && r.tile_px == 0)
+        rc = fail(c, RAD_E_LIMIT, "autotune: every candidate overflowed the bins");
+    int rc2 = set_tiles(c, rc == RAD_OK ? r.tile_px : T0, rc == RAD_OK ? r.small_tri_px : S0);
+    if (rc)
+        return rc;
+    if (rc2)
+        return rc2;
+    if (out)
+        *out = r;
     return RAD_OK;
 }
 

@@ -284,3 +284,43 @@ def test_itembuf_dense_bins_bit_exact(rad):
     d[sh] -= snap
     want = u0.astype(np.float64) + d
     assert _close(u1, want, float(np.abs(want).max()))
+
+
+def test_graph_replay_matches_eager(rad, cornell_t):
+    """rad_step replaying captured passes (graph_passes=4) tracks plain
+    launches: every replay is checked after host work and a synchronisation
+    (float atomics make the F rows order-dependent in the last bits, so the
+    comparison is 1e-4 relative on the area-weighted state)."""
+    e = rad.Solver(hemicube=32, hemicubes=4)
+    g = rad.Solver(hemicube=32, hemicubes=4, graph_passes=4)
+    e.upload(cornell_t)
+    g.upload(cornell_t)
+    a = cornell_t.area[:, None].astype(np.float64)
+    for rnd, n in enumerate((8, 4, 5, 12)):
+        say(f"graph round {rnd} passes {n}")
+        e.step(n)
+        g.step(n)
+        e.finish()
+        g.finish()
+        ue, se = e.state()
+        ug, sg = g.state()
+        tot_e = (ue + se).astype(np.float64)
+        tot_g = (ug + sg).astype(np.float64)
+        rel = np.abs(a * (tot_e - tot_g)).sum() / np.abs(a * tot_e).sum()
+        assert rel < 1e-4, (rnd, rel)
+        assert abs(e.residual() - g.residual()) <= 1e-4 * e.residual()
+
+
+def test_autotune_picks_a_candidate_and_keeps_parity(rad, boxes_small):
+    s = rad.Solver(hemicube=64, hemicubes=4)
+    s.upload(boxes_small)
+    say("autotune")
+    t, sm, ms, cands = s.autotune(2)
+    say(f"  chose tile {t} small {sm} at {ms:.3f} ms; {len(cands)} candidates")
+    assert (t, sm) in cands and ms > 0 and cands[(t, sm)] == ms
+    assert all(v < 0 or v >= ms for v in cands.values())
+    shooters = [0, 17, boxes_small.npatch - 1]
+    ids = s.render(shooters).reshape(3, -1)
+    for k, p in enumerate(shooters):
+        want = O.render_hemicube(64, boxes_small.tri, boxes_small.tri_patch, boxes_small.frame[p])
+        assert np.array_equal(ids[k], want), p
