@@ -77,6 +77,83 @@ def gather_leg(R, s, sc, args):
     return out
 
 
+def cornell_leg(R, O, area, W, shots, seed):
+    """Plain single-shooter drop-in on the GPU against the oracle's CPU pass
+    (the reference's CPU path restated), same scene, same shots, same rule."""
+    import numpy as np
+    sc = R.Scene.cornell(area, triangle_patches=True, physical=True, seed=seed)
+    s = R.Solver(hemicube=W, hemicubes=1)
+    s.upload(sc)
+    s.step(3)
+    s.finish()
+    s.upload(sc)
+    t0 = time.perf_counter()
+    s.step(shots)
+    s.finish()
+    gpu = time.perf_counter() - t0
+    ug, sg = s.state()
+    dff = O.dff_cross(W)
+    u = sc.emission.copy()
+    sh = sc.shot0.copy()
+    item = np.zeros(3 * W * W, np.uint32)
+    ff = np.zeros(sc.npatch, np.float32)
+    t0 = time.perf_counter()
+    for _ in range(shots):
+        O.plain_pass(W, 1, 1, sc, dff, u, sh, item, ff)
+    cpu = time.perf_counter() - t0
+    a = sc.area.astype(np.float64)[:, None]
+    tg, tc = (ug + sg).astype(np.float64), (u + sh).astype(np.float64)
+    diff = float((a * np.abs(tg - tc)).sum() / (a * np.abs(tc)).sum())
+    out = {"patches": sc.npatch, "hemicube": W, "shots": shots,
+           "gpu_ms": gpu * 1e3, "cpu_ms": cpu * 1e3, "cpu_cores": 1,
+           "gpu_shooters_per_s": shots / gpu, "cpu_shooters_per_s": shots / cpu,
+           "speedup": cpu / gpu, "rel_l1_gpu_vs_cpu_state": diff,
+           "residual_gpu": s.residual(), "residual_cpu": O.residual(u.astype(np.float64),
+                                                                  sc.emission, sc.area)}
+    log(f"cornell {sc.npatch} W={W}: {shots} shots GPU {gpu * 1e3:.1f} ms, CPU "
+        f"{cpu * 1e3:.0f} ms ({cpu / gpu:.0f}x); state rel L1 {diff:.1e}")
+    return out
+
+
+def dense_leg(R, args):
+    """Dense 32k x 32k matrix (hemicube 128) and the gather of 64 lighting
+    conditions x RGB (192 columns) over 32 candidate lights, BF16 MFMA with
+    FP64-residual refinement."""
+    import numpy as np
+    sc = R.Scene.boxes(nboxes=60, room=(12.0, 4.0, 12.0), target_tris=32768, nlights=16, seed=4)
+    n = sc.npatch
+    s = R.Solver(hemicube=128, hemicubes=64)
+    s.upload(sc)
+    t0 = time.perf_counter()
+    s.dense_build(R.RAD_DENSE_SHOOT)
+    t1 = time.perf_counter()
+    rng = np.random.default_rng(4)
+    cand = rng.choice(n, 32, replace=False)
+    E = np.zeros((192, n), np.float32)
+    for c in range(64):
+        E[3 * c:3 * c + 3, cand] = rng.uniform(0.0, 20.0, (3, 32)) * (rng.random(32) < 0.5)
+    s.gather(E, target=1e-2, max_outer=1, inner=1)            # warm the kernels
+    t2 = time.perf_counter()
+    B, st = s.gather(E, target=1e-4, inner=args.gather_inner)
+    t3 = time.perf_counter()
+    npad = (n + 511) // 512 * 512
+    flop = 2.0 * npad * npad * 192
+    out = {"patches": n, "hemicube": 128, "columns": 192, "build_ms": (t1 - t0) * 1e3,
+           "solve_ms": (t3 - t2) * 1e3, "outer": st.outer, "sweeps": st.sweeps,
+           "residual": st.residual, "sweep_kernel_ms": st.sweep_ms,
+           "sweep_hbm_gbs": 2.0 * npad * npad / (st.sweep_ms * 1e-3) / 1e9,
+           "sweep_mfma_tflops": flop / (st.sweep_ms * 1e-3) / 1e12,
+           "sweep_mfma_frac_of_2500": flop / (st.sweep_ms * 1e-3) / 2.5e15,
+           "resid_kernel_ms": st.resid_ms,
+           "resid_mfma_tflops": 3 * flop / (st.resid_ms * 1e-3) / 1e12}
+    s.dense_release()
+    log(f"dense 32k x 192: build {out['build_ms']:.0f} ms, solve {out['solve_ms']:.1f} ms "
+        f"({st.outer} steps, {st.sweeps} sweeps, residual {st.residual:.1e}); sweep "
+        f"{st.sweep_ms:.3f} ms = {out['sweep_hbm_gbs']:.0f} GB/s, "
+        f"{out['sweep_mfma_tflops']:.0f} TFLOP/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +170,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the K=1 leg (profiling)")
     ap.add_argument("--no-gather", action="store_true", help="skip the dense BF16 gather leg")
     ap.add_argument("--gather-inner", type=int, default=4)
+    ap.add_argument("--graph-passes", type=int, default=8,
+                    help="passes per captured HIP graph in the batched leg (0: plain launches)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the Cornell (GPU vs CPU) and dense 32k x 192 legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, so that a "
                          "counter run can scale FETCH_SIZE / WRITE_SIZE to bytes")
@@ -121,8 +203,15 @@ def main():
     names = ["select", "prepare_clear", "raster_setup", "bin", "tile_raster_reduce", "shoot"]
 
     # ---------------------------------------------------------- batched K
-    s = R.Solver(hemicube=W, hemicubes=K, small_tri_px=args.small_tri_px)
+    s = R.Solver(hemicube=W, hemicubes=K, small_tri_px=args.small_tri_px,
+                 graph_passes=args.graph_passes)
     s.upload(sc)
+    tune = None
+    if not args.no_autotune:
+        t, sm, tms, cands = s.autotune(3)
+        tune = {"tile_px": t, "small_tri_px": sm, "render_ms": tms,
+                "candidates": {f"{a}x{b}": round(v, 5) for (a, b), v in sorted(cands.items())}}
+        log(f"autotune K={K}: tile {t}, in-lane limit {sm} px ({tms:.3f} ms per render pass)")
     s.step(args.warmup)
     s.finish()
     t0 = time.perf_counter()
@@ -186,6 +275,14 @@ def main():
                 if "gather_bf16" in conv:
                     conv["speedup_gather_vs_plain"] = (conv["plain"]["ms"]
                                                        / conv["gather_bf16"]["ms"])
+
+    # -------------------------------------- Cornell configs and dense 32k
+    configs = None
+    if not args.no_configs:
+        from oracle import oracle as O
+        configs = {"cornell1500_W128_100": cornell_leg(R, O, 0.3127, 128, 100, 1),
+                   "cornell20k_W256_1000": cornell_leg(R, O, 0.02314, 256, 1000, 1),
+                   "dense32k_R192": dense_leg(R, args)}
 
     # ------------------------------------------------------------ roofline
     # Dominant kernel of the batched pass and its algorithmic bytes per launch
@@ -259,9 +356,11 @@ def main():
             "shooters_per_s": round(plain_sps, 2),
             "ms_per_pass": round(pdt * 1e3 / args.plain_passes, 5),
             "kernel_ms": {k: round(v, 5) for k, v in pkt.items()}},
-        "batched": {"shooters_per_s": round(batched_sps, 2),
+        "batched": {"shooters_per_s": round(batched_sps, 2), "autotune": tune,
+                    "graph_passes": args.graph_passes,
                     "kernel_ms": {k: round(v, 5) for k, v in kt.items()}},
         "converge": conv,
+        "configs": configs,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -168,8 +168,9 @@ int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
  * tile_px) into out and returns the tile count per hemicube in *ntiles. */
 int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32_t *tile_px);
 
-/* Autotuner of the rasterizer: times `passes` render passes (select once,
- * then render + reduce, no shooting, state unchanged) for each screen tile
+/* Autotuner of the rasterizer: times `passes` render passes of K shooters
+ * spread evenly over the patches (render + reduce, no shooting, state
+ * unchanged; the device selection is overwritten) for each screen tile
  * side {16, 32} (and 8 for K <= 2) and in-lane triangle limit
  * {16, 32, 64, 128, 256}, and keeps the fastest pair.  No counterpart in the
  * reference (its GL driver rasterizes). */

@@ -838,7 +838,19 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
     }
     bool timing = c->timing;
     c->timing = false;
-    if ((rc = select_dev(c, c->K)) == RAD_OK) {
+    // K shooters spread evenly over the patch list: early in a solve the
+    // selection holds only the few emitters, which is no sample of the work
+    std::vector<uint32_t> spread(c->K);
+    for (uint32_t s = 0; s < c->K; ++s)
+        spread[s] = (uint32_t)(((uint64_t)s * c->npatch + c->npatch / 2) / c->K) % c->npatch;
+    uint32_t kk = c->K;
+    rc = RAD_OK;
+    if (hipMemcpyAsync(c->shooters, spread.data(), (size_t)c->K * 4, hipMemcpyHostToDevice, c->st) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->nshoot, &kk, 4, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+        rc = fail(c, RAD_E_HIP, "autotune: shooter upload");
+    if (rc == RAD_OK) {
         for (uint32_t ti = 0; ti < 3 && rc == RAD_OK; ++ti) {
             if (tiles[ti] == 8 && c->K > 2)
                 continue;
