@@ -1,0 +1,221 @@
+// radhip_dropin.hpp - C++ drop-in layer over the C ABI of libradhip.
+//
+// The reference's host API for the shooting path, with the same class names,
+// member names, argument meaning and ownership rules, implemented on the
+// MI355X library (include/radhip.h).  A caller of the reference switches by
+// including this header instead of Config.h, Patch.h, Model.h,
+// PrimitiveModel.h, ModelContainer.h and FormFactors.h and using namespace
+// radhip; the GL item-buffer render and the OpenCL ProcessHemicube launch
+// of OnIdle become one Shooter::shootCycle() call.
+//
+// Differences by design: patches keep their host fields (radiosity,
+// illumination) as the reference's do, and the GPU state is synchronised
+// around the calls that need it; the 8-neighbour links and the display
+// path are not part of this layer.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "radhip.h"
+
+namespace radhip {
+
+// Vector3f of Vector.h: the fields and operations the hot path uses.
+struct Vector3f {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    Vector3f() = default;
+    Vector3f(float a, float b, float c) : x(a), y(b), z(c) {}
+    Vector3f operator+(const Vector3f &o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vector3f operator-(const Vector3f &o) const { return {x - o.x, y - o.y, z - o.z}; }
+    Vector3f operator*(const Vector3f &o) const { return {x * o.x, y * o.y, z * o.z}; }
+    Vector3f operator*(float t) const { return {x * t, y * t, z * t}; }
+    Vector3f &operator+=(const Vector3f &o)
+    {
+        x += o.x;
+        y += o.y;
+        z += o.z;
+        return *this;
+    }
+    Vector3f &operator-=(const Vector3f &o)
+    {
+        x -= o.x;
+        y -= o.y;
+        z -= o.z;
+        return *this;
+    }
+    float f_Length2() const { return x * x + y * y + z * z; }
+    float f_Length() const;
+};
+
+// Config (Config.h:7-39, Config.cpp): static settings, frozen before use.
+class Config {
+  public:
+    static void setHemicubeSide(unsigned int n);
+    static void setOCLWorkitemsX(unsigned int n);
+    static void setMaxPatchArea(double n);
+    static void setShootsPerCycle(unsigned int n);
+    static void setHemicubesCount(unsigned int n);
+    static void freeze();
+
+    static unsigned int HEMICUBE_W();
+    static unsigned int HEMICUBE_H();
+    static unsigned int PATCHVIEW_TEX_W();
+    static unsigned int PATCHVIEW_TEX_H();
+    static unsigned int PATCHVIEW_TEX_RES();
+    static double MAX_PATCH_AREA();
+    static unsigned int OCL_WORKITEMS_X();
+    static unsigned int OCL_WORKITEMS_Y();
+    static unsigned int SHOOTS_PER_CYCLE();
+    static unsigned int HEMICUBES_CNT();
+    static bool isFrozen();
+};
+
+// REFLECTIVITY of Patch.h:13.
+constexpr float REFLECTIVITY = 0.3f;
+
+// Patch (Patch.h:16-56): a quad with colour, illumination (shot) and
+// radiosity (unshot) fields.
+class Patch {
+  public:
+    Patch();
+    Patch(Vector3f v1, Vector3f v2, Vector3f v3, Vector3f v4);
+    Patch(Vector3f v1, Vector3f v2, Vector3f v3, Vector3f v4, Vector3f color);
+    Patch(Vector3f v1, Vector3f v2, Vector3f v3, Vector3f v4, Vector3f color,
+          Vector3f illumination);
+    Patch(Vector3f v1, Vector3f v2, Vector3f v3, Vector3f v4, Vector3f color,
+          Vector3f illumination, Vector3f radiosity);
+
+    // Patch::divide (Patch.cpp:47-125): children of area <= `area` (the
+    // caller owns the vector and the patches), or NULL when already small.
+    std::vector<Patch *> *divide(double area);
+    std::vector<float> getVerticesCoords();
+
+    Vector3f getCenter();
+    Vector3f getNormal();
+    Vector3f getUp();
+    Vector3f getColor() { return color; }
+    float getReflectivity() { return REFLECTIVITY; }
+
+    Vector3f radiosity;    // unshot
+    Vector3f illumination; // shot
+
+  protected:
+    Vector3f vec1, vec2, vec3, vec4;
+    Vector3f color;
+};
+
+// Model (Model.h:7-20): a list of patches, subdivided on request.
+class Model {
+  public:
+    Model();
+    virtual ~Model();
+    virtual std::vector<Patch *> *getPatches(double area = 0) = 0;
+
+  protected:
+    void subdivide(double area); // Model.cpp:27-60
+    std::vector<Patch *> *patches;
+};
+
+// PrimitiveModel (PrimitiveModel.h): the Cornell box pieces.
+class PrimitiveModel : public Model {
+  public:
+    enum { ROOM, ROOMCLOSURE, CUBE, BLOCK };
+    explicit PrimitiveModel(int type);
+    std::vector<Patch *> *getPatches(double area = 0) override;
+
+  private:
+    int type;
+    bool built = false;
+};
+
+// ModelContainer (ModelContainer.h:11-53) whose shooter selection runs on
+// the device.  updateData() also uploads the scene to a context made from the
+// frozen Config (REFERENCE shooting and selection rules).
+class ModelContainer {
+  public:
+    ModelContainer();
+    ~ModelContainer();
+
+    void load(); // ModelContainer.cpp:35-48: the Cornell box
+
+    int addModel(Model *m);
+    void removeModel(int i);
+    void updateData();
+
+    float *getVertices();
+    unsigned int getVerticesCount();
+    int *getIndices();
+    unsigned int getIndicesCount();
+    Patch **getPatches();
+    unsigned int getPatchesCount();
+
+    // ModelContainer.cpp:222-237: argmax |radiosity|^2, first index on ties
+    // (the reference's unused single variant; a host loop over the fields).
+    unsigned int getHighestRadiosityPatchId();
+    // ModelContainer.cpp:259-299 on the device: `count` ids by the
+    // reference's list rule, unfilled slots NULL / 0.
+    void getHighestRadiosityPatchesId(unsigned int count, Patch **p_emitters,
+                                      unsigned int *p_emitters_ids);
+
+    double maxPatchArea;
+
+    // library context of the uploaded scene (created by updateData)
+    rad_ctx *context();
+    void pushState(); // patch fields -> device
+    void pullState(); // device -> patch fields
+
+  protected:
+    bool needRefresh;
+    std::vector<Model *> models;
+    std::vector<Patch *> patchList;
+    std::vector<float> vertices;
+    std::vector<int> indices;
+    std::vector<float> emission0; // radiosity at upload: the residual's normalisation
+    rad_ctx *ctx;
+    unsigned int ctxK;
+};
+
+// float *precomputeHemicubeFormFactors() (FormFactors.h:20): new[] array of
+// PATCHVIEW_TEX_RES * HEMICUBES_CNT floats (HEMICUBES_CNT copies of the
+// cross-layout table), computed on the device; the caller delete[]s it.
+// Config must be frozen.
+float *precomputeHemicubeFormFactors();
+
+// The OnIdle shooting cycle (Main.cpp:1137-1309): SHOOTS_PER_CYCLE passes of
+// HEMICUBES_CNT shooters (select, render the item buffers, ProcessHemicube,
+// accumulate, shoot, emitter update), then the patch fields are updated.
+class Shooter {
+  public:
+    explicit Shooter(ModelContainer *scene);
+    // One OnIdle call's worth of passes; returns |dB| of the last shooter
+    // (lastEnergy.f_Length(), Main.cpp:1292).
+    float shootCycle();
+    // Main.cpp:1298: the cycle loop stops below 0.1.
+    bool finished() const { return lastEnergy < 0.1f && cycles > 0; }
+    double residual(); // FP64 area-weighted unshot / emitted
+    unsigned long long shooters() const { return shot; }
+
+  private:
+    ModelContainer *scene;
+    float lastEnergy = 0.0f;
+    unsigned cycles = 0;
+    unsigned long long shot = 0;
+};
+
+// The ProcessHemicube kernel's host contract (Kernel_ProcessHemicube.h:9-20,
+// Main.cpp:1207-1249) over an explicit item buffer of n_hemicubes stacked
+// hemicubes (ids: patch + 1, 0 background): writes triplets (hemicube, patch
+// id, form-factor sum) for every (hemicube, patch) seen, in hemicube then id
+// order, and returns their count (the reference's write index).  The output
+// arrays hold at least n_hemicubes * getPatchesCount() entries.
+unsigned int ProcessHemicube(ModelContainer *scene, const uint32_t *itembuf,
+                             unsigned int n_hemicubes, uint32_t *p_hemicubes, uint32_t *p_ids,
+                             float *p_energies);
+
+// Last error of this layer (the library's message, or the layer's own).
+const std::string &lastError();
+
+} // namespace radhip

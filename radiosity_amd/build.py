@@ -14,6 +14,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libradhip.so")
 BUILD = os.path.join(HERE, "_build")
+BIN = os.path.join(HERE, "bin")
+ROOT = os.path.dirname(HERE)
+
+# host programs linked against libradhip.so (rpath: the package directory)
+PROGRAMS = [
+    ("radhip_cli", os.path.join(CSRC, "rad_cli.cpp")),
+    ("dropin_check", os.path.join(ROOT, "tests", "dropin_check.cpp")),
+]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RADHIP_ARCH", "gfx950")
@@ -56,6 +64,17 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    os.makedirs(BIN, exist_ok=True)
+    for name, src in PROGRAMS:
+        if not os.path.exists(src):
+            continue
+        exe = os.path.join(BIN, name)
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+               "-o", exe + ".tmp", "-L", HERE, "-lradhip", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
     if verbose:
         print(f"built {OUT}")
     return OUT

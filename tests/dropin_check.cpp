@@ -1,0 +1,196 @@
+// dropin_check.cpp - drives the C++ drop-in layer for the tests; every mode
+// prints one JSON line.  Built next to the library by radiosity_amd.build.
+//
+//   dropin_check config W K WX SHOOTS AREA     Config::freeze arithmetic (CPU)
+//   dropin_check patches AREA                  load + updateData arrays (CPU)
+//   dropin_check dff W K                       precomputeHemicubeFormFactors
+//   dropin_check select AREA K SEED            list rule on seeded radiosities
+//   dropin_check shoot AREA W K SHOOTS CYCLES  Shooter cycles, final fields
+//   dropin_check process AREA W                ProcessHemicube triplets
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <random>
+#include <string>
+#include <vector>
+
+#include "radhip_dropin.hpp"
+
+using namespace radhip;
+
+static void vec(const char *name, const std::vector<double> &v, bool last = false)
+{
+    printf("\"%s\": [", name);
+    for (size_t i = 0; i < v.size(); ++i)
+        printf(i ? ", %.9g" : "%.9g", v[i]);
+    printf("]%s", last ? "" : ", ");
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 2)
+        return 2;
+    std::string mode = argv[1];
+    if (mode == "config" && argc == 7) {
+        Config::setHemicubeSide((unsigned)atoi(argv[2]));
+        Config::setHemicubesCount((unsigned)atoi(argv[3]));
+        Config::setOCLWorkitemsX((unsigned)atoi(argv[4]));
+        Config::setShootsPerCycle((unsigned)atoi(argv[5]));
+        Config::setMaxPatchArea(atof(argv[6]));
+        Config::freeze();
+        Config::setHemicubeSide(4); // frozen: ignored
+        printf("{\"W\": %u, \"H\": %u, \"TW\": %u, \"TH\": %u, \"RES\": %u, \"WX\": %u, "
+               "\"WY\": %u, \"shoots\": %u, \"K\": %u, \"area\": %.9g, \"frozen_err\": %d}\n",
+               Config::HEMICUBE_W(), Config::HEMICUBE_H(), Config::PATCHVIEW_TEX_W(),
+               Config::PATCHVIEW_TEX_H(), Config::PATCHVIEW_TEX_RES(), Config::OCL_WORKITEMS_X(),
+               Config::OCL_WORKITEMS_Y(), Config::SHOOTS_PER_CYCLE(), Config::HEMICUBES_CNT(),
+               Config::MAX_PATCH_AREA(), lastError().empty() ? 0 : 1);
+        return 0;
+    }
+    if (mode == "patches" && argc == 3) {
+        ModelContainer mc;
+        mc.maxPatchArea = atof(argv[2]);
+        mc.load();
+        unsigned n = mc.getPatchesCount();
+        Patch **ps = mc.getPatches();
+        std::vector<double> fr, col, rad, ill;
+        for (unsigned i : {0u, 5u, n / 2, n - 1}) {
+            Vector3f c = ps[i]->getCenter(), nn = ps[i]->getNormal(), u = ps[i]->getUp();
+            for (float f : {c.x, c.y, c.z, nn.x, nn.y, nn.z, u.x, u.y, u.z})
+                fr.push_back(f);
+        }
+        for (unsigned i = 0; i < n; ++i) {
+            Vector3f c = ps[i]->getColor();
+            col.insert(col.end(), {c.x, c.y, c.z});
+            rad.push_back(ps[i]->radiosity.x);
+            ill.push_back(ps[i]->illumination.x);
+        }
+        std::vector<double> v(mc.getVertices(), mc.getVertices() + mc.getVerticesCount());
+        std::vector<double> ix(mc.getIndices(), mc.getIndices() + mc.getIndicesCount());
+        printf("{\"n\": %u, \"nv\": %u, \"ni\": %u, ", n, mc.getVerticesCount(),
+               mc.getIndicesCount());
+        vec("frames", fr);
+        vec("colors", col);
+        vec("radiosity_r", rad);
+        vec("illumination_r", ill);
+        vec("vertices", v);
+        vec("indices", ix, true);
+        printf("}\n");
+        return 0;
+    }
+    if (mode == "dff" && argc == 4) {
+        Config::setHemicubeSide((unsigned)atoi(argv[2]));
+        Config::setHemicubesCount((unsigned)atoi(argv[3]));
+        Config::freeze();
+        float *f = precomputeHemicubeFormFactors();
+        if (!f) {
+            fprintf(stderr, "%s\n", lastError().c_str());
+            return 1;
+        }
+        size_t res = Config::PATCHVIEW_TEX_RES(), k = Config::HEMICUBES_CNT();
+        bool same = true;
+        for (size_t h = 1; h < k; ++h)
+            same = same && memcmp(f, f + h * res, res * 4) == 0;
+        std::vector<double> first(f, f + res);
+        printf("{\"copies_equal\": %d, ", same ? 1 : 0);
+        vec("table", first, true);
+        printf("}\n");
+        delete[] f;
+        return 0;
+    }
+    if ((mode == "select" && argc == 5) || (mode == "shoot" && argc == 7) ||
+        (mode == "process" && argc == 4)) {
+        unsigned W = mode == "select" ? 16 : (unsigned)atoi(argv[3]);
+        unsigned K = mode == "select" ? (unsigned)atoi(argv[3])
+                                      : (mode == "shoot" ? (unsigned)atoi(argv[4]) : 1);
+        Config::setHemicubeSide(W);
+        Config::setHemicubesCount(K);
+        if (mode == "shoot")
+            Config::setShootsPerCycle((unsigned)atoi(argv[5]));
+        Config::setMaxPatchArea(atof(argv[2]));
+        Config::freeze();
+        ModelContainer mc;
+        mc.maxPatchArea = Config::MAX_PATCH_AREA();
+        mc.load();
+        mc.updateData();
+        if (!mc.context()) {
+            fprintf(stderr, "%s\n", lastError().c_str());
+            return 1;
+        }
+        unsigned n = mc.getPatchesCount();
+        Patch **ps = mc.getPatches();
+        if (mode == "select") {
+            std::mt19937 rng((unsigned)atoi(argv[4]));
+            std::vector<double> rr;
+            for (unsigned i = 0; i < n; ++i) {
+                // a few distinct levels so ties and zeros occur
+                float v = (float)(rng() % 7) * 0.5f;
+                ps[i]->radiosity = {v, (float)(rng() % 3), 0.25f};
+                if (rng() % 5 == 0)
+                    ps[i]->radiosity = {0, 0, 0};
+                rr.insert(rr.end(), {ps[i]->radiosity.x, ps[i]->radiosity.y, ps[i]->radiosity.z});
+            }
+            std::vector<Patch *> em(K);
+            std::vector<unsigned> ids(K);
+            mc.getHighestRadiosityPatchesId(K, em.data(), ids.data());
+            std::vector<double> out(ids.begin(), ids.end()), nul;
+            for (unsigned k = 0; k < K; ++k)
+                nul.push_back(em[k] ? (double)(em[k] == ps[ids[k]]) : -1.0);
+            printf("{\"first\": %u, ", mc.getHighestRadiosityPatchId());
+            vec("ids", out);
+            vec("ptr_ok", nul);
+            vec("radiosity", rr, true);
+            printf("}\n");
+            return 0;
+        }
+        if (mode == "shoot") {
+            Shooter sh(&mc);
+            unsigned cycles = (unsigned)atoi(argv[6]);
+            float e = 0;
+            for (unsigned c = 0; c < cycles; ++c)
+                e = sh.shootCycle();
+            std::vector<double> rad, ill;
+            for (unsigned i = 0; i < n; ++i) {
+                rad.insert(rad.end(), {ps[i]->radiosity.x, ps[i]->radiosity.y, ps[i]->radiosity.z});
+                ill.insert(ill.end(),
+                           {ps[i]->illumination.x, ps[i]->illumination.y, ps[i]->illumination.z});
+            }
+            printf("{\"shooters\": %llu, \"last_energy\": %.9g, \"residual\": %.9g, ",
+                   sh.shooters(), e, sh.residual());
+            vec("radiosity", rad);
+            vec("illumination", ill, true);
+            printf("}\n");
+            return 0;
+        }
+        // process: the light's hemicube through the C ABI, then the triplets
+        rad_ctx *c = mc.context();
+        unsigned tw, th, res;
+        rad_geometry(c, &tw, &th, &res);
+        unsigned light = 0;
+        for (unsigned i = 0; i < n; ++i)
+            if (ps[i]->radiosity.x > 0) {
+                light = i;
+                break;
+            }
+        std::vector<uint32_t> item(res);
+        if (rad_render_itembuf(c, &light, 1, item.data(), nullptr) != RAD_OK) {
+            fprintf(stderr, "%s\n", rad_last_error(c));
+            return 1;
+        }
+        std::vector<uint32_t> hc(n), idv(n);
+        std::vector<float> en(n);
+        unsigned m = ProcessHemicube(&mc, item.data(), 1, hc.data(), idv.data(), en.data());
+        std::vector<double> ids(idv.begin(), idv.begin() + m), e(en.begin(), en.begin() + m),
+            h(hc.begin(), hc.begin() + m), items(item.begin(), item.end());
+        printf("{\"light\": %u, \"count\": %u, ", light, m);
+        vec("hemicubes", h);
+        vec("ids", ids);
+        vec("energies", e);
+        vec("item", items, true);
+        printf("}\n");
+        return 0;
+    }
+    fprintf(stderr, "usage: see the header of dropin_check.cpp\n");
+    return 2;
+}
