@@ -185,6 +185,35 @@ typedef struct rad_tune_result {
 int rad_autotune(rad_ctx *ctx, uint32_t passes, rad_tune_result *out_or_null);
 
 /* ---------------------------------------------------------------------------
+ * Scene packs: many independent scenes solved by one context, every kernel
+ * of a pass launched once for all of them.  No counterpart in the reference
+ * (one scene per process there); each scene follows the context's shooting
+ * mode with the POWER rule inside the scene.
+ *
+ * The scenes' arrays are concatenated: scene s owns triangles
+ * [scene_tri_beg[s], scene_tri_beg[s+1]) and patches [scene_patch_beg[s],
+ * scene_patch_beg[s+1]) (nscene + 1 offsets each, first 0, last ntri /
+ * npatch); tri_patch holds packed patch indices and each triangle's patch
+ * lies in its own scene.  scene_slots[s] (1..K, summing to at most K) is the
+ * number of shooters the scene gets per pass while it is active.  The pack
+ * replaces the context's scene; rad_set_state / rad_get_state / rad_step /
+ * rad_finish / rad_residual_fp64 work on the packed arrays, and the calls
+ * that take explicit shooters, rad_select, rad_autotune and the dense gather
+ * return RAD_E_STATE while a pack is loaded. */
+int rad_pack_upload(rad_ctx *ctx, uint32_t nscene, const uint32_t *scene_tri_beg,
+                    const uint32_t *scene_patch_beg, const uint32_t *scene_slots,
+                    const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri, uint32_t npatch,
+                    const float *patch_frame, const float *refl_rgb, const float *area);
+/* Starts a solve: per-scene normalisation sum_A (r+g+b)(E) from the emission
+ * of rad_set_state; a scene is done once its residual <= target. */
+int rad_pack_begin(rad_ctx *ctx, double target);
+/* Synchronises, then reports per scene the pass it finished in (0xFFFFFFFF:
+ * not yet) and its residual at the last selection, the passes run and the
+ * scenes still active.  Any argument may be NULL. */
+int rad_pack_status(rad_ctx *ctx, uint32_t *done_pass_or_null, double *residual_or_null,
+                    uint32_t *passes_or_null, uint32_t *remaining_or_null);
+
+/* ---------------------------------------------------------------------------
  * Dense form factors and the Jacobi gather (the precision path).
  *
  * The reference shoots one hemicube row at a time (Main.cpp:1251-1279).  The

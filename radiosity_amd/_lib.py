@@ -99,6 +99,9 @@ SIGNATURES = {
     "rad_scene_free": (None, [C.POINTER(rad_scene)]),
     "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
     "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
+    "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
+    "rad_pack_begin": (C.c_int, [_P, C.c_double]),
+    "rad_pack_status": (C.c_int, [_P, _PU, C.POINTER(C.c_double), _PU, _PU]),
     "rad_dense_build": (C.c_int, [_P, _u32]),
     "rad_dense_upload": (C.c_int, [_P, _PF]),
     "rad_dense_download": (C.c_int, [_P, _u32, _u32, _PF]),
@@ -327,6 +330,45 @@ class Solver:
         self._chk(lib().rad_autotune(self._ctx, passes, C.byref(r)), "rad_autotune")
         cands = {(r.cand_tile[k], r.cand_small[k]): r.cand_ms[k] for k in range(r.ncand)}
         return r.tile_px, r.small_tri_px, r.best_ms, cands
+
+    # ------------------------------------------------------------ packs
+    def pack_upload(self, scenes, slots):
+        """Concatenates `scenes` into one pack; slots[s] shooters per pass for
+        scene s.  Sets the state to each scene's emission / initial shot."""
+        tb = np.zeros(len(scenes) + 1, np.uint32)
+        pb = np.zeros(len(scenes) + 1, np.uint32)
+        tb[1:] = np.cumsum([sc.ntri for sc in scenes])
+        pb[1:] = np.cumsum([sc.npatch for sc in scenes])
+        tri = np.ascontiguousarray(np.concatenate([sc.tri.reshape(-1) for sc in scenes]),
+                                   np.float32)
+        tp = np.ascontiguousarray(np.concatenate(
+            [sc.tri_patch + pb[k] for k, sc in enumerate(scenes)]), np.uint32)
+        cat = lambda name: np.ascontiguousarray(  # noqa: E731
+            np.concatenate([getattr(sc, name) for sc in scenes]), np.float32)
+        frame, refl, area = cat("frame"), cat("refl"), cat("area")
+        sl = np.ascontiguousarray(slots, np.uint32)
+        self._chk(lib().rad_pack_upload(self._ctx, len(scenes), uptr(tb), uptr(pb), uptr(sl),
+                                        fptr(tri), uptr(tp), int(tb[-1]), int(pb[-1]),
+                                        fptr(frame), fptr(refl), fptr(area)), "rad_pack_upload")
+        self.npatch = int(pb[-1])
+        self.pack_offsets = (tb, pb)
+        em = cat("emission")
+        self.set_state(em, cat("shot0"), em)
+
+    def pack_begin(self, target):
+        self._chk(lib().rad_pack_begin(self._ctx, target), "rad_pack_begin")
+
+    def pack_status(self):
+        """(done_pass per scene, 0xFFFFFFFF while active; residual per scene;
+        passes; scenes remaining)."""
+        n = len(self.pack_offsets[1]) - 1
+        done = np.zeros(n, np.uint32)
+        res = np.zeros(n, np.float64)
+        passes, rem = C.c_uint32(), C.c_uint32()
+        self._chk(lib().rad_pack_status(self._ctx, uptr(done),
+                                        res.ctypes.data_as(C.POINTER(C.c_double)),
+                                        C.byref(passes), C.byref(rem)), "rad_pack_status")
+        return done, res, passes.value, rem.value
 
     # ------------------------------------------------------ dense / gather
     def dense_build(self, layout=RAD_DENSE_SHOOT):

@@ -34,6 +34,7 @@ SOURCES = [
     # (source, compiled as HIP)
     ("rad_kernels.hip", True),
     ("rad_gather.hip", True),
+    ("rad_pack.hip", True),
     ("rad_api.cpp", True),
     ("rad_scene.cpp", False),
     ("rad_dropin.cpp", False),

@@ -57,6 +57,10 @@ struct rad_ctx {
     bool rendered = false;        // keys hold the item buffer of the last render
     uint32_t rendered_k = 0;
 
+    // F row length: npatch, or the largest scene of a pack
+    uint32_t rowlen = 0;
+    PackBuffers pk{};
+
     // captured passes (rad_config.graph_passes of them) replayed by rad_step
     hipGraphExec_t gexec = nullptr;
     uint32_t gpasses = 0;
@@ -146,15 +150,15 @@ void record(rad_ctx *c, size_t idx)
 int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits)
 {
     launch_prepare(c->g, k, c->shooters, c->nshoot, c->frames, c->unshot, c->snap, c->mvp, c->st);
-    launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->npatch, c->counters, c->bb.ent_cap,
-                 c->st);
+    launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->rowlen, c->counters,
+                 c->bb.ent_cap, c->st);
     record(c, evbase + 2);
     launch_raster_setup(c->g, k, c->tri, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
-                        c->counters, c->st);
+                        c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->st);
     record(c, evbase + 3);
     launch_bin(k, c->nshoot, c->bb, c->counters, c->st);
     record(c, evbase + 4);
-    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, keys, c->dff, c->npatch,
+    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, keys, c->dff, c->rowlen, c->pk.slot_pbeg,
                 keys ? nullptr : c->ff, hits, c->counters, c->st);
     record(c, evbase + 5);
     return check_launch(c, "render kernels");
@@ -202,13 +206,22 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
 {
     int rc;
     record(c, evbase + 0);
-    if (do_select && (rc = select_dev(c, c->K)) != RAD_OK)
+    if (c->pk.nscene) {
+        launch_pack_select(c->pk, c->unshot, c->area, c->st);
+        if ((rc = check_launch(c, "pack select")))
+            return rc;
+    } else if (do_select && (rc = select_dev(c, c->K)) != RAD_OK) {
         return rc;
+    }
     record(c, evbase + 1);
     if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr)) != RAD_OK)
         return rc;
-    launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl, c->area,
-                 c->npatch, c->unshot, c->shot, c->st);
+    if (c->pk.nscene)
+        launch_pack_shoot(c->pk, c->cfg.mode, c->snap, c->ff, c->rowlen, c->refl, c->area,
+                          c->npatch, c->unshot, c->shot, c->st);
+    else
+        launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl,
+                     c->area, c->npatch, c->unshot, c->shot, c->st);
     record(c, evbase + 6);
     c->rendered = false;
     return check_launch(c, "shoot");
@@ -247,6 +260,17 @@ int need_scene(rad_ctx *c)
 {
     if (c->npatch == 0)
         return fail(c, RAD_E_STATE, "no scene uploaded (rad_scene_upload)");
+    return RAD_OK;
+}
+
+// Entry points whose F rows span all patches: not with a scene pack.
+int need_plain(rad_ctx *c)
+{
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (c->pk.nscene)
+        return fail(c, RAD_E_STATE, "not available while a scene pack is loaded");
     return RAD_OK;
 }
 
@@ -335,7 +359,10 @@ void rad_destroy(rad_ctx *c)
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
                     c->resid, c->bb.recs, c->bb.count, c->bb.start, c->bb.fill,
                     c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts,
-                    c->fhi,  c->flo};
+                    c->fhi,  c->flo,      c->pk.sc_tbeg, c->pk.sc_pbeg, c->pk.sc_slots,
+                    c->pk.sc_slot_beg, c->pk.sc_nsel, c->pk.sc_done, c->pk.sc_den, c->pk.sc_res,
+                    c->pk.cand, c->pk.patch_scene, c->pk.pass, c->pk.slot_tbeg, c->pk.slot_tcnt,
+                    c->pk.slot_pbeg};
     for (void *p : ptrs)
         dfree(p);
     for (hipEvent_t e : c->ev)
@@ -371,11 +398,24 @@ int rad_precompute_dff(rad_ctx *c, float *out)
     return RAD_OK;
 }
 
-int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
-                     uint32_t npatch, const float *frames, const float *refl, const float *area)
+} // extern "C"
+
+namespace {
+
+void pack_free(rad_ctx *c)
 {
-    if (!c)
-        return RAD_E_ARG;
+    PackBuffers &p = c->pk;
+    void *ptrs[] = {p.sc_tbeg, p.sc_pbeg, p.sc_slots, p.sc_slot_beg, p.sc_nsel, p.sc_done, p.sc_den,
+                    p.sc_res,  p.cand,    p.patch_scene, p.pass,     p.slot_tbeg, p.slot_tcnt,
+                    p.slot_pbeg};
+    for (void *q : ptrs)
+        dfree(q);
+    c->pk = PackBuffers{};
+}
+
+int check_scene(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                uint32_t npatch, const float *frames, const float *refl, const float *area)
+{
     if (!tri_xyz || !tri_patch || !frames || !refl || !area || npatch == 0)
         return fail(c, RAD_E_ARG, "scene: null array or zero patches");
     for (uint32_t t = 0; t < ntri; ++t) {
@@ -390,21 +430,29 @@ int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch
     for (uint32_t i = 0; i < npatch; ++i)
         if (!(area[i] > 0.0f))
             return fail(c, RAD_E_ARG, "scene: patch area must be > 0");
+    return RAD_OK;
+}
+
+// Device copy of a scene; the rasterizer's record regions are sized for
+// setup grids of nblk blocks per shooter and F rows of rowlen entries.
+int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                uint32_t npatch, const float *frames, const float *refl, const float *area,
+                uint32_t nblk, uint32_t rowlen)
+{
     (void)hipStreamSynchronize(c->st);
     drop_graph(c);
+    pack_free(c);
     int rc;
     size_t K = c->K;
     if ((rc = dalloc(c, &c->tri, (size_t)ntri * 9)) || (rc = dalloc(c, &c->tri_patch, ntri)) ||
         (rc = dalloc(c, &c->frames, (size_t)npatch * 9)) || (rc = dalloc(c, &c->refl, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->area, npatch)) || (rc = dalloc(c, &c->emis, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->unshot, (size_t)npatch * 3)) || (rc = dalloc(c, &c->shot, (size_t)npatch * 3)) ||
-        (rc = dalloc(c, &c->ff, K * npatch)) || (rc = dalloc(c, &c->hits, K * npatch)))
+        (rc = dalloc(c, &c->ff, K * rowlen)) || (rc = dalloc(c, &c->hits, K * rowlen)))
         return rc;
     // records: one region of kRecsPerBlock per (setup block, shooter) plus a
     // spill region; bin entries: three per record slot
-    c->bb.nblk = (ntri + 255) / 256;
-    if (c->bb.nblk == 0)
-        c->bb.nblk = 1;
+    c->bb.nblk = nblk ? nblk : 1;
     c->bb.nz = K <= 4 ? 5u : 1u;
     uint64_t regions = (uint64_t)c->bb.nblk * K * c->bb.nz * kRecsPerBlock;
     uint64_t spill = regions / 8 + 65536;
@@ -435,12 +483,30 @@ int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch
     HIPCHK(c, hipMemset(c->emis, 0, (size_t)npatch * 12));
     HIPCHK(c, hipMemset(c->unshot, 0, (size_t)npatch * 12));
     HIPCHK(c, hipMemset(c->shot, 0, (size_t)npatch * 12));
-    HIPCHK(c, hipMemset(c->ff, 0, K * npatch * 4));
-    HIPCHK(c, hipMemset(c->hits, 0, K * npatch * 4));
+    HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
+    HIPCHK(c, hipMemset(c->hits, 0, K * rowlen * 4));
     c->ntri = ntri;
     c->npatch = npatch;
+    c->rowlen = rowlen;
     c->rendered = false;
+    c->dense = false;
     return RAD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rad_scene_upload(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                     uint32_t npatch, const float *frames, const float *refl, const float *area)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = check_scene(c, tri_xyz, tri_patch, ntri, npatch, frames, refl, area)))
+        return rc;
+    return upload_impl(c, tri_xyz, tri_patch, ntri, npatch, frames, refl, area, (ntri + 255) / 256,
+                       npatch);
 }
 
 int rad_set_state(rad_ctx *c, const float *unshot, const float *shot, const float *emission)
@@ -484,7 +550,7 @@ int rad_select(rad_ctx *c, uint32_t k, uint32_t *out_ids, uint32_t *out_n)
     if (!c)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)))
+    if ((rc = need_plain(c)))
         return rc;
     if (k < 1 || k > c->K)
         return fail(c, RAD_E_ARG, "select: k must be in [1, hemicubes]");
@@ -524,7 +590,7 @@ int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_
     if (!c)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)) || (rc = upload_shooters(c, shooters, k)))
+    if ((rc = need_plain(c)) || (rc = upload_shooters(c, shooters, k)))
         return rc;
     if ((rc = render_pass(c, k, 0, c->keys, nullptr)))
         return rc;
@@ -578,7 +644,7 @@ int rad_process_hemicube(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_
     if (!c)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)))
+    if ((rc = need_plain(c)))
         return rc;
     if (!c->rendered || k != c->rendered_k)
         return fail(c, RAD_E_STATE, "process_hemicube: k must match the last render");
@@ -591,7 +657,7 @@ int rad_process_itembuf(rad_ctx *c, const uint32_t *itembuf, uint32_t k, uint32_
     if (!c || !itembuf)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)))
+    if ((rc = need_plain(c)))
         return rc;
     if (k < 1 || k > c->K)
         return fail(c, RAD_E_ARG, "process_itembuf: k must be in [1, hemicubes]");
@@ -616,7 +682,7 @@ int rad_shoot(rad_ctx *c, const uint32_t *shooters, uint32_t k)
     if (!c)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)) || (rc = upload_shooters(c, shooters, k)))
+    if ((rc = need_plain(c)) || (rc = upload_shooters(c, shooters, k)))
         return rc;
     uint32_t saveK = c->K;
     c->K = k;
@@ -823,7 +889,7 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
     if (!c)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)))
+    if ((rc = need_plain(c)))
         return rc;
     if (passes == 0)
         passes = 3;
@@ -1022,7 +1088,7 @@ int rad_dense_build(rad_ctx *c, uint32_t layout)
     if (layout > RAD_DENSE_SHOOT)
         return fail(c, RAD_E_ARG, "dense_build: unknown layout");
     int rc;
-    if ((rc = need_scene(c)) || (rc = dense_alloc(c)))
+    if ((rc = need_plain(c)) || (rc = dense_alloc(c)))
         return rc;
     bool timing = c->timing;
     c->timing = false;
@@ -1054,7 +1120,7 @@ int rad_dense_upload(rad_ctx *c, const float *m)
     if (!c || !m)
         return RAD_E_ARG;
     int rc;
-    if ((rc = need_scene(c)) || (rc = dense_alloc(c)))
+    if ((rc = need_plain(c)) || (rc = dense_alloc(c)))
         return rc;
     size_t n = (size_t)c->npatch * c->npatch;
     float *tmp = nullptr;
@@ -1234,6 +1300,145 @@ int rad_gather_solve(rad_ctx *c, const float *rhs, uint32_t R, const rad_gather_
         st.residual = res;
         st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *stats = st;
+    }
+    return RAD_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scene packs (rad_pack.hip kernels).
+extern "C" {
+
+int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
+                    const uint32_t *scene_patch_beg, const uint32_t *scene_slots,
+                    const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri, uint32_t npatch,
+                    const float *frames, const float *refl, const float *area)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (nscene == 0 || !scene_tri_beg || !scene_patch_beg || !scene_slots)
+        return fail(c, RAD_E_ARG, "pack: no scenes");
+    int rc;
+    if ((rc = check_scene(c, tri_xyz, tri_patch, ntri, npatch, frames, refl, area)))
+        return rc;
+    if (scene_tri_beg[0] != 0 || scene_tri_beg[nscene] != ntri || scene_patch_beg[0] != 0 ||
+        scene_patch_beg[nscene] != npatch)
+        return fail(c, RAD_E_ARG, "pack: offsets must start at 0 and end at ntri / npatch");
+    uint64_t slots = 0;
+    uint32_t nblk = 1, rowlen = 1, kmax = 1;
+    std::vector<uint32_t> patch_scene(npatch);
+    for (uint32_t s = 0; s < nscene; ++s) {
+        uint32_t t0 = scene_tri_beg[s], t1 = scene_tri_beg[s + 1];
+        uint32_t p0 = scene_patch_beg[s], p1 = scene_patch_beg[s + 1];
+        if (t1 < t0 || p1 <= p0)
+            return fail(c, RAD_E_ARG, "pack: scene " + std::to_string(s) +
+                                          " has decreasing offsets or no patch");
+        for (uint32_t t = t0; t < t1; ++t)
+            if (tri_patch[t] < p0 || tri_patch[t] >= p1)
+                return fail(c, RAD_E_ARG, "pack: triangle " + std::to_string(t) +
+                                              " names a patch outside its scene");
+        if (scene_slots[s] == 0 || scene_slots[s] > c->K)
+            return fail(c, RAD_E_ARG, "pack: shooters per scene must be 1..K");
+        slots += scene_slots[s];
+        nblk = std::max(nblk, (t1 - t0 + 255) / 256);
+        rowlen = std::max(rowlen, p1 - p0);
+        kmax = std::max(kmax, scene_slots[s]);
+        for (uint32_t p = p0; p < p1; ++p)
+            patch_scene[p] = s;
+    }
+    if (slots > c->K)
+        return fail(c, RAD_E_ARG, "pack: shooters per pass of all scenes exceed K");
+    if ((rc = upload_impl(c, tri_xyz, tri_patch, ntri, npatch, frames, refl, area, nblk, rowlen)))
+        return rc;
+    PackBuffers &p = c->pk;
+    size_t S = nscene;
+    if ((rc = dalloc(c, &p.sc_tbeg, S + 1)) || (rc = dalloc(c, &p.sc_pbeg, S + 1)) ||
+        (rc = dalloc(c, &p.sc_slots, S)) || (rc = dalloc(c, &p.sc_slot_beg, S)) ||
+        (rc = dalloc(c, &p.sc_nsel, S)) || (rc = dalloc(c, &p.sc_done, S)) ||
+        (rc = dalloc(c, &p.sc_den, S)) || (rc = dalloc(c, &p.sc_res, S)) ||
+        (rc = dalloc(c, &p.cand, S * kmax)) || (rc = dalloc(c, &p.patch_scene, npatch)) ||
+        (rc = dalloc(c, &p.pass, 1)) || (rc = dalloc(c, &p.slot_tbeg, c->K)) ||
+        (rc = dalloc(c, &p.slot_tcnt, c->K)) || (rc = dalloc(c, &p.slot_pbeg, c->K))) {
+        pack_free(c);
+        return rc;
+    }
+    HIPCHK(c, hipMemcpy(p.sc_tbeg, scene_tri_beg, (S + 1) * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(p.sc_pbeg, scene_patch_beg, (S + 1) * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(p.sc_slots, scene_slots, S * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(p.patch_scene, patch_scene.data(), (size_t)npatch * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(p.sc_nsel, 0, S * 4));
+    HIPCHK(c, hipMemset(p.sc_slot_beg, 0, S * 4));
+    HIPCHK(c, hipMemset(p.sc_done, 0xFF, S * 4));
+    HIPCHK(c, hipMemset(p.sc_den, 0, S * 8));
+    HIPCHK(c, hipMemset(p.sc_res, 0, S * 8));
+    HIPCHK(c, hipMemset(p.pass, 0, 4));
+    HIPCHK(c, hipMemset(p.slot_tbeg, 0, c->K * 4));
+    HIPCHK(c, hipMemset(p.slot_tcnt, 0, c->K * 4));
+    HIPCHK(c, hipMemset(p.slot_pbeg, 0, c->K * 4));
+    p.nscene = nscene;
+    p.kmax = kmax;
+    p.K = c->K;
+    p.shooters = c->shooters;
+    p.nshoot = c->nshoot;
+    return RAD_OK;
+}
+
+int rad_pack_begin(rad_ctx *c, double target)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    PackBuffers &p = c->pk;
+    if (!p.nscene)
+        return fail(c, RAD_E_STATE, "no scene pack (rad_pack_upload)");
+    size_t n = c->npatch, S = p.nscene;
+    std::vector<float> em(n * 3), ar(n);
+    std::vector<uint32_t> pb(S + 1);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(em.data(), c->emis, n * 12, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ar.data(), c->area, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(pb.data(), p.sc_pbeg, (S + 1) * 4, hipMemcpyDeviceToHost));
+    std::vector<double> den(S, 0.0);
+    for (size_t s = 0; s < S; ++s)
+        for (uint32_t i = pb[s]; i < pb[s + 1]; ++i)
+            den[s] += (double)ar[i] *
+                      ((double)em[3 * (size_t)i] + (double)em[3 * (size_t)i + 1] + (double)em[3 * (size_t)i + 2]);
+    HIPCHK(c, hipMemcpy(p.sc_den, den.data(), S * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(p.sc_done, 0xFF, S * 4));
+    HIPCHK(c, hipMemset(p.pass, 0, 4));
+    p.target = target;
+    drop_graph(c); // the target is a kernel argument of captured passes
+    return RAD_OK;
+}
+
+int rad_pack_status(rad_ctx *c, uint32_t *done_pass, double *residual, uint32_t *passes,
+                    uint32_t *remaining)
+{
+    if (!c)
+        return RAD_E_ARG;
+    PackBuffers &p = c->pk;
+    if (!p.nscene)
+        return fail(c, RAD_E_STATE, "no scene pack (rad_pack_upload)");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    int rc;
+    if ((rc = check_overflow(c)))
+        return rc;
+    std::vector<uint32_t> done(p.nscene);
+    HIPCHK(c, hipMemcpy(done.data(), p.sc_done, p.nscene * 4, hipMemcpyDeviceToHost));
+    if (done_pass)
+        memcpy(done_pass, done.data(), p.nscene * 4);
+    if (residual)
+        HIPCHK(c, hipMemcpy(residual, p.sc_res, p.nscene * 8, hipMemcpyDeviceToHost));
+    if (passes)
+        HIPCHK(c, hipMemcpy(passes, p.pass, 4, hipMemcpyDeviceToHost));
+    if (remaining) {
+        uint32_t r = 0;
+        for (uint32_t d : done)
+            r += d == 0xFFFFFFFFu;
+        *remaining = r;
     }
     return RAD_OK;
 }

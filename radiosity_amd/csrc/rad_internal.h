@@ -68,6 +68,23 @@ struct Geometry {
     uint32_t W, TW, TH, RES;
 };
 
+// Many scenes packed into one context (rad_pack_upload): per-scene ranges
+// and per-pass slot arrays (null when no pack is loaded).
+struct PackBuffers {
+    uint32_t nscene = 0, kmax = 0, K = 0;
+    double target = 1e-3;
+    uint32_t *sc_tbeg = nullptr, *sc_pbeg = nullptr; // nscene + 1 offsets
+    uint32_t *sc_slots = nullptr;                    // shooters per pass per scene
+    uint32_t *sc_slot_beg = nullptr, *sc_nsel = nullptr;
+    uint32_t *sc_done = nullptr;                     // pass a scene finished in (~0u: not yet)
+    double *sc_den = nullptr, *sc_res = nullptr;
+    uint32_t *cand = nullptr;                        // nscene x kmax
+    uint32_t *patch_scene = nullptr;                 // npatch
+    uint32_t *pass = nullptr;                        // passes selected so far
+    uint32_t *slot_tbeg = nullptr, *slot_tcnt = nullptr, *slot_pbeg = nullptr; // K
+    uint32_t *shooters = nullptr, *nshoot = nullptr; // the context's own
+};
+
 // Launchers implemented in rad_kernels.hip; every launch goes on `st`.
 void launch_dff(const Geometry &g, float *dff, hipStream_t st);
 void launch_select_top1(uint32_t mode_rule, const float *unshot, const float *area,
@@ -92,12 +109,13 @@ void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, u
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                         hipStream_t st);
+                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt, hipStream_t st);
 void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
                 hipStream_t st);
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
-                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
-                 uint32_t *hits, uint32_t *counters, hipStream_t st);
+                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t rowlen,
+                 const uint32_t *slot_pbeg, float *ff, uint32_t *hits, uint32_t *counters,
+                 hipStream_t st);
 void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot,
                    const uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
                    uint32_t *hits, uint32_t *counters, hipStream_t st);
@@ -111,6 +129,13 @@ void launch_split_keys(const uint64_t *keys, uint32_t *ids, float *depth, size_t
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);
+
+// Launchers implemented in rad_pack.hip.
+void launch_pack_select(const PackBuffers &pk, const float *unshot, const float *area,
+                        hipStream_t st);
+void launch_pack_shoot(const PackBuffers &pk, uint32_t mode, const float *snap, const float *ff,
+                       uint32_t rowlen, const float *refl, const float *area, uint32_t npatch,
+                       float *unshot, float *shot, hipStream_t st);
 
 // Launchers implemented in rad_gather.hip.
 void launch_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, uint32_t *nshoot,

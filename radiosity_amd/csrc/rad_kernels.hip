@@ -604,11 +604,15 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
 //    misses the triangle's bounding sphere by more than a margin.
 // Triangles are stored as 9 planes of ntri floats (v0.x, v0.y, ..., v2.z) so
 // consecutive lanes load consecutive addresses.
+// In a packed launch slot s draws only its scene's triangles
+// [slot_tbeg[s], slot_tbeg[s] + slot_tcnt[s]); blockIdx.x walks that range.
 __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *__restrict__ tri,
                                                       const uint32_t *__restrict__ tri_patch,
                                                       uint32_t ntri, const float *__restrict__ mvp,
                                                       const uint32_t *__restrict__ nshoot,
-                                                      BinCtx bc, uint32_t *__restrict__ blk_nrec)
+                                                      BinCtx bc, uint32_t *__restrict__ blk_nrec,
+                                                      const uint32_t *__restrict__ slot_tbeg,
+                                                      const uint32_t *__restrict__ slot_tcnt)
 {
     __shared__ uint32_t lcnt[kMaxTilesPerSlab];
     __shared__ float lv[9][256];
@@ -618,6 +622,13 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     uint32_t s = blockIdx.y;
     if (s >= *nshoot)
         return;
+    const uint32_t tb = slot_tbeg ? slot_tbeg[s] : 0u, tc = slot_tcnt ? slot_tcnt[s] : ntri;
+    if (blockIdx.x * blockDim.x >= tc) {
+        // beyond this slot's scene: an empty region for the bin fill
+        if (threadIdx.x == 0)
+            blk_nrec[(s * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x] = 0u;
+        return;
+    }
     for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
         lcnt[i] = 0u;
     if (threadIdx.x == 0) {
@@ -634,8 +645,9 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     int f0 = gridDim.z == 1 ? 0 : (int)blockIdx.z, f1 = gridDim.z == 1 ? 5 : f0 + 1;
     const float *sm = mvp + (size_t)s * kShooterStride;
     // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ntri) {
+    const uint32_t tl0 = blockIdx.x * blockDim.x;
+    if (tl0 + threadIdx.x < tc) {
+        const uint32_t t = tb + tl0 + threadIdx.x;
         float vx[3][3];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
@@ -665,7 +677,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 vx[k][c] = lv[3 * k + c][tl];
-        setup_face(sm + face * 16, vx, W, s, face, lid[tl], blockIdx.x * blockDim.x + tl, bc);
+        setup_face(sm + face * 16, vx, W, s, face, lid[tl], tb + tl0 + tl, bc);
     }
     __syncthreads();
     // flush: one global atomic per touched tile, and the block's record count
@@ -858,8 +870,11 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
 // Wave-level ProcessHemicube on 64 consecutive texels (any grouping of equal
 // ids gives the same sums; consecutive texels make runs long).  Every lane
 // of the wavefront must call it.
-__device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t npatch,
-                                          float *__restrict__ ff, uint32_t *__restrict__ hits,
+// F row entries are id - 1 - pbase for ids in [pbase + 1, pbase + rowlen]
+// (pbase: first patch of the shooter's scene in a packed launch, else 0).
+__device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t pbase,
+                                          uint32_t rowlen, float *__restrict__ ff,
+                                          uint32_t *__restrict__ hits,
                                           uint32_t *__restrict__ counters)
 {
     int lane = threadIdx.x & 63;
@@ -885,9 +900,9 @@ __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uin
     uint32_t nid = __shfl_down(id, 1), nhi = __shfl_down(hi, 1);
     bool tail = lane == 63 || nid != id || nhi != hi;
     if (tail && id != 0) {
-        uint32_t p = id - 1;
-        if (p < npatch) {
-            size_t o = (size_t)hi * npatch + p;
+        uint32_t p = id - 1 - pbase;
+        if (p < rowlen) {
+            size_t o = (size_t)hi * rowlen + p;
             atomicAdd(&ff[o], f);
             if (hits)
                 atomicAdd(&hits[o], c);
@@ -983,8 +998,8 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     const uint32_t *__restrict__ bin_count, const uint32_t *__restrict__ bin_start,
     const uint32_t *__restrict__ list, uint32_t ent_cap, uint32_t ntx, uint32_t ntiles,
     uint32_t tshift, uint32_t small_px, uint64_t *__restrict__ keys, const float *__restrict__ dff,
-    uint32_t npatch, float *__restrict__ ff, uint32_t *__restrict__ hits,
-    uint32_t *__restrict__ counters)
+    uint32_t rowlen, const uint32_t *__restrict__ slot_pbeg, float *__restrict__ ff,
+    uint32_t *__restrict__ hits, uint32_t *__restrict__ counters)
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     __shared__ uint32_t perm[256];
@@ -1066,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_tile_raster(
         if (ff) {
             uint32_t id = (uint32_t)key;
             float f = id ? dff[texel] : 0.0f;
-            reduce_wave(id, f, s, npatch, ff, hits, counters);
+            reduce_wave(id, f, s, slot_pbeg ? slot_pbeg[s] : 0u, rowlen, ff, hits, counters);
         }
     }
 }
@@ -1074,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_tile_raster(
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                         hipStream_t st)
+                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt, hipStream_t st)
 {
     BinCtx bc;
     bc.recs = bb.recs;
@@ -1092,7 +1107,8 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
     bc.ntiles = bb.ntiles;
     bc.tshift = bb.tshift;
     dim3 grid(bb.nblk, K, bb.nz);
-    k_raster_setup<<<grid, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec);
+    k_raster_setup<<<grid, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
+                                         slot_tbeg, slot_tcnt);
     bc.lnrec = nullptr;
     k_setup_clipped<<<bb.clip_blocks, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
 }
@@ -1109,13 +1125,14 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
 }
 
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
-                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
-                 uint32_t *hits, uint32_t *counters, hipStream_t st)
+                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t rowlen,
+                 const uint32_t *slot_pbeg, float *ff, uint32_t *hits, uint32_t *counters,
+                 hipStream_t st)
 {
     dim3 tg(bb.ntiles, K);
     k_tile_raster<<<tg, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start,
                                       bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px,
-                                      keys, dff, npatch, ff, hits, counters);
+                                      keys, dff, rowlen, slot_pbeg, ff, hits, counters);
 }
 
 // ----------------------------------------------------------- ProcessHemicube
