@@ -154,6 +154,72 @@ def dense_leg(R, args):
     return out
 
 
+def pack_leg(R, args):
+    """256 independent box rooms, 500-20,000 triangle patches (log-uniform,
+    seed 5), hemicube 64 / 128 / 256 by size, solved to the residual target
+    by scene packs: every kernel of a pass runs once per pack, the packs
+    (one per hemicube class, each up to 256 shooter slots) run on their own
+    streams side by side."""
+    import math
+    import numpy as np
+    rng = np.random.default_rng(5)
+    sizes = np.exp(rng.uniform(math.log(500), math.log(20000), args.pack_scenes)).astype(int)
+    scenes, Ws, ks = [], [], []
+    for i, n in enumerate(sizes):
+        side = 5.0 + 7.0 * n / 20000.0
+        sc = R.Scene.boxes(nboxes=max(2, int(n) // 400), room=(side, 3.0 + n / 10000.0, side),
+                           target_tris=int(n), nlights=4, seed=1000 + i)
+        scenes.append(sc)
+        Ws.append(64 if sc.npatch < 1500 else (128 if sc.npatch < 6000 else 256))
+        ks.append(int(min(16, max(1, math.ceil(sc.npatch / 2000)))))
+    packs = []   # (Solver, scene indices)
+    for W in (256, 128, 64):
+        idx = [i for i in range(len(scenes)) if Ws[i] == W]
+        cur, tot = [], 0
+        for i in idx + [None]:
+            if i is None or tot + ks[i] > 256:
+                if cur:
+                    s = R.Solver(hemicube=W, hemicubes=256, graph_passes=8)
+                    s.pack_upload([scenes[j] for j in cur], [ks[j] for j in cur])
+                    packs.append((s, cur))
+                cur, tot = [], 0
+            if i is not None:
+                cur.append(i)
+                tot += ks[i]
+    for s, _ in packs:
+        s.pack_begin(args.target)
+    t0 = time.perf_counter()
+    done_t = np.full(len(scenes), np.nan)
+    shooters = np.zeros(len(scenes), np.int64)
+    live = list(packs)
+    while live and time.perf_counter() - t0 < args.pack_cap_s:
+        for s, _ in live:
+            s.step(16)
+        now_live = []
+        for s, idx in live:
+            done, res, passes, rem = s.pack_status()
+            now = time.perf_counter() - t0
+            for k, i in enumerate(idx):
+                if done[k] != 0xFFFFFFFF and np.isnan(done_t[i]):
+                    done_t[i] = now
+                shooters[i] = s.pack_shooters[k]
+            if rem:
+                now_live.append((s, idx))
+        live = now_live
+    wall = time.perf_counter() - t0
+    finished = ~np.isnan(done_t)
+    out = {"scenes": len(scenes), "patches_total": int(sum(sc.npatch for sc in scenes)),
+           "packs": len(packs), "hemicube_classes": {str(W): Ws.count(W) for W in (64, 128, 256)},
+           "wall_s": wall, "finished": int(finished.sum()),
+           "shooters": int(shooters.sum()), "shooters_per_s": float(shooters.sum() / wall),
+           "completion_s_p50": float(np.percentile(done_t[finished], 50)) if finished.any() else None,
+           "completion_s_p99": float(np.percentile(done_t[finished], 99)) if finished.any() else None}
+    log(f"pack: {out['scenes']} scenes ({out['patches_total']} patches) in {len(packs)} packs: "
+        f"{out['finished']} done in {wall:.2f} s, {out['shooters_per_s']:.0f} shooters/s, "
+        f"completion p50 {out['completion_s_p50']:.2f} s p99 {out['completion_s_p99']:.2f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +239,8 @@ def main():
     ap.add_argument("--graph-passes", type=int, default=8,
                     help="passes per captured HIP graph in the batched leg (0: plain launches)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--pack-scenes", type=int, default=256)
+    ap.add_argument("--pack-cap-s", type=float, default=120.0)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the Cornell (GPU vs CPU) and dense 32k x 192 legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -282,7 +350,8 @@ def main():
         from oracle import oracle as O
         configs = {"cornell1500_W128_100": cornell_leg(R, O, 0.3127, 128, 100, 1),
                    "cornell20k_W256_1000": cornell_leg(R, O, 0.02314, 256, 1000, 1),
-                   "dense32k_R192": dense_leg(R, args)}
+                   "dense32k_R192": dense_leg(R, args),
+                   "pack256": pack_leg(R, args)}
 
     # ------------------------------------------------------------ roofline
     # Dominant kernel of the batched pass and its algorithmic bytes per launch

@@ -208,10 +208,11 @@ int rad_pack_upload(rad_ctx *ctx, uint32_t nscene, const uint32_t *scene_tri_beg
  * of rad_set_state; a scene is done once its residual <= target. */
 int rad_pack_begin(rad_ctx *ctx, double target);
 /* Synchronises, then reports per scene the pass it finished in (0xFFFFFFFF:
- * not yet) and its residual at the last selection, the passes run and the
- * scenes still active.  Any argument may be NULL. */
+ * not yet), its residual at the last selection and the shooters it has shot,
+ * the passes run and the scenes still active.  Any pointer may be NULL. */
 int rad_pack_status(rad_ctx *ctx, uint32_t *done_pass_or_null, double *residual_or_null,
-                    uint32_t *passes_or_null, uint32_t *remaining_or_null);
+                    uint32_t *shooters_or_null, uint32_t *passes_or_null,
+                    uint32_t *remaining_or_null);
 
 /* ---------------------------------------------------------------------------
  * Dense form factors and the Jacobi gather (the precision path).

@@ -101,7 +101,7 @@ SIGNATURES = {
     "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
     "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
     "rad_pack_begin": (C.c_int, [_P, C.c_double]),
-    "rad_pack_status": (C.c_int, [_P, _PU, C.POINTER(C.c_double), _PU, _PU]),
+    "rad_pack_status": (C.c_int, [_P, _PU, C.POINTER(C.c_double), _PU, _PU, _PU]),
     "rad_dense_build": (C.c_int, [_P, _u32]),
     "rad_dense_upload": (C.c_int, [_P, _PF]),
     "rad_dense_download": (C.c_int, [_P, _u32, _u32, _PF]),
@@ -360,14 +360,16 @@ class Solver:
 
     def pack_status(self):
         """(done_pass per scene, 0xFFFFFFFF while active; residual per scene;
-        passes; scenes remaining)."""
+        passes; scenes remaining).  Shooters per scene: self.pack_shooters."""
         n = len(self.pack_offsets[1]) - 1
         done = np.zeros(n, np.uint32)
         res = np.zeros(n, np.float64)
+        shot = np.zeros(n, np.uint32)
         passes, rem = C.c_uint32(), C.c_uint32()
         self._chk(lib().rad_pack_status(self._ctx, uptr(done),
-                                        res.ctypes.data_as(C.POINTER(C.c_double)),
+                                        res.ctypes.data_as(C.POINTER(C.c_double)), uptr(shot),
                                         C.byref(passes), C.byref(rem)), "rad_pack_status")
+        self.pack_shooters = shot
         return done, res, passes.value, rem.value
 
     # ------------------------------------------------------ dense / gather

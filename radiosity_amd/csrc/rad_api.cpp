@@ -361,6 +361,7 @@ void rad_destroy(rad_ctx *c)
                     c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts,
                     c->fhi,  c->flo,      c->pk.sc_tbeg, c->pk.sc_pbeg, c->pk.sc_slots,
                     c->pk.sc_slot_beg, c->pk.sc_nsel, c->pk.sc_done, c->pk.sc_den, c->pk.sc_res,
+                    c->pk.sc_shot,
                     c->pk.cand, c->pk.patch_scene, c->pk.pass, c->pk.slot_tbeg, c->pk.slot_tcnt,
                     c->pk.slot_pbeg};
     for (void *p : ptrs)
@@ -406,8 +407,8 @@ void pack_free(rad_ctx *c)
 {
     PackBuffers &p = c->pk;
     void *ptrs[] = {p.sc_tbeg, p.sc_pbeg, p.sc_slots, p.sc_slot_beg, p.sc_nsel, p.sc_done, p.sc_den,
-                    p.sc_res,  p.cand,    p.patch_scene, p.pass,     p.slot_tbeg, p.slot_tcnt,
-                    p.slot_pbeg};
+                    p.sc_res,  p.sc_shot, p.cand,    p.patch_scene, p.pass,     p.slot_tbeg,
+                    p.slot_tcnt, p.slot_pbeg};
     for (void *q : ptrs)
         dfree(q);
     c->pk = PackBuffers{};
@@ -1357,6 +1358,7 @@ int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
         (rc = dalloc(c, &p.sc_slots, S)) || (rc = dalloc(c, &p.sc_slot_beg, S)) ||
         (rc = dalloc(c, &p.sc_nsel, S)) || (rc = dalloc(c, &p.sc_done, S)) ||
         (rc = dalloc(c, &p.sc_den, S)) || (rc = dalloc(c, &p.sc_res, S)) ||
+        (rc = dalloc(c, &p.sc_shot, S)) ||
         (rc = dalloc(c, &p.cand, S * kmax)) || (rc = dalloc(c, &p.patch_scene, npatch)) ||
         (rc = dalloc(c, &p.pass, 1)) || (rc = dalloc(c, &p.slot_tbeg, c->K)) ||
         (rc = dalloc(c, &p.slot_tcnt, c->K)) || (rc = dalloc(c, &p.slot_pbeg, c->K))) {
@@ -1372,6 +1374,7 @@ int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
     HIPCHK(c, hipMemset(p.sc_done, 0xFF, S * 4));
     HIPCHK(c, hipMemset(p.sc_den, 0, S * 8));
     HIPCHK(c, hipMemset(p.sc_res, 0, S * 8));
+    HIPCHK(c, hipMemset(p.sc_shot, 0, S * 4));
     HIPCHK(c, hipMemset(p.pass, 0, 4));
     HIPCHK(c, hipMemset(p.slot_tbeg, 0, c->K * 4));
     HIPCHK(c, hipMemset(p.slot_tcnt, 0, c->K * 4));
@@ -1408,14 +1411,15 @@ int rad_pack_begin(rad_ctx *c, double target)
                       ((double)em[3 * (size_t)i] + (double)em[3 * (size_t)i + 1] + (double)em[3 * (size_t)i + 2]);
     HIPCHK(c, hipMemcpy(p.sc_den, den.data(), S * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(p.sc_done, 0xFF, S * 4));
+    HIPCHK(c, hipMemset(p.sc_shot, 0, S * 4));
     HIPCHK(c, hipMemset(p.pass, 0, 4));
     p.target = target;
     drop_graph(c); // the target is a kernel argument of captured passes
     return RAD_OK;
 }
 
-int rad_pack_status(rad_ctx *c, uint32_t *done_pass, double *residual, uint32_t *passes,
-                    uint32_t *remaining)
+int rad_pack_status(rad_ctx *c, uint32_t *done_pass, double *residual, uint32_t *shooters,
+                    uint32_t *passes, uint32_t *remaining)
 {
     if (!c)
         return RAD_E_ARG;
@@ -1432,6 +1436,8 @@ int rad_pack_status(rad_ctx *c, uint32_t *done_pass, double *residual, uint32_t 
         memcpy(done_pass, done.data(), p.nscene * 4);
     if (residual)
         HIPCHK(c, hipMemcpy(residual, p.sc_res, p.nscene * 8, hipMemcpyDeviceToHost));
+    if (shooters)
+        HIPCHK(c, hipMemcpy(shooters, p.sc_shot, p.nscene * 4, hipMemcpyDeviceToHost));
     if (passes)
         HIPCHK(c, hipMemcpy(passes, p.pass, 4, hipMemcpyDeviceToHost));
     if (remaining) {

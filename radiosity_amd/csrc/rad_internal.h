@@ -78,6 +78,7 @@ struct PackBuffers {
     uint32_t *sc_slot_beg = nullptr, *sc_nsel = nullptr;
     uint32_t *sc_done = nullptr;                     // pass a scene finished in (~0u: not yet)
     double *sc_den = nullptr, *sc_res = nullptr;
+    uint32_t *sc_shot = nullptr;                     // shooters shot per scene
     uint32_t *cand = nullptr;                        // nscene x kmax
     uint32_t *patch_scene = nullptr;                 // npatch
     uint32_t *pass = nullptr;                        // passes selected so far

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void k_pack_select(
     const double *__restrict__ sc_den, const float *__restrict__ unshot,
     const float *__restrict__ area, const uint32_t *__restrict__ pass, double target,
     uint32_t kmax, uint32_t *__restrict__ sc_done, double *__restrict__ sc_res,
-    uint32_t *__restrict__ sc_nsel, uint32_t *__restrict__ cand)
+    uint32_t *__restrict__ sc_nsel, uint32_t *__restrict__ sc_shot, uint32_t *__restrict__ cand)
 {
     __shared__ double dred[4];
     __shared__ uint64_t kred[4];
@@ -87,6 +87,7 @@ __global__ __launch_bounds__(256) void k_pack_select(
     if (threadIdx.x == 0) {
         sc_res[sc] = res;
         sc_nsel[sc] = nsel;
+        sc_shot[sc] += nsel;
     }
 }
 
@@ -195,7 +196,7 @@ void launch_pack_select(const PackBuffers &pk, const float *unshot, const float 
 {
     k_pack_select<<<pk.nscene, 256, 0, st>>>(pk.sc_pbeg, pk.sc_slots, pk.sc_den, unshot, area,
                                              pk.pass, pk.target, pk.kmax, pk.sc_done, pk.sc_res,
-                                             pk.sc_nsel, pk.cand);
+                                             pk.sc_nsel, pk.sc_shot, pk.cand);
     k_pack_slots<<<1, 1024, 0, st>>>(pk.nscene, pk.sc_tbeg, pk.sc_pbeg, pk.sc_nsel, pk.cand,
                                      pk.kmax, pk.K, pk.sc_slot_beg, pk.shooters, pk.slot_tbeg,
                                      pk.slot_tcnt, pk.slot_pbeg, pk.nshoot, pk.pass);
