@@ -241,6 +241,8 @@ int set_tiles(rad_ctx *c, uint32_t T, uint32_t small_px)
     if (c->st)
         (void)hipStreamSynchronize(c->st);
     c->bb.tshift = T == 8 ? 3 : (T == 16 ? 4 : 5);
+    if (!c->bb.setup_group)
+        c->bb.setup_group = c->K >= 8 ? 8u : c->K;
     c->bb.ntx = ntx;
     c->bb.nty = nty;
     c->bb.ntiles = ntx * nty;

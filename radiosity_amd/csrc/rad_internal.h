@@ -62,6 +62,7 @@ struct BinBuffers {
     uint32_t clip_cap;
     int clip_blocks;
     uint32_t ntx, nty, ntiles, tshift;
+    uint32_t setup_group;  // shooter slots one setup workgroup walks
 };
 
 struct Geometry {

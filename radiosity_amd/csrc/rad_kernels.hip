@@ -612,81 +612,96 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                                                       const uint32_t *__restrict__ nshoot,
                                                       BinCtx bc, uint32_t *__restrict__ blk_nrec,
                                                       const uint32_t *__restrict__ slot_tbeg,
-                                                      const uint32_t *__restrict__ slot_tcnt)
+                                                      const uint32_t *__restrict__ slot_tcnt,
+                                                      uint32_t group, uint32_t nslot)
 {
     __shared__ uint32_t lcnt[kMaxTilesPerSlab];
     __shared__ float lv[9][256];
     __shared__ uint32_t lid[256];
     __shared__ uint16_t lq[256 * 5];
     __shared__ uint32_t lnrec, lqn;
-    uint32_t s = blockIdx.y;
-    if (s >= *nshoot)
-        return;
-    const uint32_t tb = slot_tbeg ? slot_tbeg[s] : 0u, tc = slot_tcnt ? slot_tcnt[s] : ntri;
-    if (blockIdx.x * blockDim.x >= tc) {
-        // beyond this slot's scene: an empty region for the bin fill
-        if (threadIdx.x == 0)
-            blk_nrec[(s * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x] = 0u;
-        return;
-    }
-    for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
-        lcnt[i] = 0u;
-    if (threadIdx.x == 0) {
-        lnrec = 0u;
-        lqn = 0u;
-    }
-    __syncthreads();
-    bc.lcnt = lcnt;
-    bc.lnrec = &lnrec;
+    const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
+    const uint32_t tl0 = blockIdx.x * blockDim.x;
     // gridDim.z == 5 splits the faces over blocks (small K), else one block
     // walks all five faces
-    const uint32_t rb = (s * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x;
-    bc.region = rb * kRecsPerBlock;
-    int f0 = gridDim.z == 1 ? 0 : (int)blockIdx.z, f1 = gridDim.z == 1 ? 5 : f0 + 1;
-    const float *sm = mvp + (size_t)s * kShooterStride;
-    // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
-    const uint32_t tl0 = blockIdx.x * blockDim.x;
-    if (tl0 + threadIdx.x < tc) {
-        const uint32_t t = tb + tl0 + threadIdx.x;
-        float vx[3][3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                vx[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
-                lv[3 * k + c][threadIdx.x] = vx[k][c];
-            }
-        lid[threadIdx.x] = tri_patch[t] + 1u;
-        uint32_t fm = cull_faces(sm, vx, f0, f1);
-        while (fm) {
-            int f = __ffs(fm) - 1;
-            fm &= fm - 1;
-            lq[atomicAdd(&lqn, 1u)] = (uint16_t)((threadIdx.x << 3) | (uint32_t)f);
+    const int f0 = gridDim.z == 1 ? 0 : (int)blockIdx.z, f1 = gridDim.z == 1 ? 5 : f0 + 1;
+    bc.lcnt = lcnt;
+    bc.lnrec = &lnrec;
+    // The block's triangles stay in registers / LDS while it walks `group`
+    // consecutive shooter slots (reloaded only when a packed slot changes
+    // scene), so vertices are read once per group instead of once per shooter.
+    uint32_t ltb = 0xFFFFFFFFu, ltc = 0;
+    float vx[3][3];
+    for (uint32_t g = 0; g < group; ++g) {
+        const uint32_t s = blockIdx.y * group + g;
+        if (s >= ns)
+            break;
+        const uint32_t tb = slot_tbeg ? slot_tbeg[s] : 0u, tc = slot_tcnt ? slot_tcnt[s] : ntri;
+        const uint32_t rb = (s * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x;
+        if (tl0 >= tc) {
+            // beyond this slot's scene: an empty region for the bin fill
+            if (threadIdx.x == 0)
+                blk_nrec[rb] = 0u;
+            continue;
         }
-    }
-    __syncthreads();
-    // phase 2: the exact per-face pipeline over the compacted pairs
-    const uint32_t nq = lqn;
-    for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
-        uint32_t e = lq[q];
-        uint32_t tl = e >> 3;
-        int face = (int)(e & 7u);
-        float vx[3][3];
+        __syncthreads(); // the previous slot's phase 2 and flush are done with LDS
+        if (tb != ltb || tc != ltc) {
+            ltb = tb;
+            ltc = tc;
+            if (tl0 + threadIdx.x < tc) {
+                const uint32_t t = tb + tl0 + threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+                for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                vx[k][c] = lv[3 * k + c][tl];
-        setup_face(sm + face * 16, vx, W, s, face, lid[tl], tb + tl0 + tl, bc);
+                    for (int c = 0; c < 3; ++c) {
+                        vx[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
+                        lv[3 * k + c][threadIdx.x] = vx[k][c];
+                    }
+                lid[threadIdx.x] = tri_patch[t] + 1u;
+            }
+        }
+        for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
+            lcnt[i] = 0u;
+        if (threadIdx.x == 0) {
+            lnrec = 0u;
+            lqn = 0u;
+        }
+        __syncthreads();
+        bc.region = rb * kRecsPerBlock;
+        const float *sm = mvp + (size_t)s * kShooterStride;
+        // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
+        if (tl0 + threadIdx.x < tc) {
+            uint32_t fm = cull_faces(sm, vx, f0, f1);
+            while (fm) {
+                int f = __ffs(fm) - 1;
+                fm &= fm - 1;
+                lq[atomicAdd(&lqn, 1u)] = (uint16_t)((threadIdx.x << 3) | (uint32_t)f);
+            }
+        }
+        __syncthreads();
+        // phase 2: the exact per-face pipeline over the compacted pairs
+        const uint32_t nq = lqn;
+        for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
+            uint32_t e = lq[q];
+            uint32_t tl = e >> 3;
+            int face = (int)(e & 7u);
+            float wv[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    wv[k][c] = lv[3 * k + c][tl];
+            setup_face(sm + face * 16, wv, W, s, face, lid[tl], tb + tl0 + tl, bc);
+        }
+        __syncthreads();
+        // flush: one global atomic per touched tile, and the region's record count
+        uint32_t *cnt = bc.bin_count + (size_t)s * bc.ntiles;
+        for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
+            if (lcnt[i])
+                atomicAdd(&cnt[i], lcnt[i]);
+        if (threadIdx.x == 0)
+            blk_nrec[rb] = lnrec;
     }
-    __syncthreads();
-    // flush: one global atomic per touched tile, and the block's record count
-    uint32_t *cnt = bc.bin_count + (size_t)s * bc.ntiles;
-    for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
-        if (lcnt[i])
-            atomicAdd(&cnt[i], lcnt[i]);
-    if (threadIdx.x == 0)
-        blk_nrec[rb] = lnrec;
 }
 
 // Conservative culls of one triangle for one shooter; returns the mask of
@@ -1106,9 +1121,12 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
     bc.ntx = bb.ntx;
     bc.ntiles = bb.ntiles;
     bc.tshift = bb.tshift;
-    dim3 grid(bb.nblk, K, bb.nz);
+    uint32_t group = bb.setup_group ? bb.setup_group : 1u;
+    if (group > K)
+        group = K;
+    dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
     k_raster_setup<<<grid, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
-                                         slot_tbeg, slot_tcnt);
+                                         slot_tbeg, slot_tcnt, group, K);
     bc.lnrec = nullptr;
     k_setup_clipped<<<bb.clip_blocks, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
 }
@@ -1544,6 +1562,102 @@ __global__ __launch_bounds__(256) void k_topk_pass(const float *__restrict__ uns
     }
 }
 
+// ---- K <= 64: one sorted list of 64 keys per wavefront, one key per lane
+// (lane 0 largest).  A batch of 64 new keys is sorted with a shuffle bitonic
+// network and merged into the list (max against the reversed batch gives a
+// bitonic sequence holding the top 64 of both; a half-cleaner pass sorts
+// it).  Waves scan the patches, then one workgroup merges the wave lists.
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
+{
+    uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
+{
+    uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_sort_desc(uint64_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            uint64_t o = shfl_xor_u64(v, stride);
+            bool take_max = ((lane & stride) == 0) == ((lane & size) == 0);
+            v = take_max ? (v > o ? v : o) : (v < o ? v : o);
+        }
+    return v;
+}
+
+// top and b sorted descending over the lanes: the top 64 of both, sorted.
+__device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t r = shfl_u64(b, 63 - lane);
+    uint64_t v = top > r ? top : r;
+#pragma unroll
+    for (int stride = 32; stride > 0; stride >>= 1) {
+        uint64_t o = shfl_xor_u64(v, stride);
+        v = (lane & stride) == 0 ? (v > o ? v : o) : (v < o ? v : o);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_topk64_scan(const float *__restrict__ unshot,
+                                                     const float *__restrict__ area,
+                                                     uint32_t npatch, uint64_t *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+    uint64_t top = 0;
+    for (uint32_t base = gw * 64; base < npatch; base += nw * 64) {
+        uint32_t j = base + lane;
+        uint64_t key = j < npatch ? sel_key(RAD_SELECT_POWER, unshot, area, j) : 0ull;
+        if (__ballot(key != 0) == 0)
+            continue;
+        top = wave_merge_desc(top, wave_sort_desc(key));
+    }
+    out[(size_t)gw * 64 + lane] = top;
+}
+
+// Merges nlists sorted lists of 64 (in[l * 64 + lane]) per workgroup; the
+// last workgroup level (gridDim.x == 1) writes the shooters.
+__global__ __launch_bounds__(1024) void k_topk64_merge(const uint64_t *__restrict__ in,
+                                                       uint32_t nlists, uint32_t per_block,
+                                                       uint32_t k, uint64_t *__restrict__ out,
+                                                       uint32_t *__restrict__ shooters,
+                                                       uint32_t *__restrict__ nshoot)
+{
+    __shared__ uint64_t wl[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t l0 = blockIdx.x * per_block;
+    const uint32_t l1 = l0 + per_block < nlists ? l0 + per_block : nlists;
+    uint64_t top = 0;
+    for (uint32_t l = l0 + w; l < l1; l += nw)
+        top = wave_merge_desc(top, in[(size_t)l * 64 + lane]);
+    wl[w][lane] = top;
+    __syncthreads();
+    if (w != 0)
+        return;
+    for (int q = 1; q < nw; ++q)
+        top = wave_merge_desc(top, wl[q][lane]);
+    if (shooters) {
+        bool ok = (uint32_t)lane < k && (top >> 32) != 0;
+        if ((uint32_t)lane < k)
+            shooters[lane] = ok ? 0xFFFFFFFFu - (uint32_t)top : 0u;
+        uint64_t m = __ballot(ok);
+        if (lane == 0)
+            *nshoot = (uint32_t)__popcll(m);
+    } else {
+        out[(size_t)blockIdx.x * 64 + lane] = top;
+    }
+}
+
 void launch_select_topk(uint32_t rule, const float *unshot, const float *area, const float *emis,
                         uint32_t npatch, uint32_t k, uint32_t *shooters, uint32_t *nshoot,
                         uint64_t *cand, double *partials, uint32_t *counter, double *resid,
@@ -1554,6 +1668,28 @@ void launch_select_topk(uint32_t rule, const float *unshot, const float *area, c
     (void)partials;
     (void)counter;
     (void)resid;
+    if (k <= 64) {
+        // wave lists: about four batches of 64 patches per wave
+        uint32_t waves = (npatch + 255) / 256;
+        if (waves < 4)
+            waves = 4;
+        uint32_t blocks = (waves + 3) / 4;
+        if ((size_t)blocks * 4 * 64 > topk_cand_half(npatch, k))
+            blocks = (uint32_t)(topk_cand_half(npatch, k) / 256);
+        uint64_t *a = cand, *b = cand + topk_cand_half(npatch, k);
+        k_topk64_scan<<<blocks, 256, 0, st>>>(unshot, area, npatch, a);
+        uint32_t nl = blocks * 4;
+        while (nl > 256) {
+            uint32_t per = 64, nb = (nl + per - 1) / per;
+            k_topk64_merge<<<nb, 1024, 0, st>>>(a, nl, per, k, b, nullptr, nullptr);
+            uint64_t *t = a;
+            a = b;
+            b = t;
+            nl = nb;
+        }
+        k_topk64_merge<<<1, 1024, 0, st>>>(a, nl, nl, k, nullptr, shooters, nshoot);
+        return;
+    }
     // cand holds two ping-pong halves of topk_cand_size(npatch, k) entries
     uint32_t n = npatch;
     const uint64_t *in = nullptr;
@@ -1576,7 +1712,11 @@ void launch_select_topk(uint32_t rule, const float *unshot, const float *area, c
 size_t topk_cand_half(uint32_t npatch, uint32_t k)
 {
     size_t nb = ((size_t)npatch + kTopkChunk - 1) / kTopkChunk;
-    return nb * k + 1;
+    size_t bitonic = nb * k + 1;
+    // the wave-list path keeps 64 keys per wave, one wave per 256 patches
+    size_t waves = ((size_t)npatch + 255) / 256;
+    size_t lists = (waves < 4 ? 4 : waves) + 3;
+    return bitonic > lists * 64 ? bitonic : lists * 64;
 }
 
 // ----------------------------------------------- reference rule, K > 1

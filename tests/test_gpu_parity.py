@@ -221,6 +221,25 @@ def test_batched_topk_and_shoot(rad, cornell_t):
     assert _close(gu, u64, scale) and _close(gs, s64, scale)
 
 
+@pytest.mark.parametrize("K", [2, 33, 64, 100, 256])
+def test_power_topk_with_ties(rad, boxes_small, K):
+    """POWER top-K (wave lists for K <= 64, bitonic chunks above) against a
+    lexicographic sort by (-A (r+g+b), index) on a state with many ties and
+    zeros."""
+    sc = boxes_small
+    s = _solver(rad, sc, 16, K=K)
+    rng = np.random.default_rng(K)
+    u = rng.integers(0, 3, size=(sc.npatch, 3)).astype(np.float32)
+    u[rng.random(sc.npatch) < 0.3] = 0
+    s.set_state(u, np.zeros_like(u), u)
+    ids, n = s.select(K)
+    key = sc.area * u.sum(1, dtype=np.float32)
+    order = np.lexsort((np.arange(sc.npatch), -key))
+    want = [int(i) for i in order[:K] if key[i] > 0]
+    assert n == len(want) and list(ids[:n]) == want
+    assert np.all(ids[n:] == 0)
+
+
 def test_reference_rule_batched_matches_list(rad, cornell_q):
     say("reference selection K=10")
     s = _solver(rad, cornell_q, 16, K=10, mode=0, select=0)
