@@ -253,7 +253,7 @@ int set_tiles(rad_ctx *c, uint32_t T, uint32_t small_px)
     size_t nbins = (size_t)c->K * c->bb.ntiles;
     int rc;
     if ((rc = dalloc(c, &c->bb.count, nbins)) || (rc = dalloc(c, &c->bb.start, nbins)) ||
-        (rc = dalloc(c, &c->bb.fill, nbins)))
+        (rc = dalloc(c, &c->bb.fill, nbins)) || (rc = dalloc(c, &c->bb.order, nbins)))
         return rc;
     return RAD_OK;
 }
@@ -360,7 +360,7 @@ void rad_destroy(rad_ctx *c)
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
                     c->resid, c->bb.recs, c->bb.count, c->bb.start, c->bb.fill,
-                    c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts,
+                    c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts, c->bb.order,
                     c->fhi,  c->flo,      c->pk.sc_tbeg, c->pk.sc_pbeg, c->pk.sc_slots,
                     c->pk.sc_slot_beg, c->pk.sc_nsel, c->pk.sc_done, c->pk.sc_den, c->pk.sc_res,
                     c->pk.sc_shot,

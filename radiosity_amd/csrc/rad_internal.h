@@ -55,6 +55,7 @@ struct BinBuffers {
     uint64_t *bbox;        // texel bbox per record slot, read by the bin fill
     uint32_t *blk_nrec;    // records per (shooter, setup block)
     uint32_t *count, *start, *fill, *list;
+    uint32_t *order;       // bins by descending size class (tile launch order)
     uint32_t nblk;         // setup blocks per shooter and face group (256 triangles each)
     uint32_t nz;           // face groups: 5 (one face per block, small K) or 1
     uint32_t spill_base, spill_cap, ent_cap;

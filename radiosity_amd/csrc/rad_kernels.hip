@@ -769,33 +769,72 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
 // Exclusive scan of n bin counts by one block of 1024 threads (each thread
 // scans a contiguous chunk); writes start offsets, resets the fill counters and
 // stores the total in counters[kCntEntries].
+// Also orders the bins for the tile kernel by descending size class
+// (floor(log2(count)) buckets, a counting sort): the largest bins start
+// first, so the heavy horizon tiles overlap the light ones instead of
+// forming the tail of the launch.
 __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ cnt,
                                                    uint32_t *__restrict__ start,
                                                    uint32_t *__restrict__ fill, uint32_t n,
-                                                   uint32_t *__restrict__ counters)
+                                                   uint32_t *__restrict__ counters,
+                                                   uint32_t *__restrict__ order)
 {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t bucket[33], boff[33];
+    if (threadIdx.x < 33)
+        bucket[threadIdx.x] = 0u;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t per = (n + blockDim.x - 1) / blockDim.x;
     uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
     uint32_t sum = 0;
     for (uint32_t i = b0; i < b1; ++i)
         sum += cnt[i];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    // inclusive scan inside the wave, then over the wave totals
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d)
+            inc += o;
     }
-    uint32_t run = part[threadIdx.x] - sum;
+    if (lane == 63)
+        wsum[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t v = lane < (blockDim.x >> 6) ? wsum[lane] : 0u, x = v;
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            uint32_t o = __shfl_up(x, d, 64);
+            if (lane >= (uint32_t)d)
+                x += o;
+        }
+        if (lane < 16)
+            wsum[lane] = x - v; // exclusive prefix of the wave totals
+        if (lane == (blockDim.x >> 6) - 1)
+            counters[kCntEntries] = x;
+    }
+    __syncthreads();
+    uint32_t run = wsum[w] + inc - sum;
     for (uint32_t i = b0; i < b1; ++i) {
+        uint32_t c = cnt[i];
         start[i] = run;
         fill[i] = 0u;
-        run += cnt[i];
+        run += c;
+        atomicAdd(&bucket[c ? 32u - __clz(c) : 0u], 1u);
     }
-    if (threadIdx.x == blockDim.x - 1)
-        counters[kCntEntries] = part[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int k = 32; k >= 0; --k) {
+            boff[k] = acc;
+            acc += bucket[k];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = b0; i < b1; ++i) {
+        uint32_t c = cnt[i];
+        order[atomicAdd(&boff[c ? 32u - __clz(c) : 0u], 1u)] = i;
+    }
 }
 
 // Same grid as k_raster_setup: block (blk, s) puts the records of its region
@@ -820,65 +859,73 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
                                                   uint32_t *__restrict__ fill,
                                                   uint32_t *__restrict__ list, uint32_t ent_cap,
                                                   uint32_t ntx, uint32_t ntiles, uint32_t tshift,
-                                                  const uint32_t *__restrict__ nshoot)
+                                                  const uint32_t *__restrict__ nshoot,
+                                                  uint32_t group, uint32_t nslot)
 {
     __shared__ uint32_t lcnt[kMaxTilesPerSlab];
     __shared__ uint32_t lbase[kMaxTilesPerSlab];
-    const uint32_t s = blockIdx.y;
-    if (s >= *nshoot)
-        return;
-    uint32_t *fl = fill + (size_t)s * ntiles;
-    const uint32_t *stt = start + (size_t)s * ntiles;
-    if (blockIdx.x == nblk) {
-        if (blockIdx.z != 0)
-            return;
-        // spill records of this shooter: one global atomic each (rare)
-        uint32_t ns = min(counters[kCntSpill], spill_cap);
-        for (uint32_t q = threadIdx.x; q < ns; q += blockDim.x) {
-            uint32_t r = spill_base + q;
-            if ((recs[r].sf >> 3) != s)
-                continue;
+    const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
+    // the workgroup walks `group` consecutive shooter slots, like the setup
+    for (uint32_t g = 0; g < group; ++g) {
+        const uint32_t s = blockIdx.y * group + g;
+        if (s >= ns)
+            break;
+        uint32_t *fl = fill + (size_t)s * ntiles;
+        const uint32_t *stt = start + (size_t)s * ntiles;
+        if (blockIdx.x == nblk) {
+            if (blockIdx.z != 0)
+                return;
+            // spill records of this shooter: one global atomic each (rare)
+            uint32_t nsp = min(counters[kCntSpill], spill_cap);
+            for (uint32_t q = threadIdx.x; q < nsp; q += blockDim.x) {
+                uint32_t r = spill_base + q;
+                if ((recs[r].sf >> 3) != s)
+                    continue;
+                uint32_t g0, g1, h0, h1;
+                unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
+                for (uint32_t gy = h0; gy <= h1; ++gy)
+                    for (uint32_t gx = g0; gx <= g1; ++gx) {
+                        uint32_t b = gy * ntx + gx;
+                        uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
+                        if (pos < ent_cap)
+                            list[pos] = r;
+                    }
+            }
+            continue;
+        }
+        const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
+        const uint32_t region = blk * kRecsPerBlock, nrec = blk_nrec[blk];
+        if (nrec == 0)
+            continue;
+        __syncthreads(); // the previous slot is done with lcnt / lbase
+        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x)
+            lcnt[i] = 0u;
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
             uint32_t g0, g1, h0, h1;
-            unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
+            unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
+            for (uint32_t gy = h0; gy <= h1; ++gy)
+                for (uint32_t gx = g0; gx <= g1; ++gx)
+                    atomicAdd(&lcnt[gy * ntx + gx], 1u);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
+            uint32_t c = lcnt[i];
+            lbase[i] = c ? stt[i] + atomicAdd(&fl[i], c) : 0u;
+            lcnt[i] = 0u;
+        }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
+            uint32_t g0, g1, h0, h1;
+            unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx) {
                     uint32_t b = gy * ntx + gx;
-                    uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
+                    uint32_t pos = lbase[b] + atomicAdd(&lcnt[b], 1u);
                     if (pos < ent_cap)
-                        list[pos] = r;
+                        list[pos] = region + q;
                 }
         }
-        return;
-    }
-    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x)
-        lcnt[i] = 0u;
-    __syncthreads();
-    const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
-    const uint32_t region = blk * kRecsPerBlock, nrec = blk_nrec[blk];
-    for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
-        uint32_t g0, g1, h0, h1;
-        unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
-        for (uint32_t gy = h0; gy <= h1; ++gy)
-            for (uint32_t gx = g0; gx <= g1; ++gx)
-                atomicAdd(&lcnt[gy * ntx + gx], 1u);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
-        uint32_t c = lcnt[i];
-        lbase[i] = c ? stt[i] + atomicAdd(&fl[i], c) : 0u;
-        lcnt[i] = 0u;
-    }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
-        uint32_t g0, g1, h0, h1;
-        unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
-        for (uint32_t gy = h0; gy <= h1; ++gy)
-            for (uint32_t gx = g0; gx <= g1; ++gx) {
-                uint32_t b = gy * ntx + gx;
-                uint32_t pos = lbase[b] + atomicAdd(&lcnt[b], 1u);
-                if (pos < ent_cap)
-                    list[pos] = region + q;
-            }
     }
 }
 
@@ -1014,15 +1061,17 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     const uint32_t *__restrict__ list, uint32_t ent_cap, uint32_t ntx, uint32_t ntiles,
     uint32_t tshift, uint32_t small_px, uint64_t *__restrict__ keys, const float *__restrict__ dff,
     uint32_t rowlen, const uint32_t *__restrict__ slot_pbeg, float *__restrict__ ff,
-    uint32_t *__restrict__ hits, uint32_t *__restrict__ counters)
+    uint32_t *__restrict__ hits, uint32_t *__restrict__ counters,
+    const uint32_t *__restrict__ order)
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     __shared__ uint32_t perm[256];
     __shared__ uint32_t cls_n[9], cls_off[9];
-    const uint32_t s = blockIdx.y;
+    const uint32_t bin = order[blockIdx.x];
+    const uint32_t s = bin / ntiles;
     if (s >= *nshoot)
         return;
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = bin - s * ntiles;
     const uint32_t T = 1u << tshift, TT = T * T;
     const int32_t ox = (int32_t)((tile % ntx) << tshift), oy = (int32_t)((tile / ntx) << tshift);
     for (uint32_t p = threadIdx.x; p < TT; p += blockDim.x)
@@ -1135,11 +1184,14 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
                 hipStream_t st)
 {
     uint32_t nb = K * bb.ntiles;
-    k_bin_scan<<<1, 1024, 0, st>>>(bb.count, bb.start, bb.fill, nb, counters);
-    dim3 grid(bb.nblk + 1, K, bb.nz);
+    k_bin_scan<<<1, 1024, 0, st>>>(bb.count, bb.start, bb.fill, nb, counters, bb.order);
+    uint32_t group = bb.setup_group ? bb.setup_group : 1u;
+    if (group > K)
+        group = K;
+    dim3 grid(bb.nblk + 1, (K + group - 1) / group, bb.nz);
     k_bin_fill<<<grid, 256, 0, st>>>(bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base,
                                      bb.spill_cap, bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx,
-                                     bb.ntiles, bb.tshift, nshoot);
+                                     bb.ntiles, bb.tshift, nshoot, group, K);
 }
 
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
@@ -1147,10 +1199,10 @@ void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const Bi
                  const uint32_t *slot_pbeg, float *ff, uint32_t *hits, uint32_t *counters,
                  hipStream_t st)
 {
-    dim3 tg(bb.ntiles, K);
-    k_tile_raster<<<tg, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start,
-                                      bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px,
-                                      keys, dff, rowlen, slot_pbeg, ff, hits, counters);
+    k_tile_raster<<<bb.ntiles * K, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox,
+                                                 bb.count, bb.start, bb.list, bb.ent_cap, bb.ntx,
+                                                 bb.ntiles, bb.tshift, small_px, keys, dff, rowlen,
+                                                 slot_pbeg, ff, hits, counters, bb.order);
 }
 
 // ----------------------------------------------------------- ProcessHemicube
