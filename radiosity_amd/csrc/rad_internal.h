@@ -27,15 +27,17 @@ constexpr uint32_t kRecsPerBlock = 768;
 // Floats per shooter in the MVP buffer: 5 face MVPs + the culling frame.
 constexpr int kShooterStride = 96;
 
-// Setup record of one snapped (sub-)triangle of one face of one shooter.
-struct RasterRec {
+// Setup record of one snapped (sub-)triangle of one face of one shooter:
+// three 16-byte words (one vector load each).  Its texel bbox lives in the
+// parallel bbox array the binning and the tile classification read.
+struct alignas(16) RasterRec {
     int32_t x[3], y[3];   // snapped face-local window coordinates (1/256 px)
     float dzdx, dzdy, z0; // depth plane anchored at vertex 0
     uint32_t id;          // patch + 1
     uint32_t sf;          // shooter slot * 8 + face
-    uint16_t bx0, by0, bx1, by1; // inclusive texel bbox inside the hemicube slab
+    uint32_t pad;
 };
-static_assert(sizeof(RasterRec) == 52, "RasterRec layout");
+static_assert(sizeof(RasterRec) == 48, "RasterRec layout");
 
 // Counters used by one pass (reset by the clear kernel).
 enum {

@@ -418,12 +418,9 @@ __device__ static void emit_tri(const SV &a, const SV &b, const SV &c, uint32_t 
     rr.z0 = a.z;
     rr.id = id;
     rr.sf = s * 8u + (uint32_t)face;
+    rr.pad = 0u;
     // bbox in texel coordinates of the hemicube slab
     int32_t tx0 = fv.vpx + ix0, tx1 = fv.vpx + ix1, ty0 = fv.vpy + iy0, ty1 = fv.vpy + iy1;
-    rr.bx0 = (uint16_t)tx0;
-    rr.by0 = (uint16_t)ty0;
-    rr.bx1 = (uint16_t)tx1;
-    rr.by1 = (uint16_t)ty1;
     bc.recs[r] = rr;
     bc.bbox[r] = (uint64_t)(uint16_t)tx0 | ((uint64_t)(uint16_t)ty0 << 16) |
                  ((uint64_t)(uint16_t)tx1 << 32) | ((uint64_t)(uint16_t)ty1 << 48);
@@ -615,7 +612,8 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                                                       const uint32_t *__restrict__ slot_tcnt,
                                                       uint32_t group, uint32_t nslot)
 {
-    __shared__ uint32_t lcnt[kMaxTilesPerSlab];
+    extern __shared__ uint32_t dyn_lds[];
+    uint32_t *lcnt = dyn_lds; // bc.ntiles per-tile counts (dynamic LDS)
     __shared__ float lv[9][256];
     __shared__ uint32_t lid[256];
     __shared__ uint16_t lq[256 * 5];
@@ -862,8 +860,8 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
                                                   const uint32_t *__restrict__ nshoot,
                                                   uint32_t group, uint32_t nslot)
 {
-    __shared__ uint32_t lcnt[kMaxTilesPerSlab];
-    __shared__ uint32_t lbase[kMaxTilesPerSlab];
+    extern __shared__ uint32_t dyn_lds[];
+    uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + ntiles; // dynamic LDS, 2 x ntiles
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
     // the workgroup walks `group` consecutive shooter slots, like the setup
     for (uint32_t g = 0; g < group; ++g) {
@@ -988,6 +986,54 @@ __device__ static inline void draw_span(const RasterRec &r, const Edges &e, int3
     }
 }
 
+// The whole wavefront draws one record's pixels [ax0,ax1] x [ay0,ay1]
+// (texels, at most one tile) in 8 x 8 blocks, lane = (lane & 7, lane >> 3)
+// inside the block.  The edge values start from the exact 64-bit evaluation
+// at the lane's first pixel and step by 8 pixels with 64-bit adds, so every
+// inside test equals frag_key's; depth is frag_key's expression.
+__device__ static inline void draw_wave(const RasterRec &r, const Edges &e, int32_t vpx,
+                                        int32_t vpy, int32_t ax0, int32_t ax1, int32_t ay0,
+                                        int32_t ay1, int32_t ox, int32_t oy, uint32_t tshift,
+                                        uint64_t *tk)
+{
+    const int32_t lane = (int32_t)(threadIdx.x & 63), lx = lane & 7, ly = lane >> 3;
+    const int32_t px0 = (ax0 - vpx + lx) * kSub + kSub / 2, py0 = (ay0 - vpy + ly) * kSub + kSub / 2;
+    int64_t er[3], sx[3], sy[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        er[k] = (int64_t)e.A[k] * px0 + (int64_t)e.B[k] * py0 + e.C[k] + e.bias[k];
+        sx[k] = (int64_t)e.A[k] * (8 * kSub);
+        sy[k] = (int64_t)e.B[k] * (8 * kSub);
+    }
+    for (int32_t by = 0; ay0 + by <= ay1; by += 8) {
+        const int32_t ty = ay0 + by + ly;
+        const int32_t py = py0 + by * kSub;
+        const float zr = fmaf(r.dzdy, (float)(py - r.y[0]), r.z0);
+        int64_t e0 = er[0], e1 = er[1], e2 = er[2];
+        for (int32_t bx = 0; ax0 + bx <= ax1; bx += 8) {
+            const int32_t tx = ax0 + bx + lx;
+            if (tx <= ax1 && ty <= ay1 && e0 > 0 && e1 > 0 && e2 > 0) {
+                const int32_t px = px0 + bx * kSub;
+                float z = fmaf(r.dzdx, (float)(px - r.x[0]), zr);
+                if (!(z > 0.0f))
+                    z = 0.0f;
+                if (z < 1.0f) {
+                    uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | r.id;
+                    uint32_t pp = (uint32_t)((ty - oy) << tshift) + (uint32_t)(tx - ox);
+                    if (key < tk[pp])
+                        atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
+                }
+            }
+            e0 += sx[0];
+            e1 += sx[1];
+            e2 += sx[2];
+        }
+        er[0] += sy[0];
+        er[1] += sy[1];
+        er[2] += sy[2];
+    }
+}
+
 // One lane draws a record's pixels [ax0,ax1] x [ay0,ay1] (texels).  When the
 // snapped vertices span less than 64 pixels every edge value inside the bbox
 // fits in int32, so the edge functions are stepped with 32-bit adds; the
@@ -1066,6 +1112,7 @@ __global__ __launch_bounds__(256) void k_tile_raster(
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     __shared__ uint32_t perm[256];
+    __shared__ uint64_t pbb[256];
     __shared__ uint32_t cls_n[9], cls_off[9];
     const uint32_t bin = order[blockIdx.x];
     const uint32_t s = bin / ntiles;
@@ -1087,9 +1134,10 @@ __global__ __launch_bounds__(256) void k_tile_raster(
         // class of this lane's record: 0..6 for <= 1, 2, 4 ... 64 pixels,
         // 7 for larger, 8 for no record
         uint32_t i = c0 + me, cls = 8, ri = 0;
+        uint64_t bv = 0;
         if (i < n && st0 + i < ent_cap) {
             ri = list[st0 + i];
-            uint64_t bv = bbox[ri];
+            bv = bbox[ri];
             int32_t ax0 = max((int32_t)(bv & 0xFFFFu), ox);
             int32_t ay0 = max((int32_t)((bv >> 16) & 0xFFFFu), oy);
             int32_t ax1 = min((int32_t)((bv >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
@@ -1107,28 +1155,35 @@ __global__ __launch_bounds__(256) void k_tile_raster(
             }
         }
         __syncthreads();
-        if (cls < 8)
+        if (cls < 8) {
             perm[cls_off[cls] + rank] = ri;
+            pbb[cls_off[cls] + rank] = bv;
+        }
         __syncthreads();
         const uint32_t nsmall = cls_off[7], nall = cls_off[8];
         if (me < nsmall) {
             RasterRec r = recs[perm[me]];
-            int32_t ax0 = max((int32_t)r.bx0, ox), ax1 = min((int32_t)r.bx1, ox + (int32_t)T - 1);
-            int32_t ay0 = max((int32_t)r.by0, oy), ay1 = min((int32_t)r.by1, oy + (int32_t)T - 1);
+            uint64_t bb = pbb[me];
+            int32_t ax0 = max((int32_t)(bb & 0xFFFFu), ox);
+            int32_t ax1 = min((int32_t)((bb >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
+            int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
+            int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
             FaceView fv = face_view(Wi, (int)(r.sf & 7u));
             draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
         }
-        // large records: the whole wavefront per record, one pixel per lane;
-        // wave w takes records nsmall + w, nsmall + w + 4, ...
+        // large records: the whole wavefront per record in 8 x 8 pixel
+        // blocks; wave w takes records nsmall + w, nsmall + w + 4, ...
         for (uint32_t q = nsmall + (me >> 6); q < nall; q += blockDim.x >> 6) {
             RasterRec r = recs[perm[q]];
-            int32_t ax0 = max((int32_t)r.bx0, ox), ax1 = min((int32_t)r.bx1, ox + (int32_t)T - 1);
-            int32_t ay0 = max((int32_t)r.by0, oy), ay1 = min((int32_t)r.by1, oy + (int32_t)T - 1);
-            uint32_t w = (uint32_t)(ax1 - ax0 + 1), area = w * (uint32_t)(ay1 - ay0 + 1);
+            uint64_t bb = pbb[q];
+            int32_t ax0 = max((int32_t)(bb & 0xFFFFu), ox);
+            int32_t ax1 = min((int32_t)((bb >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
+            int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
+            int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
             FaceView fv = face_view(Wi, (int)(r.sf & 7u));
             Edges e;
             make_edges(r.x, r.y, e);
-            draw_span(r, e, fv.vpx, fv.vpy, ax0, ay0, w, area, lane, 64, ox, oy, tshift, tk);
+            draw_wave(r, e, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
         }
         __syncthreads();
     }
@@ -1174,7 +1229,8 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
     if (group > K)
         group = K;
     dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
-    k_raster_setup<<<grid, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
+    k_raster_setup<<<grid, 256, bb.ntiles * sizeof(uint32_t), st>>>(
+        g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
                                          slot_tbeg, slot_tcnt, group, K);
     bc.lnrec = nullptr;
     k_setup_clipped<<<bb.clip_blocks, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
@@ -1189,7 +1245,8 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
     if (group > K)
         group = K;
     dim3 grid(bb.nblk + 1, (K + group - 1) / group, bb.nz);
-    k_bin_fill<<<grid, 256, 0, st>>>(bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base,
+    k_bin_fill<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
+        bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base,
                                      bb.spill_cap, bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx,
                                      bb.ntiles, bb.tshift, nshoot, group, K);
 }
@@ -1319,21 +1376,31 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
         r2 = refl[3 * (size_t)j + 2];
         inv_aj = 1.0f / area[j];
     }
-    for (uint32_t s = 0; s < n; ++s) {
-        float f = ff[(size_t)s * npatch + j];
-        if (f == 0.0f)
-            continue;
-        float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
-        if (mode == RAD_MODE_REFERENCE) {
-            // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
-            u0 += d0 * f * 0.3f * sc[4 * s + 0];
-            u1 += d1 * f * 0.3f * sc[4 * s + 1];
-            u2 += d2 * f * 0.3f * sc[4 * s + 2];
-        } else {
-            float g = f * (sc[4 * s + 3] * inv_aj);
-            u0 += r0 * d0 * g;
-            u1 += r1 * d1 * g;
-            u2 += r2 * d2 * g;
+    // F entries are read 8 shooters at a time (independent loads in flight),
+    // then added in shooter order
+    for (uint32_t s0 = 0; s0 < n; s0 += 8) {
+        float fb[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            fb[q] = s0 + q < n ? ff[(size_t)(s0 + q) * npatch + j] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t s = s0 + q;
+            const float f = fb[q];
+            if (f == 0.0f)
+                continue;
+            float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
+            if (mode == RAD_MODE_REFERENCE) {
+                // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
+                u0 += d0 * f * 0.3f * sc[4 * s + 0];
+                u1 += d1 * f * 0.3f * sc[4 * s + 1];
+                u2 += d2 * f * 0.3f * sc[4 * s + 2];
+            } else {
+                float g = f * (sc[4 * s + 3] * inv_aj);
+                u0 += r0 * d0 * g;
+                u1 += r1 * d1 * g;
+                u2 += r2 * d2 * g;
+            }
         }
     }
     // Main.cpp:1287-1295
