@@ -991,11 +991,12 @@ namespace {
 uint32_t dense_pad(uint32_t n) { return (n + 511u) & ~511u; }
 
 // Split-K factor of the gather product: enough workgroups (128 rows each)
-// to give every CU two, npad / S stays a multiple of 64.
-uint32_t gather_split(uint32_t npad)
+// to give every CU two; with many columns, 8 k slices (one per XCD, so each
+// XCD's L2 holds its slice of X).  npad / S stays a multiple of 64.
+uint32_t gather_split(uint32_t npad, uint32_t R)
 {
     uint32_t rows = npad / 128, S = 1;
-    while (S < 8 && rows * S < 512)
+    while (S < 8 && (rows * S < 512 || (R > 16 && S < 8)))
         S *= 2;
     return S;
 }
@@ -1052,7 +1053,7 @@ int gather_alloc(rad_ctx *c, GatherBufs &g, uint32_t R)
 {
     g.rt = gemm_tiles(R);
     g.rp = g.rt * 16;
-    g.S = gather_split(c->npad);
+    g.S = gather_split(c->npad, R);
     size_t plane = (size_t)g.rp * c->npad;
     int rc;
     if ((rc = dalloc(c, &g.yp, plane * g.S)) || (rc = dalloc(c, &g.E, plane)) ||

@@ -20,6 +20,7 @@ namespace rad {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t bf16_rne(float f)
 {
@@ -176,7 +177,8 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
                                               const uint16_t *__restrict__ flo,
                                               const uint16_t *__restrict__ xhi,
                                               const uint16_t *__restrict__ xlo, uint32_t npad,
-                                              uint32_t kspan, float *__restrict__ yp)
+                                              uint32_t kspan, uint32_t nslice,
+                                              float *__restrict__ yp)
 {
     constexpr int NB = SPLIT ? 2 : 1;
     constexpr int LROW = 72;               // bf16 per LDS row: 64 + 8 pad (144 B)
@@ -186,8 +188,12 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, kg = lane >> 4;
-    const uint32_t i0 = blockIdx.x * 128 + wave * 32;
-    const uint32_t kbeg = blockIdx.y * kspan, kend = kbeg + kspan;
+    // 1-D grid, k slice fastest: consecutive workgroups go to different XCDs
+    // (round robin), so with S dividing 8 every XCD works on one k slice and
+    // keeps that slice of X in its own L2
+    const uint32_t slice = blockIdx.x % nslice, rtile = blockIdx.x / nslice;
+    const uint32_t i0 = rtile * 128 + wave * 32;
+    const uint32_t kbeg = slice * kspan, kend = kbeg + kspan;
 
     f32x4 acc[2][RT];
 #pragma unroll
@@ -214,14 +220,24 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
             }
         }
     };
+    // the matrix is streamed once per product: with several column tiles
+    // non-temporal loads keep it from evicting X out of L2
+    auto ntload = [](const uint16_t *p) {
+        if constexpr (RT > 1) {
+            u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
+            return uint4{v.x, v.y, v.z, v.w};
+        } else {
+            return *(const uint4 *)p;
+        }
+    };
     auto load_a = [&](uint32_t kc) {
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-            ar[0][0][ks] = *(const uint4 *)(fhi + arow0 + kc + ks * 32);
-            ar[0][1][ks] = *(const uint4 *)(fhi + arow1 + kc + ks * 32);
+            ar[0][0][ks] = ntload(fhi + arow0 + kc + ks * 32);
+            ar[0][1][ks] = ntload(fhi + arow1 + kc + ks * 32);
             if (SPLIT) {
-                ar[NB - 1][0][ks] = *(const uint4 *)(flo + arow0 + kc + ks * 32);
-                ar[NB - 1][1][ks] = *(const uint4 *)(flo + arow1 + kc + ks * 32);
+                ar[NB - 1][0][ks] = ntload(flo + arow0 + kc + ks * 32);
+                ar[NB - 1][1][ks] = ntload(flo + arow1 + kc + ks * 32);
             }
         }
     };
@@ -272,7 +288,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
         }
     }
     // lane holds rows 4*kg .. 4*kg+3 of column m of each 16x16 tile
-    float *out = yp + (size_t)blockIdx.y * (RT * 16) * npad;
+    float *out = yp + (size_t)slice * (RT * 16) * npad;
 #pragma unroll
     for (int f = 0; f < 2; ++f)
 #pragma unroll
@@ -459,12 +475,12 @@ template <int RT>
 static void gemm_rt(bool split, const uint16_t *fhi, const uint16_t *flo, const uint16_t *xhi,
                     const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp, hipStream_t st)
 {
-    dim3 grid(npad / 128, S);
+    const uint32_t nwg = (npad / 128) * S;
     uint32_t kspan = npad / S;
     if (split)
-        k_gemm<RT, true><<<grid, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, yp);
+        k_gemm<RT, true><<<nwg, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
     else
-        k_gemm<RT, false><<<grid, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, yp);
+        k_gemm<RT, false><<<nwg, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
 }
 
 void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
