@@ -184,6 +184,17 @@ typedef struct rad_tune_result {
 } rad_tune_result;
 int rad_autotune(rad_ctx *ctx, uint32_t passes, rad_tune_result *out_or_null);
 
+/* Checkpoint / resume of the solver state: the successor of the reference's
+ * .rr save and load (Main.cpp:1460-1647, LoadingModel.cpp:4-26), which dumped
+ * raw Patch structs.  The file is portable: magic "RADCKPT1", version,
+ * patch and triangle counts, shooting mode, a 64-bit FNV-1a hash of the
+ * uploaded scene arrays, the unshot / shot / emission planes (f32, one plane
+ * per channel) and an FNV-1a checksum.  Loading requires the same scene
+ * (hash) and mode and replaces the state as rad_set_state does; a mismatch,
+ * truncation or corruption is RAD_E_ARG with the reason in rad_last_error. */
+int rad_checkpoint_save(rad_ctx *ctx, const char *path);
+int rad_checkpoint_load(rad_ctx *ctx, const char *path);
+
 /* ---------------------------------------------------------------------------
  * Scene packs: many independent scenes solved by one context, every kernel
  * of a pass launched once for all of them.  No counterpart in the reference

@@ -162,6 +162,13 @@ class ModelContainer {
 
     double maxPatchArea;
 
+    // SaveToFile / LoadFromFile of Main.cpp:1460-1647 with a path instead
+    // of the file dialog: the patch fields in the library's portable
+    // checkpoint format (rad_checkpoint_save / rad_checkpoint_load); loading
+    // requires the same scene and writes the fields back.
+    bool saveToFile(const char *path);
+    bool loadFromFile(const char *path);
+
     // library context of the uploaded scene (created by updateData)
     rad_ctx *context();
     void pushState(); // patch fields -> device

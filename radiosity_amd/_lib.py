@@ -102,6 +102,8 @@ SIGNATURES = {
     "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
     "rad_pack_begin": (C.c_int, [_P, C.c_double]),
     "rad_pack_status": (C.c_int, [_P, _PU, C.POINTER(C.c_double), _PU, _PU, _PU]),
+    "rad_checkpoint_save": (C.c_int, [_P, C.c_char_p]),
+    "rad_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
     "rad_dense_build": (C.c_int, [_P, _u32]),
     "rad_dense_upload": (C.c_int, [_P, _PF]),
     "rad_dense_download": (C.c_int, [_P, _u32, _u32, _PF]),
@@ -322,6 +324,12 @@ class Solver:
         n = C.c_uint64()
         self._chk(lib().rad_kernel_times(self._ctx, enable, out, C.byref(n)), "rad_kernel_times")
         return list(out), n.value
+
+    def checkpoint_save(self, path):
+        self._chk(lib().rad_checkpoint_save(self._ctx, os.fsencode(path)), "rad_checkpoint_save")
+
+    def checkpoint_load(self, path):
+        self._chk(lib().rad_checkpoint_load(self._ctx, os.fsencode(path)), "rad_checkpoint_load")
 
     def autotune(self, passes=3):
         """Picks the fastest (tile_px, small_tri_px); returns (tile, small,

@@ -59,6 +59,7 @@ struct rad_ctx {
 
     // F row length: npatch, or the largest scene of a pack
     uint32_t rowlen = 0;
+    uint64_t geom_hash = 0;        // FNV-1a of the uploaded scene arrays (checkpoints)
     PackBuffers pk{};
 
     // captured passes (rad_config.graph_passes of them) replayed by rad_step
@@ -85,6 +86,17 @@ int fail(rad_ctx *c, int code, const std::string &msg)
     if (c)
         c->err = msg;
     return code;
+}
+
+// 64-bit FNV-1a over a byte range (scene identity of a checkpoint).
+uint64_t fnv1a(uint64_t h, const void *p, size_t n)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 0x100000001B3ull;
+    }
+    return h;
 }
 
 // Buffers a captured pass refers to change (scene upload, retiling): the
@@ -491,6 +503,15 @@ int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uin
     c->ntri = ntri;
     c->npatch = npatch;
     c->rowlen = rowlen;
+    uint64_t h = 0xCBF29CE484222325ull;
+    h = fnv1a(h, &ntri, 4);
+    h = fnv1a(h, &npatch, 4);
+    h = fnv1a(h, tri_xyz, (size_t)ntri * 36);
+    h = fnv1a(h, tri_patch, (size_t)ntri * 4);
+    h = fnv1a(h, frames, (size_t)npatch * 36);
+    h = fnv1a(h, refl, (size_t)npatch * 12);
+    h = fnv1a(h, area, (size_t)npatch * 4);
+    c->geom_hash = h;
     c->rendered = false;
     c->dense = false;
     return RAD_OK;
@@ -1450,6 +1471,110 @@ int rad_pack_status(rad_ctx *c, uint32_t *done_pass, double *residual, uint32_t 
         *remaining = r;
     }
     return RAD_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Checkpoints: the solver state in a portable file (the successor of the
+// reference's .rr dump of raw Patch structs, Main.cpp:1460-1647).
+//   "RADCKPT1" | u32 version 1 | u32 npatch | u32 ntri | u32 mode |
+//   u64 scene hash | f32 unshot[3][npatch] | f32 shot[3][npatch] |
+//   f32 emission[3][npatch] | u64 FNV-1a of everything before it
+// Little-endian, channels as separate planes.
+extern "C" {
+
+int rad_checkpoint_save(rad_ctx *c, const char *path)
+{
+    if (!c || !path)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    const size_t n = c->npatch;
+    std::vector<float> u(n * 3), sh(n * 3), em(n * 3), buf(n * 9);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(u.data(), c->unshot, n * 12, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(sh.data(), c->shot, n * 12, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(em.data(), c->emis, n * 12, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i)
+        for (int ch = 0; ch < 3; ++ch) {
+            buf[(0 + ch) * n + i] = u[i * 3 + ch];
+            buf[(3 + ch) * n + i] = sh[i * 3 + ch];
+            buf[(6 + ch) * n + i] = em[i * 3 + ch];
+        }
+    std::vector<unsigned char> out;
+    auto put = [&](const void *p, size_t k) {
+        const unsigned char *b = (const unsigned char *)p;
+        out.insert(out.end(), b, b + k);
+    };
+    const uint32_t ver = 1, np = c->npatch, nt = c->ntri, mode = c->cfg.mode;
+    put("RADCKPT1", 8);
+    put(&ver, 4);
+    put(&np, 4);
+    put(&nt, 4);
+    put(&mode, 4);
+    put(&c->geom_hash, 8);
+    put(buf.data(), buf.size() * 4);
+    uint64_t sum = fnv1a(0xCBF29CE484222325ull, out.data(), out.size());
+    put(&sum, 8);
+    FILE *f = fopen(path, "wb");
+    if (!f)
+        return fail(c, RAD_E_ARG, std::string("checkpoint: cannot open ") + path);
+    size_t w = fwrite(out.data(), 1, out.size(), f);
+    int cl = fclose(f);
+    if (w != out.size() || cl != 0)
+        return fail(c, RAD_E_ARG, std::string("checkpoint: write failed for ") + path);
+    return RAD_OK;
+}
+
+int rad_checkpoint_load(rad_ctx *c, const char *path)
+{
+    if (!c || !path)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    FILE *f = fopen(path, "rb");
+    if (!f)
+        return fail(c, RAD_E_ARG, std::string("checkpoint: cannot open ") + path);
+    std::vector<unsigned char> in;
+    unsigned char tmp[65536];
+    size_t r;
+    while ((r = fread(tmp, 1, sizeof(tmp), f)) > 0)
+        in.insert(in.end(), tmp, tmp + r);
+    fclose(f);
+    const size_t n = c->npatch, head = 8 + 4 * 4 + 8, body = n * 9 * 4;
+    if (in.size() < head + 8 || memcmp(in.data(), "RADCKPT1", 8) != 0)
+        return fail(c, RAD_E_ARG, "checkpoint: not a checkpoint file");
+    uint32_t ver, np, nt, mode;
+    uint64_t hash, sum;
+    memcpy(&ver, &in[8], 4);
+    memcpy(&np, &in[12], 4);
+    memcpy(&nt, &in[16], 4);
+    memcpy(&mode, &in[20], 4);
+    memcpy(&hash, &in[24], 8);
+    if (ver != 1)
+        return fail(c, RAD_E_ARG, "checkpoint: unknown version " + std::to_string(ver));
+    if (np != c->npatch || nt != c->ntri || hash != c->geom_hash)
+        return fail(c, RAD_E_ARG, "checkpoint: written for another scene (patch count, triangle "
+                                  "count or geometry hash differ)");
+    if (in.size() != head + body + 8)
+        return fail(c, RAD_E_ARG, "checkpoint: truncated or oversized file");
+    memcpy(&sum, &in[head + body], 8);
+    if (sum != fnv1a(0xCBF29CE484222325ull, in.data(), head + body))
+        return fail(c, RAD_E_ARG, "checkpoint: checksum mismatch (corrupted file)");
+    if (mode != c->cfg.mode)
+        return fail(c, RAD_E_ARG, "checkpoint: written in another shooting mode");
+    std::vector<float> buf(n * 9), u(n * 3), sh(n * 3), em(n * 3);
+    memcpy(buf.data(), &in[head], body);
+    for (size_t i = 0; i < n; ++i)
+        for (int ch = 0; ch < 3; ++ch) {
+            u[i * 3 + ch] = buf[(0 + ch) * n + i];
+            sh[i * 3 + ch] = buf[(3 + ch) * n + i];
+            em[i * 3 + ch] = buf[(6 + ch) * n + i];
+        }
+    return rad_set_state(c, u.data(), sh.data(), em.data());
 }
 
 } // extern "C"

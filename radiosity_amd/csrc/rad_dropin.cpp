@@ -446,6 +446,26 @@ void ModelContainer::pullState()
     }
 }
 
+bool ModelContainer::saveToFile(const char *path)
+{
+    rad_ctx *c = context();
+    if (!c)
+        return false;
+    pushState();
+    return libErr(c, rad_checkpoint_save(c, path), "rad_checkpoint_save") == RAD_OK;
+}
+
+bool ModelContainer::loadFromFile(const char *path)
+{
+    rad_ctx *c = context();
+    if (!c)
+        return false;
+    if (libErr(c, rad_checkpoint_load(c, path), "rad_checkpoint_load") != RAD_OK)
+        return false;
+    pullState();
+    return true;
+}
+
 unsigned int ModelContainer::getHighestRadiosityPatchId()
 {
     if (needRefresh)

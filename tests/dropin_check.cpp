@@ -7,6 +7,7 @@
 //   dropin_check select AREA K SEED            list rule on seeded radiosities
 //   dropin_check shoot AREA W K SHOOTS CYCLES  Shooter cycles, final fields
 //   dropin_check process AREA W                ProcessHemicube triplets
+//   dropin_check resume AREA W PATH            shoot, saveToFile, loadFromFile
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -97,6 +98,32 @@ int main(int argc, char **argv)
         vec("table", first, true);
         printf("}\n");
         delete[] f;
+        return 0;
+    }
+    if (mode == "resume" && argc == 5) {
+        Config::setHemicubeSide((unsigned)atoi(argv[3]));
+        Config::setHemicubesCount(4);
+        Config::setShootsPerCycle(5);
+        Config::setMaxPatchArea(atof(argv[2]));
+        Config::freeze();
+        ModelContainer a, b;
+        a.maxPatchArea = b.maxPatchArea = Config::MAX_PATCH_AREA();
+        a.load();
+        b.load();
+        a.updateData();
+        b.updateData();
+        Shooter sh(&a);
+        sh.shootCycle();
+        bool saved = a.saveToFile(argv[4]);
+        bool loaded = b.loadFromFile(argv[4]);
+        unsigned n = a.getPatchesCount(), same = 0;
+        for (unsigned i = 0; i < n; ++i) {
+            Patch *p = a.getPatches()[i], *q = b.getPatches()[i];
+            same += memcmp(&p->radiosity, &q->radiosity, 12) == 0 &&
+                    memcmp(&p->illumination, &q->illumination, 12) == 0;
+        }
+        printf("{\"saved\": %d, \"loaded\": %d, \"n\": %u, \"same\": %u}\n", saved ? 1 : 0,
+               loaded ? 1 : 0, n, same);
         return 0;
     }
     if ((mode == "select" && argc == 5) || (mode == "shoot" && argc == 7) ||
