@@ -348,7 +348,10 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         ncu = prop.multiProcessorCount;
     c->sel_blocks = ncu * 2;
     // screen tiles: 32x32 when enough hemicubes fill the GPU, else 16x16
+    // (32x32 also when a hemicube would have more tiles than the bins hold)
     uint32_t T = cfg->tile_px ? cfg->tile_px : (c->K >= 4 ? 32u : 16u);
+    if (!cfg->tile_px && ((c->g.TW + T - 1) / T) * ((c->g.TH + T - 1) / T) > kMaxTilesPerSlab)
+        T = 32;
     if ((rc = set_tiles(c, T, cfg->small_tri_px ? cfg->small_tri_px : 64u))) {
         rad_destroy(c);
         return rc;

@@ -343,3 +343,46 @@ def test_autotune_picks_a_candidate_and_keeps_parity(rad, boxes_small):
     for k, p in enumerate(shooters):
         want = O.render_hemicube(64, boxes_small.tri, boxes_small.tri_patch, boxes_small.frame[p])
         assert np.array_equal(ids[k], want), p
+
+
+def test_itembuf_bit_exact_max_hemicube(rad, cornell_q):
+    """W = RAD_MAX_W (1024): 2048 x 1536 texels per hemicube, 3072 tiles."""
+    say("itembuf cornell quads W=1024")
+    light = int(np.argmax(cornell_q.emission.sum(1)))
+    _render_case(rad, cornell_q, 1024, [light, cornell_q.npatch // 2])
+
+
+def test_max_batch_rows_equal_oracle(rad, cornell_t):
+    """K = RAD_MAX_K (256) shooters in one pass: every F row equals the
+    oracle's FP64 histogram of that shooter's hemicube."""
+    W, K = 16, 256
+    sc = cornell_t
+    s = _solver(rad, sc, W, K=K)
+    rng = np.random.default_rng(7)
+    sh = [int(x) for x in rng.choice(sc.npatch, K, replace=False)]
+    say(f"K={K} render + process")
+    item = s.render(sh)
+    hits, ff = s.process(K)
+    h_o, ff_o = O.histogram(W, K, item, O.dff_cross(W), sc.npatch)
+    assert np.array_equal(hits, h_o)
+    assert np.allclose(ff, ff_o, rtol=RTOL, atol=1e-9)
+    res = s.tex_res
+    for k in (0, 77, 255):
+        want = O.render_hemicube(W, sc.tri, sc.tri_patch, sc.frame[sh[k]])
+        assert np.array_equal(item[k * res:(k + 1) * res], want), k
+
+
+def test_empty_and_degenerate_scenes(rad):
+    """The unsubdivided Cornell box with one triangle collapsed to a point:
+    passes run and the item buffer still equals the oracle's."""
+    one = rad.Scene.cornell(0.0, triangle_patches=True, physical=True, seed=1)
+    tri = one.tri.reshape(-1, 3, 3).copy()
+    tri[1] = tri[1][[0, 0, 0]]                      # a degenerate triangle (one point)
+    sc = rad.Scene(tri.reshape(-1), one.tri_patch, one.frame, one.refl, one.area,
+                   one.emission, one.shot0)
+    s = _solver(rad, sc, 16, K=4)
+    s.step(3)
+    s.finish()
+    it = s.render([1])
+    want = O.render_hemicube(16, sc.tri, sc.tri_patch, sc.frame[1])
+    assert np.array_equal(it, want)
