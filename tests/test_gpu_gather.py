@@ -40,7 +40,7 @@ def built(rad, cornell):
     return s, F, G
 
 
-@pytest.mark.parametrize("R", [3, 20, 192])
+@pytest.mark.parametrize("R", [3, 20, 48, 100, 192])
 def test_dense_apply_random(rad, cornell, R):
     say(f"dense apply R={R}")
     rng = np.random.default_rng(R)
