@@ -164,16 +164,16 @@ __global__ void k_dense_to_f32(const uint16_t *__restrict__ fhi, const uint16_t 
 
 // ------------------------------------------------------------------ GEMM
 
-// YP[s] = F[:, kspan slice s] X[slice s]  for 128 rows per workgroup.
+// YP[s] = F[:, kspan slice s] X[slice s]  for 32 rows per wave.
 // Wave w owns rows i0 .. i0+31 (two 16-row A fragments) and all RT column
 // tiles of 16.  Per 64-wide k chunk the X slice (RT*16 rows x 64 bf16) is
-// staged in LDS and shared by the four waves; the A fragments go straight
+// staged in LDS and shared by the waves; the A fragments go straight
 // from HBM to registers (each F element is used by one wave only), with the
 // next chunk's loads issued before the current chunk's MFMAs.
 // SPLIT accumulates Fhi Xhi + Fhi Xlo + Flo Xhi (about 16 significant bits
 // of F and X) for the refinement residual.
-template <int RT, bool SPLIT>
-__global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
+template <int RT, bool SPLIT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict__ fhi,
                                               const uint16_t *__restrict__ flo,
                                               const uint16_t *__restrict__ xhi,
                                               const uint16_t *__restrict__ xlo, uint32_t npad,
@@ -183,7 +183,8 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
     constexpr int NB = SPLIT ? 2 : 1;
     constexpr int LROW = 72;               // bf16 per LDS row: 64 + 8 pad (144 B)
     constexpr int UNITS = RT * 16 * 8;     // 16-byte units of one X chunk
-    constexpr int UPT = (UNITS + 255) / 256;
+    constexpr int NT = WAVES * 64;
+    constexpr int UPT = (UNITS + NT - 1) / NT;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NB][RT * 16][LROW];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
     // (round robin), so with S dividing 8 every XCD works on one k slice and
     // keeps that slice of X in its own L2
     const uint32_t slice = blockIdx.x % nslice, rtile = blockIdx.x / nslice;
-    const uint32_t i0 = rtile * 128 + wave * 32;
+    const uint32_t i0 = rtile * (WAVES * 32) + wave * 32;
     const uint32_t kbeg = slice * kspan, kend = kbeg + kspan;
 
     f32x4 acc[2][RT];
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
     auto load_x = [&](uint32_t kc) {
 #pragma unroll
         for (int u = 0; u < UPT; ++u) {
-            int idx = tid + u * 256;
+            int idx = tid + u * NT;
             if (idx < UNITS) {
                 size_t o = (size_t)(idx >> 3) * npad + kc + (idx & 7) * 8;
                 xr[0][u] = *(const uint4 *)(xhi + o);
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ fhi,
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < UPT; ++u) {
-            int idx = tid + u * 256;
+            int idx = tid + u * NT;
             if (idx < UNITS) {
 #pragma unroll
                 for (int b = 0; b < NB; ++b)
@@ -471,16 +472,19 @@ uint32_t gemm_tiles(uint32_t R)
     return 12;
 }
 
+// Wide products use 8 waves (256 rows) per workgroup so every X chunk staged
+// in LDS serves twice the rows; narrow ones 4 waves (more workgroups).
 template <int RT>
 static void gemm_rt(bool split, const uint16_t *fhi, const uint16_t *flo, const uint16_t *xhi,
                     const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp, hipStream_t st)
 {
-    const uint32_t nwg = (npad / 128) * S;
+    constexpr int WV = RT >= 4 ? 8 : 4;
+    const uint32_t nwg = (npad / (WV * 32)) * S;
     uint32_t kspan = npad / S;
     if (split)
-        k_gemm<RT, true><<<nwg, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
+        k_gemm<RT, true, WV><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
     else
-        k_gemm<RT, false><<<nwg, 256, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
+        k_gemm<RT, false, WV><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
 }
 
 void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
