@@ -171,7 +171,9 @@ __device__ static void face_mvp(const float *fr, int face, float *mvp)
 // One thread per (slot, face): snapshot dB (Main.cpp:1161) and the MVP.
 // Per shooter the mvp buffer holds kShooterStride floats: five column-major
 // MVPs, then the culling frame used by k_raster_setup: eye, unit normal n,
-// unit up u (made orthogonal to n) and n x u.
+// unit up u (made orthogonal to n) and n x u; then a flag (1 when the frame
+// is degenerate or its up is not orthogonal to the normal within 2e-5, which
+// turns the conservative culls off for that shooter).
 __global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
                           const uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
                           const float *__restrict__ unshot, float *__restrict__ snap,
@@ -200,6 +202,14 @@ __global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
         float cf[12] = {fr[0], fr[1], fr[2], n.x, n.y, n.z, u.x, u.y, u.z, v.x, v.y, v.z};
         for (int k = 0; k < 12; ++k)
             out[80 + k] = cf[k];
+        // the culls take the five face axes as n, +-u, +-(n x u); face_mvp
+        // aims the side faces along the raw up, so they hold only while up
+        // is orthogonal to n
+        float ln = sqrtf(fr[3] * fr[3] + fr[4] * fr[4] + fr[5] * fr[5]);
+        float lu = sqrtf(fr[6] * fr[6] + fr[7] * fr[7] + fr[8] * fr[8]);
+        float cosnu = fabsf(fr[3] * fr[6] + fr[4] * fr[7] + fr[5] * fr[8]) / (ln * lu);
+        bool ok = ln > 0.0f && lu > 0.0f && cosnu <= 2e-5f && isfinite(fr[0] + fr[1] + fr[2]);
+        out[92] = ok ? 0.0f : 1.0f;
     }
     float m[16];
     face_mvp(fr, (int)face, m);
@@ -716,6 +726,8 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
 __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
                                       int f0, int f1)
 {
+    if (sm[92] != 0.0f)
+        return ((1u << f1) - 1u) & ~((1u << f0) - 1u);
     const float ex = sm[80], ey = sm[81], ez = sm[82];
     const float nx = sm[83], ny = sm[84], nz = sm[85];
     const float ux = sm[86], uy = sm[87], uz = sm[88];

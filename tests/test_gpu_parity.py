@@ -386,3 +386,90 @@ def test_empty_and_degenerate_scenes(rad):
     it = s.render([1])
     want = O.render_hemicube(16, sc.tri, sc.tri_patch, sc.frame[1])
     assert np.array_equal(it, want)
+
+
+def _sliver_scene(rad, rng, n_per_shooter=1500):
+    """Cornell triangles plus, around four shooters, triangles whose planes
+    pass close to the shooter's eye (nearly edge-on, both windings) and tiny
+    ones: back-facing triangles whose snapped area can flip sign.  Each added
+    triangle is its own patch."""
+    base = rad.Scene.cornell(0.3127, triangle_patches=True, physical=True, seed=1)
+    shooters = [int(x) for x in rng.choice(base.npatch, 4, replace=False)]
+    tris = []
+    for p in shooters:
+        e = base.frame[p, 0:3].astype(np.float64)
+        n = base.frame[p, 3:6].astype(np.float64)
+        n /= np.linalg.norm(n)
+        for _ in range(n_per_shooter):
+            while True:
+                dr = rng.normal(size=3)
+                dr /= np.linalg.norm(dr)
+                if dr @ n > 0.05:
+                    break
+            D = 10 ** rng.uniform(-1.7, 0.5)
+            nrm = np.cross(dr, rng.normal(size=3))
+            nrm /= np.linalg.norm(nrm)
+            kind = rng.integers(4)
+            if kind == 0:      # nearly edge-on, moderate size
+                s = D * 10 ** rng.uniform(-3, -0.7)
+                h = D * 10 ** rng.uniform(-7, -2)
+            elif kind == 1:    # tiny, any orientation
+                s = D * 10 ** rng.uniform(-5, -2.5)
+                h = D * 10 ** rng.uniform(-3, -1)
+            elif kind == 2:    # almost exactly edge-on
+                s = D * 10 ** rng.uniform(-2, -0.5)
+                h = D * 10 ** rng.uniform(-9, -5)
+            else:              # around the snapping-error scale
+                s = D * 10 ** rng.uniform(-3, -1)
+                h = 10 ** rng.uniform(-1, 1) * 5.66 * (1 / 512) * D * D / (s * 32)
+            h *= rng.choice([-1.0, 1.0])
+            a = np.cross(nrm, dr)
+            c = e + dr * D + h * nrm
+            tris.append(np.concatenate([c + s * (rng.normal() * a + rng.normal() * dr)
+                                        for _ in range(3)]))
+    extra = np.array(tris, np.float32)
+    ex = extra.reshape(-1, 3, 3).astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(ex[:, 1] - ex[:, 0], ex[:, 2] - ex[:, 0]), axis=1)
+    keep = area > 1e-12
+    extra, ex, area = extra[keep], ex[keep], area[keep]
+    m = len(extra)
+    nrm = np.cross(ex[:, 1] - ex[:, 0], ex[:, 2] - ex[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    up = ex[:, 1] - ex[:, 0]
+    up /= np.linalg.norm(up, axis=1, keepdims=True)
+    frame = np.concatenate([ex.mean(1), nrm, up], axis=1).astype(np.float32)
+    npb = base.npatch
+    sc = rad.Scene(np.concatenate([base.tri, extra]),
+                   np.concatenate([base.tri_patch, np.arange(npb, npb + m, dtype=np.uint32)]),
+                   np.concatenate([base.frame, frame]),
+                   np.concatenate([base.refl, np.full((m, 3), 0.5, np.float32)]),
+                   np.concatenate([base.area, area.astype(np.float32)]),
+                   np.concatenate([base.emission, np.zeros((m, 3), np.float32)]),
+                   np.concatenate([base.shot0, np.zeros((m, 3), np.float32)]))
+    return sc, shooters
+
+
+@pytest.mark.parametrize("W", [16, 64, 256])
+def test_itembuf_bit_exact_slivers(rad, W):
+    """Nearly edge-on and tiny triangles around the shooters (many of them
+    back-facing with a snapped area that flips sign): the conservative culls
+    must not change a texel."""
+    say(f"itembuf slivers W={W}")
+    sc, shooters = _sliver_scene(rad, np.random.default_rng(W))
+    _render_case(rad, sc, W, shooters)
+
+
+def test_itembuf_bit_exact_skewed_frame(rad, cornell_t):
+    """A shooter frame whose up is not orthogonal to its normal (the side
+    faces are then skewed): the culls that assume orthogonal face axes are
+    switched off for it and the item buffer still equals the oracle's."""
+    sc = cornell_t
+    frame = sc.frame.copy()
+    p = int(np.argmax(sc.emission.sum(1)))
+    q = 700
+    for i, t in ((p, 0.3), (q, -0.05)):
+        n = frame[i, 3:6] / np.linalg.norm(frame[i, 3:6])
+        frame[i, 6:9] = frame[i, 6:9] + t * n
+    sk = rad.Scene(sc.tri, sc.tri_patch, frame, sc.refl, sc.area, sc.emission, sc.shot0)
+    say("itembuf skewed frames W=64")
+    _render_case(rad, sk, 64, [p, q, 11])
