@@ -288,6 +288,54 @@ typedef struct rad_gather_stats {
 int rad_gather_solve(rad_ctx *ctx, const float *rhs, uint32_t R, const rad_gather_opts *opts,
                      float *out, rad_gather_stats *stats);
 
+/* ---------------------------------------------------------------------------
+ * Scene ingest on the device and vertex-colour export
+ * ------------------------------------------------------------------------ */
+
+/* Builds the scene on the device from source faces (nface*12 floats: A, B,
+ * C, D each; a triangle is A B C C): replaces Model::subdivide + Patch::divide
+ * (Model.cpp:27-60, Patch.cpp:47-222) and ModelContainer::updateData
+ * (ModelContainer.cpp:81-155) followed by rad_scene_upload.  The host only
+ * plans each face (whether and how it splits); one kernel writes every
+ * child's triangles, frame, area, colour, radiosity and illumination
+ * (inherited from its face, Patch.cpp:119-122; null arrays are 0) and its
+ * neighbour links into the context.  mode is RAD_FACES_QUADS or
+ * RAD_FACES_TRIANGLES (radhip_scene.h).  The arrays equal rad_scene_faces'
+ * bit for bit; the state is unshot = emission, shot = shot0.  Counts are
+ * returned through the optional out pointers.  A face that yields a patch of
+ * zero area is RAD_E_ARG (no scene is kept). */
+int rad_scene_ingest(rad_ctx *ctx, const float *quads, uint32_t nface, const float *color,
+                     const float *emission, const float *shot0, double max_area, int mode,
+                     uint32_t *out_ntri, uint32_t *out_npatch, uint32_t *out_nquad);
+
+/* Copies the scene on the device back in rad_scene_upload's layout (any
+ * pointer may be null): tri_xyz ntri*9, tri_patch ntri, frames npatch*9,
+ * refl npatch*3, area npatch. */
+int rad_scene_download(rad_ctx *ctx, float *tri_xyz, uint32_t *tri_patch, float *frames,
+                       float *refl, float *area);
+
+/* Quad topology of an uploaded scene (rad_scene_ingest sets it itself):
+ * patch_quad (npatch, starting at 0 and stepping by 0 or 1, so the patches of
+ * a quad are consecutive) and quad_nbr (nquad*8 neighbour quads in the slot
+ * order of Patch::neighbours, as rad_scene.quad_nbr). */
+int rad_set_quads(rad_ctx *ctx, const uint32_t *patch_quad, uint32_t nquad,
+                  const uint32_t *quad_nbr);
+int rad_get_quads(rad_ctx *ctx, uint32_t *nquad, uint32_t *patch_quad, uint32_t *quad_nbr);
+
+/* Smoothed vertex colours of every quad (Colors::smoothShadePatch,
+ * Colors.cpp:195-262, the per-frame loop of Main.cpp:1319-1340): out
+ * (nquad*12) holds corners A, B, C, D (left-bottom, right-bottom, right-top,
+ * left-top), each the mean of colour * (illumination + radiosity) over the
+ * quad and its three neighbours at that corner.  A quad of several patches
+ * (RAD_FACES_TRIANGLES) uses their area-weighted mean. */
+int rad_vertex_colors(rad_ctx *ctx, float *out);
+
+/* Item buffer of a viewpoint that is not a patch: frame = eye, view
+ * direction, up (9 floats); out_ids (TEX_RES) as rad_render_itembuf for one
+ * shooter.  The FRONT face is the 90-degree perspective image along the
+ * view direction. */
+int rad_render_view(rad_ctx *ctx, const float *frame, uint32_t *out_ids);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,8 @@ RAD_SELECT_REFERENCE = 0
 RAD_SELECT_POWER = 1
 RAD_DENSE_ROWS = 0
 RAD_DENSE_SHOOT = 1
+RAD_FACES_QUADS = 0
+RAD_FACES_TRIANGLES = 1
 
 
 class RadError(RuntimeError):
@@ -42,7 +44,8 @@ class rad_scene(C.Structure):
                 ("tri", C.POINTER(C.c_float)), ("tri_patch", C.POINTER(C.c_uint32)),
                 ("frame", C.POINTER(C.c_float)), ("refl", C.POINTER(C.c_float)),
                 ("area", C.POINTER(C.c_float)), ("emission", C.POINTER(C.c_float)),
-                ("shot0", C.POINTER(C.c_float)), ("patch_quad", C.POINTER(C.c_uint32))]
+                ("shot0", C.POINTER(C.c_float)), ("patch_quad", C.POINTER(C.c_uint32)),
+                ("quad_nbr", C.POINTER(C.c_uint32))]
 
 
 class rad_gather_opts(C.Structure):
@@ -97,6 +100,18 @@ SIGNATURES = {
     "rad_scene_boxes": (C.c_int, [_u32, C.c_float, C.c_float, C.c_float, _u32, _u32,
                                   C.c_uint64, C.POINTER(C.POINTER(rad_scene))]),
     "rad_scene_free": (None, [C.POINTER(rad_scene)]),
+    "rad_scene_ingest": (C.c_int, [_P, _PF, _u32, _PF, _PF, _PF, C.c_double, C.c_int,
+                                   C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
+    "rad_scene_download": (C.c_int, [_P, _PF, _PU, _PF, _PF, _PF]),
+    "rad_set_quads": (C.c_int, [_P, _PU, _u32, _PU]),
+    "rad_get_quads": (C.c_int, [_P, C.POINTER(_u32), _PU, _PU]),
+    "rad_vertex_colors": (C.c_int, [_P, _PF]),
+    "rad_render_view": (C.c_int, [_P, _PF, _PU]),
+    "rad_obj_parse": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(_u32)]),
+    "rad_obj_free": (None, [C.POINTER(C.c_float)]),
+    "rad_cornell_faces": (C.c_int, [_PF, _PF, _PF, _PF, C.POINTER(_u32)]),
+    "rad_scene_faces": (C.c_int, [_PF, _u32, _PF, _PF, _PF, C.c_double, C.c_int,
+                                  C.POINTER(C.POINTER(rad_scene))]),
     "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
     "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
     "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
@@ -149,7 +164,8 @@ def _arr(ptr, n, dtype, width=1):
 class Scene:
     """Host copy of a generated scene (numpy arrays)."""
 
-    def __init__(self, tri, tri_patch, frame, refl, area, emission, shot0, patch_quad=None):
+    def __init__(self, tri, tri_patch, frame, refl, area, emission, shot0, patch_quad=None,
+                 quad_nbr=None):
         self.tri = np.ascontiguousarray(tri, dtype=np.float32).reshape(-1, 9)
         self.tri_patch = np.ascontiguousarray(tri_patch, dtype=np.uint32)
         self.frame = np.ascontiguousarray(frame, dtype=np.float32).reshape(-1, 9)
@@ -158,6 +174,7 @@ class Scene:
         self.emission = np.ascontiguousarray(emission, dtype=np.float32).reshape(-1, 3)
         self.shot0 = np.ascontiguousarray(shot0, dtype=np.float32).reshape(-1, 3)
         self.patch_quad = patch_quad
+        self.quad_nbr = quad_nbr
 
     @property
     def ntri(self):
@@ -173,7 +190,8 @@ class Scene:
         out = Scene(_arr(s.tri, s.ntri, np.float32, 9), _arr(s.tri_patch, s.ntri, np.uint32),
                     _arr(s.frame, s.npatch, np.float32, 9), _arr(s.refl, s.npatch, np.float32, 3),
                     _arr(s.area, s.npatch, np.float32), _arr(s.emission, s.npatch, np.float32, 3),
-                    _arr(s.shot0, s.npatch, np.float32, 3), _arr(s.patch_quad, s.npatch, np.uint32))
+                    _arr(s.shot0, s.npatch, np.float32, 3), _arr(s.patch_quad, s.npatch, np.uint32),
+                    _arr(s.quad_nbr, s.nquad, np.uint32, 8))
         out.nquad = s.nquad
         lib().rad_scene_free(p)
         return out
@@ -188,6 +206,20 @@ class Scene:
         return Scene._from_c(p)
 
     @staticmethod
+    def faces(quads, max_area, mode=0, color=None, emission=None, shot0=None):
+        """Scene from source faces (n x 12: A, B, C, D; rad_scene_faces)."""
+        q = np.ascontiguousarray(quads, np.float32).reshape(-1, 12)
+        opt = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+               for a in (color, emission, shot0)]
+        p = C.POINTER(rad_scene)()
+        rc = lib().rad_scene_faces(fptr(q), q.shape[0], *[fptr(a) if a is not None else None
+                                                          for a in opt],
+                                   float(max_area), int(mode), C.byref(p))
+        if rc != RAD_OK:
+            raise RadError(f"rad_scene_faces failed ({rc})")
+        return Scene._from_c(p)
+
+    @staticmethod
     def boxes(nboxes=400, room=(20.0, 5.0, 20.0), target_tris=200000, nlights=16, seed=3):
         p = C.POINTER(rad_scene)()
         rc = lib().rad_scene_boxes(int(nboxes), float(room[0]), float(room[1]), float(room[2]),
@@ -195,6 +227,29 @@ class Scene:
         if rc != RAD_OK:
             raise RadError(f"rad_scene_boxes failed ({rc})")
         return Scene._from_c(p)
+
+
+def obj_parse(path):
+    """Source faces of a Wavefront OBJ file (rad_obj_parse): (faces n x 12,
+    complete) where complete is False when a bad vertex line ended the parse."""
+    q = C.POINTER(C.c_float)()
+    n = _u32(0)
+    rc = lib().rad_obj_parse(os.fsencode(path), C.byref(q), C.byref(n))
+    if rc != RAD_OK and not q:
+        raise RadError(f"rad_obj_parse failed ({rc}): {path}")
+    out = _arr(q, n.value, np.float32, 12)
+    lib().rad_obj_free(q)
+    return out, rc == RAD_OK
+
+
+def cornell_faces():
+    """The 17 Cornell source quads with colour, radiosity, illumination."""
+    n = _u32(0)
+    lib().rad_cornell_faces(None, None, None, None, C.byref(n))
+    q = np.zeros((n.value, 12), np.float32)
+    c, e, s = (np.zeros((n.value, 3), np.float32) for _ in range(3))
+    lib().rad_cornell_faces(fptr(q), fptr(c), fptr(e), fptr(s), C.byref(n))
+    return q, c, e, s
 
 
 class Solver:
@@ -241,8 +296,67 @@ class Solver:
                                          sc.npatch, fptr(sc.frame), fptr(sc.refl), fptr(sc.area)),
                   "rad_scene_upload")
         self.npatch = sc.npatch
+        self.ntri = sc.ntri
         if state:
             self.set_state(sc.emission, sc.shot0, sc.emission)
+
+    def ingest(self, quads, max_area, mode=0, color=None, emission=None, shot0=None):
+        """Builds the scene on the device from source faces (rad_scene_ingest);
+        returns (ntri, npatch, nquad)."""
+        q = np.ascontiguousarray(quads, np.float32).reshape(-1, 12)
+        opt = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+               for a in (color, emission, shot0)]
+        nt, np_, nq = _u32(0), _u32(0), _u32(0)
+        self._chk(lib().rad_scene_ingest(self._ctx, fptr(q), q.shape[0], fptr(opt[0]),
+                                         fptr(opt[1]), fptr(opt[2]), float(max_area), int(mode),
+                                         C.byref(nt), C.byref(np_), C.byref(nq)),
+                  "rad_scene_ingest")
+        self.npatch = np_.value
+        self.ntri = nt.value
+        return nt.value, np_.value, nq.value
+
+    def download(self):
+        """The device scene as a Scene (state arrays from rad_get_state)."""
+        nt, np_ = self.ntri, self.npatch
+        tri = np.zeros((nt, 9), np.float32)
+        tp = np.zeros(nt, np.uint32)
+        fr = np.zeros((np_, 9), np.float32)
+        rf = np.zeros((np_, 3), np.float32)
+        ar = np.zeros(np_, np.float32)
+        self._chk(lib().rad_scene_download(self._ctx, fptr(tri), uptr(tp), fptr(fr), fptr(rf),
+                                           fptr(ar)), "rad_scene_download")
+        u, sh = self.state()
+        pq, nb = self.quads()
+        return Scene(tri, tp, fr, rf, ar, u, sh, pq, nb)
+
+    def set_quads(self, patch_quad, quad_nbr):
+        pq = np.ascontiguousarray(patch_quad, np.uint32)
+        nb = np.ascontiguousarray(quad_nbr, np.uint32).reshape(-1, 8)
+        self._chk(lib().rad_set_quads(self._ctx, uptr(pq), nb.shape[0], uptr(nb)),
+                  "rad_set_quads")
+
+    def quads(self):
+        """(patch_quad, quad_nbr) of the device scene, or (None, None)."""
+        n = _u32(0)
+        if lib().rad_get_quads(self._ctx, C.byref(n), None, None) != RAD_OK:
+            return None, None
+        pq = np.zeros(self.npatch, np.uint32)
+        nb = np.zeros((n.value, 8), np.uint32)
+        self._chk(lib().rad_get_quads(self._ctx, C.byref(n), uptr(pq), uptr(nb)), "rad_get_quads")
+        return pq, nb
+
+    def vertex_colors(self):
+        n = _u32(0)
+        self._chk(lib().rad_get_quads(self._ctx, C.byref(n), None, None), "rad_get_quads")
+        out = np.zeros((n.value, 4, 3), np.float32)
+        self._chk(lib().rad_vertex_colors(self._ctx, fptr(out)), "rad_vertex_colors")
+        return out
+
+    def render_view(self, frame):
+        fr = np.ascontiguousarray(frame, np.float32).reshape(9)
+        ids = np.zeros(self.tex_res, np.uint32)
+        self._chk(lib().rad_render_view(self._ctx, fptr(fr), uptr(ids)), "rad_render_view")
+        return ids
 
     def set_state(self, unshot, shot, emission=None):
         u = np.ascontiguousarray(unshot, np.float32)

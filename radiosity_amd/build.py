@@ -35,8 +35,10 @@ SOURCES = [
     ("rad_kernels.hip", True),
     ("rad_gather.hip", True),
     ("rad_pack.hip", True),
+    ("rad_ingest.hip", True),
     ("rad_api.cpp", True),
     ("rad_scene.cpp", False),
+    ("rad_obj.cpp", False),
     ("rad_dropin.cpp", False),
 ]
 

@@ -14,7 +14,9 @@
 #include <string>
 #include <vector>
 
+#include "../../include/radhip_scene.h"
 #include "rad_internal.h"
+#include "rad_patch.h"
 
 using namespace rad;
 
@@ -60,6 +62,16 @@ struct rad_ctx {
     // F row length: npatch, or the largest scene of a pack
     uint32_t rowlen = 0;
     uint64_t geom_hash = 0;        // FNV-1a of the uploaded scene arrays (checkpoints)
+    bool hash_valid = false;       // geom_hash computed (ingested scenes hash lazily)
+
+    // quad topology (rad_scene_ingest or rad_set_quads; nquad 0 when absent)
+    uint32_t nquad = 0;
+    uint32_t *patch_quad = nullptr; // npatch
+    uint32_t *quad_pbeg = nullptr;  // nquad + 1: first patch of each quad
+    uint32_t *quad_nbr = nullptr;   // nquad * 8
+    float *qval = nullptr;          // nquad * 3 (vertex colours)
+    float *viewfr = nullptr;        // 9: frame of rad_render_view
+    uint32_t *view_sh = nullptr;    // 1: shooter slot 0 -> frame 0 of viewfr
     PackBuffers pk{};
 
     // captured passes (rad_config.graph_passes of them) replayed by rad_step
@@ -159,9 +171,11 @@ void record(rad_ctx *c, size_t idx)
 // Renders the hemicubes of the shooters currently in c->shooters/nshoot
 // (k slots): keys != null writes the item buffer, otherwise the tiles are
 // reduced into F rows (and hit counts when hits != null).
-int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits)
+int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits,
+                const uint32_t *shooters = nullptr, const float *frames = nullptr)
 {
-    launch_prepare(c->g, k, c->shooters, c->nshoot, c->frames, c->unshot, c->snap, c->mvp, c->st);
+    launch_prepare(c->g, k, shooters ? shooters : c->shooters, c->nshoot,
+                   frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->st);
     launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->rowlen, c->counters,
                  c->bb.ent_cap, c->st);
     record(c, evbase + 2);
@@ -380,7 +394,8 @@ void rad_destroy(rad_ctx *c)
                     c->pk.sc_slot_beg, c->pk.sc_nsel, c->pk.sc_done, c->pk.sc_den, c->pk.sc_res,
                     c->pk.sc_shot,
                     c->pk.cand, c->pk.patch_scene, c->pk.pass, c->pk.slot_tbeg, c->pk.slot_tcnt,
-                    c->pk.slot_pbeg};
+                    c->pk.slot_pbeg, c->patch_quad, c->quad_pbeg, c->quad_nbr, c->qval,
+                    c->viewfr, c->view_sh};
     for (void *p : ptrs)
         dfree(p);
     for (hipEvent_t e : c->ev)
@@ -453,13 +468,17 @@ int check_scene(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uin
 
 // Device copy of a scene; the rasterizer's record regions are sized for
 // setup grids of nblk blocks per shooter and F rows of rowlen entries.
-int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
-                uint32_t npatch, const float *frames, const float *refl, const float *area,
-                uint32_t nblk, uint32_t rowlen)
+// Device buffers of a scene of ntri triangles and npatch patches (contents
+// undefined), the rasterizer's record regions for setup grids of nblk blocks
+// per shooter, and F rows of rowlen entries.
+int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint32_t rowlen)
 {
     (void)hipStreamSynchronize(c->st);
     drop_graph(c);
     pack_free(c);
+    c->nquad = 0;
+    c->ntri = c->npatch = 0;
+    c->hash_valid = false;
     int rc;
     size_t K = c->K;
     if ((rc = dalloc(c, &c->tri, (size_t)ntri * 9)) || (rc = dalloc(c, &c->tri_patch, ntri)) ||
@@ -488,6 +507,40 @@ int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uin
         (rc = dalloc(c, &c->bb.clipq, c->bb.clip_cap)) ||
         (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
         return rc;
+    HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
+    HIPCHK(c, hipMemset(c->hits, 0, K * rowlen * 4));
+    c->ntri = ntri;
+    c->npatch = npatch;
+    c->rowlen = rowlen;
+    c->rendered = false;
+    c->dense = false;
+    return RAD_OK;
+}
+
+// 64-bit FNV-1a of a scene's arrays as the host hands them over (triangles
+// interleaved, then patch ids, frames, reflectances, areas).
+uint64_t scene_hash(uint32_t ntri, uint32_t npatch, const float *tri_xyz,
+                    const uint32_t *tri_patch, const float *frames, const float *refl,
+                    const float *area)
+{
+    uint64_t h = 0xCBF29CE484222325ull;
+    h = fnv1a(h, &ntri, 4);
+    h = fnv1a(h, &npatch, 4);
+    h = fnv1a(h, tri_xyz, (size_t)ntri * 36);
+    h = fnv1a(h, tri_patch, (size_t)ntri * 4);
+    h = fnv1a(h, frames, (size_t)npatch * 36);
+    h = fnv1a(h, refl, (size_t)npatch * 12);
+    h = fnv1a(h, area, (size_t)npatch * 4);
+    return h;
+}
+
+int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                uint32_t npatch, const float *frames, const float *refl, const float *area,
+                uint32_t nblk, uint32_t rowlen)
+{
+    int rc;
+    if ((rc = scene_alloc(c, ntri, npatch, nblk, rowlen)))
+        return rc;
     // triangles -> 9 planes
     std::vector<float> planes((size_t)ntri * 9);
     for (uint32_t t = 0; t < ntri; ++t)
@@ -501,22 +554,32 @@ int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uin
     HIPCHK(c, hipMemset(c->emis, 0, (size_t)npatch * 12));
     HIPCHK(c, hipMemset(c->unshot, 0, (size_t)npatch * 12));
     HIPCHK(c, hipMemset(c->shot, 0, (size_t)npatch * 12));
-    HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
-    HIPCHK(c, hipMemset(c->hits, 0, K * rowlen * 4));
-    c->ntri = ntri;
-    c->npatch = npatch;
-    c->rowlen = rowlen;
-    uint64_t h = 0xCBF29CE484222325ull;
-    h = fnv1a(h, &ntri, 4);
-    h = fnv1a(h, &npatch, 4);
-    h = fnv1a(h, tri_xyz, (size_t)ntri * 36);
-    h = fnv1a(h, tri_patch, (size_t)ntri * 4);
-    h = fnv1a(h, frames, (size_t)npatch * 36);
-    h = fnv1a(h, refl, (size_t)npatch * 12);
-    h = fnv1a(h, area, (size_t)npatch * 4);
-    c->geom_hash = h;
-    c->rendered = false;
-    c->dense = false;
+    c->geom_hash = scene_hash(ntri, npatch, tri_xyz, tri_patch, frames, refl, area);
+    c->hash_valid = true;
+    return RAD_OK;
+}
+
+// The scene hash of what is on the device (an ingested scene is hashed on
+// first use: its arrays are read back in the upload layout).
+int ensure_hash(rad_ctx *c)
+{
+    if (c->hash_valid)
+        return RAD_OK;
+    const size_t nt = c->ntri, np = c->npatch;
+    std::vector<float> planes(nt * 9), tri(nt * 9), fr(np * 9), rf(np * 3), ar(np);
+    std::vector<uint32_t> tp(nt);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(planes.data(), c->tri, nt * 36, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(tp.data(), c->tri_patch, nt * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(fr.data(), c->frames, np * 36, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(rf.data(), c->refl, np * 12, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ar.data(), c->area, np * 4, hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 9; ++k)
+            tri[t * 9 + k] = planes[(size_t)k * nt + t];
+    c->geom_hash = scene_hash(c->ntri, c->npatch, tri.data(), tp.data(), fr.data(), rf.data(),
+                              ar.data());
+    c->hash_valid = true;
     return RAD_OK;
 }
 
@@ -1494,6 +1557,8 @@ int rad_checkpoint_save(rad_ctx *c, const char *path)
     int rc;
     if ((rc = need_scene(c)))
         return rc;
+    if ((rc = ensure_hash(c)))
+        return rc;
     const size_t n = c->npatch;
     std::vector<float> u(n * 3), sh(n * 3), em(n * 3), buf(n * 9);
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1559,6 +1624,8 @@ int rad_checkpoint_load(rad_ctx *c, const char *path)
     memcpy(&hash, &in[24], 8);
     if (ver != 1)
         return fail(c, RAD_E_ARG, "checkpoint: unknown version " + std::to_string(ver));
+    if ((rc = ensure_hash(c)))
+        return rc;
     if (np != c->npatch || nt != c->ntri || hash != c->geom_hash)
         return fail(c, RAD_E_ARG, "checkpoint: written for another scene (patch count, triangle "
                                   "count or geometry hash differ)");
@@ -1578,6 +1645,233 @@ int rad_checkpoint_load(rad_ctx *c, const char *path)
             em[i * 3 + ch] = buf[(6 + ch) * n + i];
         }
     return rad_set_state(c, u.data(), sh.data(), em.data());
+}
+
+// ------------------------------------------------------------ scene ingest
+
+int rad_scene_ingest(rad_ctx *c, const float *quads, uint32_t nface, const float *color,
+                     const float *emission, const float *shot0, double max_area, int mode,
+                     uint32_t *out_ntri, uint32_t *out_npatch, uint32_t *out_nquad)
+{
+    if (!c)
+        return RAD_E_ARG;
+    if (!quads || nface == 0 || !(max_area >= 0.0) ||
+        (mode != RAD_FACES_QUADS && mode != RAD_FACES_TRIANGLES))
+        return fail(c, RAD_E_ARG, "ingest: null faces, zero faces, negative area or bad mode");
+    for (uint64_t i = 0; i < (uint64_t)nface * 12; ++i)
+        if (!std::isfinite(quads[i]) || std::fabs(quads[i]) > 1.0e4f)
+            return fail(c, RAD_E_ARG, "ingest: face coordinate not finite or beyond 1e4");
+    std::vector<radpatch::FacePlan> plan;
+    uint32_t nq, nt, np;
+    if (!radpatch::plan_faces(quads, nface, max_area, mode, plan, nq, nt, np))
+        return fail(c, RAD_E_LIMIT, "ingest: more than 2^31 triangles");
+    if (np == 0) // Patch::divide yields no children for a quad with a zero-length edge
+        return fail(c, RAD_E_ARG, "ingest: the faces yield no patches");
+    int rc;
+    if ((rc = scene_alloc(c, nt, np, (nt + 255) / 256, np)))
+        return rc;
+    if ((rc = dalloc(c, &c->patch_quad, np)) || (rc = dalloc(c, &c->quad_pbeg, (size_t)nq + 1)) ||
+        (rc = dalloc(c, &c->quad_nbr, (size_t)nq * 8)) || (rc = dalloc(c, &c->qval, (size_t)nq * 3)))
+        return rc;
+    // inputs of the kernel: faces, plans, per-face colour / radiosity / illumination
+    float *dfaces = nullptr, *dcol = nullptr, *dem = nullptr, *dsh = nullptr;
+    FacePlanDev *dplan = nullptr;
+    std::vector<FacePlanDev> hp(nface);
+    for (uint32_t f = 0; f < nface; ++f)
+        hp[f] = {plan[f].native, plan[f].kx, plan[f].ky, plan[f].qbeg, plan[f].tbeg, plan[f].pbeg};
+    auto cleanup = [&]() {
+        dfree(dfaces);
+        dfree(dcol);
+        dfree(dem);
+        dfree(dsh);
+        dfree(dplan);
+    };
+    if ((rc = dalloc(c, &dfaces, (size_t)nface * 12)) || (rc = dalloc(c, &dplan, nface)) ||
+        (color && (rc = dalloc(c, &dcol, (size_t)nface * 3))) ||
+        (emission && (rc = dalloc(c, &dem, (size_t)nface * 3))) ||
+        (shot0 && (rc = dalloc(c, &dsh, (size_t)nface * 3)))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(dfaces, quads, (size_t)nface * 48, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dplan, hp.data(), (size_t)nface * sizeof(FacePlanDev),
+                           hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess && color)
+        e = hipMemcpyAsync(dcol, color, (size_t)nface * 12, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess && emission)
+        e = hipMemcpyAsync(dem, emission, (size_t)nface * 12, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess && shot0)
+        e = hipMemcpyAsync(dsh, shot0, (size_t)nface * 12, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->quad_pbeg + nq, &np, 4, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) {
+        launch_ingest(nface, nq, dfaces, dplan, dcol, dem, dsh, (uint32_t)mode, nt, c->tri,
+                      c->tri_patch, c->frames, c->refl, c->area, c->emis, c->unshot, c->shot,
+                      c->patch_quad, c->quad_pbeg, c->quad_nbr, c->st);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->st);
+    cleanup();
+    if (e != hipSuccess) {
+        c->npatch = c->ntri = 0;
+        return fail(c, RAD_E_HIP, std::string("ingest: ") + hipGetErrorString(e));
+    }
+    // a patch of zero area would break the area-normalised shooting
+    std::vector<float> ar(np);
+    HIPCHK(c, hipMemcpy(ar.data(), c->area, (size_t)np * 4, hipMemcpyDeviceToHost));
+    for (uint32_t p = 0; p < np; ++p)
+        if (!(ar[p] > 0.0f)) {
+            c->npatch = c->ntri = 0;
+            return fail(c, RAD_E_ARG, "ingest: patch " + std::to_string(p) +
+                                          " has zero area (degenerate face)");
+        }
+    c->nquad = nq;
+    if (out_ntri)
+        *out_ntri = nt;
+    if (out_npatch)
+        *out_npatch = np;
+    if (out_nquad)
+        *out_nquad = nq;
+    return RAD_OK;
+}
+
+int rad_scene_download(rad_ctx *c, float *tri_xyz, uint32_t *tri_patch, float *frames,
+                       float *refl, float *area)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    const size_t nt = c->ntri, np = c->npatch;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (tri_xyz) {
+        std::vector<float> planes(nt * 9);
+        HIPCHK(c, hipMemcpy(planes.data(), c->tri, nt * 36, hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 9; ++k)
+                tri_xyz[t * 9 + k] = planes[(size_t)k * nt + t];
+    }
+    if (tri_patch)
+        HIPCHK(c, hipMemcpy(tri_patch, c->tri_patch, nt * 4, hipMemcpyDeviceToHost));
+    if (frames)
+        HIPCHK(c, hipMemcpy(frames, c->frames, np * 36, hipMemcpyDeviceToHost));
+    if (refl)
+        HIPCHK(c, hipMemcpy(refl, c->refl, np * 12, hipMemcpyDeviceToHost));
+    if (area)
+        HIPCHK(c, hipMemcpy(area, c->area, np * 4, hipMemcpyDeviceToHost));
+    return RAD_OK;
+}
+
+int rad_set_quads(rad_ctx *c, const uint32_t *patch_quad, uint32_t nquad, const uint32_t *quad_nbr)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    if (!patch_quad || !quad_nbr || nquad == 0 || nquad > c->npatch)
+        return fail(c, RAD_E_ARG, "quads: null arrays or quad count out of range");
+    const uint32_t np = c->npatch;
+    std::vector<uint32_t> pbeg(nquad + 1, 0);
+    if (patch_quad[0] != 0 || patch_quad[np - 1] != nquad - 1)
+        return fail(c, RAD_E_ARG, "quads: patch_quad must run from 0 to nquad - 1");
+    for (uint32_t p = 1; p < np; ++p) {
+        uint32_t d = patch_quad[p] - patch_quad[p - 1];
+        if (patch_quad[p] < patch_quad[p - 1] || d > 1)
+            return fail(c, RAD_E_ARG, "quads: patch_quad must step by 0 or 1");
+        if (d == 1)
+            pbeg[patch_quad[p]] = p;
+    }
+    pbeg[nquad] = np;
+    for (uint64_t i = 0; i < (uint64_t)nquad * 8; ++i)
+        if (quad_nbr[i] >= nquad)
+            return fail(c, RAD_E_ARG, "quads: neighbour index out of range");
+    if ((rc = dalloc(c, &c->patch_quad, np)) || (rc = dalloc(c, &c->quad_pbeg, (size_t)nquad + 1)) ||
+        (rc = dalloc(c, &c->quad_nbr, (size_t)nquad * 8)) || (rc = dalloc(c, &c->qval, (size_t)nquad * 3)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->patch_quad, patch_quad, (size_t)np * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->quad_pbeg, pbeg.data(), ((size_t)nquad + 1) * 4,
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->quad_nbr, quad_nbr, (size_t)nquad * 32, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->nquad = nquad;
+    return RAD_OK;
+}
+
+int rad_get_quads(rad_ctx *c, uint32_t *nquad, uint32_t *patch_quad, uint32_t *quad_nbr)
+{
+    if (!c || !nquad)
+        return RAD_E_ARG;
+    *nquad = c->nquad;
+    if (c->nquad == 0)
+        return fail(c, RAD_E_STATE, "no quad topology (rad_scene_ingest or rad_set_quads)");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (patch_quad)
+        HIPCHK(c, hipMemcpy(patch_quad, c->patch_quad, (size_t)c->npatch * 4, hipMemcpyDeviceToHost));
+    if (quad_nbr)
+        HIPCHK(c, hipMemcpy(quad_nbr, c->quad_nbr, (size_t)c->nquad * 32, hipMemcpyDeviceToHost));
+    return RAD_OK;
+}
+
+int rad_vertex_colors(rad_ctx *c, float *out)
+{
+    if (!c || !out)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_plain(c)))
+        return rc;
+    if (c->nquad == 0)
+        return fail(c, RAD_E_STATE, "no quad topology (rad_scene_ingest or rad_set_quads)");
+    float *dout = nullptr;
+    if ((rc = dalloc(c, &dout, (size_t)c->nquad * 12)))
+        return rc;
+    launch_vertex_colors(c->nquad, c->quad_pbeg, c->quad_nbr, c->refl, c->area, c->unshot,
+                         c->shot, c->qval, dout, c->st);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, (size_t)c->nquad * 48, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->st);
+    dfree(dout);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("vertex colours: ") + hipGetErrorString(e));
+    return RAD_OK;
+}
+
+int rad_render_view(rad_ctx *c, const float *frame, uint32_t *out_ids)
+{
+    if (!c || !frame || !out_ids)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_plain(c)))
+        return rc;
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(frame[k]))
+            return fail(c, RAD_E_ARG, "render_view: frame not finite");
+    if ((!c->viewfr && (rc = dalloc(c, &c->viewfr, 9))) ||
+        (!c->view_sh && (rc = dalloc(c, &c->view_sh, 1))))
+        return rc;
+    const uint32_t zero = 0, one = 1;
+    HIPCHK(c, hipMemcpyAsync(c->viewfr, frame, 36, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->view_sh, &zero, 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->nshoot, &one, 4, hipMemcpyHostToDevice, c->st));
+    if ((rc = render_pass(c, 1, 0, c->keys, nullptr, c->view_sh, c->viewfr)))
+        return rc;
+    c->rendered = false; // the keys do not hold a shooter's hemicube
+    uint32_t *dids = nullptr;
+    if ((rc = dalloc(c, &dids, c->g.RES)))
+        return rc;
+    launch_split_keys(c->keys, dids, nullptr, c->g.RES, c->st);
+    hipError_t e = hipMemcpyAsync(out_ids, dids, (size_t)c->g.RES * 4, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->st);
+    dfree(dids);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("render_view: ") + hipGetErrorString(e));
+    return check_overflow(c);
 }
 
 } // extern "C"

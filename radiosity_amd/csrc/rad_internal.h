@@ -72,6 +72,13 @@ struct Geometry {
     uint32_t W, TW, TH, RES;
 };
 
+// How one source face splits (planned on the host, rad_patch.h FacePlan):
+// native triangle grid of side kx, or a kx x ky quad grid; where its child
+// quads, triangles and patches start.
+struct FacePlanDev {
+    uint32_t native, kx, ky, qbeg, tbeg, pbeg;
+};
+
 // Many scenes packed into one context (rad_pack_upload): per-scene ranges
 // and per-pass slot arrays (null when no pack is loaded).
 struct PackBuffers {
@@ -168,5 +175,15 @@ void launch_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *src,
                         uint16_t *xhi, uint16_t *xlo, hipStream_t st);
 void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
                          const float *yp, float *out, hipStream_t st);
+
+// Launchers implemented in rad_ingest.hip.
+void launch_ingest(uint32_t nface, uint32_t nquad, const float *faces, const FacePlanDev *plan,
+                   const float *color, const float *emission, const float *shot0, uint32_t mode,
+                   uint32_t ntri, float *tri, uint32_t *tri_patch, float *frames, float *refl,
+                   float *area, float *emis, float *unshot, float *shot, uint32_t *patch_quad,
+                   uint32_t *quad_pbeg, uint32_t *quad_nbr, hipStream_t st);
+void launch_vertex_colors(uint32_t nquad, const uint32_t *quad_pbeg, const uint32_t *quad_nbr,
+                          const float *refl, const float *area, const float *unshot,
+                          const float *shot, float *val, float *out, hipStream_t st);
 
 } // namespace rad

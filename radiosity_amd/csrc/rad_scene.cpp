@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 #include <vector>
 
@@ -46,13 +47,14 @@ struct Rng {
     float uni(float lo, float hi) { return lo + (hi - lo) * (float)((next() >> 40) * (1.0 / 16777216.0)); }
 };
 
-rad_scene *alloc_scene(uint32_t ntri, uint32_t npatch)
+rad_scene *alloc_scene(uint32_t ntri, uint32_t npatch, uint32_t nquad)
 {
     rad_scene *s = (rad_scene *)calloc(1, sizeof(rad_scene));
     if (!s)
         return nullptr;
     s->ntri = ntri;
     s->npatch = npatch;
+    s->nquad = nquad;
     s->tri = (float *)calloc((size_t)ntri * 9 + 1, sizeof(float));
     s->tri_patch = (uint32_t *)calloc((size_t)ntri + 1, sizeof(uint32_t));
     s->frame = (float *)calloc((size_t)npatch * 9 + 1, sizeof(float));
@@ -61,31 +63,27 @@ rad_scene *alloc_scene(uint32_t ntri, uint32_t npatch)
     s->emission = (float *)calloc((size_t)npatch * 3 + 1, sizeof(float));
     s->shot0 = (float *)calloc((size_t)npatch * 3 + 1, sizeof(float));
     s->patch_quad = (uint32_t *)calloc((size_t)npatch + 1, sizeof(uint32_t));
+    s->quad_nbr = (uint32_t *)calloc((size_t)nquad * 8 + 1, sizeof(uint32_t));
     if (!s->tri || !s->tri_patch || !s->frame || !s->refl || !s->area || !s->emission ||
-        !s->shot0 || !s->patch_quad) {
+        !s->shot0 || !s->patch_quad || !s->quad_nbr) {
         rad_scene_free(s);
         return nullptr;
     }
     return s;
 }
 
-float tri_area(V3 a, V3 b, V3 c)
-{
-    V3 n = cross(b - a, c - a);
-    return (float)(0.5 * sqrt((double)n.x * n.x + (double)n.y * n.y + (double)n.z * n.z));
-}
-
 // Flattens quads like ModelContainer::updateData (ModelContainer.cpp:81-155):
-// triangles (0,1,2), (0,2,3) per quad.  Per-patch values come from `fill`.
+// triangles (0,1,2), (0,2,3) per quad.  Per-patch values come from `fill`;
+// nbr (8 per quad, or empty) becomes quad_nbr.
 template <typename Fill>
-rad_scene *flatten(const std::vector<Quad> &quads, bool tri_patches, Fill fill)
+rad_scene *flatten(const std::vector<Quad> &quads, bool tri_patches, Fill fill,
+                   const std::vector<unsigned> &nbr)
 {
     uint32_t nq = (uint32_t)quads.size();
     uint32_t np = tri_patches ? 2 * nq : nq;
-    rad_scene *s = alloc_scene(2 * nq, np);
+    rad_scene *s = alloc_scene(2 * nq, np, nq);
     if (!s)
         return nullptr;
-    s->nquad = nq;
     for (uint32_t q = 0; q < nq; ++q) {
         const Quad &Q = quads[q];
         const V3 *v = Q.v;
@@ -94,36 +92,21 @@ rad_scene *flatten(const std::vector<Quad> &quads, bool tri_patches, Fill fill)
             memcpy(&s->tri[(size_t)(2 * q) * 9 + 3 * k], &t0[k], 12);
             memcpy(&s->tri[(size_t)(2 * q + 1) * 9 + 3 * k], &t1[k], 12);
         }
+        for (int k = 0; k < 8; ++k)
+            s->quad_nbr[8 * (size_t)q + k] = nbr.empty() ? q : nbr[8 * (size_t)q + k];
         if (tri_patches) {
             s->tri_patch[2 * q] = 2 * q;
             s->tri_patch[2 * q + 1] = 2 * q + 1;
             for (int h = 0; h < 2; ++h) {
-                const V3 *t = h ? t1 : t0;
                 uint32_t p = 2 * q + h;
-                // triangle frame: centroid, (c-a) x (b-a), up c-a - the quad
-                // rule of Patch.cpp:253-276 applied to the triangle's edges
-                V3 eye = {(t[0].x + t[1].x + t[2].x) / 3.0f, (t[0].y + t[1].y + t[2].y) / 3.0f,
-                          (t[0].z + t[1].z + t[2].z) / 3.0f};
-                V3 n = vcross(t[1] - t[0], t[2] - t[0]);
-                V3 up = t[2] - t[0];
-                float fr[9] = {eye.x, eye.y, eye.z, n.x, n.y, n.z, up.x, up.y, up.z};
-                memcpy(&s->frame[(size_t)p * 9], fr, sizeof(fr));
-                s->area[p] = tri_area(t[0], t[1], t[2]);
+                tri_patch(h ? t1 : t0, &s->frame[(size_t)p * 9], s->area[p]);
                 s->patch_quad[p] = q;
                 fill(s, p, Q, h);
             }
         } else {
             s->tri_patch[2 * q] = q;
             s->tri_patch[2 * q + 1] = q;
-            // Patch.cpp:253-276
-            V3 eye = {(v[0].x + v[1].x + v[2].x + v[3].x) / 4.0f,
-                      (v[0].y + v[1].y + v[2].y + v[3].y) / 4.0f,
-                      (v[0].z + v[1].z + v[2].z + v[3].z) / 4.0f};
-            V3 n = vcross(v[1] - v[0], v[3] - v[0]);
-            V3 up = v[3] - v[0];
-            float fr[9] = {eye.x, eye.y, eye.z, n.x, n.y, n.z, up.x, up.y, up.z};
-            memcpy(&s->frame[(size_t)q * 9], fr, sizeof(fr));
-            s->area[q] = tri_area(v[0], v[1], v[2]) + tri_area(v[0], v[2], v[3]);
+            quad_patch(v, &s->frame[(size_t)q * 9], s->area[q]);
             s->patch_quad[q] = q;
             fill(s, q, Q, 0);
         }
@@ -131,12 +114,17 @@ rad_scene *flatten(const std::vector<Quad> &quads, bool tri_patches, Fill fill)
     return s;
 }
 
-// One model's quads subdivided like Model::getPatches(area) (PrimitiveModel.cpp:79-84).
-std::vector<Quad> cornell_quads(double max_area, bool keep_light)
+// Each model's quads subdivided like Model::getPatches(area)
+// (PrimitiveModel.cpp:79-84), concatenated in ModelContainer::load order,
+// with the neighbour links of every quad.
+std::vector<Quad> cornell_quads(double max_area, bool keep_light, std::vector<unsigned> &nbr)
 {
     std::vector<Quad> all;
     for (const std::vector<Quad> &m : cornell_models()) {
-        std::vector<Quad> d = subdivide(m, max_area, keep_light);
+        std::vector<unsigned> nb;
+        std::vector<Quad> d = subdivide(m, max_area, keep_light, &nb);
+        for (unsigned x : nb)
+            nbr.push_back(x + (unsigned)all.size());
         all.insert(all.end(), d.begin(), d.end());
     }
     return all;
@@ -164,6 +152,25 @@ Quad face_quad(V3 o, V3 e1, V3 e2, V3 n, int group)
     return q;
 }
 
+// Child c of face f (plan pl): its corners, four even for a triangle
+// (the last one repeated), and its triangles (2, or 1 for a native one).
+int face_child(const float *fv, const FacePlan &pl, uint32_t c, V3 *v)
+{
+    V3 src[4] = {face_vertex(fv, 0), face_vertex(fv, 1), face_vertex(fv, 2), face_vertex(fv, 3)};
+    if (pl.native) {
+        tri_grid_child(src[0], src[1], src[2], pl.kx, c, v);
+        v[3] = v[2];
+        return 1;
+    }
+    if (pl.kx * pl.ky == 1) {
+        for (int k = 0; k < 4; ++k)
+            v[k] = src[k];
+    } else {
+        divide_child(src, pl.kx, pl.ky, c, v);
+    }
+    return 2;
+}
+
 } // namespace
 
 extern "C" {
@@ -180,6 +187,7 @@ void rad_scene_free(rad_scene *s)
     free(s->emission);
     free(s->shot0);
     free(s->patch_quad);
+    free(s->quad_nbr);
     free(s);
 }
 
@@ -189,7 +197,8 @@ int rad_scene_cornell(double max_area, int triangle_patches, int physical, uint6
     if (!out || !(max_area >= 0.0))
         return RAD_E_ARG;
     *out = nullptr;
-    std::vector<Quad> quads = cornell_quads(max_area, physical != 0);
+    std::vector<unsigned> nbr;
+    std::vector<Quad> quads = cornell_quads(max_area, physical != 0, nbr);
     Rng rng(seed);
     rad_scene *s = flatten(quads, triangle_patches != 0, [&](rad_scene *sc, uint32_t p, const Quad &q, int) {
         if (physical) {
@@ -212,7 +221,7 @@ int rad_scene_cornell(double max_area, int triangle_patches, int physical, uint6
             sc->shot0[3 * p + 1] = q.illum.y;
             sc->shot0[3 * p + 2] = q.illum.z;
         }
-    });
+    }, nbr);
     if (!s)
         return RAD_E_NOMEM;
     *out = s;
@@ -281,7 +290,8 @@ int rad_scene_boxes(uint32_t nboxes, float rx, float ry, float rz, uint32_t targ
     } else {
         hi = 0;
     }
-    std::vector<Quad> quads = subdivide(base, hi, false);
+    std::vector<unsigned> nbr;
+    std::vector<Quad> quads = subdivide(base, hi, false, &nbr);
     // lights: nlights distinct ceiling quads, both triangles of the first
     // half of picks ... one triangle per pick
     std::vector<uint32_t> ceiling;
@@ -305,9 +315,95 @@ int rad_scene_boxes(uint32_t nboxes, float rx, float ry, float rz, uint32_t targ
             sc->emission[3 * p + 1] = lr.uni(5.0f, 20.0f);
             sc->emission[3 * p + 2] = lr.uni(5.0f, 20.0f);
         }
-    });
+    }, nbr);
     if (!s)
         return RAD_E_NOMEM;
+    *out = s;
+    return RAD_OK;
+}
+
+int rad_cornell_faces(float *quads, float *color, float *emission, float *shot0,
+                      uint32_t *nface)
+{
+    if (!nface)
+        return RAD_E_ARG;
+    uint32_t n = 0;
+    for (const std::vector<Quad> &m : cornell_models())
+        for (const Quad &q : m) {
+            if (quads)
+                for (int k = 0; k < 4; ++k)
+                    memcpy(quads + 12 * (size_t)n + 3 * k, &q.v[k], 12);
+            if (color)
+                memcpy(color + 3 * (size_t)n, &q.color, 12);
+            if (emission)
+                memcpy(emission + 3 * (size_t)n, &q.rad, 12);
+            if (shot0)
+                memcpy(shot0 + 3 * (size_t)n, &q.illum, 12);
+            ++n;
+        }
+    *nface = n;
+    return RAD_OK;
+}
+
+int rad_scene_faces(const float *quads, uint32_t nface, const float *color, const float *emission,
+                    const float *shot0, double max_area, int mode, rad_scene **out)
+{
+    if (!out)
+        return RAD_E_ARG;
+    *out = nullptr;
+    if (!quads || nface == 0 || !(max_area >= 0.0) || (mode != kFacesQuads && mode != kFacesTriangles))
+        return RAD_E_ARG;
+    for (uint64_t i = 0; i < (uint64_t)nface * 12; ++i)
+        if (!std::isfinite(quads[i]) || std::fabs(quads[i]) > 1.0e4f)
+            return RAD_E_ARG;
+    std::vector<FacePlan> plan;
+    uint32_t nq, nt, np;
+    if (!plan_faces(quads, nface, max_area, mode, plan, nq, nt, np))
+        return RAD_E_LIMIT;
+    if (np == 0) // Patch::divide yields no children for a quad with a zero-length edge
+        return RAD_E_ARG;
+    rad_scene *s = alloc_scene(nt, np, nq);
+    if (!s)
+        return RAD_E_NOMEM;
+    for (uint32_t f = 0; f < nface; ++f) {
+        const FacePlan &pl = plan[f];
+        const float *fv = quads + 12 * (size_t)f;
+        uint32_t kids = pl.native ? pl.kx * pl.kx : pl.kx * pl.ky;
+        for (uint32_t c = 0; c < kids; ++c) {
+            V3 v[4];
+            int ntc = face_child(fv, pl, c, v);
+            uint32_t q = pl.qbeg + c;
+            unsigned nb[8];
+            if (pl.native)
+                for (int k = 0; k < 8; ++k)
+                    nb[k] = c;
+            else
+                divide_neighbours(pl.kx, pl.ky, c, nb);
+            for (int k = 0; k < 8; ++k)
+                s->quad_nbr[8 * (size_t)q + k] = pl.qbeg + nb[k];
+            V3 t[2][3] = {{v[0], v[1], v[2]}, {v[0], v[2], v[3]}};
+            uint32_t t0 = pl.tbeg + (uint32_t)ntc * c;
+            uint32_t p0 = mode == kFacesQuads ? pl.pbeg + c : pl.pbeg + (uint32_t)ntc * c;
+            for (int h = 0; h < ntc; ++h) {
+                memcpy(&s->tri[(size_t)(t0 + h) * 9], t[h], 36);
+                uint32_t p = mode == kFacesQuads ? p0 : p0 + h;
+                s->tri_patch[t0 + h] = p;
+                if (mode == kFacesQuads && h == 1)
+                    continue;
+                if (mode == kFacesQuads)
+                    quad_patch(v, &s->frame[(size_t)p * 9], s->area[p]);
+                else
+                    tri_patch(t[h], &s->frame[(size_t)p * 9], s->area[p]);
+                s->patch_quad[p] = q;
+                // Patch.cpp:119-122: children inherit colour, illumination, radiosity
+                for (int k = 0; k < 3; ++k) {
+                    s->refl[3 * (size_t)p + k] = color ? color[3 * (size_t)f + k] : 0.0f;
+                    s->emission[3 * (size_t)p + k] = emission ? emission[3 * (size_t)f + k] : 0.0f;
+                    s->shot0[3 * (size_t)p + k] = shot0 ? shot0[3 * (size_t)f + k] : 0.0f;
+                }
+            }
+        }
+    }
     *out = s;
     return RAD_OK;
 }
