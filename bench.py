@@ -154,6 +154,34 @@ def dense_leg(R, args):
     return out
 
 
+def ingest_leg(R):
+    """The step before the path at 200k patches: the Cornell faces split into
+    about 200,000 triangle patches, built on the host and uploaded
+    (rad_scene_faces + rad_scene_upload + rad_set_state) against built on the
+    device (rad_scene_ingest); both give the same arrays."""
+    import numpy as np
+    q, c, e, sh = R.cornell_faces()
+    area = 0.00226
+    s = R.Solver(hemicube=256, hemicubes=64)
+    s.ingest(q, area, R.RAD_FACES_TRIANGLES, c, e, sh)            # warm
+    t0 = time.perf_counter()
+    host = R.Scene.faces(q, area, R.RAD_FACES_TRIANGLES, c, e, sh)
+    t1 = time.perf_counter()
+    s.upload(host)
+    t2 = time.perf_counter()
+    nt, npch, nq = s.ingest(q, area, R.RAD_FACES_TRIANGLES, c, e, sh)
+    t3 = time.perf_counter()
+    same = bool(np.array_equal(s.download().tri, host.tri))
+    out = {"faces": int(q.shape[0]), "patches": npch, "triangles": nt, "quads": nq,
+           "host_build_ms": (t1 - t0) * 1e3, "host_upload_ms": (t2 - t1) * 1e3,
+           "device_ingest_ms": (t3 - t2) * 1e3, "same_triangles": same,
+           "speedup": (t2 - t0) / (t3 - t2)}
+    log(f"ingest {npch} patches: host build {out['host_build_ms']:.1f} ms + upload "
+        f"{out['host_upload_ms']:.1f} ms vs device {out['device_ingest_ms']:.1f} ms "
+        f"({out['speedup']:.1f}x), same triangles {same}")
+    return out
+
+
 def pack_leg(R, args):
     """256 independent box rooms, 500-20,000 triangle patches (log-uniform,
     seed 5), hemicube 64 / 128 / 256 by size, solved to the residual target
@@ -351,6 +379,7 @@ def main():
         configs = {"cornell1500_W128_100": cornell_leg(R, O, 0.3127, 128, 100, 1),
                    "cornell20k_W256_1000": cornell_leg(R, O, 0.02314, 256, 1000, 1),
                    "dense32k_R192": dense_leg(R, args),
+                   "ingest200k": ingest_leg(R),
                    "pack256": pack_leg(R, args)}
 
     # ------------------------------------------------------------ roofline

@@ -10,8 +10,8 @@
 //
 // Differences by design: patches keep their host fields (radiosity,
 // illumination) as the reference's do, and the GPU state is synchronised
-// around the calls that need it; the 8-neighbour links and the display
-// path are not part of this layer.
+// around the calls that need it; the GL display path is not part of this
+// layer (its vertex colours are, through smoothShadeAll).
 #pragma once
 
 #include <stdint.h>
@@ -32,6 +32,12 @@ struct Vector3f {
     Vector3f operator-(const Vector3f &o) const { return {x - o.x, y - o.y, z - o.z}; }
     Vector3f operator*(const Vector3f &o) const { return {x * o.x, y * o.y, z * o.z}; }
     Vector3f operator*(float t) const { return {x * t, y * t, z * t}; }
+    // Vector.h:448-452: a multiply by the reciprocal
+    Vector3f operator/(float t) const
+    {
+        t = 1 / t;
+        return {x * t, y * t, z * t};
+    }
     Vector3f &operator+=(const Vector3f &o)
     {
         x += o.x;
@@ -98,9 +104,18 @@ class Patch {
     Vector3f getUp();
     Vector3f getColor() { return color; }
     float getReflectivity() { return REFLECTIVITY; }
+    Patch **getNeighbours() { return neighbours; }
 
     Vector3f radiosity;    // unshot
     Vector3f illumination; // shot
+
+    // Patch.h:48-49: links to the 8 surrounding patches clockwise from the
+    // top-left (a patch without a neighbour there links to itself; NULL
+    // until Patch::divide, Model::subdivide or updateData sets them), and
+    // their indices in the scene (filled by updateData).
+    unsigned int relativeNeighbours[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    Patch *neighbours[8] = {nullptr, nullptr, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, nullptr};
 
   protected:
     Vector3f vec1, vec2, vec3, vec4;
@@ -130,6 +145,25 @@ class PrimitiveModel : public Model {
     int type;
     bool built = false;
 };
+
+// WaveFrontModel (WaveFrontModel.h): the faces of an OBJ file
+// (rad_obj_parse; coordinates / 1000, triangles as quads A B C C), black
+// patches as the reference makes them (Patch(v1, v2, v3, v4)).  getPatches
+// subdivides on every call with area > 0, as WaveFrontModel.cpp:145-151 does.
+class WaveFrontModel : public Model {
+  public:
+    explicit WaveFrontModel(std::string filename);
+    bool parse(std::string filename);
+    std::vector<Patch *> *getPatches(double area = 0) override;
+};
+
+// Colors::smoothShadePatch (Colors.cpp:195-262): the 12 vertex colour
+// components of a patch (corners 1..4 = left-bottom, right-bottom,
+// right-top, left-top), each the mean of colour * (illumination +
+// radiosity) over the patch and its three neighbours at that corner.
+namespace Colors {
+void smoothShadePatch(float *colors, Patch *p);
+}
 
 // ModelContainer (ModelContainer.h:11-53) whose shooter selection runs on
 // the device.  updateData() also uploads the scene to a context made from the
@@ -168,6 +202,11 @@ class ModelContainer {
     // requires the same scene and writes the fields back.
     bool saveToFile(const char *path);
     bool loadFromFile(const char *path);
+
+    // The vertex-colour loop of the display (Main.cpp:1319-1336) for every
+    // patch at once on the device, from the device state: colors receives
+    // getPatchesCount() * 12 floats in Colors::smoothShadePatch's layout.
+    bool smoothShadeAll(float *colors);
 
     // library context of the uploaded scene (created by updateData)
     rad_ctx *context();

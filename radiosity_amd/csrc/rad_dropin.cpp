@@ -9,9 +9,11 @@
 
 #include <algorithm>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/radhip_dropin.hpp"
+#include "../../include/radhip_scene.h"
 #include "rad_patch.h"
 
 namespace radhip {
@@ -118,8 +120,9 @@ bool Config::isFrozen() { return cfg.frozen; }
 // ------------------------------------------------------------------- Patch
 
 Patch::Patch() {}
+// Patch.cpp:8-13: a patch made from corners alone is black
 Patch::Patch(Vector3f v1, Vector3f v2, Vector3f v3, Vector3f v4)
-    : vec1(v1), vec2(v2), vec3(v3), vec4(v4), color(1, 1, 1)
+    : vec1(v1), vec2(v2), vec3(v3), vec4(v4), color(0, 0, 0)
 {
 }
 Patch::Patch(Vector3f v1, Vector3f v2, Vector3f v3, Vector3f v4, Vector3f c)
@@ -143,15 +146,25 @@ std::vector<Patch *> *Patch::divide(double area)
     q.v[1] = toV3(vec2);
     q.v[2] = toV3(vec3);
     q.v[3] = toV3(vec4);
-    std::vector<radpatch::Quad> kids;
-    if (!radpatch::divide(q, area, kids))
+    unsigned kx, ky;
+    if (!radpatch::divide_grid(q, area, kx, ky))
         return nullptr;
     auto *out = new std::vector<Patch *>();
-    out->reserve(kids.size());
+    out->reserve((size_t)kx * ky);
     // Patch.cpp:119-122: children inherit colour, illumination and radiosity
-    for (const radpatch::Quad &k : kids)
-        out->push_back(new Patch(fromV3(k.v[0]), fromV3(k.v[1]), fromV3(k.v[2]), fromV3(k.v[3]),
-                                 color, illumination, radiosity));
+    for (unsigned i = 0; i < kx * ky; ++i) {
+        radpatch::V3 v[4];
+        radpatch::divide_child(q.v, kx, ky, i, v);
+        out->push_back(new Patch(fromV3(v[0]), fromV3(v[1]), fromV3(v[2]), fromV3(v[3]), color,
+                                 illumination, radiosity));
+    }
+    // Patch.cpp:127-222
+    for (unsigned i = 0; i < kx * ky; ++i) {
+        unsigned nb[8];
+        radpatch::divide_neighbours(kx, ky, i, nb);
+        for (int k = 0; k < 8; ++k)
+            (*out)[i]->neighbours[k] = (*out)[nb[k]];
+    }
     return out;
 }
 
@@ -213,6 +226,10 @@ void Model::subdivide(double area)
             delete kids;
             delete p;
         } else {
+            // Model.cpp:42-48: an undivided patch links to itself
+            for (Patch *&n : p->neighbours)
+                if (!n)
+                    n = p;
             out->push_back(p);
         }
     }
@@ -235,6 +252,66 @@ std::vector<Patch *> *PrimitiveModel::getPatches(double area)
         subdivide(area);
     }
     return patches;
+}
+
+WaveFrontModel::WaveFrontModel(std::string filename)
+{
+    if (!parse(filename))
+        setErr("Unable to load model '" + filename + "'");
+}
+
+// WaveFrontModel.cpp:15-136 through rad_obj_parse; the faces read before a
+// bad vertex line are kept, as the reference keeps them.
+bool WaveFrontModel::parse(std::string filename)
+{
+    float *q = nullptr;
+    uint32_t n = 0;
+    int rc = rad_obj_parse(filename.c_str(), &q, &n);
+    for (uint32_t f = 0; f < n; ++f) {
+        const float *v = q + 12 * (size_t)f;
+        patches->push_back(new Patch({v[0], v[1], v[2]}, {v[3], v[4], v[5]}, {v[6], v[7], v[8]},
+                                     {v[9], v[10], v[11]}));
+    }
+    rad_obj_free(q);
+    return rc == RAD_OK;
+}
+
+// WaveFrontModel.cpp:145-151
+std::vector<Patch *> *WaveFrontModel::getPatches(double area)
+{
+    if (area > 0)
+        subdivide(area);
+    return patches;
+}
+
+// Colors.cpp:195-262
+void Colors::smoothShadePatch(float *colors, Patch *p)
+{
+    auto val = [](Patch *q) { return q->getColor() * (q->illumination + q->radiosity); };
+    Patch **n = p->neighbours;
+    Vector3f lt = val(p), rt = val(p), rb = val(p), lb = val(p);
+    lt += val(n[7]);
+    lt += val(n[0]);
+    lt += val(n[1]);
+    lt = lt / 4;
+    rt += val(n[1]);
+    rt += val(n[2]);
+    rt += val(n[3]);
+    rt = rt / 4;
+    rb += val(n[3]);
+    rb += val(n[4]);
+    rb += val(n[5]);
+    rb = rb / 4;
+    lb += val(n[5]);
+    lb += val(n[6]);
+    lb += val(n[7]);
+    lb = lb / 4;
+    const Vector3f out[4] = {lb, rb, rt, lt};
+    for (int k = 0; k < 4; ++k) {
+        colors[3 * k] = out[k].x;
+        colors[3 * k + 1] = out[k].y;
+        colors[3 * k + 2] = out[k].z;
+    }
 }
 
 // ---------------------------------------------------------- ModelContainer
@@ -293,6 +370,10 @@ void ModelContainer::updateData()
             vertices.insert(vertices.end(), v.begin(), v.end());
             for (int k : {0, 1, 2, 0, 2, 3})
                 indices.push_back(offset + k);
+            // ModelContainer.cpp:112-116
+            for (Patch *&n : p->neighbours)
+                if (!n)
+                    n = p;
             patchList.push_back(p);
             offset += 4;
         }
@@ -364,7 +445,33 @@ void ModelContainer::updateData()
         s[3 * i + 2] = p->illumination.z;
     }
     emission0 = u;
-    libErr(ctx, rad_set_state(ctx, u.data(), s.data(), u.data()), "rad_set_state");
+    if (libErr(ctx, rad_set_state(ctx, u.data(), s.data(), u.data()), "rad_set_state") != RAD_OK)
+        return;
+    // neighbour links as scene indices (a link to a patch outside the scene
+    // counts as a link to itself); every patch is its own quad
+    std::unordered_map<const Patch *, uint32_t> index;
+    index.reserve(n);
+    for (uint32_t i = 0; i < n; ++i)
+        index[patchList[i]] = i;
+    std::vector<uint32_t> pq(n), nbr((size_t)n * 8);
+    for (uint32_t i = 0; i < n; ++i) {
+        pq[i] = i;
+        for (int k = 0; k < 8; ++k) {
+            auto it = index.find(patchList[i]->neighbours[k]);
+            uint32_t j = it == index.end() ? i : it->second;
+            patchList[i]->relativeNeighbours[k] = j;
+            nbr[(size_t)i * 8 + k] = j;
+        }
+    }
+    libErr(ctx, rad_set_quads(ctx, pq.data(), n, nbr.data()), "rad_set_quads");
+}
+
+bool ModelContainer::smoothShadeAll(float *colors)
+{
+    rad_ctx *c = context();
+    if (!c || patchList.empty() || !colors)
+        return false;
+    return libErr(c, rad_vertex_colors(c, colors), "rad_vertex_colors") == RAD_OK;
 }
 
 float *ModelContainer::getVertices()

@@ -8,10 +8,14 @@
 //   dropin_check shoot AREA W K SHOOTS CYCLES  Shooter cycles, final fields
 //   dropin_check process AREA W                ProcessHemicube triplets
 //   dropin_check resume AREA W PATH            shoot, saveToFile, loadFromFile
+//   dropin_check obj PATH AREA                 WaveFrontModel patches and links (CPU)
+//   dropin_check smooth AREA W CYCLES          smoothShadePatch vs smoothShadeAll
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <random>
 #include <string>
 #include <vector>
@@ -98,6 +102,54 @@ int main(int argc, char **argv)
         vec("table", first, true);
         printf("}\n");
         delete[] f;
+        return 0;
+    }
+    if (mode == "obj" && argc == 4) {
+        WaveFrontModel m(argv[2]);
+        std::vector<Patch *> *ps = m.getPatches(atof(argv[3]));
+        std::vector<double> v, nb;
+        for (size_t i = 0; i < ps->size(); ++i) {
+            std::vector<float> c = (*ps)[i]->getVerticesCoords();
+            v.insert(v.end(), c.begin(), c.end());
+            for (Patch *n : (*ps)[i]->neighbours) {
+                size_t j = std::find(ps->begin(), ps->end(), n) - ps->begin();
+                nb.push_back(j == ps->size() ? -1.0 : (double)j);
+            }
+        }
+        Vector3f col = ps->empty() ? Vector3f() : (*ps)[0]->getColor();
+        printf("{\"n\": %zu, \"error\": %d, \"color0\": [%g, %g, %g], ", ps->size(),
+               lastError().empty() ? 0 : 1, col.x, col.y, col.z);
+        vec("vertices", v);
+        vec("neighbours", nb, true);
+        printf("}\n");
+        return 0;
+    }
+    if (mode == "smooth" && argc == 5) {
+        Config::setHemicubeSide((unsigned)atoi(argv[3]));
+        Config::setHemicubesCount(4);
+        Config::setShootsPerCycle(5);
+        Config::setMaxPatchArea(atof(argv[2]));
+        Config::freeze();
+        ModelContainer mc;
+        mc.maxPatchArea = Config::MAX_PATCH_AREA();
+        mc.load();
+        mc.updateData();
+        Shooter sh(&mc);
+        for (int c = 0; c < atoi(argv[4]); ++c)
+            sh.shootCycle();
+        unsigned n = mc.getPatchesCount();
+        std::vector<float> dev((size_t)n * 12), host((size_t)n * 12);
+        bool ok = mc.smoothShadeAll(dev.data());
+        for (unsigned i = 0; i < n; ++i)
+            Colors::smoothShadePatch(&host[(size_t)i * 12], mc.getPatches()[i]);
+        unsigned same = 0;
+        double mx = 0;
+        for (size_t i = 0; i < host.size(); ++i) {
+            same += memcmp(&host[i], &dev[i], 4) == 0;
+            mx = std::max(mx, (double)std::fabs(host[i]));
+        }
+        printf("{\"ok\": %d, \"n\": %u, \"values\": %zu, \"same\": %u, \"max\": %g}\n",
+               ok ? 1 : 0, n, host.size(), same, mx);
         return 0;
     }
     if (mode == "resume" && argc == 5) {

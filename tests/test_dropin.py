@@ -64,6 +64,36 @@ def test_cli_rejects_bad_arguments():
     assert p.returncode == 2 and "unknown parameter" in p.stderr
 
 
+SMALL_OBJ = (
+    "v 0 0 0\nv 2000 0 0\nv 2000 0 1500\nv 0 0 1500\nv 1000 800 700\n"
+    "f 1 2 3 4\nf 1/1 2/2 5/5\nf 4 3 5\n"
+)
+
+
+def test_dropin_wavefront_model(rad, tmp_path):
+    """WaveFrontModel: the OBJ faces subdivided by Patch::divide, black,
+    with the neighbour links of Patch.cpp:127-222, as the face-scene builder
+    produces them."""
+    path = tmp_path / "small.obj"
+    path.write_text(SMALL_OBJ)
+    d = run("dropin_check", "obj", str(path), 0.2)
+    faces, ok = rad.obj_parse(str(path))
+    assert ok and d["error"] == 0 and d["color0"] == [0, 0, 0]
+    sc = rad.Scene.faces(faces, 0.2, rad.RAD_FACES_QUADS)
+    assert d["n"] == sc.nquad
+    tri = sc.tri.reshape(-1, 2, 3, 3)
+    quads = np.concatenate([tri[:, 0], tri[:, 1, 2:3]], axis=1)
+    assert np.array_equal(np.array(d["vertices"], np.float32), quads.reshape(-1))
+    assert np.array_equal(np.array(d["neighbours"]).reshape(-1, 8), sc.quad_nbr)
+    d = run("dropin_check", "obj", str(tmp_path / "missing.obj"), 0.2)
+    assert d["n"] == 0 and d["error"] == 1
+
+
+def test_cli_rejects_bad_view():
+    p = run("radhip_cli", "view", "1,2,3", ok=False)
+    assert p.returncode == 2 and "view takes 9" in p.stderr
+
+
 # ------------------------------------------------------------------- GPU
 
 @pytest.mark.gpu
@@ -132,3 +162,46 @@ def test_cli_runs_to_the_reference_stop_rule():
     d = json.loads(p.stdout.strip().splitlines()[-1])
     assert d["last_energy"] < 0.1 and d["cycles"] < 200
     assert d["shooters"] > 0 and 0 <= d["residual"] < 1
+
+
+@pytest.mark.gpu
+def test_dropin_smooth_shade_device_equals_host():
+    """ModelContainer::smoothShadeAll (device) equals Colors::smoothShadePatch
+    over the patch fields (host) for every patch, bit for bit."""
+    d = run("dropin_check", "smooth", 0.3127, 64, 3)
+    assert d["ok"] == 1 and d["values"] == 12 * d["n"]
+    assert d["same"] == d["values"] and d["max"] > 0
+
+
+@pytest.mark.gpu
+def test_cli_exports_ply_and_ppm(tmp_path):
+    ply, ppm = tmp_path / "out.ply", tmp_path / "out.ppm"
+    p = subprocess.run([os.path.join(BIN, "radhip_cli"), "area", "0.5", "hemicube", "64",
+                        "shoots", "20", "hemicubes", "4", "cycles", "30", "json", "1",
+                        "ply", str(ply), "ppm", str(ppm)],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    n = json.loads(p.stdout.strip().splitlines()[-1])["patches"]
+    data = ply.read_bytes()
+    head, body = data.split(b"end_header\n", 1)
+    assert f"element vertex {4 * n}".encode() in head and f"element face {n}".encode() in head
+    assert len(body) == 4 * n * 15 + n * 17
+    img = ppm.read_bytes()
+    assert img.startswith(b"P6\n64 64\n255\n")
+    pix = np.frombuffer(img[len(b"P6\n64 64\n255\n"):], np.uint8)
+    assert pix.size == 64 * 64 * 3 and pix.max() > 0
+    # the open side is closed by ROOMCLOSURE: inside the box no pixel is background
+    assert np.count_nonzero(pix.reshape(-1, 3).max(1)) > 0.9 * 64 * 64
+
+
+@pytest.mark.gpu
+def test_cli_loads_obj(tmp_path):
+    path = tmp_path / "small.obj"
+    path.write_text(SMALL_OBJ)
+    p = subprocess.run([os.path.join(BIN, "radhip_cli"), "obj", str(path), "area", "0.2",
+                        "hemicube", "32", "json", "1"],
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    d = json.loads(p.stdout.strip().splitlines()[-1])
+    assert d["patches"] > 3 and d["sum_rgb"] == [0, 0, 0]   # black, nothing emits
+
