@@ -342,7 +342,7 @@ static inline int32_t floor_shift8(int32_t v) { return v >> 8; } /* arithmetic *
 
 static uint64_t raster_tri(uint32_t W, const sv *a, const sv *b, const sv *c,
                            const int32_t *sc_local, uint32_t id, const int32_t *vp,
-                           uint32_t TW, uint32_t *item, float *zbuf)
+                           uint32_t TW, uint32_t *item, float *zbuf, int flags)
 {
     int64_t area2 = (int64_t)(b->x - a->x) * (c->y - a->y) -
                     (int64_t)(c->x - a->x) * (b->y - a->y);
@@ -400,6 +400,8 @@ static uint64_t raster_tri(uint32_t W, const sv *a, const sv *b, const sv *c,
                 z = 0.0f;
             if (!(z < 1.0f))
                 continue;
+            if (flags & ORC_VAR_DEPTH24) /* stored as round(z (2^24 - 1)) */
+                z = (float)((double)lrint((double)z * 16777215.0) / 16777215.0);
             uint32_t tx = (uint32_t)(vp[0] + ix), ty = (uint32_t)(vp[1] + iy);
             uint32_t texel = tx + ty * TW;
             if (z < zbuf[texel]) {
@@ -411,9 +413,34 @@ static uint64_t raster_tri(uint32_t W, const sv *a, const sv *b, const sv *c,
     return frags;
 }
 
+static uint64_t render_impl(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                            uint32_t ntri, const float *frame, uint32_t *item,
+                            float *depth_or_null, int flags);
+
 uint64_t orc_render_hemicube(uint32_t W, const float *tri, const uint32_t *tri_patch,
                              uint32_t ntri, const float *frame, uint32_t *item,
                              float *depth_or_null)
+{
+    return render_impl(W, tri, tri_patch, ntri, frame, item, depth_or_null, 0);
+}
+
+/* The same render with arithmetic a GL driver may use instead, to measure
+ * how many item-buffer texels such differences change (flags may be
+ * combined):
+ *   ORC_VAR_DEPTH24  depth rounded to 24-bit fixed point before the GL_LESS
+ *                    test (the reference's GL_DEPTH_COMPONENT24, Main.cpp:288)
+ *   ORC_VAR_FMA      clip coordinates as fused multiply-adds
+ *   ORC_VAR_DIVIDE   x / w instead of x * (1 / w) */
+uint64_t orc_render_hemicube_variant(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                                     uint32_t ntri, const float *frame, uint32_t *item,
+                                     int flags)
+{
+    return render_impl(W, tri, tri_patch, ntri, frame, item, NULL, flags);
+}
+
+static uint64_t render_impl(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                            uint32_t ntri, const float *frame, uint32_t *item,
+                            float *depth_or_null, int flags)
 {
     uint32_t TW, TH, RES;
     orc_geometry(W, &TW, &TH, &RES);
@@ -441,8 +468,12 @@ uint64_t orc_render_hemicube(uint32_t W, const float *tri, const uint32_t *tri_p
                 float x = tri[t * 9 + k * 3 + 0], y = tri[t * 9 + k * 3 + 1],
                       z = tri[t * 9 + k * 3 + 2];
                 for (int r = 0; r < 4; ++r)
-                    poly[k].c[r] = m[0 * 4 + r] * x + m[1 * 4 + r] * y +
-                                   m[2 * 4 + r] * z + m[3 * 4 + r];
+                    poly[k].c[r] = (flags & ORC_VAR_FMA)
+                                       ? fmaf(m[0 * 4 + r], x,
+                                              fmaf(m[1 * 4 + r], y, fmaf(m[2 * 4 + r], z,
+                                                                         m[3 * 4 + r])))
+                                       : m[0 * 4 + r] * x + m[1 * 4 + r] * y +
+                                             m[2 * 4 + r] * z + m[3 * 4 + r];
             }
             int n = clip_polygon(poly, 3);
             if (n < 3)
@@ -451,6 +482,11 @@ uint64_t orc_render_hemicube(uint32_t W, const float *tri, const uint32_t *tri_p
             for (int k = 0; k < n; ++k) {
                 float iw = 1.0f / poly[k].c[3];
                 float xn = poly[k].c[0] * iw, yn = poly[k].c[1] * iw, zn = poly[k].c[2] * iw;
+                if (flags & ORC_VAR_DIVIDE) {
+                    xn = poly[k].c[0] / poly[k].c[3];
+                    yn = poly[k].c[1] / poly[k].c[3];
+                    zn = poly[k].c[2] / poly[k].c[3];
+                }
                 float xl = (xn + 1.0f) * halfW, yl = (yn + 1.0f) * halfW;
                 s[k].x = (int32_t)rintf(xl * (float)ORC_SUB);
                 s[k].y = (int32_t)rintf(yl * (float)ORC_SUB);
@@ -458,7 +494,8 @@ uint64_t orc_render_hemicube(uint32_t W, const float *tri, const uint32_t *tri_p
             }
             uint32_t id = tri_patch[t] + 1u;
             for (int k = 1; k + 1 < n; ++k)
-                frags += raster_tri(W, &s[0], &s[k], &s[k + 1], scl, id, vp, TW, item, zbuf);
+                frags += raster_tri(W, &s[0], &s[k], &s[k + 1], scl, id, vp, TW, item, zbuf,
+                                    flags);
         }
     }
     if (!depth_or_null)

@@ -56,6 +56,12 @@ void orc_face_viewport(uint32_t W, int face, int32_t *vp4, int32_t *sc4);
 uint64_t orc_render_hemicube(uint32_t W, const float *tri, const uint32_t *tri_patch,
                              uint32_t ntri, const float *frame9,
                              uint32_t *item, float *depth_or_null);
+#define ORC_VAR_DEPTH24 1
+#define ORC_VAR_FMA 2
+#define ORC_VAR_DIVIDE 4
+uint64_t orc_render_hemicube_variant(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                                     uint32_t ntri, const float *frame, uint32_t *item,
+                                     int flags);
 
 /* Kernel_ProcessHemicube.h:23-69 run-length form + host accumulation
  * Main.cpp:1251-1269 for K stacked hemicubes (rows TEX_H*K).

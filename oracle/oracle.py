@@ -27,6 +27,7 @@ _SIG = {
     "orc_face_mvp": (None, [_PF, C.c_int, _PF]),
     "orc_face_viewport": (None, [_u32, C.c_int, _PI, _PI]),
     "orc_render_hemicube": (C.c_uint64, [_u32, _PF, _PU, _u32, _PF, _PU, _PF]),
+    "orc_render_hemicube_variant": (C.c_uint64, [_u32, _PF, _PU, _u32, _PF, _PU, C.c_int]),
     "orc_process_hemicube_runs": (C.c_uint64, [_u32, _u32, _u32, _PU, _PF, _u32, _PF,
                                                C.POINTER(C.c_uint64)]),
     "orc_hemicube_histogram": (None, [_u32, _u32, _PU, _PF, _u32, _PU, _PD]),
@@ -125,6 +126,20 @@ def render_hemicube(W, tri, tri_patch, frame, depth=False):
     z = np.zeros(res, np.float32) if depth else None
     frags = lib().orc_render_hemicube(W, _f(tri), _u(tp), tri.shape[0], _f(fr), _u(item), _f(z))
     return (item, z, frags) if depth else item
+
+
+VAR_DEPTH24, VAR_FMA, VAR_DIVIDE = 1, 2, 4
+
+
+def render_hemicube_variant(W, tri, tri_patch, frame, flags):
+    """render_hemicube with the arithmetic of another implementation
+    (VAR_* flags): 24-bit depth, FMA transform, divide by w."""
+    tri = np.ascontiguousarray(tri, np.float32).reshape(-1, 9)
+    tp = np.ascontiguousarray(tri_patch, np.uint32)
+    fr = np.ascontiguousarray(frame, np.float32).reshape(9)
+    item = np.zeros(geometry(W)[2], np.uint32)
+    lib().orc_render_hemicube_variant(W, _f(tri), _u(tp), tri.shape[0], _f(fr), _u(item), flags)
+    return item
 
 
 def process_runs(W, K, item, dff, npatch, workitems_x=4):

@@ -500,3 +500,43 @@ def test_gather_jacobi_residual_measure(orc):
     err = float((area[None, :] * np.abs(B1 - B2)).sum()) / den
     # ||B* - B_k|| <= q/(1-q) ||B_k - B_k-1||, q = max rho * max row sum < 0.8
     assert err <= 1e-3 * 0.8 / 0.2 * 1.5
+
+
+# ------------------------------------------ item buffer vs a GL driver's arithmetic
+
+def _on_boundary(a, b, W):
+    """Every texel where a and b differ has, in a 3 x 3 neighbourhood of one
+    of the two buffers, both of its ids: it lies on the edge between them."""
+    TW, TH, _ = O.geometry(W)
+    A, B = a.reshape(TH, TW), b.reshape(TH, TW)
+    for y, x in zip(*np.nonzero(A != B)):
+        ys, xs = slice(max(0, y - 1), y + 2), slice(max(0, x - 1), x + 2)
+        ids = {int(A[y, x]), int(B[y, x])}
+        if not (ids <= set(A[ys, xs].ravel()) or ids <= set(B[ys, xs].ravel())):
+            return False
+    return True
+
+
+@pytest.mark.parametrize("flags", [O.VAR_DEPTH24, O.VAR_FMA, O.VAR_DIVIDE,
+                                   O.VAR_DEPTH24 | O.VAR_FMA | O.VAR_DIVIDE])
+def test_item_buffer_sensitivity_to_driver_arithmetic(rad, flags):
+    """The GL item buffer of the reference is not reproducible here, so bound
+    how much it could differ: render with the arithmetic a driver may use
+    instead (24-bit depth as the reference's GL_DEPTH_COMPONENT24, FMA
+    transforms, a divide by w) and count the changed texels.  They stay
+    below 0.01 % and lie on triangle edges only."""
+    sc = rad.Scene.cornell(0.3127, triangle_patches=False, physical=False)
+    rng = np.random.default_rng(1)
+    diff = total = 0
+    for p in rng.choice(sc.npatch, 40, replace=False):
+        a = O.render_hemicube(128, sc.tri, sc.tri_patch, sc.frame[p])
+        b = O.render_hemicube_variant(128, sc.tri, sc.tri_patch, sc.frame[p], flags)
+        diff += int((a != b).sum())
+        total += a.size
+        assert _on_boundary(a, b, 128)
+    print(f"flags {flags}: {diff} of {total} texels differ ({100 * diff / total:.4f} %)")
+    assert diff / total < 1e-4
+    if flags == O.VAR_DEPTH24:
+        # float depth in [0.5, 1) is coarser than 24-bit fixed point, so the
+        # rounding keeps every order and every tie
+        assert diff == 0
