@@ -82,49 +82,64 @@ __global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t
 // Shooting-consistent layout: row j of G holds G_ji = F_ij A_i / A_j, the
 // factor PHYSICAL shooting applies when shooter i sends to receiver j
 // (k_shoot), so B = E + diag(rho) G B is the fixed point of the shooting
-// iteration.  Pass rows row0 .. row0+k-1 of F become columns of G: one lane
-// writes 8 consecutive columns (16 B) of one row j, never a column outside
-// [row0, row0+k).
-__global__ void k_dense_store_t(const float *__restrict__ ff, uint32_t k, uint32_t row0,
-                                uint32_t npatch, uint32_t npad, const float *__restrict__ area,
-                                uint16_t *__restrict__ fhi, uint16_t *__restrict__ flo)
+// iteration.  Pass rows row0 .. row0+k-1 of F become columns of G.  A block
+// takes 64 receivers; for each chunk of 64 shooters it reads the F entries
+// along the rows (coalesced), scales and splits them, transposes them
+// through LDS and writes 128-byte runs of each G row (8 lanes x 16 B of hi
+// and of lo).  Only the k columns of this pass are written.
+__global__ __launch_bounds__(256) void k_dense_store_t(const float *__restrict__ ff, uint32_t k,
+                                                       uint32_t row0, uint32_t npatch,
+                                                       uint32_t npad,
+                                                       const float *__restrict__ area,
+                                                       uint16_t *__restrict__ fhi,
+                                                       uint16_t *__restrict__ flo)
 {
-    uint32_t groups = (k + 7) / 8;
-    size_t total = (size_t)npatch * groups;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
-         e += (size_t)gridDim.x * blockDim.x) {
-        uint32_t g = (uint32_t)(e / npatch), j = (uint32_t)(e - (size_t)g * npatch);
-        float inv_aj = 1.0f / area[j];
-        uint16_t h[8], l[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            uint32_t s = g * 8 + q;
+    __shared__ uint32_t tile[64][65]; // hi | lo << 16 per (receiver, shooter)
+    const uint32_t t = threadIdx.x, jl = t & 63;
+    const uint32_t j0 = blockIdx.x * 64, j = j0 + jl;
+    const bool jok = j < npatch;
+    const float inv_aj = jok ? 1.0f / area[j] : 0.0f;
+    for (uint32_t c0 = 0; c0 < k; c0 += 64) {
+        const uint32_t nc = min(64u, k - c0);
+        for (uint32_t sl = t >> 6; sl < 64; sl += 4) {
+            const uint32_t sh = c0 + sl;
             float f = 0.0f;
-            if (s < k)
-                f = ff[(size_t)s * npatch + j] * area[row0 + s] * inv_aj;
-            h[q] = bf16_rne(f);
-            l[q] = bf16_rne(f - bf16_val(h[q]));
+            if (jok && sl < nc)
+                f = ff[(size_t)sh * npatch + j] * area[row0 + sh] * inv_aj;
+            const uint16_t h = bf16_rne(f), l = bf16_rne(f - bf16_val(h));
+            tile[jl][sl] = (uint32_t)h | ((uint32_t)l << 16);
         }
-        size_t o = (size_t)j * npad + row0 + g * 8;
-        if ((row0 & 7u) == 0 && g * 8 + 8 <= k) {
-            uint4 vh, vl;
-            vh.x = h[0] | ((uint32_t)h[1] << 16);
-            vh.y = h[2] | ((uint32_t)h[3] << 16);
-            vh.z = h[4] | ((uint32_t)h[5] << 16);
-            vh.w = h[6] | ((uint32_t)h[7] << 16);
-            vl.x = l[0] | ((uint32_t)l[1] << 16);
-            vl.y = l[2] | ((uint32_t)l[3] << 16);
-            vl.z = l[4] | ((uint32_t)l[5] << 16);
-            vl.w = l[6] | ((uint32_t)l[7] << 16);
-            *(uint4 *)(fhi + o) = vh;
-            *(uint4 *)(flo + o) = vl;
-        } else {
-            // ragged group: only the k columns of this pass
-            for (uint32_t q = 0; q < 8 && g * 8 + q < k; ++q) {
-                fhi[o + q] = h[q];
-                flo[o + q] = l[q];
+        __syncthreads();
+        const uint32_t s8 = 8 * (t & 7);
+        for (uint32_t r = t >> 3; r < 64; r += 32) {
+            const uint32_t jj = j0 + r;
+            if (jj >= npatch || s8 >= nc)
+                continue;
+            const size_t o = (size_t)jj * npad + row0 + c0 + s8;
+            uint32_t w[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                w[e] = tile[r][s8 + e];
+            if (((row0 + c0) & 7u) == 0 && s8 + 8 <= nc) {
+                uint4 vh, vl;
+                vh.x = (w[0] & 0xFFFFu) | (w[1] << 16);
+                vh.y = (w[2] & 0xFFFFu) | (w[3] << 16);
+                vh.z = (w[4] & 0xFFFFu) | (w[5] << 16);
+                vh.w = (w[6] & 0xFFFFu) | (w[7] << 16);
+                vl.x = (w[0] >> 16) | (w[1] & 0xFFFF0000u);
+                vl.y = (w[2] >> 16) | (w[3] & 0xFFFF0000u);
+                vl.z = (w[4] >> 16) | (w[5] & 0xFFFF0000u);
+                vl.w = (w[6] >> 16) | (w[7] & 0xFFFF0000u);
+                *(uint4 *)(fhi + o) = vh;
+                *(uint4 *)(flo + o) = vl;
+            } else {
+                for (uint32_t e = 0; e < 8 && s8 + e < nc; ++e) {
+                    fhi[o + e] = (uint16_t)(w[e] & 0xFFFFu);
+                    flo[o + e] = (uint16_t)(w[e] >> 16);
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -440,10 +455,8 @@ void launch_dense_store_t(const float *ff, uint32_t k, uint32_t row0, uint32_t n
                           uint32_t npad, const float *area, uint16_t *fhi, uint16_t *flo,
                           hipStream_t st)
 {
-    size_t total = (size_t)npatch * ((k + 7) / 8);
-    size_t b = (total + 255) / 256;
-    k_dense_store_t<<<(unsigned)(b > 4096 ? 4096 : b), 256, 0, st>>>(ff, k, row0, npatch, npad,
-                                                                    area, fhi, flo);
+    k_dense_store_t<<<(npatch + 63) / 64, 256, 0, st>>>(ff, k, row0, npatch, npad, area, fhi,
+                                                        flo);
 }
 
 void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint32_t npatch,
