@@ -534,13 +534,9 @@ uint64_t scene_hash(uint32_t ntri, uint32_t npatch, const float *tri_xyz,
     return h;
 }
 
-int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
-                uint32_t npatch, const float *frames, const float *refl, const float *area,
-                uint32_t nblk, uint32_t rowlen)
+int upload_copy(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                uint32_t npatch, const float *frames, const float *refl, const float *area)
 {
-    int rc;
-    if ((rc = scene_alloc(c, ntri, npatch, nblk, rowlen)))
-        return rc;
     // triangles -> 9 planes
     std::vector<float> planes((size_t)ntri * 9);
     for (uint32_t t = 0; t < ntri; ++t)
@@ -556,6 +552,26 @@ int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uin
     HIPCHK(c, hipMemset(c->shot, 0, (size_t)npatch * 12));
     c->geom_hash = scene_hash(ntri, npatch, tri_xyz, tri_patch, frames, refl, area);
     c->hash_valid = true;
+    return RAD_OK;
+}
+
+// A scene whose buffers were allocated but not filled is no scene.
+int drop_scene(rad_ctx *c, int rc)
+{
+    c->ntri = c->npatch = c->nquad = 0;
+    c->hash_valid = false;
+    return rc;
+}
+
+int upload_impl(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri,
+                uint32_t npatch, const float *frames, const float *refl, const float *area,
+                uint32_t nblk, uint32_t rowlen)
+{
+    int rc;
+    if ((rc = scene_alloc(c, ntri, npatch, nblk, rowlen)))
+        return drop_scene(c, rc);
+    if ((rc = upload_copy(c, tri_xyz, tri_patch, ntri, npatch, frames, refl, area)))
+        return drop_scene(c, rc);
     return RAD_OK;
 }
 
@@ -1452,23 +1468,38 @@ int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
         (rc = dalloc(c, &p.cand, S * kmax)) || (rc = dalloc(c, &p.patch_scene, npatch)) ||
         (rc = dalloc(c, &p.pass, 1)) || (rc = dalloc(c, &p.slot_tbeg, c->K)) ||
         (rc = dalloc(c, &p.slot_tcnt, c->K)) || (rc = dalloc(c, &p.slot_pbeg, c->K))) {
+        // the F rows are sized for the largest scene, not for a plain scene
         pack_free(c);
-        return rc;
+        return drop_scene(c, rc);
     }
-    HIPCHK(c, hipMemcpy(p.sc_tbeg, scene_tri_beg, (S + 1) * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(p.sc_pbeg, scene_patch_beg, (S + 1) * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(p.sc_slots, scene_slots, S * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(p.patch_scene, patch_scene.data(), (size_t)npatch * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(p.sc_nsel, 0, S * 4));
-    HIPCHK(c, hipMemset(p.sc_slot_beg, 0, S * 4));
-    HIPCHK(c, hipMemset(p.sc_done, 0xFF, S * 4));
-    HIPCHK(c, hipMemset(p.sc_den, 0, S * 8));
-    HIPCHK(c, hipMemset(p.sc_res, 0, S * 8));
-    HIPCHK(c, hipMemset(p.sc_shot, 0, S * 4));
-    HIPCHK(c, hipMemset(p.pass, 0, 4));
-    HIPCHK(c, hipMemset(p.slot_tbeg, 0, c->K * 4));
-    HIPCHK(c, hipMemset(p.slot_tcnt, 0, c->K * 4));
-    HIPCHK(c, hipMemset(p.slot_pbeg, 0, c->K * 4));
+    hipError_t e = hipSuccess;
+    auto cp = [&](void *d, const void *h, size_t n) {
+        if (e == hipSuccess)
+            e = hipMemcpy(d, h, n, hipMemcpyHostToDevice);
+    };
+    auto set = [&](void *d, int v, size_t n) {
+        if (e == hipSuccess)
+            e = hipMemset(d, v, n);
+    };
+    cp(p.sc_tbeg, scene_tri_beg, (S + 1) * 4);
+    cp(p.sc_pbeg, scene_patch_beg, (S + 1) * 4);
+    cp(p.sc_slots, scene_slots, S * 4);
+    cp(p.patch_scene, patch_scene.data(), (size_t)npatch * 4);
+    set(p.sc_nsel, 0, S * 4);
+    set(p.sc_slot_beg, 0, S * 4);
+    set(p.sc_done, 0xFF, S * 4);
+    set(p.sc_den, 0, S * 8);
+    set(p.sc_res, 0, S * 8);
+    set(p.sc_shot, 0, S * 4);
+    set(p.pass, 0, 4);
+    set(p.slot_tbeg, 0, c->K * 4);
+    set(p.slot_tcnt, 0, c->K * 4);
+    set(p.slot_pbeg, 0, c->K * 4);
+    if (e != hipSuccess) {
+        pack_free(c);
+        return drop_scene(c, fail(c, RAD_E_HIP, std::string("pack upload: ") +
+                                                    hipGetErrorString(e)));
+    }
     p.nscene = nscene;
     p.kmax = kmax;
     p.K = c->K;
@@ -1669,10 +1700,10 @@ int rad_scene_ingest(rad_ctx *c, const float *quads, uint32_t nface, const float
         return fail(c, RAD_E_ARG, "ingest: the faces yield no patches");
     int rc;
     if ((rc = scene_alloc(c, nt, np, (nt + 255) / 256, np)))
-        return rc;
+        return drop_scene(c, rc);
     if ((rc = dalloc(c, &c->patch_quad, np)) || (rc = dalloc(c, &c->quad_pbeg, (size_t)nq + 1)) ||
         (rc = dalloc(c, &c->quad_nbr, (size_t)nq * 8)) || (rc = dalloc(c, &c->qval, (size_t)nq * 3)))
-        return rc;
+        return drop_scene(c, rc);
     // inputs of the kernel: faces, plans, per-face colour / radiosity / illumination
     float *dfaces = nullptr, *dcol = nullptr, *dem = nullptr, *dsh = nullptr;
     FacePlanDev *dplan = nullptr;
@@ -1691,7 +1722,7 @@ int rad_scene_ingest(rad_ctx *c, const float *quads, uint32_t nface, const float
         (emission && (rc = dalloc(c, &dem, (size_t)nface * 3))) ||
         (shot0 && (rc = dalloc(c, &dsh, (size_t)nface * 3)))) {
         cleanup();
-        return rc;
+        return drop_scene(c, rc);
     }
     hipError_t e = hipMemcpyAsync(dfaces, quads, (size_t)nface * 48, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess)
@@ -1714,19 +1745,17 @@ int rad_scene_ingest(rad_ctx *c, const float *quads, uint32_t nface, const float
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->st);
     cleanup();
-    if (e != hipSuccess) {
-        c->npatch = c->ntri = 0;
-        return fail(c, RAD_E_HIP, std::string("ingest: ") + hipGetErrorString(e));
-    }
+    if (e != hipSuccess)
+        return drop_scene(c, fail(c, RAD_E_HIP, std::string("ingest: ") + hipGetErrorString(e)));
     // a patch of zero area would break the area-normalised shooting
     std::vector<float> ar(np);
-    HIPCHK(c, hipMemcpy(ar.data(), c->area, (size_t)np * 4, hipMemcpyDeviceToHost));
+    e = hipMemcpy(ar.data(), c->area, (size_t)np * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return drop_scene(c, fail(c, RAD_E_HIP, std::string("ingest: ") + hipGetErrorString(e)));
     for (uint32_t p = 0; p < np; ++p)
-        if (!(ar[p] > 0.0f)) {
-            c->npatch = c->ntri = 0;
-            return fail(c, RAD_E_ARG, "ingest: patch " + std::to_string(p) +
-                                          " has zero area (degenerate face)");
-        }
+        if (!(ar[p] > 0.0f))
+            return drop_scene(c, fail(c, RAD_E_ARG, "ingest: patch " + std::to_string(p) +
+                                                        " has zero area (degenerate face)"));
     c->nquad = nq;
     if (out_ntri)
         *out_ntri = nt;
@@ -1789,6 +1818,7 @@ int rad_set_quads(rad_ctx *c, const uint32_t *patch_quad, uint32_t nquad, const 
     for (uint64_t i = 0; i < (uint64_t)nquad * 8; ++i)
         if (quad_nbr[i] >= nquad)
             return fail(c, RAD_E_ARG, "quads: neighbour index out of range");
+    c->nquad = 0; // until the new arrays are complete
     if ((rc = dalloc(c, &c->patch_quad, np)) || (rc = dalloc(c, &c->quad_pbeg, (size_t)nquad + 1)) ||
         (rc = dalloc(c, &c->quad_nbr, (size_t)nquad * 8)) || (rc = dalloc(c, &c->qval, (size_t)nquad * 3)))
         return rc;
