@@ -10,6 +10,10 @@ shooters (select, render, reduce, shoot).  Reported:
   roofline       the dominant kernel, timed with HIP events on the library's
                  stream, against the MI355X HBM peak
   cpu_baseline   the oracle's single-thread CPU pass on the same scene
+  cpu_baseline_multicore  the same pass on all host cores (OpenMP)
+  rasterizer     (triangle, hemicube) pairs set up per second and covered
+                 fragments tested per second (fragments per hemicube counted
+                 by the oracle on sample shooters)
 
     python bench.py [--steps 50] [--warmup 5] [--no-converge] [--cpu-sample-s 10]
 """
@@ -23,6 +27,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def log(*a):
@@ -411,7 +426,7 @@ def main():
             "traffic": measured_traffic(W, K).get(dom)}
 
     # --------------------------------------------------------- CPU baseline
-    cpu = None
+    cpu = cpu_mt = raster = None
     if args.cpu_sample_s > 0:
         from oracle import oracle as O
         dff = O.dff_cross(W)
@@ -427,10 +442,40 @@ def main():
             el = time.perf_counter() - t0
             if el >= args.cpu_sample_s and n >= 3:
                 break
+        model = cpu_model()
         cpu = {"value": n / el, "unit": "shooters/s", "cores": 1, "kind": "port",
                "sample": f"{n} plain passes (K=1, W={W}) of the oracle on the same 200k-patch "
-                         f"scene from the initial state, {el:.1f} s"}
+                         f"scene from the initial state, {el:.1f} s",
+               "cpu_model": model, "nproc": os.cpu_count()}
         log(f"cpu oracle: {n / el:.2f} shooters/s")
+        # the same passes on all the host cores this process may use (OpenMP)
+        threads = max(1, min(len(os.sched_getaffinity(0)),
+                             int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()))
+        u = sc.emission.copy()
+        sh = sc.shot0.copy()
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            O.plain_pass_mt(W, 1, 1, sc, dff, u, sh, threads, item, ff)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= args.cpu_sample_s and n >= 3:
+                break
+        cpu_mt = {"value": n / el, "unit": "shooters/s", "cores": threads, "kind": "port",
+                  "sample": f"{n} plain passes (K=1, W={W}) of the oracle's OpenMP pass on the "
+                            f"same scene from the initial state, {el:.1f} s",
+                  "cpu_model": model, "nproc": os.cpu_count()}
+        log(f"cpu oracle, {threads} threads: {n / el:.2f} shooters/s")
+        # covered fragments per hemicube on sample shooters, for fragments/s
+        frags = [O.render_hemicube_mt(W, sc.tri, sc.tri_patch, sc.frame[p], threads)[1]
+                 for p in np.random.default_rng(11).choice(sc.npatch, 4, replace=False)]
+        fph = float(np.mean(frags))
+        raster = {"triangle_hemicubes_per_s": sc.ntri * batched_sps,
+                  "fragments_per_hemicube": fph,
+                  "fragments_per_s": fph * batched_sps,
+                  "sample": "fragments counted by the oracle on 4 seeded shooters"}
+        log(f"rasterizer: {sc.ntri * batched_sps / 1e9:.1f} G triangle-hemicubes/s, "
+            f"{fph * batched_sps / 1e9:.1f} G fragments/s ({fph:.0f} per hemicube)")
 
     out = {
         "metric": "shooters/sec + ms to 1e-3 unshot energy at 200k patches, plain FP32 vs "
@@ -461,6 +506,8 @@ def main():
         "configs": configs,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_multicore": cpu_mt,
+        "rasterizer": raster,
     }
     print(json.dumps(out), flush=True)
 

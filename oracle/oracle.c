@@ -30,6 +30,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -438,18 +439,14 @@ uint64_t orc_render_hemicube_variant(uint32_t W, const float *tri, const uint32_
     return render_impl(W, tri, tri_patch, ntri, frame, item, NULL, flags);
 }
 
-static uint64_t render_impl(uint32_t W, const float *tri, const uint32_t *tri_patch,
-                            uint32_t ntri, const float *frame, uint32_t *item,
-                            float *depth_or_null, int flags)
+/* Draws triangles [t0, t1) of the five faces into item / zbuf (already
+ * cleared), in draw order. */
+static uint64_t render_range(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                             uint32_t t0, uint32_t t1, const float *frame, uint32_t *item,
+                             float *zbuf, int flags)
 {
     uint32_t TW, TH, RES;
     orc_geometry(W, &TW, &TH, &RES);
-    float *zbuf = depth_or_null ? depth_or_null : (float *)malloc(sizeof(float) * RES);
-    /* Main.cpp:1148 glClear: colour 0, depth 1.0 */
-    for (uint32_t i = 0; i < RES; ++i) {
-        zbuf[i] = 1.0f;
-        item[i] = 0;
-    }
     float halfW = 0.5f * (float)W;
     uint64_t frags = 0;
     for (int face = 0; face < 5; ++face) {
@@ -462,7 +459,7 @@ static uint64_t render_impl(uint32_t W, const float *tri, const uint32_t *tri_pa
         scl[1] = sc[1] - vp[1];
         scl[2] = sc[0] + sc[2] - vp[0];
         scl[3] = sc[1] + sc[3] - vp[1];
-        for (uint32_t t = 0; t < ntri; ++t) {
+        for (uint32_t t = t0; t < t1; ++t) {
             cv4 poly[12];
             for (int k = 0; k < 3; ++k) {
                 float x = tri[t * 9 + k * 3 + 0], y = tri[t * 9 + k * 3 + 1],
@@ -498,8 +495,69 @@ static uint64_t render_impl(uint32_t W, const float *tri, const uint32_t *tri_pa
                                     flags);
         }
     }
+    return frags;
+}
+
+static uint64_t render_impl(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                            uint32_t ntri, const float *frame, uint32_t *item,
+                            float *depth_or_null, int flags)
+{
+    uint32_t TW, TH, RES;
+    orc_geometry(W, &TW, &TH, &RES);
+    float *zbuf = depth_or_null ? depth_or_null : (float *)malloc(sizeof(float) * RES);
+    /* Main.cpp:1148 glClear: colour 0, depth 1.0 */
+    for (uint32_t i = 0; i < RES; ++i) {
+        zbuf[i] = 1.0f;
+        item[i] = 0;
+    }
+    uint64_t frags = render_range(W, tri, tri_patch, 0, ntri, frame, item, zbuf, flags);
     if (!depth_or_null)
         free(zbuf);
+    return frags;
+}
+
+/* orc_render_hemicube on nthreads OpenMP threads: thread t draws the t-th
+ * contiguous slice of the triangles into its own buffers, and the buffers
+ * are merged texel by texel in slice order keeping the first strictly
+ * nearer fragment - the GL_LESS result of drawing all triangles in order.
+ * The CPU baseline's multi-core leg; equal to orc_render_hemicube. */
+uint64_t orc_render_hemicube_mt(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                                uint32_t ntri, const float *frame, uint32_t *item,
+                                int nthreads)
+{
+    uint32_t TW, TH, RES;
+    orc_geometry(W, &TW, &TH, &RES);
+    if (nthreads < 1)
+        nthreads = 1;
+    uint32_t *items = (uint32_t *)malloc(sizeof(uint32_t) * RES * (size_t)nthreads);
+    float *zs = (float *)malloc(sizeof(float) * RES * (size_t)nthreads);
+    uint64_t frags = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : frags)
+    {
+        int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        uint32_t *it = items + (size_t)t * RES;
+        float *z = zs + (size_t)t * RES;
+        for (uint32_t i = 0; i < RES; ++i) {
+            z[i] = 1.0f;
+            it[i] = 0;
+        }
+        uint32_t a = (uint32_t)((uint64_t)ntri * t / nt), b = (uint32_t)((uint64_t)ntri * (t + 1) / nt);
+        frags += render_range(W, tri, tri_patch, a, b, frame, it, z, 0);
+#pragma omp barrier
+#pragma omp for
+        for (uint32_t i = 0; i < RES; ++i) {
+            float bz = zs[i];
+            uint32_t bi = items[i];
+            for (int q = 1; q < nt; ++q)
+                if (zs[(size_t)q * RES + i] < bz) {
+                    bz = zs[(size_t)q * RES + i];
+                    bi = items[(size_t)q * RES + i];
+                }
+            item[i] = bi;
+        }
+    }
+    free(items);
+    free(zs);
     return frags;
 }
 
@@ -675,11 +733,39 @@ double orc_residual(const double *unshot, const float *emission, const float *ar
     return den > 0 ? num / den : 0.0;
 }
 
+static uint32_t plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule, const float *tri,
+                           const uint32_t *tri_patch, uint32_t ntri, const float *frames,
+                           const float *refl, const float *area, uint32_t npatch,
+                           const float *dff, float *unshot, float *shot, uint32_t *item,
+                           float *ff, int nthreads);
+
 uint32_t orc_plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule,
                         const float *tri, const uint32_t *tri_patch, uint32_t ntri,
                         const float *frames, const float *refl, const float *area,
                         uint32_t npatch, const float *dff, float *unshot, float *shot,
                         uint32_t *item, float *ff)
+{
+    return plain_pass(W, mode, select_rule, tri, tri_patch, ntri, frames, refl, area, npatch,
+                      dff, unshot, shot, item, ff, 0);
+}
+
+/* orc_plain_pass with the hemicube rendered by orc_render_hemicube_mt and
+ * the shooting loop split over the same threads; same results. */
+uint32_t orc_plain_pass_mt(uint32_t W, uint32_t mode, uint32_t select_rule,
+                           const float *tri, const uint32_t *tri_patch, uint32_t ntri,
+                           const float *frames, const float *refl, const float *area,
+                           uint32_t npatch, const float *dff, float *unshot, float *shot,
+                           uint32_t *item, float *ff, int nthreads)
+{
+    return plain_pass(W, mode, select_rule, tri, tri_patch, ntri, frames, refl, area, npatch,
+                      dff, unshot, shot, item, ff, nthreads < 1 ? 1 : nthreads);
+}
+
+static uint32_t plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule, const float *tri,
+                           const uint32_t *tri_patch, uint32_t ntri, const float *frames,
+                           const float *refl, const float *area, uint32_t npatch,
+                           const float *dff, float *unshot, float *shot, uint32_t *item,
+                           float *ff, int nthreads)
 {
     uint32_t i;
     if (select_rule == 0)
@@ -687,10 +773,14 @@ uint32_t orc_plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule,
     else
         i = orc_select_argmax_power(unshot, area, npatch);
     float snap[3] = {unshot[3 * i], unshot[3 * i + 1], unshot[3 * i + 2]};
-    orc_render_hemicube(W, tri, tri_patch, ntri, frames + 9 * (size_t)i, item, NULL);
+    if (nthreads)
+        orc_render_hemicube_mt(W, tri, tri_patch, ntri, frames + 9 * (size_t)i, item, nthreads);
+    else
+        orc_render_hemicube(W, tri, tri_patch, ntri, frames + 9 * (size_t)i, item, NULL);
     memset(ff, 0, sizeof(float) * npatch);
     orc_process_hemicube_runs(W, 1, 4, item, dff, npatch, ff, NULL);
     /* Main.cpp:1272-1275 in float, as the reference's Patch fields */
+#pragma omp parallel for num_threads(nthreads ? nthreads : 1) if (nthreads > 1)
     for (uint32_t j = 0; j < npatch; ++j) {
         float f = ff[j];
         if (f == 0.0f)

@@ -59,6 +59,9 @@ uint64_t orc_render_hemicube(uint32_t W, const float *tri, const uint32_t *tri_p
 #define ORC_VAR_DEPTH24 1
 #define ORC_VAR_FMA 2
 #define ORC_VAR_DIVIDE 4
+uint64_t orc_render_hemicube_mt(uint32_t W, const float *tri, const uint32_t *tri_patch,
+                                uint32_t ntri, const float *frame, uint32_t *item,
+                                int nthreads);
 uint64_t orc_render_hemicube_variant(uint32_t W, const float *tri, const uint32_t *tri_patch,
                                      uint32_t ntri, const float *frame, uint32_t *item,
                                      int flags);
@@ -109,6 +112,11 @@ uint32_t orc_plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule,
                         uint32_t npatch, const float *dff_cross,
                         float *unshot, float *shot,
                         uint32_t *item_scratch, float *ff_scratch);
+uint32_t orc_plain_pass_mt(uint32_t W, uint32_t mode, uint32_t select_rule,
+                           const float *tri, const uint32_t *tri_patch, uint32_t ntri,
+                           const float *frames, const float *refl, const float *area,
+                           uint32_t npatch, const float *dff, float *unshot, float *shot,
+                           uint32_t *item, float *ff, int nthreads);
 
 /* FP64 Jacobi for the gather B = E + diag(rho) M B (no reference
  * counterpart: the fixed point the shooting loop of Main.cpp:1251-1295

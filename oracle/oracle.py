@@ -28,6 +28,9 @@ _SIG = {
     "orc_face_viewport": (None, [_u32, C.c_int, _PI, _PI]),
     "orc_render_hemicube": (C.c_uint64, [_u32, _PF, _PU, _u32, _PF, _PU, _PF]),
     "orc_render_hemicube_variant": (C.c_uint64, [_u32, _PF, _PU, _u32, _PF, _PU, C.c_int]),
+    "orc_render_hemicube_mt": (C.c_uint64, [_u32, _PF, _PU, _u32, _PF, _PU, C.c_int]),
+    "orc_plain_pass_mt": (_u32, [_u32, _u32, _u32, _PF, _PU, _u32, _PF, _PF, _PF, _u32, _PF,
+                                 _PF, _PF, _PU, _PF, C.c_int]),
     "orc_process_hemicube_runs": (C.c_uint64, [_u32, _u32, _u32, _PU, _PF, _u32, _PF,
                                                C.POINTER(C.c_uint64)]),
     "orc_hemicube_histogram": (None, [_u32, _u32, _PU, _PF, _u32, _PU, _PD]),
@@ -142,6 +145,18 @@ def render_hemicube_variant(W, tri, tri_patch, frame, flags):
     return item
 
 
+def render_hemicube_mt(W, tri, tri_patch, frame, nthreads):
+    """render_hemicube on nthreads OpenMP threads (same item buffer);
+    returns (item, fragments)."""
+    tri = np.ascontiguousarray(tri, np.float32).reshape(-1, 9)
+    tp = np.ascontiguousarray(tri_patch, np.uint32)
+    fr = np.ascontiguousarray(frame, np.float32).reshape(9)
+    item = np.zeros(geometry(W)[2], np.uint32)
+    frags = lib().orc_render_hemicube_mt(W, _f(tri), _u(tp), tri.shape[0], _f(fr), _u(item),
+                                         int(nthreads))
+    return item, frags
+
+
 def process_runs(W, K, item, dff, npatch, workitems_x=4):
     item = np.ascontiguousarray(item, np.uint32)
     ff = np.zeros(K * npatch, np.float32)
@@ -194,6 +209,19 @@ def plain_pass(W, mode, rule, sc, dff, unshot, shot, item_scratch=None, ff_scrat
     return lib().orc_plain_pass(W, mode, rule, _f(sc.tri), _u(sc.tri_patch), sc.ntri,
                                 _f(sc.frame), _f(sc.refl), _f(sc.area), sc.npatch, _f(dff),
                                 _f(unshot), _f(shot), _u(item_scratch), _f(ff_scratch))
+
+
+def plain_pass_mt(W, mode, rule, sc, dff, unshot, shot, nthreads, item_scratch=None,
+                  ff_scratch=None):
+    """plain_pass with the render and the shooting loop on nthreads threads."""
+    if item_scratch is None:
+        item_scratch = np.zeros(geometry(W)[2], np.uint32)
+    if ff_scratch is None:
+        ff_scratch = np.zeros(sc.npatch, np.float32)
+    return lib().orc_plain_pass_mt(W, mode, rule, _f(sc.tri), _u(sc.tri_patch), sc.ntri,
+                                   _f(sc.frame), _f(sc.refl), _f(sc.area), sc.npatch, _f(dff),
+                                   _f(unshot), _f(shot), _u(item_scratch), _f(ff_scratch),
+                                   int(nthreads))
 
 
 def gather_jacobi(M, refl, area, E, tol=1e-13, max_it=100000):

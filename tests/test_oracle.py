@@ -540,3 +540,28 @@ def test_item_buffer_sensitivity_to_driver_arithmetic(rad, flags):
         # float depth in [0.5, 1) is coarser than 24-bit fixed point, so the
         # rounding keeps every order and every tie
         assert diff == 0
+
+
+# --------------------------------------------- multi-core CPU baseline leg
+
+@pytest.mark.parametrize("threads", [2, 5, 8])
+def test_mt_render_equals_single_thread(rad, threads):
+    """The OpenMP render (triangle slices merged in slice order with the
+    strict depth test) equals the single-thread GL_LESS render bit for bit,
+    and so does a plain pass built on it."""
+    sc = rad.Scene.boxes(nboxes=30, room=(8.0, 3.0, 8.0), target_tris=6000, nlights=4, seed=3)
+    rng = np.random.default_rng(threads)
+    for p in rng.choice(sc.npatch, 3, replace=False):
+        a, fa = O.render_hemicube(64, sc.tri, sc.tri_patch, sc.frame[p], depth=True)[0::2]
+        b, fb = O.render_hemicube_mt(64, sc.tri, sc.tri_patch, sc.frame[p], threads)
+        assert np.array_equal(a, b) and fa == fb
+    dff = O.dff_cross(64)
+    u1, s1 = sc.emission.copy(), sc.shot0.copy()
+    u2, s2 = sc.emission.copy(), sc.shot0.copy()
+    for _ in range(4):
+        i1 = O.plain_pass(64, 1, 1, sc, dff, u1, s1)
+        i2 = O.plain_pass_mt(64, 1, 1, sc, dff, u2, s2, threads)
+        assert i1 == i2
+    assert np.array_equal(u1.view(np.uint32), u2.view(np.uint32))
+    assert np.array_equal(s1.view(np.uint32), s2.view(np.uint32))
+
