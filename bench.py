@@ -29,6 +29,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def host_threads():
+    """Host cores this process may use (OMP_NUM_THREADS caps it)."""
+    return max(1, min(len(os.sched_getaffinity(0)),
+                      int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()))
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -57,6 +63,27 @@ def measured_traffic(W, K):
     except OSError:
         pass
     return {}
+
+
+def stop_rule_leg(s, sc, chunk, cap):
+    """The reference's own stop rule (Main.cpp:1298): passes until |dB| of
+    the last shooter of a pass falls below 0.1, checked every `chunk`
+    passes, from the initial state."""
+    s.upload(sc)
+    ms = 0.0
+    passes = 0
+    st = None
+    while passes < cap:
+        st = s.run(chunk, 0.0, check_every=chunk)
+        ms += st.seconds * 1e3
+        passes += st.passes
+        if st.last_energy < 0.1:
+            break
+    out = {"ms": ms, "passes": passes, "last_energy": float(st.last_energy),
+           "residual": st.residual, "stopped": bool(st.last_energy < 0.1), "check_every": chunk}
+    log(f"reference stop rule (|dB| < 0.1): {'after' if out['stopped'] else 'not within'} "
+        f"{passes} passes, {ms:.0f} ms, residual {st.residual:.3e}")
+    return out
 
 
 def gather_leg(R, s, sc, args):
@@ -116,17 +143,26 @@ def cornell_leg(R, O, area, W, shots, seed):
     for _ in range(shots):
         O.plain_pass(W, 1, 1, sc, dff, u, sh, item, ff)
     cpu = time.perf_counter() - t0
+    threads = host_threads()
+    u2, sh2 = sc.emission.copy(), sc.shot0.copy()
+    t0 = time.perf_counter()
+    for _ in range(shots):
+        O.plain_pass_mt(W, 1, 1, sc, dff, u2, sh2, threads, item, ff)
+    cpu_mt = time.perf_counter() - t0
     a = sc.area.astype(np.float64)[:, None]
     tg, tc = (ug + sg).astype(np.float64), (u + sh).astype(np.float64)
     diff = float((a * np.abs(tg - tc)).sum() / (a * np.abs(tc)).sum())
     out = {"patches": sc.npatch, "hemicube": W, "shots": shots,
            "gpu_ms": gpu * 1e3, "cpu_ms": cpu * 1e3, "cpu_cores": 1,
+           "cpu_multicore_ms": cpu_mt * 1e3, "cpu_multicore_cores": threads,
+           "speedup_vs_multicore": cpu_mt / gpu,
            "gpu_shooters_per_s": shots / gpu, "cpu_shooters_per_s": shots / cpu,
            "speedup": cpu / gpu, "rel_l1_gpu_vs_cpu_state": diff,
            "residual_gpu": s.residual(), "residual_cpu": O.residual(u.astype(np.float64),
                                                                   sc.emission, sc.area)}
     log(f"cornell {sc.npatch} W={W}: {shots} shots GPU {gpu * 1e3:.1f} ms, CPU "
-        f"{cpu * 1e3:.0f} ms ({cpu / gpu:.0f}x); state rel L1 {diff:.1e}")
+        f"{cpu * 1e3:.0f} ms ({cpu / gpu:.0f}x), CPU {threads} threads {cpu_mt * 1e3:.0f} ms "
+        f"({cpu_mt / gpu:.0f}x); state rel L1 {diff:.1e}")
     return out
 
 
@@ -371,6 +407,7 @@ def main():
             f"residual {st.residual:.3e}")
         if not args.no_gather:
             conv["gather_bf16"] = gather_leg(R, s, sc, args)
+        conv["batched_reference_stop"] = stop_rule_leg(s, sc, 1, 20000)
         if p is not None:
             p.upload(sc)
             cap = int(max(1000, plain_sps * args.plain_cap_s))
@@ -449,8 +486,7 @@ def main():
                "cpu_model": model, "nproc": os.cpu_count()}
         log(f"cpu oracle: {n / el:.2f} shooters/s")
         # the same passes on all the host cores this process may use (OpenMP)
-        threads = max(1, min(len(os.sched_getaffinity(0)),
-                             int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()))
+        threads = host_threads()
         u = sc.emission.copy()
         sh = sc.shot0.copy()
         n = 0
