@@ -233,6 +233,42 @@ def ingest_leg(R):
     return out
 
 
+def profile_leg(R, name):
+    """One path run briefly, for rocprofv3 (kernel trace or counters):
+      dense32k    the 32,760-patch dense build and one refinement step of
+                  the 192-column gather (4 BF16 sweeps + one two-half
+                  residual product), three times
+      reduce200k  64 item buffers of the 200k room (hemicube 256) reduced by
+                  the standalone ProcessHemicube kernel, five times"""
+    import numpy as np
+    if name == "dense32k":
+        sc = R.Scene.boxes(nboxes=60, room=(12.0, 4.0, 12.0), target_tris=32768, nlights=16,
+                           seed=4)
+        s = R.Solver(hemicube=128, hemicubes=64)
+        s.upload(sc)
+        s.dense_build(R.RAD_DENSE_SHOOT)
+        rng = np.random.default_rng(4)
+        cand = rng.choice(sc.npatch, 32, replace=False)
+        E = np.zeros((192, sc.npatch), np.float32)
+        for c in range(64):
+            E[3 * c:3 * c + 3, cand] = rng.uniform(0.0, 20.0, (3, 32)) * (rng.random(32) < 0.5)
+        for _ in range(3):
+            _, st = s.gather(E, target=1e-30, max_outer=1, inner=4)
+        s.dense_release()
+        log(f"profile dense32k: sweep {st.sweep_ms:.3f} ms, residual product {st.resid_ms:.3f} ms")
+    elif name == "reduce200k":
+        sc = R.Scene.boxes()
+        s = R.Solver(hemicube=256, hemicubes=64, tile_px=32, small_tri_px=64)
+        s.upload(sc)
+        shooters = np.random.default_rng(12).choice(sc.npatch, 64, replace=False)
+        s.render(shooters)
+        for _ in range(5):
+            s.process(64)
+        log("profile reduce200k: 64 item buffers reduced 5 times")
+    else:
+        raise SystemExit(f"unknown profile leg {name}")
+
+
 def pack_leg(R, args):
     """256 independent box rooms, 500-20,000 triangle patches (log-uniform,
     seed 5), hemicube 64 / 128 / 256 by size, solved to the residual target
@@ -325,6 +361,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, so that a "
                          "counter run can scale FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--profile", default="",
+                    help="run one path briefly for rocprofv3: dense32k or reduce200k")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
@@ -333,6 +371,9 @@ def main():
 
     import numpy as np
     import radiosity_amd as R
+    if args.profile:
+        profile_leg(R, args.profile)
+        return
 
     if args.copy_calibration_mib:
         import torch

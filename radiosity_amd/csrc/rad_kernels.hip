@@ -1275,11 +1275,44 @@ void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const Bi
 }
 
 // ----------------------------------------------------------- ProcessHemicube
-// One texel per lane.  A wavefront covers 64 consecutive texels of a row;
-// equal (hemicube, id) neighbours form runs (the span walk of
-// Kernel_ProcessHemicube.h:36-49), summed with a segmented scan; the last
-// lane of each run adds the run's ΔFF sum and pixel count to the patch's
-// entry (the host accumulation of Main.cpp:1257-1269 fused in).
+// A workgroup takes kReduceChunk consecutive texels of one hemicube, one
+// texel per lane per step.  Equal ids in a wavefront's 64 consecutive texels
+// form runs (the span walk of Kernel_ProcessHemicube.h:36-49) summed with a
+// segmented scan; each run's ΔFF sum and pixel count go into an LDS hash
+// table keyed by id, which also merges the runs of the chunk's other rows;
+// the table is then flushed with one global atomic per distinct id (the host
+// accumulation of Main.cpp:1257-1269 fused in).  A run that finds no free
+// slot adds to the global entry directly.
+constexpr uint32_t kReduceChunk = 4096, kReduceSlots = 2048;
+
+// Segmented sum of (f, c) over the runs of equal id in a wavefront; returns
+// true in the last lane of each run with an id, which then holds the run's
+// sums.  Every lane of the wavefront must call it.
+__device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t pid = __shfl_up(id, 1);
+    const bool head = lane == 0 || pid != id;
+    int ss = head ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(ss, d);
+        if (lane >= d && o > ss)
+            ss = o;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        float fo = __shfl_up(f, d);
+        uint32_t co = __shfl_up(c, d);
+        if (lane - d >= ss) {
+            f += fo;
+            c += co;
+        }
+    }
+    const uint32_t nid = __shfl_down(id, 1);
+    return (lane == 63 || nid != id) && id != 0;
+}
+
 __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__restrict__ nshoot,
                                                 const uint64_t *__restrict__ keys,
                                                 const float *__restrict__ dff, uint32_t npatch,
@@ -1287,53 +1320,63 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
                                                 uint32_t *__restrict__ hits,
                                                 uint32_t *__restrict__ counters)
 {
-    size_t total = (size_t)(*nshoot) * RES;
-    int lane = threadIdx.x & 63;
-    for (size_t base = (size_t)blockIdx.x * blockDim.x; base < total;
-         base += (size_t)gridDim.x * blockDim.x) {
-        size_t t = base + threadIdx.x;
-        uint32_t id = 0, hi = 0, c = 0;
+    __shared__ uint32_t hkey[kReduceSlots], hcnt[kReduceSlots];
+    __shared__ float hsum[kReduceSlots];
+    const uint32_t chunks = (RES + kReduceChunk - 1) / kReduceChunk;
+    const uint32_t hi = blockIdx.x / chunks, c0 = (blockIdx.x - hi * chunks) * kReduceChunk;
+    if (hi >= *nshoot)
+        return;
+    for (uint32_t q = threadIdx.x; q < kReduceSlots; q += blockDim.x) {
+        hkey[q] = 0u;
+        hcnt[q] = 0u;
+        hsum[q] = 0.0f;
+    }
+    __syncthreads();
+    const uint64_t *kp = keys + (size_t)hi * RES;
+    const uint32_t c1 = min(RES, c0 + kReduceChunk);
+    for (uint32_t base = c0; base < c1; base += blockDim.x) {
+        const uint32_t t = base + threadIdx.x;
+        uint32_t id = 0, c = 0;
         float f = 0.0f;
-        if (t < total) {
-            id = (uint32_t)keys[t];
-            hi = (uint32_t)(t / RES);
+        if (t < c1) {
+            id = (uint32_t)kp[t];
             if (id) {
-                f = dff[t - (size_t)hi * RES];
+                f = dff[t];
                 c = 1;
             }
         }
-        uint32_t pid = __shfl_up(id, 1), phi = __shfl_up(hi, 1);
-        bool head = lane == 0 || pid != id || phi != hi;
-        int ss = head ? lane : 0;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            int o = __shfl_up(ss, d);
-            if (lane >= d && o > ss)
-                ss = o;
-        }
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            float fo = __shfl_up(f, d);
-            uint32_t co = __shfl_up(c, d);
-            if (lane - d >= ss) {
-                f += fo;
-                c += co;
-            }
-        }
-        uint32_t nid = __shfl_down(id, 1), nhi = __shfl_down(hi, 1);
-        bool tail = lane == 63 || nid != id || nhi != hi;
-        if (tail && id != 0) {
-            uint32_t p = id - 1;
-            if (p < npatch) {
-                size_t o = (size_t)hi * npatch + p;
-                atomicAdd(&ff[o], f);
-                if (hits)
-                    atomicAdd(&hits[o], c);
-            } else {
-                // Main.cpp:1259-1262: unknown patch id, counted and skipped
+        if (wave_runs(id, f, c)) {
+            uint32_t h = (id * 2654435761u) >> 21; // kReduceSlots = 2^11
+            // Main.cpp:1259-1262: an unknown patch id is counted (once per
+            // run) and skipped
+            bool done = id - 1 >= npatch;
+            if (done)
                 atomicAdd(&counters[kCntUnknown], 1u);
+            for (int probe = 0; probe < 16 && !done; ++probe, h = (h + 1) & (kReduceSlots - 1)) {
+                uint32_t k = hkey[h];
+                if (k == 0u)
+                    k = atomicCAS(&hkey[h], 0u, id);
+                if (k == 0u || k == id) {
+                    atomicAdd(&hsum[h], f);
+                    atomicAdd(&hcnt[h], c);
+                    done = true;
+                }
+            }
+            if (!done) { // no free slot: straight to the global entry
+                atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], f);
+                if (hits)
+                    atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], c);
             }
         }
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < kReduceSlots; q += blockDim.x) {
+        const uint32_t id = hkey[q];
+        if (id == 0u)
+            continue;
+        atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], hsum[q]);
+        if (hits)
+            atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], hcnt[q]);
     }
 }
 
@@ -1341,12 +1384,8 @@ void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot, const 
                    const float *dff, uint32_t npatch, float *ff, uint32_t *hits,
                    uint32_t *counters, hipStream_t st)
 {
-    size_t total = (size_t)K * g.RES;
-    size_t blocks = (total + 255) / 256;
-    if (blocks > 16384)
-        blocks = 16384;
-    k_reduce<<<(unsigned)blocks, 256, 0, st>>>(g.RES, nshoot, keys, dff, npatch, ff, hits,
-                                               counters);
+    const uint32_t chunks = (g.RES + kReduceChunk - 1) / kReduceChunk;
+    k_reduce<<<K * chunks, 256, 0, st>>>(g.RES, nshoot, keys, dff, npatch, ff, hits, counters);
 }
 
 // ------------------------------------------------------------------ shooting
