@@ -473,3 +473,34 @@ def test_itembuf_bit_exact_skewed_frame(rad, cornell_t):
     sk = rad.Scene(sc.tri, sc.tri_patch, frame, sc.refl, sc.area, sc.emission, sc.shot0)
     say("itembuf skewed frames W=64")
     _render_case(rad, sk, 64, [p, q, 11])
+
+
+@pytest.mark.parametrize("W", [64, 128])
+def test_itembuf_bit_exact_random_soup(rad, W):
+    """A soup of random triangles around the shooters: large ones spanning
+    several faces, ones crossing the near plane and the shooter's plane,
+    ones behind the eye, both windings.  Exercises the clipped path and
+    every cull next to its exact test."""
+    rng = np.random.default_rng(100 + W)
+    n = 3000
+    centre = rng.uniform(-3.0, 3.0, (n, 3))
+    big = rng.random(n) < 0.04
+    scale = np.where(big, 10 ** rng.uniform(-0.5, 0.3, n), 10 ** rng.uniform(-2.5, -0.8, n))
+    tri = (centre[:, None, :] + rng.normal(size=(n, 3, 3)) * scale[:, None, None]).astype(np.float32)
+    t64 = tri.astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(t64[:, 1] - t64[:, 0], t64[:, 2] - t64[:, 0]), axis=1)
+    keep = area > 1e-10
+    tri, t64, area = tri[keep], t64[keep], area[keep]
+    m = len(tri)
+    nrm = np.cross(t64[:, 1] - t64[:, 0], t64[:, 2] - t64[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    up = t64[:, 2] - t64[:, 0]
+    up /= np.linalg.norm(up, axis=1, keepdims=True)
+    frame = np.concatenate([t64.mean(1), nrm, up], axis=1).astype(np.float32)
+    sc = rad.Scene(tri.reshape(-1), np.arange(m, dtype=np.uint32), frame,
+                   np.full((m, 3), 0.5, np.float32), area.astype(np.float32),
+                   np.zeros((m, 3), np.float32), np.zeros((m, 3), np.float32))
+    small = np.nonzero(area < 0.01)[0]
+    shooters = [int(x) for x in rng.choice(small, 4, replace=False)]
+    say(f"itembuf random soup W={W}: {m} triangles, shooters {shooters}")
+    _render_case(rad, sc, W, shooters)
