@@ -99,9 +99,15 @@ def gather_leg(R, s, sc, args):
     t1 = time.perf_counter()
     B, st = s.gather(target=args.target, inner=args.gather_inner)
     t2 = time.perf_counter()
+    # the same solve run to the floor of the FP64 residual: the error of the
+    # 1e-3 answer against the converged fixed point of the stored matrix
+    B7, st7 = s.gather(target=1e-7, inner=args.gather_inner, max_outer=20)
     s.dense_release()
     a = sc.area.astype(np.float64)[None, :]
     diff = float((a * np.abs(B - shoot_B)).sum() / (a * np.abs(shoot_B)).sum())
+    B7 = B7.astype(np.float64)
+    err7 = float((a * np.abs(B - B7)).sum() / (a * np.abs(B7)).sum())
+    errmax = float(np.max(np.abs(B - B7)) / np.max(np.abs(B7)))
     npad = (sc.npatch + 511) // 512 * 512
     sweep_gbs = 2.0 * npad * npad / (st.sweep_ms * 1e-3) / 1e9 if st.sweep_ms else None
     resid_gbs = 4.0 * npad * npad / (st.resid_ms * 1e-3) / 1e9 if st.resid_ms else None
@@ -111,11 +117,14 @@ def gather_leg(R, s, sc, args):
            "sweep_kernel_ms": st.sweep_ms, "sweep_hbm_gbs": sweep_gbs,
            "resid_kernel_ms": st.resid_ms, "resid_hbm_gbs": resid_gbs,
            "matrix_bytes": 4 * npad * npad,
-           "rel_l1_vs_batched_shooting": diff}
+           "rel_l1_vs_batched_shooting": diff,
+           "rel_l1_vs_converged": err7, "max_rel_vs_converged": errmax,
+           "converged_residual": st7.residual}
     log(f"gather bf16 to {args.target}: {out['ms'] / 1e3:.2f} s (build {out['build_ms'] / 1e3:.2f}"
         f" s, solve {out['solve_ms']:.0f} ms, {st.outer} steps, {st.sweeps} sweeps), residual "
         f"{st.residual:.2e}; sweep {st.sweep_ms:.2f} ms ({sweep_gbs:.0f} GB/s), residual product "
-        f"{st.resid_ms:.2f} ms ({resid_gbs:.0f} GB/s); vs shooting {diff:.2e}")
+        f"{st.resid_ms:.2f} ms ({resid_gbs:.0f} GB/s); vs shooting {diff:.2e}; vs the solve "
+        f"converged to {st7.residual:.1e}: {err7:.2e} rel L1, {errmax:.2e} max")
     return out
 
 
@@ -166,6 +175,37 @@ def cornell_leg(R, O, area, W, shots, seed):
     return out
 
 
+def fp64_gather_check(s, sc, E, B, K):
+    """Bound on the error of a gather solution B (R x n) against the exact
+    solution of B = E + diag(rho) G B, G_ji = F_ij A_i/A_j, built in FP64 from
+    the FP32 form-factor rows the hemicube kernels render (before any BF16
+    rounding).  One FP64 product on the host gives the residual
+    r = E + rho G B - B.  In the area-weighted norm |x|_A = sum_j A_j |x_j|
+    the map x -> rho G x contracts by q = max_c max_i sum_j rho_jc F_ij (< 1:
+    a hemicube row sums to about 1), so |B - B*|_A <= |r|_A / (1 - q)."""
+    import numpy as np
+    n = sc.npatch
+    F = np.empty((n, n), np.float32)
+    for b in range(0, n, K):
+        F[b:b + K] = s.ff_rows(np.arange(b, min(n, b + K), dtype=np.uint32))
+    G = F.T.astype(np.float64)                                  # G[j, i] = F[i, j]
+    del F
+    A = sc.area.astype(np.float64)
+    G *= A[None, :]
+    G /= A[:, None]
+    R_ = E.shape[0]
+    rho = sc.refl.astype(np.float64)[:, np.arange(R_) % 3]       # n x R
+    Bt = B.T.astype(np.float64)
+    r = E.T.astype(np.float64) + rho * (G @ Bt) - Bt
+    # sum_j rho_jc F_ij = (sum_j G_ji rho_jc A_j) / A_i
+    q = float(np.max((G.T @ (sc.refl.astype(np.float64) * A[:, None])) / A[:, None]))
+    del G
+    rl1 = float((A[:, None] * np.abs(r)).sum() / (A[:, None] * np.abs(Bt)).sum())
+    out = {"fp64_residual_rel_l1": rl1, "contraction_q": q,
+           "error_bound_rel_l1": rl1 / (1.0 - q) if q < 1 else None}
+    return out
+
+
 def dense_leg(R, args):
     """Dense 32k x 32k matrix (hemicube 128) and the gather of 64 lighting
     conditions x RGB (192 columns) over 32 candidate lights, BF16 MFMA with
@@ -202,6 +242,14 @@ def dense_leg(R, args):
         f"({st.outer} steps, {st.sweeps} sweeps, residual {st.residual:.1e}); sweep "
         f"{st.sweep_ms:.3f} ms = {out['sweep_hbm_gbs']:.0f} GB/s, "
         f"{out['sweep_mfma_tflops']:.0f} TFLOP/s")
+    # error bound against the exact FP64 solution for the FP32 form factors
+    t4 = time.perf_counter()
+    chk = fp64_gather_check(s, sc, E, B, 64)
+    chk["seconds"] = time.perf_counter() - t4
+    out["fp64_check"] = chk
+    log(f"dense 32k x 192 against the FP64 problem of the FP32 form factors: residual "
+        f"{chk['fp64_residual_rel_l1']:.2e} rel L1 (area-weighted), error bound "
+        f"{chk['error_bound_rel_l1']} (contraction {chk['contraction_q']:.3f})")
     return out
 
 

@@ -392,6 +392,17 @@ class Solver:
                   "rad_process_hemicube")
         return hits.reshape(k, self.npatch), ff.reshape(k, self.npatch)
 
+    def ff_rows(self, shooters):
+        """FP32 form-factor rows (K x npatch) of the given shooters: the item
+        buffers stay on the device and are reduced there."""
+        sh = np.ascontiguousarray(shooters, np.uint32)
+        k = len(sh)
+        self._chk(lib().rad_render_itembuf(self._ctx, uptr(sh), k, None, None),
+                  "rad_render_itembuf")
+        ff = np.zeros(k * self.npatch, np.float32)
+        self._chk(lib().rad_process_hemicube(self._ctx, k, None, fptr(ff)), "rad_process_hemicube")
+        return ff.reshape(k, self.npatch)
+
     def process_itembuf(self, item, k):
         it = np.ascontiguousarray(item, np.uint32)
         hits = np.zeros(k * self.npatch, np.uint32)
