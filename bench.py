@@ -413,8 +413,8 @@ def main():
                     help="run one path briefly for rocprofv3: dense32k or reduce200k")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding shooters "
+              "across GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
