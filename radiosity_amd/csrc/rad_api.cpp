@@ -32,6 +32,7 @@ struct rad_ctx {
     uint32_t ntri = 0, npatch = 0;
     float *tri = nullptr;          // 9 planes of ntri
     uint32_t *tri_patch = nullptr; // ntri
+    float4 *tsph = nullptr;        // ntri bounding spheres (centre, radius)
     float *frames = nullptr;       // npatch*9
     float *refl = nullptr;         // npatch*3
     float *area = nullptr;         // npatch
@@ -179,7 +180,7 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t 
     launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->rowlen, c->counters,
                  c->bb.ent_cap, c->st);
     record(c, evbase + 2);
-    launch_raster_setup(c->g, k, c->tri, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
+    launch_raster_setup(c->g, k, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
                         c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->st);
     record(c, evbase + 3);
     launch_bin(k, c->nshoot, c->bb, c->counters, c->st);
@@ -385,7 +386,7 @@ void rad_destroy(rad_ctx *c)
     if (c->st)
         (void)hipStreamSynchronize(c->st);
     drop_graph(c);
-    void *ptrs[] = {c->tri,  c->tri_patch, c->frames, c->refl,     c->area,   c->emis,
+    void *ptrs[] = {c->tri,  c->tri_patch, c->tsph, c->frames, c->refl,     c->area,   c->emis,
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
                     c->resid, c->bb.recs, c->bb.count, c->bb.start, c->bb.fill,
@@ -482,6 +483,7 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
     int rc;
     size_t K = c->K;
     if ((rc = dalloc(c, &c->tri, (size_t)ntri * 9)) || (rc = dalloc(c, &c->tri_patch, ntri)) ||
+        (rc = dalloc(c, &c->tsph, ntri)) ||
         (rc = dalloc(c, &c->frames, (size_t)npatch * 9)) || (rc = dalloc(c, &c->refl, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->area, npatch)) || (rc = dalloc(c, &c->emis, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->unshot, (size_t)npatch * 3)) || (rc = dalloc(c, &c->shot, (size_t)npatch * 3)) ||
@@ -544,6 +546,8 @@ int upload_copy(rad_ctx *c, const float *tri_xyz, const uint32_t *tri_patch, uin
             planes[(size_t)k * ntri + t] = tri_xyz[(size_t)t * 9 + k];
     HIPCHK(c, hipMemcpy(c->tri, planes.data(), planes.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->tri_patch, tri_patch, (size_t)ntri * 4, hipMemcpyHostToDevice));
+    launch_tri_sphere(c->tri, ntri, c->tsph, c->st);
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpy(c->frames, frames, (size_t)npatch * 36, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->refl, refl, (size_t)npatch * 12, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->area, area, (size_t)npatch * 4, hipMemcpyHostToDevice));
@@ -1740,6 +1744,7 @@ int rad_scene_ingest(rad_ctx *c, const float *quads, uint32_t nface, const float
         launch_ingest(nface, nq, dfaces, dplan, dcol, dem, dsh, (uint32_t)mode, nt, c->tri,
                       c->tri_patch, c->frames, c->refl, c->area, c->emis, c->unshot, c->shot,
                       c->patch_quad, c->quad_pbeg, c->quad_nbr, c->st);
+        launch_tri_sphere(c->tri, nt, c->tsph, c->st);
         e = hipGetLastError();
     }
     if (e == hipSuccess)

@@ -118,10 +118,11 @@ void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, u
 // Binned render of the nshoot (<= K) hemicubes in three launches:
 // setup (records + bin counts), bin (scan + fill), tile (LDS depth test; keys
 // != null writes the item buffer, ff != null reduces into F rows and hits).
-void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
+void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
                          const uint32_t *slot_tbeg, const uint32_t *slot_tcnt, hipStream_t st);
+void launch_tri_sphere(const float *tri, uint32_t ntri, float4 *sph, hipStream_t st);
 void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
                 hipStream_t st);
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,

@@ -589,7 +589,37 @@ __global__ __launch_bounds__(256) void k_setup_clipped(uint32_t W, const float *
 }
 
 __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
-                                      int f0, int f1);
+                                      float4 sph, int f0, int f1);
+
+// Bounding sphere of every triangle, once per scene: the centroid as stored
+// and the largest vertex distance from it, inflated by 1e-5 relative so it
+// bounds the vertices whatever the rounding of the distances.
+__global__ __launch_bounds__(256) void k_tri_sphere(const float *__restrict__ tri, uint32_t ntri,
+                                                    float4 *__restrict__ sph)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri)
+        return;
+    float v[3][3];
+    for (int k = 0; k < 3; ++k)
+        for (int c = 0; c < 3; ++c)
+            v[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
+    float cx = (v[0][0] + v[1][0] + v[2][0]) * (1.0f / 3.0f);
+    float cy = (v[0][1] + v[1][1] + v[2][1]) * (1.0f / 3.0f);
+    float cz = (v[0][2] + v[1][2] + v[2][2]) * (1.0f / 3.0f);
+    float r2 = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        float dx = v[k][0] - cx, dy = v[k][1] - cy, dz = v[k][2] - cz;
+        r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
+    }
+    sph[t] = make_float4(cx, cy, cz, sqrtf(r2) * 1.00001f);
+}
+
+void launch_tri_sphere(const float *tri, uint32_t ntri, float4 *sph, hipStream_t st)
+{
+    if (ntri)
+        k_tri_sphere<<<(ntri + 255) / 256, 256, 0, st>>>(tri, ntri, sph);
+}
 
 // One workgroup per (256 triangles, shooter[, face]).  Phase 1: one lane
 // per triangle runs the conservative culls (cull_faces) and queues the
@@ -599,21 +629,10 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
 // flushed with one global atomic per touched tile.  Triangles are stored as 9
 // planes of ntri floats (v0.x, v0.y, ..., v2.z) so consecutive lanes load
 // consecutive addresses.
-__device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
-                                      int f0, int f1);
-
-// One lane per (triangle, shooter), looping over the five faces.  Two
-// conservative culls skip work the exact per-face test would reject anyway:
-//  - a triangle strictly below the shooter's patch plane (beyond a float
-//    rounding margin) projects into the scissored-away half of every side
-//    face and behind the near plane of the front face;
-//  - a face whose frustum (90 degree pyramid around its axis, near 0.01)
-//    misses the triangle's bounding sphere by more than a margin.
-// Triangles are stored as 9 planes of ntri floats (v0.x, v0.y, ..., v2.z) so
-// consecutive lanes load consecutive addresses.
 // In a packed launch slot s draws only its scene's triangles
 // [slot_tbeg[s], slot_tbeg[s] + slot_tcnt[s]); blockIdx.x walks that range.
 __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *__restrict__ tri,
+                                                      const float4 *__restrict__ tsph,
                                                       const uint32_t *__restrict__ tri_patch,
                                                       uint32_t ntri, const float *__restrict__ mvp,
                                                       const uint32_t *__restrict__ nshoot,
@@ -640,6 +659,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     // scene), so vertices are read once per group instead of once per shooter.
     uint32_t ltb = 0xFFFFFFFFu, ltc = 0;
     float vx[3][3];
+    float4 sph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (uint32_t g = 0; g < group; ++g) {
         const uint32_t s = blockIdx.y * group + g;
         if (s >= ns)
@@ -666,6 +686,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                         lv[3 * k + c][threadIdx.x] = vx[k][c];
                     }
                 lid[threadIdx.x] = tri_patch[t] + 1u;
+                sph = tsph[t];
             }
         }
         for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
@@ -679,7 +700,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
         const float *sm = mvp + (size_t)s * kShooterStride;
         // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
         if (tl0 + threadIdx.x < tc) {
-            uint32_t fm = cull_faces(sm, vx, f0, f1);
+            uint32_t fm = cull_faces(sm, vx, sph, f0, f1);
             while (fm) {
                 int f = __ffs(fm) - 1;
                 fm &= fm - 1;
@@ -713,18 +734,25 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
 }
 
 // Conservative culls of one triangle for one shooter; returns the mask of
-// faces in [f0, f1) the exact pipeline must look at.
-//  - a triangle with no vertex above the shooter's patch plane by more than
+// faces in [f0, f1) the exact pipeline must look at.  The triangle's
+// bounding sphere (centre, radius; k_tri_sphere) gives the centre's height
+// pa over the shooter's plane and its offsets along the face axes, with a
+// margin mq for float rounding:
+//  - a sphere wholly below the plane (pa + rho + mq < 0) puts every vertex
+//    strictly below it: in the side faces the triangle lies in the
+//    scissored-away half, in the front face behind the near plane (0.01);
+//  - a sphere straddling the band just above the plane falls back to the
+//    per-vertex test: no vertex above the plane by more than
 //    tol = min(1e-6 |v - eye|_1, 0.0099) (which also bounds the float error
-//    of the height): in the side faces it lies in the scissored-away half or
-//    within tol/|v - eye| * W/2 << 0.5 pixel of the horizon, where the
-//    nearest kept pixel centre is half a pixel away; in the front face it is
-//    nearer than the near plane (0.01).  This removes the shooter's own
-//    plane (floor under a floor patch) before any per-face work;
+//    of the height), so in the side faces the triangle lies in the
+//    scissored-away half or within tol/|v - eye| * W/2 << 0.5 pixel of the
+//    horizon, where the nearest kept pixel centre is half a pixel away, and
+//    in the front face it is nearer than the near plane.  This removes the
+//    shooter's own plane (floor under a floor patch) before any per-face work;
 //  - a face whose frustum (90 degree pyramid around its axis, near 0.01)
-//    misses the triangle's bounding sphere by more than a margin.
+//    misses the sphere by more than a margin.
 __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float (&vx)[3][3],
-                                      int f0, int f1)
+                                      float4 sph, int f0, int f1)
 {
     if (sm[92] != 0.0f)
         return ((1u << f1) - 1u) & ~((1u << f0) - 1u);
@@ -732,33 +760,26 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
     const float nx = sm[83], ny = sm[84], nz = sm[85];
     const float ux = sm[86], uy = sm[87], uz = sm[88];
     const float wx = sm[89], wy = sm[90], wz = sm[91];
-    float d[3][3];
-    bool below = true;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        d[k][0] = vx[k][0] - ex;
-        d[k][1] = vx[k][1] - ey;
-        d[k][2] = vx[k][2] - ez;
-        float h = d[k][0] * nx + d[k][1] * ny + d[k][2] * nz;
-        float tol = fminf(1e-6f * (fabsf(d[k][0]) + fabsf(d[k][1]) + fabsf(d[k][2])), 0.0099f);
-        below = below && (h < tol);
-    }
-    if (below)
+    const float cx = sph.x - ex, cy = sph.y - ey, cz = sph.z - ez, rho = sph.w;
+    const float pa = cx * nx + cy * ny + cz * nz;
+    const float pb = cx * ux + cy * uy + cz * uz;
+    const float pc = cx * wx + cy * wy + cz * wz;
+    const float mq = 1e-4f * (fabsf(pa) + fabsf(pb) + fabsf(pc) + rho) + 1e-6f;
+    if (pa + rho + mq < 0.0f)
         return 0u;
-    float cx = (d[0][0] + d[1][0] + d[2][0]) * (1.0f / 3.0f);
-    float cy = (d[0][1] + d[1][1] + d[2][1]) * (1.0f / 3.0f);
-    float cz = (d[0][2] + d[1][2] + d[2][2]) * (1.0f / 3.0f);
-    float r2 = 0.0f;
+    if (pa - rho - mq <= 0.0099f) {
+        bool below = true;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        float qx = d[k][0] - cx, qy = d[k][1] - cy, qz = d[k][2] - cz;
-        r2 = fmaxf(r2, qx * qx + qy * qy + qz * qz);
+        for (int k = 0; k < 3; ++k) {
+            float dx = vx[k][0] - ex, dy = vx[k][1] - ey, dz = vx[k][2] - ez;
+            float h = dx * nx + dy * ny + dz * nz;
+            float tol = fminf(1e-6f * (fabsf(dx) + fabsf(dy) + fabsf(dz)), 0.0099f);
+            below = below && (h < tol);
+        }
+        if (below)
+            return 0u;
     }
-    float rho = sqrtf(r2);
-    float pa = cx * nx + cy * ny + cz * nz;
-    float pb = cx * ux + cy * uy + cz * uz;
-    float pc = cx * wx + cy * wy + cz * wz;
-    float m = rho * 1.4143f + 1e-4f * (fabsf(pa) + fabsf(pb) + fabsf(pc) + rho) + 1e-6f;
+    const float m = rho * 1.4143f + mq;
     uint32_t mask = 0u;
     for (int face = f0; face < f1; ++face) {
         float q, s1, s2;
@@ -1217,7 +1238,7 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     }
 }
 
-void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
+void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
                          const uint32_t *slot_tbeg, const uint32_t *slot_tcnt, hipStream_t st)
@@ -1242,7 +1263,7 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri,
         group = K;
     dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
     k_raster_setup<<<grid, 256, bb.ntiles * sizeof(uint32_t), st>>>(
-        g.W, tri, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
+        g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
                                          slot_tbeg, slot_tcnt, group, K);
     bc.lnrec = nullptr;
     k_setup_clipped<<<bb.clip_blocks, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
