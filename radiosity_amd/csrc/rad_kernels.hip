@@ -1126,13 +1126,15 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
 }
 
 // One workgroup per bin = (shooter, tile).  The bin's records are taken 256
-// at a time and counting-sorted in LDS by their pixel count inside the tile
-// (power-of-two classes), so each wavefront holds records of about the same
-// size: records of at most 64 pixels are then drawn one per lane with
-// incremental edge functions, larger ones by the whole wavefront, one pixel
-// per lane.  Depth is resolved in LDS with 64-bit atomicMin.  Then the tile
-// either goes to the key buffer (keys != null) or is reduced into F rows and
-// hit counts (ff != null).
+// at a time, one per lane.  A record covering at most small_px pixels of the
+// tile is drawn by its lane with incremental edge functions; larger ones go
+// to an LDS queue that the wavefronts then draw one record per wavefront, one
+// pixel per lane.  (Sorting the records by size first, so that a wavefront's
+// lanes draw records of about the same size, cost more in loads and barriers
+// than the lane divergence it saved.)  Depth is resolved in LDS with 64-bit
+// atomicMin, so the drawing order does not matter.  Then the tile either goes
+// to the key buffer (keys != null) or is reduced into F rows and hit counts
+// (ff != null).
 __global__ __launch_bounds__(256) void k_tile_raster(
     uint32_t W, uint32_t TW, uint32_t TH, uint32_t RES, const uint32_t *__restrict__ nshoot,
     const RasterRec *__restrict__ recs, const uint64_t *__restrict__ bbox,
@@ -1146,7 +1148,7 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     __shared__ uint32_t perm[256];
     __shared__ uint64_t pbb[256];
-    __shared__ uint32_t cls_n[9], cls_off[9];
+    __shared__ uint32_t nbig;
     const uint32_t bin = order[blockIdx.x];
     const uint32_t s = bin / ntiles;
     if (s >= *nshoot)
@@ -1159,60 +1161,41 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     const size_t b = (size_t)s * ntiles + tile;
     const uint32_t n = bin_count[b], st0 = bin_start[b];
     const int32_t Wi = (int32_t)W;
-    const uint32_t lane = threadIdx.x & 63, me = threadIdx.x;
+    const uint32_t me = threadIdx.x;
     for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
-        if (me < 9)
-            cls_n[me] = 0u;
+        if (me == 0)
+            nbig = 0u;
         __syncthreads();
-        // class of this lane's record: 0..6 for <= 1, 2, 4 ... 64 pixels,
-        // 7 for larger, 8 for no record
-        uint32_t i = c0 + me, cls = 8, ri = 0;
-        uint64_t bv = 0;
+        const uint32_t i = c0 + me;
         if (i < n && st0 + i < ent_cap) {
-            ri = list[st0 + i];
-            bv = bbox[ri];
-            int32_t ax0 = max((int32_t)(bv & 0xFFFFu), ox);
-            int32_t ay0 = max((int32_t)((bv >> 16) & 0xFFFFu), oy);
-            int32_t ax1 = min((int32_t)((bv >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
-            int32_t ay1 = min((int32_t)(bv >> 48), oy + (int32_t)T - 1);
-            uint32_t area = (uint32_t)(ax1 - ax0 + 1) * (uint32_t)(ay1 - ay0 + 1);
-            cls = area <= 1 ? 0u : (area > small_px ? 7u : min(6u, 32u - __clz(area - 1)));
-        }
-        uint32_t rank = atomicAdd(&cls_n[cls], 1u);
-        __syncthreads();
-        if (me == 0) {
-            uint32_t acc = 0;
-            for (int k = 0; k < 9; ++k) {
-                cls_off[k] = acc;
-                acc += cls_n[k];
+            const uint32_t ri = list[st0 + i];
+            const uint64_t bv = bbox[ri];
+            const int32_t ax0 = max((int32_t)(bv & 0xFFFFu), ox);
+            const int32_t ay0 = max((int32_t)((bv >> 16) & 0xFFFFu), oy);
+            const int32_t ax1 = min((int32_t)((bv >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
+            const int32_t ay1 = min((int32_t)(bv >> 48), oy + (int32_t)T - 1);
+            const uint32_t area = (uint32_t)(ax1 - ax0 + 1) * (uint32_t)(ay1 - ay0 + 1);
+            if (area <= small_px) {
+                RasterRec r = recs[ri];
+                FaceView fv = face_view(Wi, (int)(r.sf & 7u));
+                draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
+            } else {
+                const uint32_t q = atomicAdd(&nbig, 1u);
+                perm[q] = ri;
+                pbb[q] = bv;
             }
         }
         __syncthreads();
-        if (cls < 8) {
-            perm[cls_off[cls] + rank] = ri;
-            pbb[cls_off[cls] + rank] = bv;
-        }
-        __syncthreads();
-        const uint32_t nsmall = cls_off[7], nall = cls_off[8];
-        if (me < nsmall) {
-            RasterRec r = recs[perm[me]];
-            uint64_t bb = pbb[me];
-            int32_t ax0 = max((int32_t)(bb & 0xFFFFu), ox);
-            int32_t ax1 = min((int32_t)((bb >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
-            int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
-            int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
-            FaceView fv = face_view(Wi, (int)(r.sf & 7u));
-            draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
-        }
         // large records: the whole wavefront per record in 8 x 8 pixel
-        // blocks; wave w takes records nsmall + w, nsmall + w + 4, ...
-        for (uint32_t q = nsmall + (me >> 6); q < nall; q += blockDim.x >> 6) {
+        // blocks; wave w takes queue entries w, w + 4, ...
+        const uint32_t nall = nbig;
+        for (uint32_t q = me >> 6; q < nall; q += blockDim.x >> 6) {
             RasterRec r = recs[perm[q]];
-            uint64_t bb = pbb[q];
-            int32_t ax0 = max((int32_t)(bb & 0xFFFFu), ox);
-            int32_t ax1 = min((int32_t)((bb >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
-            int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
-            int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
+            const uint64_t bb = pbb[q];
+            const int32_t ax0 = max((int32_t)(bb & 0xFFFFu), ox);
+            const int32_t ax1 = min((int32_t)((bb >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
+            const int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
+            const int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
             FaceView fv = face_view(Wi, (int)(r.sf & 7u));
             Edges e;
             make_edges(r.x, r.y, e);
