@@ -960,9 +960,57 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
     }
 }
 
-// Wave-level ProcessHemicube on 64 consecutive texels (any grouping of equal
-// ids gives the same sums; consecutive texels make runs long).  Every lane
-// of the wavefront must call it.
+// Segmented sums over the runs of equal id in a wavefront, on DPP lane
+// moves (no LDS round trip).  A run starts where the id differs from the
+// previous lane's; ss = the lane of the last run start at or before this
+// lane, read off the wavefront's mask of run starts.  The sums are a
+// Hillis-Steele scan inside each row of 16 lanes (row_shr 1, 2, 4, 8; a lane
+// adds lane - d only if lane - d >= ss, so only its own run), then the
+// row_bcast steps carry the sum at lane 15 / 47 into rows 1 / 3 and at lane 31
+// into rows 2 and 3, again only into lanes whose run began at or before that
+// lane.  Lanes a DPP move cannot read from give 0.  Returns true in the last
+// lane of each run with an id (id != 0), which then holds the run's sums of f
+// (and of c when with_c).  Every lane of the wavefront must call it.
+template <int CTRL, int ROWS> __device__ static inline float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
+}
+template <int CTRL, int ROWS> __device__ static inline uint32_t dpp_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, true);
+}
+
+template <int CTRL, int ROWS>
+__device__ static inline void seg_step(float &f, uint32_t &c, bool take, bool with_c)
+{
+    const float fo = dpp_f<CTRL, ROWS>(f);
+    f += take ? fo : 0.0f;
+    if (with_c) {
+        const uint32_t co = dpp_u<CTRL, ROWS>(c);
+        c += take ? co : 0u;
+    }
+}
+
+__device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c, bool with_c)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t pid = dpp_u<0x138, 0xF>(id); // wave_shr:1
+    const bool head = lane == 0 || pid != id;
+    const uint64_t hm = __ballot(head);
+    const int ss = 63 - __clzll(hm & ((2ull << lane) - 1ull));
+    seg_step<0x111, 0xF>(f, c, lane - 1 >= ss, with_c); // row_shr:1
+    seg_step<0x112, 0xF>(f, c, lane - 2 >= ss, with_c); // row_shr:2
+    seg_step<0x114, 0xF>(f, c, lane - 4 >= ss, with_c); // row_shr:4
+    seg_step<0x118, 0xF>(f, c, lane - 8 >= ss, with_c); // row_shr:8
+    seg_step<0x142, 0xA>(f, c, (lane & ~15) - 1 >= ss, with_c); // row_bcast:15
+    seg_step<0x143, 0xC>(f, c, 31 >= ss, with_c);               // row_bcast:31
+    const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
+    return tail && id != 0;
+}
+
+// Wave-level ProcessHemicube on 64 consecutive texels of one shooter's
+// hemicube hi (any grouping of equal ids gives the same sums; consecutive
+// texels make runs long).  Every lane of the wavefront must call it.
 // F row entries are id - 1 - pbase for ids in [pbase + 1, pbase + rowlen]
 // (pbase: first patch of the shooter's scene in a packed launch, else 0).
 __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t pbase,
@@ -970,29 +1018,8 @@ __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uin
                                           uint32_t *__restrict__ hits,
                                           uint32_t *__restrict__ counters)
 {
-    int lane = threadIdx.x & 63;
     uint32_t c = id ? 1u : 0u;
-    uint32_t pid = __shfl_up(id, 1), phi = __shfl_up(hi, 1);
-    bool head = lane == 0 || pid != id || phi != hi;
-    int ss = head ? lane : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int o = __shfl_up(ss, d);
-        if (lane >= d && o > ss)
-            ss = o;
-    }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        float fo = __shfl_up(f, d);
-        uint32_t co = __shfl_up(c, d);
-        if (lane - d >= ss) {
-            f += fo;
-            c += co;
-        }
-    }
-    uint32_t nid = __shfl_down(id, 1), nhi = __shfl_down(hi, 1);
-    bool tail = lane == 63 || nid != id || nhi != hi;
-    if (tail && id != 0) {
+    if (wave_runs(id, f, c, hits != nullptr)) {
         uint32_t p = id - 1 - pbase;
         if (p < rowlen) {
             size_t o = (size_t)hi * rowlen + p;
@@ -1014,7 +1041,7 @@ __device__ static inline void draw_span(const RasterRec &r, const Edges &e, int3
         int32_t tx = ax0 + (int32_t)(p % w), ty = ay0 + (int32_t)(p / w);
         uint64_t key = frag_key(e, r, tx - vpx, ty - vpy);
         uint32_t pp = (uint32_t)((ty - oy) << tshift) + (uint32_t)(tx - ox);
-        if (key && key < tk[pp])
+        if (key)
             atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
     }
 }
@@ -1053,8 +1080,7 @@ __device__ static inline void draw_wave(const RasterRec &r, const Edges &e, int3
                 if (z < 1.0f) {
                     uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | r.id;
                     uint32_t pp = (uint32_t)((ty - oy) << tshift) + (uint32_t)(tx - ox);
-                    if (key < tk[pp])
-                        atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
+                    atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
                 }
             }
             e0 += sx[0];
@@ -1111,8 +1137,7 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
                 if (z < 1.0f) {
                     uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | r.id;
                     uint32_t pp = prow + (uint32_t)tx;
-                    if (key < tk[pp])
-                        atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
+                    atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
                 }
             }
             e0 += dA[0];
@@ -1132,7 +1157,9 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
 // pixel per lane.  (Sorting the records by size first, so that a wavefront's
 // lanes draw records of about the same size, cost more in loads and barriers
 // than the lane divergence it saved.)  Depth is resolved in LDS with 64-bit
-// atomicMin, so the drawing order does not matter.  Then the tile either goes
+// atomicMin, so the drawing order does not matter; the atomic returns nothing
+// and is issued without first reading the texel, so no fragment waits on an
+// LDS read.  Then the tile either goes
 // to the key buffer (keys != null) or is reduced into F rows and hit counts
 // (ff != null).
 __global__ __launch_bounds__(256) void k_tile_raster(
@@ -1289,34 +1316,6 @@ void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const Bi
 // slot adds to the global entry directly.
 constexpr uint32_t kReduceChunk = 4096, kReduceSlots = 2048;
 
-// Segmented sum of (f, c) over the runs of equal id in a wavefront; returns
-// true in the last lane of each run with an id, which then holds the run's
-// sums.  Every lane of the wavefront must call it.
-__device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c)
-{
-    const int lane = threadIdx.x & 63;
-    const uint32_t pid = __shfl_up(id, 1);
-    const bool head = lane == 0 || pid != id;
-    int ss = head ? lane : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int o = __shfl_up(ss, d);
-        if (lane >= d && o > ss)
-            ss = o;
-    }
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        float fo = __shfl_up(f, d);
-        uint32_t co = __shfl_up(c, d);
-        if (lane - d >= ss) {
-            f += fo;
-            c += co;
-        }
-    }
-    const uint32_t nid = __shfl_down(id, 1);
-    return (lane == 63 || nid != id) && id != 0;
-}
-
 __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__restrict__ nshoot,
                                                 const uint64_t *__restrict__ keys,
                                                 const float *__restrict__ dff, uint32_t npatch,
@@ -1349,7 +1348,7 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
                 c = 1;
             }
         }
-        if (wave_runs(id, f, c)) {
+        if (wave_runs(id, f, c, true)) {
             uint32_t h = (id * 2654435761u) >> 21; // kReduceSlots = 2^11
             // Main.cpp:1259-1262: an unknown patch id is counted (once per
             // run) and skipped
