@@ -515,9 +515,21 @@ __device__ static inline void transform3(const float *mm, const float (&vx)[3][3
 
 // Exact per-face pipeline of one triangle up to the record.  Triangles that
 // need clipping (near plane or guard band) are queued for k_setup_clipped.
-__device__ static inline void setup_face(const float *mm, const float (&vx)[3][3], uint32_t W,
-                                         uint32_t s, int face, uint32_t id, uint32_t t,
-                                         const BinCtx &bc)
+// Scissor rectangle of a face in clip space (x / w, y / w) with a one-pixel
+// margin: (lo_x, hi_x, lo_y, hi_y).
+__device__ static inline float4 face_scissor_clip(int32_t W, int face)
+{
+    FaceView fv = face_view(W, face);
+    return make_float4(2.0f * (float)(fv.sx0 - 1) / (float)W - 1.0f,
+                       2.0f * (float)(fv.sx1 + 1) / (float)W - 1.0f,
+                       2.0f * (float)(fv.sy0 - 1) / (float)W - 1.0f,
+                       2.0f * (float)(fv.sy1 + 1) / (float)W - 1.0f);
+}
+
+// mm: the face's MVP, scis: face_scissor_clip of the face.
+__device__ static inline void setup_face(const float *mm, float4 scis, const float (&vx)[3][3],
+                                         uint32_t W, uint32_t s, int face, uint32_t id,
+                                         uint32_t t, const BinCtx &bc)
 {
     CV4 poly[3];
     transform3(mm, vx, poly);
@@ -545,10 +557,7 @@ __device__ static inline void setup_face(const float *mm, const float (&vx)[3][3
     float halfW = 0.5f * (float)W;
     // All vertices in front of the near plane (w > 0): reject against the
     // scissor rectangle in clip space with a one-pixel margin.
-    float lo_x = 2.0f * (float)(fv.sx0 - 1) / (float)W - 1.0f;
-    float hi_x = 2.0f * (float)(fv.sx1 + 1) / (float)W - 1.0f;
-    float lo_y = 2.0f * (float)(fv.sy0 - 1) / (float)W - 1.0f;
-    float hi_y = 2.0f * (float)(fv.sy1 + 1) / (float)W - 1.0f;
+    const float lo_x = scis.x, hi_x = scis.y, lo_y = scis.z, hi_y = scis.w;
     bool lx = true, hx = true, ly = true, hy = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -647,7 +656,11 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     __shared__ uint32_t lid[256];
     __shared__ uint16_t lq[256 * 5];
     __shared__ uint32_t lnrec, lqn;
+    __shared__ float4 lscis[5];
+    __shared__ float lmvp[80]; // the current shooter's five face MVPs
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
+    if (threadIdx.x < 5)
+        lscis[threadIdx.x] = face_scissor_clip((int32_t)W, (int)threadIdx.x);
     const uint32_t tl0 = blockIdx.x * blockDim.x;
     // gridDim.z == 5 splits the faces over blocks (small K), else one block
     // walks all five faces
@@ -691,13 +704,15 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
         }
         for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
             lcnt[i] = 0u;
+        const float *sm = mvp + (size_t)s * kShooterStride;
         if (threadIdx.x == 0) {
             lnrec = 0u;
             lqn = 0u;
         }
+        if (threadIdx.x < 80)
+            lmvp[threadIdx.x] = sm[threadIdx.x];
         __syncthreads();
         bc.region = rb * kRecsPerBlock;
-        const float *sm = mvp + (size_t)s * kShooterStride;
         // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
         if (tl0 + threadIdx.x < tc) {
             uint32_t fm = cull_faces(sm, vx, sph, f0, f1);
@@ -720,7 +735,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                     wv[k][c] = lv[3 * k + c][tl];
-            setup_face(sm + face * 16, wv, W, s, face, lid[tl], tb + tl0 + tl, bc);
+            setup_face(lmvp + face * 16, lscis[face], wv, W, s, face, lid[tl], tb + tl0 + tl, bc);
         }
         __syncthreads();
         // flush: one global atomic per touched tile, and the region's record count
