@@ -450,53 +450,70 @@ __device__ static void emit_tri(const SV &a, const SV &b, const SV &c, uint32_t 
 
 // Sutherland-Hodgman against the near plane and the guard band; an
 // intersection is interpolated from the inside vertex towards the outside.
+// The polygons live in LDS (lp: [2][kClipMax][64], one column per lane of
+// the 64-thread block) rather than in dynamically indexed private arrays,
+// which went to scratch memory.  A convex polygon gains at most one vertex
+// per plane (3 + 5 = 8); more sign changes along the outline than two can
+// only come from vertices within rounding of a plane, and a polygon that
+// would outgrow kClipMax is dropped and flagged as an overflow.
+constexpr int kClipMax = 12;
 __device__ static void emit_clipped(const CV4 *in3, float halfW, uint32_t id, const FaceView &fv,
-                                   uint32_t s, int face, const BinCtx &bc)
+                                   uint32_t s, int face, const BinCtx &bc, CV4 (*lp)[kClipMax][64],
+                                   uint32_t ln)
 {
-    CV4 poly[12], tmp[12];
-    poly[0] = in3[0];
-    poly[1] = in3[1];
-    poly[2] = in3[2];
-    int n = 3;
+    lp[0][0][ln] = in3[0];
+    lp[0][1][ln] = in3[1];
+    lp[0][2][ln] = in3[2];
+    int n = 3, cur = 0;
     for (int plane = 0; plane < 5; ++plane) {
-        float d[12];
         int nin = 0;
-        for (int i = 0; i < n; ++i) {
-            d[i] = clip_dist(poly[i], plane);
-            nin += d[i] >= 0.0f;
-        }
+        for (int i = 0; i < n; ++i)
+            nin += clip_dist(lp[cur][i][ln], plane) >= 0.0f;
         if (nin == n)
             continue;
         if (nin == 0)
             return;
         int m = 0;
+        CV4 P = lp[cur][0][ln];
+        float dP = clip_dist(P, plane);
+        const CV4 P0 = P;
+        const float d0 = dP;
         for (int i = 0; i < n; ++i) {
-            int j = (i + 1 == n) ? 0 : i + 1;
-            bool ini = d[i] >= 0.0f, inj = d[j] >= 0.0f;
+            const bool last = i + 1 == n;
+            CV4 Q = last ? P0 : lp[cur][i + 1][ln];
+            float dQ = last ? d0 : clip_dist(Q, plane);
+            bool ini = dP >= 0.0f, inj = dQ >= 0.0f;
+            if (m + 2 > kClipMax) {
+                atomicOr(&bc.counters[kCntOverflow], 1u);
+                return;
+            }
             if (ini)
-                tmp[m++] = poly[i];
+                lp[cur ^ 1][m++][ln] = P;
             if (ini != inj) {
-                const CV4 &A = ini ? poly[i] : poly[j];
-                const CV4 &B = ini ? poly[j] : poly[i];
-                float da = ini ? d[i] : d[j], db = ini ? d[j] : d[i];
+                const CV4 &A = ini ? P : Q;
+                const CV4 &B = ini ? Q : P;
+                float da = ini ? dP : dQ, db = ini ? dQ : dP;
                 float t = da / (da - db);
                 CV4 r;
                 for (int c = 0; c < 4; ++c)
                     r.c[c] = A.c[c] + t * (B.c[c] - A.c[c]);
-                tmp[m++] = r;
+                lp[cur ^ 1][m++][ln] = r;
             }
+            P = Q;
+            dP = dQ;
         }
         n = m;
-        for (int i = 0; i < n; ++i)
-            poly[i] = tmp[i];
+        cur ^= 1;
     }
     if (n < 3)
         return;
-    SV sv[12];
-    for (int k = 0; k < n; ++k)
-        sv[k] = project(poly[k], halfW);
-    for (int k = 1; k + 1 < n; ++k)
-        emit_tri(sv[0], sv[k], sv[k + 1], id, fv, s, face, bc);
+    const SV a = project(lp[cur][0][ln], halfW);
+    SV b = project(lp[cur][1][ln], halfW);
+    for (int k = 1; k + 1 < n; ++k) {
+        const SV c = project(lp[cur][k + 1][ln], halfW);
+        emit_tri(a, b, c, id, fv, s, face, bc);
+        b = c;
+    }
 }
 
 // Clip-space transform of a triangle: MVP (column-major) times (x, y, z, 1),
@@ -576,11 +593,12 @@ __device__ static inline void setup_face(const float *mm, float4 scis, const flo
 // Queued (triangle, shooter, face) jobs that cross the near plane or the
 // guard band: same transform, then Sutherland-Hodgman and a fan; records go
 // to the spill region.
-__global__ __launch_bounds__(256) void k_setup_clipped(uint32_t W, const float *__restrict__ tri,
+__global__ __launch_bounds__(64) void k_setup_clipped(uint32_t W, const float *__restrict__ tri,
                                                        const uint32_t *__restrict__ tri_patch,
                                                        uint32_t ntri, const float *__restrict__ mvp,
                                                        BinCtx bc)
 {
+    __shared__ CV4 lp[2][kClipMax][64];
     uint32_t n = min(bc.counters[kCntClip], bc.clip_cap);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
         uint2 job = bc.clipq[q];
@@ -593,7 +611,7 @@ __global__ __launch_bounds__(256) void k_setup_clipped(uint32_t W, const float *
         CV4 poly[3];
         transform3(mvp + (size_t)s * kShooterStride + face * 16, vx, poly);
         FaceView fv = face_view((int32_t)W, face);
-        emit_clipped(poly, 0.5f * (float)W, tri_patch[t] + 1u, fv, s, face, bc);
+        emit_clipped(poly, 0.5f * (float)W, tri_patch[t] + 1u, fv, s, face, bc, lp, threadIdx.x);
     }
 }
 
@@ -812,64 +830,105 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
     return mask;
 }
 
-// Exclusive scan of n bin counts by one block of 1024 threads (each thread
-// scans a contiguous chunk); writes start offsets, resets the fill counters and
-// stores the total in counters[kCntEntries].
+// Exclusive scan of n bin counts by one block of 1024 threads; writes start
+// offsets, resets the fill counters and stores the total in
+// counters[kCntEntries].  The counts go through LDS in chunks of
+// kScanChunk: every thread issues its kScanPer coalesced loads at once, then
+// scans kScanPer contiguous counts from LDS, so a chunk costs one memory
+// latency instead of one per count.
 // Also orders the bins for the tile kernel by descending size class
 // (floor(log2(count)) buckets, a counting sort): the largest bins start
 // first, so the heavy horizon tiles overlap the light ones instead of
 // forming the tail of the launch.
+constexpr uint32_t kScanPer = 8, kScanChunk = 1024 * kScanPer;
+
+__device__ static inline uint32_t size_class(uint32_t c)
+{
+    return c ? 32u - __clz(c) : 0u;
+}
+
 __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ cnt,
                                                    uint32_t *__restrict__ start,
                                                    uint32_t *__restrict__ fill, uint32_t n,
                                                    uint32_t *__restrict__ counters,
                                                    uint32_t *__restrict__ order)
 {
+    __shared__ uint32_t lc[kScanChunk];
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t bucket[33], boff[33];
-    if (threadIdx.x < 33)
-        bucket[threadIdx.x] = 0u;
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-    uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
-    uint32_t sum = 0;
-    for (uint32_t i = b0; i < b1; ++i)
-        sum += cnt[i];
-    // inclusive scan inside the wave, then over the wave totals
-    uint32_t inc = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(inc, d, 64);
-        if (lane >= (uint32_t)d)
-            inc += o;
-    }
-    if (lane == 63)
-        wsum[w] = inc;
+    __shared__ uint32_t carry;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < 33)
+        bucket[tid] = 0u;
+    if (tid == 0)
+        carry = 0u;
     __syncthreads();
-    if (w == 0) {
-        uint32_t v = lane < (blockDim.x >> 6) ? wsum[lane] : 0u, x = v;
+    for (uint32_t c0 = 0; c0 < n; c0 += kScanChunk) {
+        uint32_t v[kScanPer];
 #pragma unroll
-        for (int d = 1; d < 16; d <<= 1) {
-            uint32_t o = __shfl_up(x, d, 64);
-            if (lane >= (uint32_t)d)
-                x += o;
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t i = c0 + k * 1024u + tid;
+            v[k] = i < n ? cnt[i] : 0u;
         }
-        if (lane < 16)
-            wsum[lane] = x - v; // exclusive prefix of the wave totals
-        if (lane == (blockDim.x >> 6) - 1)
-            counters[kCntEntries] = x;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t i = c0 + k * 1024u + tid;
+            lc[k * 1024u + tid] = v[k];
+            if (i < n) {
+                fill[i] = 0u;
+                atomicAdd(&bucket[size_class(v[k])], 1u);
+            }
+        }
+        __syncthreads();
+        uint32_t x[kScanPer], sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            x[k] = lc[tid * kScanPer + k];
+            sum += x[k];
+        }
+        // inclusive scan inside the wave, then over the wave totals
+        uint32_t inc = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d)
+                inc += o;
+        }
+        if (lane == 63)
+            wsum[w] = inc;
+        const uint32_t base = carry;
+        __syncthreads();
+        if (w == 0) {
+            uint32_t y = lane < 16 ? wsum[lane] : 0u, z = y;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                uint32_t o = __shfl_up(z, d, 64);
+                if (lane >= (uint32_t)d)
+                    z += o;
+            }
+            if (lane < 16)
+                wsum[lane] = z - y; // exclusive prefix of the wave totals
+            if (lane == 15)
+                carry = base + z;
+        }
+        __syncthreads();
+        uint32_t run = base + wsum[w] + inc - sum;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            lc[tid * kScanPer + k] = run;
+            run += x[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            const uint32_t i = c0 + k * 1024u + tid;
+            if (i < n)
+                start[i] = lc[k * 1024u + tid];
+        }
+        __syncthreads(); // lc and wsum are reused by the next chunk
     }
-    __syncthreads();
-    uint32_t run = wsum[w] + inc - sum;
-    for (uint32_t i = b0; i < b1; ++i) {
-        uint32_t c = cnt[i];
-        start[i] = run;
-        fill[i] = 0u;
-        run += c;
-        atomicAdd(&bucket[c ? 32u - __clz(c) : 0u], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
+        counters[kCntEntries] = carry;
         uint32_t acc = 0;
         for (int k = 32; k >= 0; --k) {
             boff[k] = acc;
@@ -877,10 +936,14 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
         }
     }
     __syncthreads();
-    for (uint32_t i = b0; i < b1; ++i) {
-        uint32_t c = cnt[i];
-        order[atomicAdd(&boff[c ? 32u - __clz(c) : 0u], 1u)] = i;
-    }
+    // Each thread ranks a contiguous run of bins, one at a time: consecutive
+    // launches of a class then come from bins about n / 1024 apart (tiles far
+    // apart on the hemicube).  That ran measurably faster in the tile kernel
+    // than neighbouring tiles launched together, and than batching the loads.
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = tid * per, b1 = min(n, b0 + per);
+    for (uint32_t i = b0; i < b1; ++i)
+        order[atomicAdd(&boff[size_class(cnt[i])], 1u)] = i;
 }
 
 // Same grid as k_raster_setup: block (blk, s) puts the records of its region
@@ -911,16 +974,15 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
     extern __shared__ uint32_t dyn_lds[];
     uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + ntiles; // dynamic LDS, 2 x ntiles
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
-    // the workgroup walks `group` consecutive shooter slots, like the setup
-    for (uint32_t g = 0; g < group; ++g) {
-        const uint32_t s = blockIdx.y * group + g;
-        if (s >= ns)
-            break;
-        uint32_t *fl = fill + (size_t)s * ntiles;
-        const uint32_t *stt = start + (size_t)s * ntiles;
-        if (blockIdx.x == nblk) {
-            if (blockIdx.z != 0)
-                return;
+    if (blockIdx.x == nblk) {
+        if (blockIdx.z != 0)
+            return;
+        for (uint32_t g = 0; g < group; ++g) {
+            const uint32_t s = blockIdx.y * group + g;
+            if (s >= ns)
+                break;
+            uint32_t *fl = fill + (size_t)s * ntiles;
+            const uint32_t *stt = start + (size_t)s * ntiles;
             // spill records of this shooter: one global atomic each (rare)
             uint32_t nsp = min(counters[kCntSpill], spill_cap);
             for (uint32_t q = threadIdx.x; q < nsp; q += blockDim.x) {
@@ -937,19 +999,54 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
                             list[pos] = r;
                     }
             }
-            continue;
         }
+        return;
+    }
+    // The workgroup walks `group` consecutive shooter slots, like the setup.
+    // Each lane holds the bboxes of its kFillPer region slots in registers;
+    // the next slot's count and bboxes are loaded while the current slot is
+    // binned, so a slot costs no load latency of its own.
+    constexpr uint32_t kFillPer = kRecsPerBlock / 256;
+    static_assert(kRecsPerBlock % 256 == 0, "k_bin_fill runs 256 threads per region");
+    uint64_t bv[kFillPer];
+    uint32_t nrec = 0;
+    auto prefetch = [&](uint32_t s) {
         const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
-        const uint32_t region = blk * kRecsPerBlock, nrec = blk_nrec[blk];
-        if (nrec == 0)
+        nrec = blk_nrec[blk];
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k) {
+            const uint32_t q = threadIdx.x + 256u * k;
+            bv[k] = q < nrec ? bbox[(size_t)blk * kRecsPerBlock + q] : 0ull;
+        }
+    };
+    if (blockIdx.y * group < ns)
+        prefetch(blockIdx.y * group);
+    for (uint32_t g = 0; g < group; ++g) {
+        const uint32_t s = blockIdx.y * group + g;
+        if (s >= ns)
+            break;
+        uint32_t *fl = fill + (size_t)s * ntiles;
+        const uint32_t *stt = start + (size_t)s * ntiles;
+        const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
+        const uint32_t region = blk * kRecsPerBlock, n = nrec;
+        uint64_t cur[kFillPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k)
+            cur[k] = bv[k];
+        if (g + 1 < group && s + 1 < ns)
+            prefetch(s + 1);
+        if (n == 0)
             continue;
         __syncthreads(); // the previous slot is done with lcnt / lbase
         for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x)
             lcnt[i] = 0u;
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k) {
+            if (threadIdx.x + 256u * k >= n)
+                break;
             uint32_t g0, g1, h0, h1;
-            unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
+            unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx)
                     atomicAdd(&lcnt[gy * ntx + gx], 1u);
@@ -961,9 +1058,13 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             lcnt[i] = 0u;
         }
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < nrec; q += blockDim.x) {
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k) {
+            const uint32_t q = threadIdx.x + 256u * k;
+            if (q >= n)
+                break;
             uint32_t g0, g1, h0, h1;
-            unpack_bbox(bbox[region + q], tshift, g0, g1, h0, h1);
+            unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx) {
                     uint32_t b = gy * ntx + gx;
@@ -1291,7 +1392,7 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const 
         g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
                                          slot_tbeg, slot_tcnt, group, K);
     bc.lnrec = nullptr;
-    k_setup_clipped<<<bb.clip_blocks, 256, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
+    k_setup_clipped<<<bb.clip_blocks, 64, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
 }
 
 void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
@@ -1754,7 +1855,10 @@ __global__ __launch_bounds__(256) void k_topk_pass(const float *__restrict__ uns
 // (lane 0 largest).  A batch of 64 new keys is sorted with a shuffle bitonic
 // network and merged into the list (max against the reversed batch gives a
 // bitonic sequence holding the top 64 of both; a half-cleaner pass sorts
-// it).  Waves scan the patches, then one workgroup merges the wave lists.
+// it).  Waves scan the patches (a batch with no key above the wave's 64th
+// is skipped), each workgroup merges its wave lists in LDS, then one
+// workgroup merges the workgroup lists.  The keys are distinct (they hold
+// the patch index), so the result does not depend on the merge order.
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
     uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
@@ -1802,15 +1906,25 @@ __global__ __launch_bounds__(256) void k_topk64_scan(const float *__restrict__ u
     const int lane = threadIdx.x & 63;
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (blockDim.x >> 6);
-    uint64_t top = 0;
+    __shared__ uint64_t wl[4][64];
+    uint64_t top = 0, thr = 0; // thr: the list's 64th key
     for (uint32_t base = gw * 64; base < npatch; base += nw * 64) {
         uint32_t j = base + lane;
         uint64_t key = j < npatch ? sel_key(RAD_SELECT_POWER, unshot, area, j) : 0ull;
-        if (__ballot(key != 0) == 0)
+        if (__ballot(key > thr) == 0)
             continue;
         top = wave_merge_desc(top, wave_sort_desc(key));
+        thr = shfl_u64(top, 63);
     }
-    out[(size_t)gw * 64 + lane] = top;
+    // the workgroup's four lists: two pairwise merge levels in LDS
+    const int w = threadIdx.x >> 6;
+    wl[w][lane] = top;
+    __syncthreads();
+    if (w < 2)
+        wl[w][lane] = wave_merge_desc(top, wl[w + 2][lane]);
+    __syncthreads();
+    if (w == 0)
+        out[(size_t)blockIdx.x * 64 + lane] = wave_merge_desc(wl[0][lane], wl[1][lane]);
 }
 
 // Merges nlists sorted lists of 64 (in[l * 64 + lane]) per workgroup; the
@@ -1825,15 +1939,28 @@ __global__ __launch_bounds__(1024) void k_topk64_merge(const uint64_t *__restric
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint32_t l0 = blockIdx.x * per_block;
     const uint32_t l1 = l0 + per_block < nlists ? l0 + per_block : nlists;
-    uint64_t top = 0;
-    for (uint32_t l = l0 + w; l < l1; l += nw)
-        top = wave_merge_desc(top, in[(size_t)l * 64 + lane]);
+    uint64_t top = 0, thr = 0;
+    // a list whose largest key is not above the 64th of the running list
+    // cannot contribute
+    for (uint32_t l = l0 + w; l < l1; l += nw) {
+        const uint64_t v = in[(size_t)l * 64 + lane];
+        if (shfl_u64(v, 0) <= thr)
+            continue;
+        top = wave_merge_desc(top, v);
+        thr = shfl_u64(top, 63);
+    }
     wl[w][lane] = top;
-    __syncthreads();
+    // the 16 wave lists: pairwise merge levels in LDS
+    for (int h = nw >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (w < h)
+            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
+        __syncthreads();
+        if (w < h)
+            wl[w][lane] = top;
+    }
     if (w != 0)
         return;
-    for (int q = 1; q < nw; ++q)
-        top = wave_merge_desc(top, wl[q][lane]);
     if (shooters) {
         bool ok = (uint32_t)lane < k && (top >> 32) != 0;
         if ((uint32_t)lane < k)
@@ -1866,7 +1993,7 @@ void launch_select_topk(uint32_t rule, const float *unshot, const float *area, c
             blocks = (uint32_t)(topk_cand_half(npatch, k) / 256);
         uint64_t *a = cand, *b = cand + topk_cand_half(npatch, k);
         k_topk64_scan<<<blocks, 256, 0, st>>>(unshot, area, npatch, a);
-        uint32_t nl = blocks * 4;
+        uint32_t nl = blocks;
         while (nl > 256) {
             uint32_t per = 64, nb = (nl + per - 1) / per;
             k_topk64_merge<<<nb, 1024, 0, st>>>(a, nl, per, k, b, nullptr, nullptr);
