@@ -1213,7 +1213,11 @@ __device__ static inline void draw_wave(const RasterRec &r, const Edges &e, int3
 // snapped vertices span less than 64 pixels every edge value inside the bbox
 // fits in int32, so the edge functions are stepped with 32-bit adds; the
 // values equal the 64-bit evaluation exactly.  Depth is the same
-// fmaf(dzdx, px-x0, fmaf(dzdy, py-y0, z0)) with the inner term per row.
+// fmaf(dzdx, px-x0, fmaf(dzdy, py-y0, z0)) with the inner term per row;
+// (float)(px - x0) is stepped by 256.0f, exact because every value is an
+// integer below 2^24.  Every bbox pixel issues its ds_min_u64 (a pixel
+// outside the triangle or the depth range with the neutral key ~0), which
+// ran faster than branching around the atomic.
 __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t vpy,
                                         int32_t ax0, int32_t ax1, int32_t ay0, int32_t ay1,
                                         int32_t ox, int32_t oy, uint32_t tshift, uint64_t *tk)
@@ -1240,22 +1244,21 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
         dB[k] = B * kSub;
     }
     int32_t py = py0;
-    for (int32_t ty = ay0; ty <= ay1; ++ty, py += kSub) {
+    const uint32_t w = (uint32_t)(ax1 - ax0 + 1), rstride = 1u << tshift;
+    const float fx0 = (float)(px0 - r.x[0]);
+    uint32_t prow = (uint32_t)((ay0 - oy) << tshift) + (uint32_t)(ax0 - ox);
+    for (int32_t ty = ay0; ty <= ay1; ++ty, py += kSub, prow += rstride) {
         float zr = fmaf(r.dzdy, (float)(py - r.y[0]), r.z0);
         int32_t e0 = v[0], e1 = v[1], e2 = v[2];
-        int32_t px = px0;
-        uint32_t prow = (uint32_t)((ty - oy) << tshift) - (uint32_t)ox;
-        for (int32_t tx = ax0; tx <= ax1; ++tx, px += kSub) {
-            if ((e0 | e1 | e2) >= 0) {
-                float z = fmaf(r.dzdx, (float)(px - r.x[0]), zr);
-                if (!(z > 0.0f))
-                    z = 0.0f;
-                if (z < 1.0f) {
-                    uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | r.id;
-                    uint32_t pp = prow + (uint32_t)tx;
-                    atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
-                }
-            }
+        float fx = fx0;
+        unsigned long long *p = (unsigned long long *)(tk + prow), *pend = p + w;
+        for (; p != pend; ++p, fx += 256.0f) {
+            float z = fmaf(r.dzdx, fx, zr);
+            if (!(z > 0.0f))
+                z = 0.0f;
+            const bool ok = (e0 | e1 | e2) >= 0 && z < 1.0f;
+            atomicMin(p, ok ? (unsigned long long)(((uint64_t)__float_as_uint(z) << 32) | r.id)
+                            : ~0ull);
             e0 += dA[0];
             e1 += dA[1];
             e2 += dA[2];
