@@ -110,6 +110,39 @@ def test_itembuf_tuning_knobs_invariant(rad, boxes_small):
     assert np.array_equal(outs[0][:len(it_o)], it_o)
 
 
+def test_many_bins_itembuf_and_fused_shoot(rad, boxes_small):
+    # K=64 shooters with 8-pixel tiles: 64 x 192 = 12,288 (shooter, tile)
+    # bins, so the bin scan runs over more than one LDS chunk, the bin fill
+    # walks full groups of 8 shooter slots, and the clipper sees the near
+    # plane of many shooters.  Item buffers bit-exact; the fused pass's
+    # shooting update against the oracle's F rows.
+    say("many bins: W=64 K=64 tile 8")
+    W, K = 64, 64
+    sc = boxes_small
+    rng = np.random.default_rng(7)
+    sh = [int(x) for x in rng.choice(sc.npatch, K, replace=False)]
+    s = rad.Solver(hemicube=W, hemicubes=K, tile_px=8, small_tri_px=16)
+    s.upload(sc)
+    item = s.render(sh)
+    res = s.tex_res
+    for k, p in enumerate(sh):
+        it_o = O.render_hemicube(W, sc.tri, sc.tri_patch, sc.frame[p])
+        assert np.array_equal(item[k * res:(k + 1) * res], it_o), (k, p)
+    u0 = rng.random((sc.npatch, 3)).astype(np.float32)
+    s.set_state(u0, np.zeros_like(u0), sc.emission)
+    u0, _ = s.state()
+    s.shoot(sh)
+    u1, _ = s.state()
+    _, ff_o = O.histogram(W, K, item, O.dff_cross(W), sc.npatch)
+    snap = u0[sh].astype(np.float64)
+    d = np.zeros_like(u0, dtype=np.float64)
+    for k, i in enumerate(sh):
+        d += sc.refl * snap[k] * ff_o[k][:, None] * (sc.area[i] / sc.area)[:, None]
+    d[sh] -= snap
+    want = u0.astype(np.float64) + d
+    assert _close(u1, want, float(np.abs(want).max()))
+
+
 def test_fused_reduce_matches_item_buffer_reduce(rad, cornell_t):
     # the shooting pass reduces tiles in the raster kernel; its F rows must
     # equal the reduction of the written item buffer
@@ -238,6 +271,27 @@ def test_power_topk_with_ties(rad, boxes_small, K):
     want = [int(i) for i in order[:K] if key[i] > 0]
     assert n == len(want) and list(ids[:n]) == want
     assert np.all(ids[n:] == 0)
+
+
+def test_power_topk_large_scene(rad):
+    """More than 262,144 patches: the scan leaves more than 256 workgroup
+    lists, so the top-64 path runs an intermediate merge level before the
+    final one.  Same lexicographic reference as above."""
+    sc = rad.Scene.boxes(nboxes=60, room=(12.0, 4.0, 12.0), target_tris=300000, nlights=4, seed=5)
+    assert sc.npatch > 262144
+    for K in (7, 64):
+        say(f"top-{K} of {sc.npatch} patches")
+        s = _solver(rad, sc, 16, K=K)
+        rng = np.random.default_rng(100 + K)
+        u = rng.integers(0, 4, size=(sc.npatch, 3)).astype(np.float32)
+        u[rng.random(sc.npatch) < 0.5] = 0
+        s.set_state(u, np.zeros_like(u), u)
+        ids, n = s.select(K)
+        key = sc.area * u.sum(1, dtype=np.float32)
+        order = np.lexsort((np.arange(sc.npatch), -key))
+        want = [int(i) for i in order[:K] if key[i] > 0]
+        assert n == len(want) and list(ids[:n]) == want
+        del s
 
 
 def test_reference_rule_batched_matches_list(rad, cornell_q):
