@@ -179,6 +179,18 @@ __global__ void k_dense_to_f32(const uint16_t *__restrict__ fhi, const uint16_t 
 
 // ------------------------------------------------------------------ GEMM
 
+// One 16-byte A fragment load: non-temporal when the product has several
+// column tiles (the matrix is read once, X should stay in L2).
+template <int RT> __device__ __forceinline__ uint4 gemm_aload(const uint16_t *p)
+{
+    if constexpr (RT > 1) {
+        u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
+        return uint4{v.x, v.y, v.z, v.w};
+    } else {
+        return *(const uint4 *)p;
+    }
+}
+
 // YP[s] = F[:, kspan slice s] X[slice s]  for 32 rows per wave.
 // Wave w owns rows i0 .. i0+31 (two 16-row A fragments) and all RT column
 // tiles of 16.  Per 64-wide k chunk the X slice (RT*16 rows x 64 bf16) is
@@ -200,6 +212,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     constexpr int UNITS = RT * 16 * 8;     // 16-byte units of one X chunk
     constexpr int NT = WAVES * 64;
     constexpr int UPT = (UNITS + NT - 1) / NT;
+    // every thread stages UPT units when NT divides UNITS: no guard, so the
+    // staging registers are not demoted to scratch memory
+    constexpr bool FULL = UNITS % NT == 0;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NB][RT * 16][LROW];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -221,54 +236,59 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     const size_t arow0 = (size_t)(i0 + m) * npad + 8 * kg;
     const size_t arow1 = arow0 + (size_t)16 * npad;
 
-    uint4 xr[NB][UPT];
+    static_assert(UPT <= 3, "three staging units per thread at most");
+    // staging registers of the X chunk units tid, tid + NT, tid + 2 NT (hi,
+    // lo): named variables, which stay in VGPRs where an array was kept in
+    // scratch memory
+    uint4 xh0, xh1, xh2, xl0, xl1, xl2;
     uint4 ar[NB][2][2];
 
-    auto load_x = [&](uint32_t kc) {
-#pragma unroll
-        for (int u = 0; u < UPT; ++u) {
-            int idx = tid + u * NT;
-            if (idx < UNITS) {
-                size_t o = (size_t)(idx >> 3) * npad + kc + (idx & 7) * 8;
-                xr[0][u] = *(const uint4 *)(xhi + o);
-                if (SPLIT)
-                    xr[NB - 1][u] = *(const uint4 *)(xlo + o);
-            }
-        }
-    };
-    // the matrix is streamed once per product: with several column tiles
-    // non-temporal loads keep it from evicting X out of L2
-    auto ntload = [](const uint16_t *p) {
-        if constexpr (RT > 1) {
-            u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
-            return uint4{v.x, v.y, v.z, v.w};
-        } else {
-            return *(const uint4 *)p;
-        }
-    };
-    auto load_a = [&](uint32_t kc) {
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            ar[0][0][ks] = ntload(fhi + arow0 + kc + ks * 32);
-            ar[0][1][ks] = ntload(fhi + arow1 + kc + ks * 32);
-            if (SPLIT) {
-                ar[NB - 1][0][ks] = ntload(flo + arow0 + kc + ks * 32);
-                ar[NB - 1][1][ks] = ntload(flo + arow1 + kc + ks * 32);
-            }
-        }
-    };
+    // X chunk kc into registers (staged to LDS at the top of the next
+    // iteration) and the A fragments of chunk kc.  The matrix is streamed once per product: with several column tiles
+    // non-temporal loads keep it from evicting X out of L2.
+#define RAD_GEMM_XUNIT(u_, h_, l_, kc_)                                                \
+    if (FULL || tid + (u_) * NT < UNITS) {                                             \
+        const int idx = tid + (u_) * NT;                                               \
+        const size_t o = (size_t)(idx >> 3) * npad + (kc_) + (idx & 7) * 8;            \
+        h_ = *(const uint4 *)(xhi + o);                                                \
+        if (SPLIT)                                                                     \
+            l_ = *(const uint4 *)(xlo + o);                                            \
+    }
+#define RAD_GEMM_LOAD_X(kc_)                                                           \
+    {                                                                                  \
+        RAD_GEMM_XUNIT(0, xh0, xl0, kc_)                                               \
+        if constexpr (UPT > 1) {                                                       \
+            RAD_GEMM_XUNIT(1, xh1, xl1, kc_)                                           \
+        }                                                                              \
+        if constexpr (UPT > 2) {                                                       \
+            RAD_GEMM_XUNIT(2, xh2, xl2, kc_)                                           \
+        }                                                                              \
+    }
+#define RAD_GEMM_LOAD_A(kc_)                                                           \
+    _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                                   \
+    {                                                                                  \
+        ar[0][0][ks] = gemm_aload<RT>(fhi + arow0 + (kc_) + ks * 32);                  \
+        ar[0][1][ks] = gemm_aload<RT>(fhi + arow1 + (kc_) + ks * 32);                  \
+        if (SPLIT) {                                                                   \
+            ar[NB - 1][0][ks] = gemm_aload<RT>(flo + arow0 + (kc_) + ks * 32);         \
+            ar[NB - 1][1][ks] = gemm_aload<RT>(flo + arow1 + (kc_) + ks * 32);         \
+        }                                                                              \
+    }
 
-    load_x(kbeg);
-    load_a(kbeg);
+    RAD_GEMM_LOAD_X(kbeg)
+    RAD_GEMM_LOAD_A(kbeg)
     for (uint32_t kc = kbeg; kc < kend; kc += 64) {
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < UPT; ++u) {
             int idx = tid + u * NT;
-            if (idx < UNITS) {
-#pragma unroll
-                for (int b = 0; b < NB; ++b)
-                    *(uint4 *)&lds[b][idx >> 3][(idx & 7) * 8] = xr[b][u];
+            if (FULL || idx < UNITS) {
+                const uint4 h = u == 0 ? xh0 : u == 1 ? xh1 : xh2;
+                *(uint4 *)&lds[0][idx >> 3][(idx & 7) * 8] = h;
+                if (SPLIT) {
+                    const uint4 l = u == 0 ? xl0 : u == 1 ? xl1 : xl2;
+                    *(uint4 *)&lds[NB - 1][idx >> 3][(idx & 7) * 8] = l;
+                }
             }
         }
         __syncthreads();
@@ -281,8 +301,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
                 for (int ks = 0; ks < 2; ++ks)
                     ac[b][f][ks] = ar[b][f][ks];
         if (kc + 64 < kend) {
-            load_x(kc + 64);
-            load_a(kc + 64);
+            RAD_GEMM_LOAD_X(kc + 64)
+            RAD_GEMM_LOAD_A(kc + 64)
         }
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
@@ -311,6 +331,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
         for (int t = 0; t < RT; ++t)
             *(f32x4 *)(out + (size_t)(t * 16 + m) * npad + i0 + f * 16 + 4 * kg) = acc[f][t];
 }
+#undef RAD_GEMM_LOAD_X
+#undef RAD_GEMM_XUNIT
+#undef RAD_GEMM_LOAD_A
 
 // ------------------------------------------------------- vector updates
 
