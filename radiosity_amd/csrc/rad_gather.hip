@@ -215,7 +215,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     // every thread stages UPT units when NT divides UNITS: no guard, so the
     // staging registers are not demoted to scratch memory
     constexpr bool FULL = UNITS % NT == 0;
-    __shared__ __attribute__((aligned(16))) uint16_t lds[NB][RT * 16][LROW];
+    // two X buffers when they fit (one product half): chunk c + 1 is staged
+    // while other waves still read chunk c, one barrier per chunk
+    constexpr int DB = NB == 1 ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[DB][NB][RT * 16][LROW];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, kg = lane >> 4;
@@ -277,17 +280,22 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
 
     RAD_GEMM_LOAD_X(kbeg)
     RAD_GEMM_LOAD_A(kbeg)
-    for (uint32_t kc = kbeg; kc < kend; kc += 64) {
-        __syncthreads();
+    for (uint32_t kc = kbeg, ci = 0; kc < kend; kc += 64, ++ci) {
+        const int cb = DB == 2 ? (int)(ci & 1u) : 0;
+        // one buffer: every wave must be done with the previous chunk; two:
+        // the buffer written now was last read two chunks ago, before the
+        // barrier of the previous chunk
+        if (DB == 1)
+            __syncthreads();
 #pragma unroll
         for (int u = 0; u < UPT; ++u) {
             int idx = tid + u * NT;
             if (FULL || idx < UNITS) {
                 const uint4 h = u == 0 ? xh0 : u == 1 ? xh1 : xh2;
-                *(uint4 *)&lds[0][idx >> 3][(idx & 7) * 8] = h;
+                *(uint4 *)&lds[cb][0][idx >> 3][(idx & 7) * 8] = h;
                 if (SPLIT) {
                     const uint4 l = u == 0 ? xl0 : u == 1 ? xl1 : xl2;
-                    *(uint4 *)&lds[NB - 1][idx >> 3][(idx & 7) * 8] = l;
+                    *(uint4 *)&lds[cb][NB - 1][idx >> 3][(idx & 7) * 8] = l;
                 }
             }
         }
@@ -308,13 +316,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
         for (int ks = 0; ks < 2; ++ks) {
 #pragma unroll
             for (int t = 0; t < RT; ++t) {
-                bf16x8 bh = *(const bf16x8 *)&lds[0][t * 16 + m][ks * 32 + 8 * kg];
+                bf16x8 bh = *(const bf16x8 *)&lds[cb][0][t * 16 + m][ks * 32 + 8 * kg];
 #pragma unroll
                 for (int f = 0; f < 2; ++f) {
                     bf16x8 ah = __builtin_bit_cast(bf16x8, ac[0][f][ks]);
                     acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[f][t], 0, 0, 0);
                     if (SPLIT) {
-                        bf16x8 bl = *(const bf16x8 *)&lds[NB - 1][t * 16 + m][ks * 32 + 8 * kg];
+                        bf16x8 bl = *(const bf16x8 *)&lds[cb][NB - 1][t * 16 + m][ks * 32 + 8 * kg];
                         bf16x8 al = __builtin_bit_cast(bf16x8, ac[NB - 1][f][ks]);
                         acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[f][t], 0, 0, 0);
                         acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[f][t], 0, 0, 0);
