@@ -1433,7 +1433,7 @@ void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const Bi
 // the table is then flushed with one global atomic per distinct id (the host
 // accumulation of Main.cpp:1257-1269 fused in).  A run that finds no free
 // slot adds to the global entry directly.
-constexpr uint32_t kReduceChunk = 4096, kReduceSlots = 2048;
+constexpr uint32_t kReduceChunk = 4096, kReduceSlots = 2048, kReduceBatch = 8;
 
 __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__restrict__ nshoot,
                                                 const uint64_t *__restrict__ keys,
@@ -1456,38 +1456,44 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
     __syncthreads();
     const uint64_t *kp = keys + (size_t)hi * RES;
     const uint32_t c1 = min(RES, c0 + kReduceChunk);
-    for (uint32_t base = c0; base < c1; base += blockDim.x) {
-        const uint32_t t = base + threadIdx.x;
-        uint32_t id = 0, c = 0;
-        float f = 0.0f;
-        if (t < c1) {
-            id = (uint32_t)kp[t];
-            if (id) {
-                f = dff[t];
-                c = 1;
-            }
+    // kReduceBatch texels per lane are loaded together (keys and weights,
+    // independent of each other), then reduced one after the other
+    for (uint32_t base = c0; base < c1; base += blockDim.x * kReduceBatch) {
+        uint32_t ids[kReduceBatch];
+        float fs[kReduceBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kReduceBatch; ++b) {
+            const uint32_t t = base + b * blockDim.x + threadIdx.x;
+            ids[b] = t < c1 ? (uint32_t)kp[t] : 0u;
+            fs[b] = t < c1 ? dff[t] : 0.0f;
         }
-        if (wave_runs(id, f, c, true)) {
-            uint32_t h = (id * 2654435761u) >> 21; // kReduceSlots = 2^11
-            // Main.cpp:1259-1262: an unknown patch id is counted (once per
-            // run) and skipped
-            bool done = id - 1 >= npatch;
-            if (done)
-                atomicAdd(&counters[kCntUnknown], 1u);
-            for (int probe = 0; probe < 16 && !done; ++probe, h = (h + 1) & (kReduceSlots - 1)) {
-                uint32_t k = hkey[h];
-                if (k == 0u)
-                    k = atomicCAS(&hkey[h], 0u, id);
-                if (k == 0u || k == id) {
-                    atomicAdd(&hsum[h], f);
-                    atomicAdd(&hcnt[h], c);
-                    done = true;
+#pragma unroll
+        for (uint32_t b = 0; b < kReduceBatch; ++b) {
+            const uint32_t id = ids[b];
+            uint32_t c = id ? 1u : 0u;
+            float f = id ? fs[b] : 0.0f;
+            if (wave_runs(id, f, c, true)) {
+                uint32_t h = (id * 2654435761u) >> 21; // kReduceSlots = 2^11
+                // Main.cpp:1259-1262: an unknown patch id is counted (once per
+                // run) and skipped
+                bool done = id - 1 >= npatch;
+                if (done)
+                    atomicAdd(&counters[kCntUnknown], 1u);
+                for (int probe = 0; probe < 16 && !done; ++probe, h = (h + 1) & (kReduceSlots - 1)) {
+                    uint32_t k = hkey[h];
+                    if (k == 0u)
+                        k = atomicCAS(&hkey[h], 0u, id);
+                    if (k == 0u || k == id) {
+                        atomicAdd(&hsum[h], f);
+                        atomicAdd(&hcnt[h], c);
+                        done = true;
+                    }
                 }
-            }
-            if (!done) { // no free slot: straight to the global entry
-                atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], f);
-                if (hits)
-                    atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], c);
+                if (!done) { // no free slot: straight to the global entry
+                    atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], f);
+                    if (hits)
+                        atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], c);
+                }
             }
         }
     }
