@@ -1908,14 +1908,16 @@ __device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_topk64_scan(const float *__restrict__ unshot,
+constexpr uint32_t kTopkScanWaves = 8;
+
+__global__ __launch_bounds__(kTopkScanWaves * 64) void k_topk64_scan(const float *__restrict__ unshot,
                                                      const float *__restrict__ area,
                                                      uint32_t npatch, uint64_t *__restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t nw = gridDim.x * (blockDim.x >> 6);
-    __shared__ uint64_t wl[4][64];
+    __shared__ uint64_t wl[kTopkScanWaves][64];
     uint64_t top = 0, thr = 0; // thr: the list's 64th key
     for (uint32_t base = gw * 64; base < npatch; base += nw * 64) {
         uint32_t j = base + lane;
@@ -1925,15 +1927,19 @@ __global__ __launch_bounds__(256) void k_topk64_scan(const float *__restrict__ u
         top = wave_merge_desc(top, wave_sort_desc(key));
         thr = shfl_u64(top, 63);
     }
-    // the workgroup's four lists: two pairwise merge levels in LDS
+    // the workgroup's wave lists: pairwise merge levels in LDS
     const int w = threadIdx.x >> 6;
     wl[w][lane] = top;
-    __syncthreads();
-    if (w < 2)
-        wl[w][lane] = wave_merge_desc(top, wl[w + 2][lane]);
-    __syncthreads();
+    for (int h = (int)kTopkScanWaves / 2; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (w < h)
+            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
+        __syncthreads();
+        if (w < h)
+            wl[w][lane] = top;
+    }
     if (w == 0)
-        out[(size_t)blockIdx.x * 64 + lane] = wave_merge_desc(wl[0][lane], wl[1][lane]);
+        out[(size_t)blockIdx.x * 64 + lane] = top;
 }
 
 // Merges nlists sorted lists of 64 (in[l * 64 + lane]) per workgroup; the
@@ -1995,13 +2001,13 @@ void launch_select_topk(uint32_t rule, const float *unshot, const float *area, c
     if (k <= 64) {
         // wave lists: about four batches of 64 patches per wave
         uint32_t waves = (npatch + 255) / 256;
-        if (waves < 4)
-            waves = 4;
-        uint32_t blocks = (waves + 3) / 4;
-        if ((size_t)blocks * 4 * 64 > topk_cand_half(npatch, k))
-            blocks = (uint32_t)(topk_cand_half(npatch, k) / 256);
+        if (waves < kTopkScanWaves)
+            waves = kTopkScanWaves;
+        uint32_t blocks = (waves + kTopkScanWaves - 1) / kTopkScanWaves;
+        if ((size_t)blocks * 64 > topk_cand_half(npatch, k))
+            blocks = (uint32_t)(topk_cand_half(npatch, k) / 64);
         uint64_t *a = cand, *b = cand + topk_cand_half(npatch, k);
-        k_topk64_scan<<<blocks, 256, 0, st>>>(unshot, area, npatch, a);
+        k_topk64_scan<<<blocks, kTopkScanWaves * 64, 0, st>>>(unshot, area, npatch, a);
         uint32_t nl = blocks;
         while (nl > 256) {
             uint32_t per = 64, nb = (nl + per - 1) / per;

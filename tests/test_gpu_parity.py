@@ -274,11 +274,12 @@ def test_power_topk_with_ties(rad, boxes_small, K):
 
 
 def test_power_topk_large_scene(rad):
-    """More than 262,144 patches: the scan leaves more than 256 workgroup
-    lists, so the top-64 path runs an intermediate merge level before the
-    final one.  Same lexicographic reference as above."""
-    sc = rad.Scene.boxes(nboxes=60, room=(12.0, 4.0, 12.0), target_tris=300000, nlights=4, seed=5)
-    assert sc.npatch > 262144
+    """More than 524,288 patches: the scan (8 wavefronts of 256 patches per
+    workgroup) leaves more than 256 workgroup lists, so the top-64 path runs
+    an intermediate merge level before the final one.  Same lexicographic
+    reference as above."""
+    sc = rad.Scene.boxes(nboxes=60, room=(12.0, 4.0, 12.0), target_tris=600000, nlights=4, seed=5)
+    assert sc.npatch > 524288
     for K in (7, 64):
         say(f"top-{K} of {sc.npatch} patches")
         s = _solver(rad, sc, 16, K=K)
