@@ -41,6 +41,10 @@ struct rad_ctx {
     float *shot = nullptr;         // npatch*3
     float *ff = nullptr;           // K*npatch
     uint32_t *hits = nullptr;      // K*npatch
+    // every F row entry is zero at this point of the stream (set by a clear
+    // or by the shoot kernels, which zero what they read; false after any
+    // kernel that accumulates into F); lets a pass skip clearing the rows
+    bool ff_zero = false;
 
     // per pass
     float *dff = nullptr;          // RES
@@ -177,8 +181,10 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t 
 {
     launch_prepare(c->g, k, shooters ? shooters : c->shooters, c->nshoot,
                    frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->st);
-    launch_clear(k, c->bb.count, c->bb.ntiles, c->ff, hits, c->rowlen, c->counters,
-                 c->bb.ent_cap, c->st);
+    const bool keep_ff = c->ff_zero && !hits;
+    launch_clear(k, c->bb.count, c->bb.ntiles, keep_ff ? nullptr : c->ff, hits, c->rowlen,
+                 c->counters, c->bb.ent_cap, c->st);
+    c->ff_zero = true;
     record(c, evbase + 2);
     launch_raster_setup(c->g, k, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
                         c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->st);
@@ -187,6 +193,8 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t 
     record(c, evbase + 4);
     launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, keys, c->dff, c->rowlen, c->pk.slot_pbeg,
                 keys ? nullptr : c->ff, hits, c->counters, c->st);
+    if (!keys)
+        c->ff_zero = false;
     record(c, evbase + 5);
     return check_launch(c, "render kernels");
 }
@@ -249,6 +257,7 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
     else
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl,
                      c->area, c->npatch, c->unshot, c->shot, c->st);
+    c->ff_zero = true; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
     c->rendered = false;
     return check_launch(c, "shoot");
@@ -510,6 +519,7 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
         (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
         return rc;
     HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
+    c->ff_zero = true;
     HIPCHK(c, hipMemset(c->hits, 0, K * rowlen * 4));
     c->ntri = ntri;
     c->npatch = npatch;
@@ -738,6 +748,7 @@ static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
     HIPCHK(c, hipMemsetAsync(c->hits, 0, nf * 4, c->st));
     HIPCHK(c, hipMemsetAsync(c->counters, 0, kCntN * 4, c->st));
     launch_reduce(c->g, k, c->nshoot, c->keys, c->dff, c->npatch, c->ff, c->hits, c->counters, c->st);
+    c->ff_zero = false;
     int rc;
     if ((rc = check_launch(c, "reduce")))
         return rc;
@@ -846,6 +857,16 @@ static int step_impl(rad_ctx *c, uint32_t passes)
         return rc;
     uint32_t G = c->cfg.graph_passes;
     if (!c->timing && G && passes >= G) {
+        // captured passes assume zero F rows at their start (the first one
+        // skips clearing them); every replay ends with a shoot, which leaves
+        // them zero again
+        if (!c->ff_zero) {
+            launch_clear(c->K, c->bb.count, c->bb.ntiles, c->ff, nullptr, c->rowlen, c->counters,
+                         c->bb.ent_cap, c->st);
+            if ((rc = check_launch(c, "clear")))
+                return rc;
+            c->ff_zero = true;
+        }
         if (!c->gexec || c->gpasses != G)
             if ((rc = capture_graph(c, G)))
                 return rc;

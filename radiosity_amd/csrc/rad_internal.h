@@ -133,7 +133,7 @@ void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot,
                    const uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
                    uint32_t *hits, uint32_t *counters, hipStream_t st);
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
-                  const uint32_t *nshoot, const float *snap, const float *ff,
+                  const uint32_t *nshoot, const float *snap, float *ff,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
                   float *shot, hipStream_t st);
 void launch_ids_to_keys(const uint32_t *ids, uint64_t *keys, size_t n, hipStream_t st);
@@ -146,7 +146,7 @@ void launch_residual(const float *unshot, const float *area, const float *emis,
 // Launchers implemented in rad_pack.hip.
 void launch_pack_select(const PackBuffers &pk, const float *unshot, const float *area,
                         hipStream_t st);
-void launch_pack_shoot(const PackBuffers &pk, uint32_t mode, const float *snap, const float *ff,
+void launch_pack_shoot(const PackBuffers &pk, uint32_t mode, const float *snap, float *ff,
                        uint32_t rowlen, const float *refl, const float *area, uint32_t npatch,
                        float *unshot, float *shot, hipStream_t st);
 

@@ -248,7 +248,8 @@ __global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *_
     for (size_t k = i; k < nbins; k += stride)
         bin_count[k] = 0u;
     for (size_t k = i; k < nff; k += stride) {
-        ff[k] = 0.0f;
+        if (ff)
+            ff[k] = 0.0f;
         if (hits)
             hits[k] = 0u;
     }
@@ -257,7 +258,8 @@ __global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *_
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
                   uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st)
 {
-    size_t nbins = (size_t)K * ntiles, nff = (size_t)K * npatch;
+    // ff null: the F rows are known to be zero (the last shoot cleared them)
+    size_t nbins = (size_t)K * ntiles, nff = ff || hits ? (size_t)K * npatch : 0;
     size_t n = nbins > nff ? nbins : nff;
     size_t blocks = (n + 255) / 256;
     if (blocks > 8192)
@@ -1519,11 +1521,12 @@ void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot, const 
 // ------------------------------------------------------------------ shooting
 // Main.cpp:1251-1295 for all receivers at once: one thread per receiver j
 // accumulates the contributions of the pass's shooters, then the shooters
-// move their snapshot from unshot to shot.
+// move their snapshot from unshot to shot.  Every nonzero F entry it reads
+// is set back to zero, so the next pass finds the F rows cleared.
 __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__restrict__ shooters,
                                                const uint32_t *__restrict__ nshoot,
                                                const float *__restrict__ snap,
-                                               const float *__restrict__ ff,
+                                               float *__restrict__ ff,
                                                const float *__restrict__ refl,
                                                const float *__restrict__ area, uint32_t npatch,
                                                float *__restrict__ unshot,
@@ -1568,6 +1571,8 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
             const float f = fb[q];
             if (f == 0.0f)
                 continue;
+            // leave the F rows zero for the next pass (its clear skips them)
+            ff[(size_t)s * npatch + j] = 0.0f;
             float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
             if (mode == RAD_MODE_REFERENCE) {
                 // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
@@ -1600,7 +1605,7 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
 }
 
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
-                  const float *snap, const float *ff, const float *refl, const float *area,
+                  const float *snap, float *ff, const float *refl, const float *area,
                   uint32_t npatch, float *unshot, float *shot, hipStream_t st)
 {
     (void)K;

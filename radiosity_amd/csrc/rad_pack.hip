@@ -137,10 +137,11 @@ __global__ __launch_bounds__(1024) void k_pack_slots(
 }
 
 // k_shoot for a pack: receiver j takes the slots of its own scene, F row
-// entry j - first patch of the scene (rows are rowlen long).
+// entry j - first patch of the scene (rows are rowlen long).  Like k_shoot
+// it sets every nonzero F entry it reads back to zero.
 __global__ __launch_bounds__(256) void k_pack_shoot(
     uint32_t mode, const uint32_t *__restrict__ shooters, const uint32_t *__restrict__ nshoot,
-    const float *__restrict__ snap, const float *__restrict__ ff, uint32_t rowlen,
+    const float *__restrict__ snap, float *__restrict__ ff, uint32_t rowlen,
     const uint32_t *__restrict__ patch_scene, const uint32_t *__restrict__ sc_pbeg,
     const uint32_t *__restrict__ sc_slot_beg, const uint32_t *__restrict__ sc_nsel,
     const float *__restrict__ refl, const float *__restrict__ area, uint32_t npatch,
@@ -162,6 +163,7 @@ __global__ __launch_bounds__(256) void k_pack_shoot(
         float f = ff[(size_t)s * rowlen + jl];
         if (f == 0.0f)
             continue;
+        ff[(size_t)s * rowlen + jl] = 0.0f; // F rows left zero for the next pass
         uint32_t i = shooters[s];
         float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
         if (mode == RAD_MODE_REFERENCE) {
@@ -202,7 +204,7 @@ void launch_pack_select(const PackBuffers &pk, const float *unshot, const float 
                                      pk.slot_tcnt, pk.slot_pbeg, pk.nshoot, pk.pass);
 }
 
-void launch_pack_shoot(const PackBuffers &pk, uint32_t mode, const float *snap, const float *ff,
+void launch_pack_shoot(const PackBuffers &pk, uint32_t mode, const float *snap, float *ff,
                        uint32_t rowlen, const float *refl, const float *area, uint32_t npatch,
                        float *unshot, float *shot, hipStream_t st)
 {

@@ -360,6 +360,35 @@ def test_itembuf_dense_bins_bit_exact(rad):
     assert _close(u1, want, float(np.abs(want).max()))
 
 
+def test_f_rows_cleared_after_other_writers(rad, cornell_t):
+    """The shoot kernels leave the F rows zero and the next pass skips
+    clearing them; a kernel that accumulates into F without a shoot after it
+    (the standalone reduction, an autotune) must make the next pass clear
+    again, eager or replayed.  Passes interrupted that way end where
+    uninterrupted ones do (1e-4 relative, float atomics)."""
+    ref = rad.Solver(hemicube=32, hemicubes=4)
+    ref.upload(cornell_t)
+    ref.step(12)
+    ref.finish()
+    ur, sr = ref.state()
+    a = cornell_t.area[:, None].astype(np.float64)
+    tot_r = (ur + sr).astype(np.float64)
+    for graph in (0, 4):
+        say(f"F rows dirtied between passes, graph_passes={graph}")
+        s = rad.Solver(hemicube=32, hemicubes=4, graph_passes=graph)
+        s.upload(cornell_t)
+        s.step(4)
+        s.ff_rows([0, 5, 9, 100])          # standalone reduction into F
+        s.step(4)
+        s.autotune(passes=1)               # render passes without a shoot
+        s.step(4)
+        s.finish()
+        u, sh = s.state()
+        tot = (u + sh).astype(np.float64)
+        rel = np.abs(a * (tot - tot_r)).sum() / np.abs(a * tot_r).sum()
+        assert rel < 1e-4, (graph, rel)
+
+
 def test_graph_replay_matches_eager(rad, cornell_t):
     """rad_step replaying captured passes (graph_passes=4) tracks plain
     launches: every replay is checked after host work and a synchronisation
