@@ -179,11 +179,11 @@ void record(rad_ctx *c, size_t idx)
 int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits,
                 const uint32_t *shooters = nullptr, const float *frames = nullptr)
 {
-    launch_prepare(c->g, k, shooters ? shooters : c->shooters, c->nshoot,
-                   frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->st);
     const bool keep_ff = c->ff_zero && !hits;
-    launch_clear(k, c->bb.count, c->bb.ntiles, keep_ff ? nullptr : c->ff, hits, c->rowlen,
-                 c->counters, c->bb.ent_cap, c->st);
+    launch_prepare_clear(k, shooters ? shooters : c->shooters, c->nshoot,
+                         frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->bb.count,
+                         c->bb.ntiles, keep_ff ? nullptr : c->ff, hits, c->rowlen, c->counters,
+                         c->bb.ent_cap, c->st);
     c->ff_zero = true;
     record(c, evbase + 2);
     launch_raster_setup(c->g, k, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,

@@ -113,6 +113,10 @@ void launch_select_reference(const float *unshot, uint32_t npatch, uint32_t k,
 void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
                     const uint32_t *nshoot, const float *frames, const float *unshot,
                     float *snap, float *mvp, hipStream_t st);
+void launch_prepare_clear(uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
+                          const float *frames, const float *unshot, float *snap, float *mvp,
+                          uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
+                          uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st);
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
                   uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st);
 // Binned render of the nshoot (<= K) hemicubes in three launches:

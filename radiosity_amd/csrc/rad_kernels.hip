@@ -174,12 +174,12 @@ __device__ static void face_mvp(const float *fr, int face, float *mvp)
 // unit up u (made orthogonal to n) and n x u; then a flag (1 when the frame
 // is degenerate or its up is not orthogonal to the normal within 2e-5, which
 // turns the conservative culls off for that shooter).
-__global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
-                          const uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
-                          const float *__restrict__ unshot, float *__restrict__ snap,
-                          float *__restrict__ mvp)
+__device__ static void prepare_item(uint32_t t, uint32_t K, const uint32_t *__restrict__ shooters,
+                                    const uint32_t *__restrict__ nshoot,
+                                    const float *__restrict__ frames,
+                                    const float *__restrict__ unshot, float *__restrict__ snap,
+                                    float *__restrict__ mvp)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= K * 5)
         return;
     uint32_t s = t / 5, face = t % 5;
@@ -217,6 +217,15 @@ __global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
         out[face * 16 + k] = m[k];
 }
 
+__global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
+                          const uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
+                          const float *__restrict__ unshot, float *__restrict__ snap,
+                          float *__restrict__ mvp)
+{
+    prepare_item(blockIdx.x * blockDim.x + threadIdx.x, K, shooters, nshoot, frames, unshot, snap,
+                 mvp);
+}
+
 void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
                     const uint32_t *nshoot, const float *frames, const float *unshot,
                     float *snap, float *mvp, hipStream_t st)
@@ -231,12 +240,11 @@ void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
 // reset.  A kernel, not hipMemsetAsync, so a pass can be captured in a graph.
 // The previous pass's overflow state is folded into the sticky counter,
 // which only the host clears, so a multi-pass call reports any lost pass.
-__global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *__restrict__ ff,
-                        uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters,
-                        uint32_t ent_cap)
+__device__ static void clear_range(size_t i, size_t stride, uint32_t *__restrict__ bin_count,
+                                   size_t nbins, float *__restrict__ ff,
+                                   uint32_t *__restrict__ hits, size_t nff,
+                                   uint32_t *__restrict__ counters, uint32_t ent_cap)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
     if (i == 0) {
         bool bad = counters[kCntOverflow] != 0 || counters[kCntEntries] > ent_cap;
         for (int k = 0; k < kCntN; ++k)
@@ -253,6 +261,52 @@ __global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *_
         if (hits)
             hits[k] = 0u;
     }
+}
+
+__global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *__restrict__ ff,
+                        uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters,
+                        uint32_t ent_cap)
+{
+    clear_range((size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x,
+                bin_count, nbins, ff, hits, nff, counters, ent_cap);
+}
+
+// k_prepare and k_clear of a pass in one launch (the first np workgroups
+// prepare, the others clear): both are small, so one launch saves a kernel's
+// start-up and drain.
+__global__ __launch_bounds__(256) void k_prepare_clear(
+    uint32_t np, uint32_t K, const uint32_t *__restrict__ shooters,
+    const uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
+    const float *__restrict__ unshot, float *__restrict__ snap, float *__restrict__ mvp,
+    uint32_t *__restrict__ bin_count, size_t nbins, float *__restrict__ ff,
+    uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters, uint32_t ent_cap)
+{
+    if (blockIdx.x < np) {
+        prepare_item(blockIdx.x * blockDim.x + threadIdx.x, K, shooters, nshoot, frames, unshot,
+                     snap, mvp);
+        return;
+    }
+    clear_range((size_t)(blockIdx.x - np) * blockDim.x + threadIdx.x,
+                (size_t)(gridDim.x - np) * blockDim.x, bin_count, nbins, ff, hits, nff, counters,
+                ent_cap);
+}
+
+void launch_prepare_clear(uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
+                          const float *frames, const float *unshot, float *snap, float *mvp,
+                          uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
+                          uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st)
+{
+    const uint32_t np = (K * 5 + 255) / 256;
+    size_t nbins = (size_t)K * ntiles, nff = ff || hits ? (size_t)K * npatch : 0;
+    size_t n = nbins > nff ? nbins : nff;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    if (blocks == 0)
+        blocks = 1;
+    k_prepare_clear<<<(unsigned)(np + blocks), 256, 0, st>>>(np, K, shooters, nshoot, frames,
+                                                            unshot, snap, mvp, bin_count, nbins,
+                                                            ff, hits, nff, counters, ent_cap);
 }
 
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
