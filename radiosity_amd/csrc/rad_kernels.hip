@@ -7,17 +7,21 @@
 // correctly-rounded defaults.
 //
 // Kernels (one pass over K shooters):
-//   k_select_*     shooter selection (+ FP64 residual partial sums)
-//   k_prepare      dB snapshot and the five face MVPs per shooter
-//   k_clear        F rows to 0, bin counts and pass counters reset
+//   k_select_* / k_topk*   shooter selection (+ FP64 residual partial sums)
+//   k_prepare_clear  dB snapshot and the five face MVPs per shooter; bin
+//                  counts and pass counters reset, F rows zeroed unless the
+//                  previous shoot left them zero
 //   k_raster_setup one lane per (triangle, shooter) over the five faces:
 //                  cull, transform, clip, snap; records counted into tile bins
+//   k_setup_clipped  the few triangle-faces that cross the near plane or the
+//                  guard band
 //   k_bin_scan / k_bin_fill   bin offsets and record lists
 //   k_tile_raster  one workgroup per (shooter, tile): depth test in LDS, then
 //                  the tile's item buffer out or ProcessHemicube fused
 //   k_reduce       ProcessHemicube over a whole item buffer (wave-level
 //                  run-length reduction into form factors and hit counts)
-//   k_shoot        energy shooting into unshot RGB, emitter update
+//   k_shoot        energy shooting into unshot RGB, emitter update; zeroes
+//                  the F entries it reads
 //
 // Depth test: each texel holds a 64-bit key (float depth bits << 32 | id).
 // atomicMin of keys gives the nearest fragment, ties to the smallest id,
@@ -236,8 +240,9 @@ void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
 }
 
 // --------------------------------------------------------------------- clear
-// Per pass: F rows (and hit counts) to zero, bin counts and pass counters
-// reset.  A kernel, not hipMemsetAsync, so a pass can be captured in a graph.
+// Per pass: bin counts and pass counters reset, F rows (ff non-null) and hit
+// counts (hits non-null) to zero.  A kernel, not hipMemsetAsync, so a pass
+// can be captured in a graph.
 // The previous pass's overflow state is folded into the sticky counter,
 // which only the host clears, so a multi-pass call reports any lost pass.
 __device__ static void clear_range(size_t i, size_t stride, uint32_t *__restrict__ bin_count,
