@@ -730,13 +730,13 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                                                       uint32_t group, uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[];
-    uint32_t *lcnt = dyn_lds; // bc.ntiles per-tile counts (dynamic LDS)
+    uint32_t *lcnt = dyn_lds; // 2 x bc.ntiles per-tile counts (dynamic LDS)
     __shared__ float lv[9][256];
     __shared__ uint32_t lid[256];
-    __shared__ uint16_t lq[256 * 5];
-    __shared__ uint32_t lnrec, lqn;
+    __shared__ uint16_t lq[256 * 5 * 2];
+    __shared__ uint32_t lnrec2[2], lqn;
     __shared__ float4 lscis[5];
-    __shared__ float lmvp[80]; // the current shooter's five face MVPs
+    __shared__ float lmvp[2][80]; // the current shooters' five face MVPs
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
     if (threadIdx.x < 5)
         lscis[threadIdx.x] = face_scissor_clip((int32_t)W, (int)threadIdx.x);
@@ -744,27 +744,38 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     // gridDim.z == 5 splits the faces over blocks (small K), else one block
     // walks all five faces
     const int f0 = gridDim.z == 1 ? 0 : (int)blockIdx.z, f1 = gridDim.z == 1 ? 5 : f0 + 1;
-    bc.lcnt = lcnt;
-    bc.lnrec = &lnrec;
+    // record region of shooter slot sl
+    auto rbof = [&](uint32_t sl) { return (sl * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x; };
     // The block's triangles stay in registers / LDS while it walks `group`
     // consecutive shooter slots (reloaded only when a packed slot changes
     // scene), so vertices are read once per group instead of once per shooter.
+    // Two slots that draw the same triangles are taken together: their culls
+    // share one queue and one exact phase, halving the barriers and the
+    // per-slot set-up; each keeps its own record region, tile counts and
+    // record count.
     uint32_t ltb = 0xFFFFFFFFu, ltc = 0;
     float vx[3][3];
     float4 sph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (uint32_t g = 0; g < group; ++g) {
+    for (uint32_t g = 0; g < group;) {
         const uint32_t s = blockIdx.y * group + g;
         if (s >= ns)
             break;
         const uint32_t tb = slot_tbeg ? slot_tbeg[s] : 0u, tc = slot_tcnt ? slot_tcnt[s] : ntri;
-        const uint32_t rb = (s * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x;
+        uint32_t nsh = 1;
+        if (g + 1 < group && s + 1 < ns) {
+            const uint32_t tb1 = slot_tbeg ? slot_tbeg[s + 1] : 0u;
+            const uint32_t tc1 = slot_tcnt ? slot_tcnt[s + 1] : ntri;
+            if (tb1 == tb && tc1 == tc)
+                nsh = 2;
+        }
         if (tl0 >= tc) {
-            // beyond this slot's scene: an empty region for the bin fill
-            if (threadIdx.x == 0)
-                blk_nrec[rb] = 0u;
+            // beyond the slots' scene: empty regions for the bin fill
+            if (threadIdx.x < nsh)
+                blk_nrec[rbof(s + threadIdx.x)] = 0u;
+            g += nsh;
             continue;
         }
-        __syncthreads(); // the previous slot's phase 2 and flush are done with LDS
+        __syncthreads(); // the previous slots' phase 2 and flush are done with LDS
         if (tb != ltb || tc != ltc) {
             ltb = tb;
             ltc = tc;
@@ -781,32 +792,35 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                 sph = tsph[t];
             }
         }
-        for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
+        for (uint32_t i = threadIdx.x; i < nsh * bc.ntiles; i += blockDim.x)
             lcnt[i] = 0u;
-        const float *sm = mvp + (size_t)s * kShooterStride;
-        if (threadIdx.x == 0) {
-            lnrec = 0u;
+        if (threadIdx.x < 2)
+            lnrec2[threadIdx.x] = 0u;
+        if (threadIdx.x == 0)
             lqn = 0u;
-        }
-        if (threadIdx.x < 80)
-            lmvp[threadIdx.x] = sm[threadIdx.x];
+        if (threadIdx.x < 80 * nsh)
+            lmvp[threadIdx.x / 80][threadIdx.x % 80] =
+                mvp[(size_t)(s + threadIdx.x / 80) * kShooterStride + threadIdx.x % 80];
         __syncthreads();
-        bc.region = rb * kRecsPerBlock;
-        // phase 1: cheap culls; surviving (triangle, face) pairs go to an LDS queue
+        // phase 1: cheap culls; surviving (triangle, slot, face) triples go to
+        // an LDS queue
         if (tl0 + threadIdx.x < tc) {
-            uint32_t fm = cull_faces(sm, vx, sph, f0, f1);
-            while (fm) {
-                int f = __ffs(fm) - 1;
-                fm &= fm - 1;
-                lq[atomicAdd(&lqn, 1u)] = (uint16_t)((threadIdx.x << 3) | (uint32_t)f);
+            for (uint32_t sh = 0; sh < nsh; ++sh) {
+                uint32_t fm = cull_faces(mvp + (size_t)(s + sh) * kShooterStride, vx, sph, f0, f1);
+                while (fm) {
+                    int f = __ffs(fm) - 1;
+                    fm &= fm - 1;
+                    lq[atomicAdd(&lqn, 1u)] =
+                        (uint16_t)((threadIdx.x << 4) | (sh << 3) | (uint32_t)f);
+                }
             }
         }
         __syncthreads();
-        // phase 2: the exact per-face pipeline over the compacted pairs
+        // phase 2: the exact per-face pipeline over the compacted triples
         const uint32_t nq = lqn;
         for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
             uint32_t e = lq[q];
-            uint32_t tl = e >> 3;
+            uint32_t tl = e >> 4, sh = (e >> 3) & 1u;
             int face = (int)(e & 7u);
             float wv[3][3];
 #pragma unroll
@@ -814,16 +828,24 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                     wv[k][c] = lv[3 * k + c][tl];
-            setup_face(lmvp + face * 16, lscis[face], wv, W, s, face, lid[tl], tb + tl0 + tl, bc);
+            BinCtx b2 = bc;
+            b2.region = rbof(s + sh) * kRecsPerBlock;
+            b2.lnrec = &lnrec2[sh];
+            b2.lcnt = lcnt + sh * bc.ntiles;
+            setup_face(lmvp[sh] + face * 16, lscis[face], wv, W, s + sh, face, lid[tl],
+                       tb + tl0 + tl, b2);
         }
         __syncthreads();
-        // flush: one global atomic per touched tile, and the region's record count
-        uint32_t *cnt = bc.bin_count + (size_t)s * bc.ntiles;
-        for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
-            if (lcnt[i])
-                atomicAdd(&cnt[i], lcnt[i]);
-        if (threadIdx.x == 0)
-            blk_nrec[rb] = lnrec;
+        // flush: one global atomic per touched tile, and the regions' record counts
+        for (uint32_t sh = 0; sh < nsh; ++sh) {
+            uint32_t *cnt = bc.bin_count + (size_t)(s + sh) * bc.ntiles;
+            for (uint32_t i = threadIdx.x; i < bc.ntiles; i += blockDim.x)
+                if (lcnt[sh * bc.ntiles + i])
+                    atomicAdd(&cnt[i], lcnt[sh * bc.ntiles + i]);
+        }
+        if (threadIdx.x < nsh)
+            blk_nrec[rbof(s + threadIdx.x)] = lnrec2[threadIdx.x];
+        g += nsh;
     }
 }
 
@@ -1452,7 +1474,7 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const 
     if (group > K)
         group = K;
     dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
-    k_raster_setup<<<grid, 256, bb.ntiles * sizeof(uint32_t), st>>>(
+    k_raster_setup<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
         g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
                                          slot_tbeg, slot_tcnt, group, K);
     bc.lnrec = nullptr;
