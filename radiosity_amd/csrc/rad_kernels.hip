@@ -730,13 +730,13 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                                                       uint32_t group, uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[];
-    uint32_t *lcnt = dyn_lds; // 2 x bc.ntiles per-tile counts (dynamic LDS)
+    uint32_t *lcnt = dyn_lds; // 4 x bc.ntiles per-tile counts (dynamic LDS)
     __shared__ float lv[9][256];
     __shared__ uint32_t lid[256];
-    __shared__ uint16_t lq[256 * 5 * 2];
-    __shared__ uint32_t lnrec2[2], lqn;
+    __shared__ uint16_t lq[256 * 5 * 4];
+    __shared__ uint32_t lnrec2[4], lqn;
     __shared__ float4 lscis[5];
-    __shared__ float lmvp[2][80]; // the current shooters' five face MVPs
+    __shared__ float lmvp[4][80]; // the current shooters' five face MVPs
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
     if (threadIdx.x < 5)
         lscis[threadIdx.x] = face_scissor_clip((int32_t)W, (int)threadIdx.x);
@@ -749,9 +749,9 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     // The block's triangles stay in registers / LDS while it walks `group`
     // consecutive shooter slots (reloaded only when a packed slot changes
     // scene), so vertices are read once per group instead of once per shooter.
-    // Two slots that draw the same triangles are taken together: their culls
-    // share one queue and one exact phase, halving the barriers and the
-    // per-slot set-up; each keeps its own record region, tile counts and
+    // Up to four slots that draw the same triangles are taken together: their
+    // culls share one queue and one exact phase, which divides the barriers
+    // and the per-slot set-up by the slot count; each keeps its own record region, tile counts and
     // record count.
     uint32_t ltb = 0xFFFFFFFFu, ltc = 0;
     float vx[3][3];
@@ -762,11 +762,12 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
             break;
         const uint32_t tb = slot_tbeg ? slot_tbeg[s] : 0u, tc = slot_tcnt ? slot_tcnt[s] : ntri;
         uint32_t nsh = 1;
-        if (g + 1 < group && s + 1 < ns) {
-            const uint32_t tb1 = slot_tbeg ? slot_tbeg[s + 1] : 0u;
-            const uint32_t tc1 = slot_tcnt ? slot_tcnt[s + 1] : ntri;
-            if (tb1 == tb && tc1 == tc)
-                nsh = 2;
+        while (nsh < 4 && g + nsh < group && s + nsh < ns) {
+            const uint32_t tb1 = slot_tbeg ? slot_tbeg[s + nsh] : 0u;
+            const uint32_t tc1 = slot_tcnt ? slot_tcnt[s + nsh] : ntri;
+            if (tb1 != tb || tc1 != tc)
+                break;
+            ++nsh;
         }
         if (tl0 >= tc) {
             // beyond the slots' scene: empty regions for the bin fill
@@ -794,13 +795,12 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
         }
         for (uint32_t i = threadIdx.x; i < nsh * bc.ntiles; i += blockDim.x)
             lcnt[i] = 0u;
-        if (threadIdx.x < 2)
+        if (threadIdx.x < 4)
             lnrec2[threadIdx.x] = 0u;
         if (threadIdx.x == 0)
             lqn = 0u;
-        if (threadIdx.x < 80 * nsh)
-            lmvp[threadIdx.x / 80][threadIdx.x % 80] =
-                mvp[(size_t)(s + threadIdx.x / 80) * kShooterStride + threadIdx.x % 80];
+        for (uint32_t i = threadIdx.x; i < 80 * nsh; i += blockDim.x)
+            lmvp[i / 80][i % 80] = mvp[(size_t)(s + i / 80) * kShooterStride + i % 80];
         __syncthreads();
         // phase 1: cheap culls; surviving (triangle, slot, face) triples go to
         // an LDS queue
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                     int f = __ffs(fm) - 1;
                     fm &= fm - 1;
                     lq[atomicAdd(&lqn, 1u)] =
-                        (uint16_t)((threadIdx.x << 4) | (sh << 3) | (uint32_t)f);
+                        (uint16_t)((threadIdx.x << 5) | (sh << 3) | (uint32_t)f);
                 }
             }
         }
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
         const uint32_t nq = lqn;
         for (uint32_t q = threadIdx.x; q < nq; q += blockDim.x) {
             uint32_t e = lq[q];
-            uint32_t tl = e >> 4, sh = (e >> 3) & 1u;
+            uint32_t tl = e >> 5, sh = (e >> 3) & 3u;
             int face = (int)(e & 7u);
             float wv[3][3];
 #pragma unroll
@@ -1474,7 +1474,7 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const 
     if (group > K)
         group = K;
     dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
-    k_raster_setup<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
+    k_raster_setup<<<grid, 256, 4 * bb.ntiles * sizeof(uint32_t), st>>>(
         g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
                                          slot_tbeg, slot_tcnt, group, K);
     bc.lnrec = nullptr;
