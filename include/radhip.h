@@ -121,6 +121,21 @@ int rad_process_hemicube(rad_ctx *ctx, uint32_t k, uint32_t *out_hits_or_null,
 int rad_process_itembuf(rad_ctx *ctx, const uint32_t *itembuf, uint32_t k,
                         uint32_t *out_hits_or_null, float *out_ff_or_null);
 
+/* The ProcessHemicube kernel's own output (Kernel_ProcessHemicube.h:9-69,
+ * launched at Main.cpp:1207-1249) over the k hemicubes last rendered (or
+ * given to rad_process_itembuf): per-run triplets (hemicube, patch id,
+ * ΔFF sum of the run) for every run of a nonzero id along the spans of
+ * TEX_W / ocl_workitems_x texels (Config.cpp:43-45), each work-item reserving
+ * 4 slots at a time and padding its last block with (0, 0, 0), exactly as
+ * the reference kernel writes p_hemicubes / p_ids / p_energies.  The stream
+ * is in work-item order (row, then span), one of the orders the reference's
+ * atomics can produce.  *out_count gets the write index (a multiple of 4,
+ * the reference's n_last_index, Main.cpp:1231-1233) even when it exceeds cap
+ * (then RAD_E_LIMIT and nothing is copied).  Ids are item - 1 whatever their
+ * range; the reference's host loop skips those >= npatch (Main.cpp:1259). */
+int rad_process_runs(rad_ctx *ctx, uint32_t k, uint32_t cap, uint32_t *out_hemicubes,
+                     uint32_t *out_ids, float *out_energies, uint32_t *out_count);
+
 /* One pass of the OnIdle shoot loop (Main.cpp:1137-1295) for the k given
  * shooters: snapshot, render, reduce, shoot, emitter update. */
 int rad_shoot(rad_ctx *ctx, const uint32_t *shooters, uint32_t k);
@@ -130,18 +145,50 @@ int rad_shoot(rad_ctx *ctx, const uint32_t *shooters, uint32_t k);
 int rad_step(rad_ctx *ctx, uint32_t passes);
 
 /* Runs passes until the FP64 residual <= residual_target or max_passes;
- * the residual is checked every check_every passes.  Mirrors the stop test of
- * Main.cpp:1297-1303 plus the residual rule. */
+ * the residual is checked every check_every passes (the residual rule; the
+ * reference's own per-pass stop test is rad_run_cycle). */
 typedef struct rad_stats {
     uint64_t passes;         /* passes executed */
-    uint64_t shooters;       /* shooters shot (<= passes*K) */
+    uint64_t shooters;       /* shooters shot, counted on the device (<= passes*K) */
     double residual;         /* final FP64 residual */
     double seconds;          /* wall time of the loop */
-    float last_energy;       /* |dB| of the last shooter (Main.cpp:1292,1298) */
+    float last_energy;       /* lastEnergy.f_Length() of the last pass (Main.cpp:1288-1298):
+                                |radiosity| of its last shooter before the emitter update */
     uint32_t reserved;
 } rad_stats;
 int rad_run(rad_ctx *ctx, uint64_t max_passes, double residual_target,
             uint32_t check_every, rad_stats *out);
+
+/* One OnIdle cycle (Main.cpp:1137-1303): up to `passes` passes (select K
+ * with the context rule, render, reduce, shoot) with the reference's stop
+ * test after every pass: the cycle ends with the pass whose last shooter's
+ * radiosity before its emitter update (lastEnergy, Main.cpp:1288-1292) has
+ * f_Length() < stop_energy (0.1 in the reference, Main.cpp:1298; 0 turns
+ * the test off).  The test runs on the device, so captured graph replays
+ * stop too: passes after the stopping one select nobody.  The host
+ * synchronises once per call. */
+typedef struct rad_cycle_stats {
+    uint64_t passes;         /* passes that shot, the stopping pass included */
+    uint64_t shooters;       /* shooters shot */
+    float last_energy[3];    /* lastEnergy of the last pass that shot */
+    uint32_t stopped;        /* 1: the stop test fired in this cycle */
+} rad_cycle_stats;
+int rad_run_cycle(rad_ctx *ctx, uint32_t passes, double stop_energy, rad_cycle_stats *out);
+
+/* Deterministic form-factor rows (no counterpart in the reference, whose
+ * triplets are appended in the order the OpenCL work-items finish,
+ * Kernel_ProcessHemicube.h:52-58, and summed in that order on the host,
+ * Main.cpp:1257-1266): on != 0 accumulates every F entry in 64-bit fixed
+ * point (units of 2^-52) with integer atomics, whose sum does not depend on
+ * their order, and converts the rows to float once per pass; passes, item
+ * buffer reductions and states are then identical bit for bit from run to
+ * run.  Costs 8 bytes per F entry of device memory and one conversion pass
+ * over the rows.  0 returns to float atomics (the default). */
+int rad_set_deterministic(rad_ctx *ctx, int on);
+
+/* Passes that shot and shooters shot since rad_create, as counted by the
+ * shoot kernels on the device (synchronises).  Either pointer may be NULL. */
+int rad_shot_count(rad_ctx *ctx, uint64_t *passes_or_null, uint64_t *shooters_or_null);
 
 /* glFinish/clFinish (Main.cpp:1202, 1228): waits for all work queued on the
  * context's stream; RAD_E_LIMIT if a pass since the last check lost
@@ -167,6 +214,15 @@ int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
  * counts of the last pass (hemicubes * tiles entries, row-major tiles of
  * tile_px) into out and returns the tile count per hemicube in *ntiles. */
 int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32_t *tile_px);
+
+/* Diagnostics (no counterpart in the reference): the rasterizer's pass
+ * counters after the last pass (synchronises): out[0] and [1] unused,
+ * [2] nonzero if a record or clipped polygon did not fit,
+ * [3] bin entries, [4] item-buffer ids beyond npatch, [5] records in the
+ * spill region, [6] triangle-faces queued for clipping, [7] nonzero if an
+ * earlier pass overflowed.  Capacities: out_caps (optional, 3 entries) gets
+ * the spill capacity, the bin-entry capacity and the clip-queue capacity. */
+int rad_debug_counters(rad_ctx *ctx, uint32_t *out8, uint32_t *out_caps_or_null);
 
 /* Autotuner of the rasterizer: times `passes` render passes of K shooters
  * spread evenly over the patches (render + reduce, no shooting, state

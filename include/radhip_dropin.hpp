@@ -230,33 +230,43 @@ class ModelContainer {
 // Config must be frozen.
 float *precomputeHemicubeFormFactors();
 
-// The OnIdle shooting cycle (Main.cpp:1137-1309): SHOOTS_PER_CYCLE passes of
-// HEMICUBES_CNT shooters (select, render the item buffers, ProcessHemicube,
-// accumulate, shoot, emitter update), then the patch fields are updated.
+// The OnIdle shooting cycle (Main.cpp:1137-1309): up to SHOOTS_PER_CYCLE
+// passes of HEMICUBES_CNT shooters (select, render the item buffers,
+// ProcessHemicube, accumulate, shoot, emitter update), each followed by the
+// reference's stop test, then the patch fields are updated.
 class Shooter {
   public:
     explicit Shooter(ModelContainer *scene);
-    // One OnIdle call's worth of passes; returns |dB| of the last shooter
-    // (lastEnergy.f_Length(), Main.cpp:1292).
+    // One OnIdle call: passes until SHOOTS_PER_CYCLE or until the pass whose
+    // lastEnergy.f_Length() < 0.1 (Main.cpp:1137, 1288-1300; computeRadiosity
+    // then stays false and later calls shoot nothing).  Returns that length
+    // for the cycle's last pass, or -1 when the library failed (failed() is
+    // then true and lastError() says why).
     float shootCycle();
-    // Main.cpp:1298: the cycle loop stops below 0.1.
-    bool finished() const { return lastEnergy < 0.1f && cycles > 0; }
+    // computeRadiosity == false: the stop test fired (Main.cpp:1298-1300).
+    bool finished() const { return stopped; }
+    bool failed() const { return error; }
     double residual(); // FP64 area-weighted unshot / emitted
     unsigned long long shooters() const { return shot; }
+    unsigned long long passes() const { return npasses; }
 
   private:
     ModelContainer *scene;
     float lastEnergy = 0.0f;
+    bool stopped = false, error = false;
     unsigned cycles = 0;
-    unsigned long long shot = 0;
+    unsigned long long shot = 0, npasses = 0;
 };
 
-// The ProcessHemicube kernel's host contract (Kernel_ProcessHemicube.h:9-20,
+// The ProcessHemicube kernel's host contract (Kernel_ProcessHemicube.h:9-69,
 // Main.cpp:1207-1249) over an explicit item buffer of n_hemicubes stacked
-// hemicubes (ids: patch + 1, 0 background): writes triplets (hemicube, patch
-// id, form-factor sum) for every (hemicube, patch) seen, in hemicube then id
-// order, and returns their count (the reference's write index).  The output
-// arrays hold at least n_hemicubes * getPatchesCount() entries.
+// hemicubes (ids: patch + 1, 0 background): writes the kernel's triplets
+// (hemicube, patch id, ΔFF sum of one run along a span of
+// PATCHVIEW_TEX_W / OCL_WORKITEMS_X texels), 4 slots reserved at a time per
+// span with (0, 0, 0) padding, and returns their count (the reference's write
+// index n_last_index, a multiple of 4).  The output arrays hold
+// PATCHVIEW_TEX_RES * n_hemicubes entries, as the reference allocates them
+// (Main.cpp:605-607); a longer stream returns 0 with lastError() set.
 unsigned int ProcessHemicube(ModelContainer *scene, const uint32_t *itembuf,
                              unsigned int n_hemicubes, uint32_t *p_hemicubes, uint32_t *p_ids,
                              float *p_energies);

@@ -605,6 +605,59 @@ uint64_t orc_process_hemicube_runs(uint32_t W, uint32_t K, uint32_t wix,
     return emitted;
 }
 
+uint64_t orc_process_hemicube_triplets(uint32_t W, uint32_t K, uint32_t wix,
+                                       const uint32_t *item, const float *dff, uint64_t cap,
+                                       uint32_t *hemis, uint32_t *ids, float *energies)
+{
+    uint32_t TW, TH, RES;
+    orc_geometry(W, &TW, &TH, &RES);
+    uint32_t nx = wix < TW ? wix : TW; /* Config.cpp:43-45 */
+    uint32_t span = TW / nx;
+    uint64_t count = 0; /* the write index, advanced 4 slots at a time */
+    for (uint32_t y = 0; y < TH * K; ++y) {
+        const float *frow = dff + (size_t)(y % TH) * TW;
+        for (uint32_t gx = 0; gx < nx; ++gx) {
+            /* Kernel_ProcessHemicube.h:23-69, one work-item */
+            uint32_t x0 = gx * span < TW - 1 ? gx * span : TW - 1;
+            uint32_t x1 = x0 + span < TW ? x0 + span : TW;
+            uint32_t space = 0;
+            uint64_t wid = 0;
+            while (x0 < x1) {
+                uint32_t id = item[(size_t)y * TW + x0];
+                float e = 0;
+                while (x0 < x1 && item[(size_t)y * TW + x0] == id) {
+                    e += frow[x0];
+                    ++x0;
+                }
+                if (id > 0) {
+                    if (!space) {
+                        wid = count;
+                        count += 4;
+                        space = 3;
+                    } else {
+                        ++wid;
+                        --space;
+                    }
+                    if (wid < cap) {
+                        hemis[wid] = y / TH;
+                        ids[wid] = id - 1;
+                        energies[wid] = e;
+                    }
+                }
+            }
+            for (; space > 0; --space) {
+                ++wid;
+                if (wid < cap) {
+                    hemis[wid] = 0;
+                    ids[wid] = 0;
+                    energies[wid] = 0.0f;
+                }
+            }
+        }
+    }
+    return count;
+}
+
 void orc_hemicube_histogram(uint32_t W, uint32_t K, const uint32_t *item,
                             const float *dff, uint32_t npatch, uint32_t *hits,
                             double *ff64)
@@ -797,6 +850,101 @@ static uint32_t plain_pass(uint32_t W, uint32_t mode, uint32_t select_rule, cons
         unshot[3 * i + c] -= snap[c];
     }
     return i;
+}
+
+/* POWER top-K: patches with A (r+g+b) > 0 by that key, descending, equal
+ * keys by index ascending; returns how many (<= count) were filled. */
+uint32_t orc_select_topk_power(const float *unshot, const float *area, uint32_t npatch,
+                               uint32_t count, uint32_t *ids)
+{
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < npatch; ++i) {
+        float k = area[i] * (unshot[3 * i] + unshot[3 * i + 1] + unshot[3 * i + 2]);
+        if (!(k > 0.0f))
+            continue;
+        /* insertion into the sorted prefix ids[0..n) */
+        uint32_t pos = n < count ? n : count;
+        while (pos > 0) {
+            uint32_t q = ids[pos - 1];
+            float kq = area[q] * (unshot[3 * q] + unshot[3 * q + 1] + unshot[3 * q + 2]);
+            if (kq >= k)
+                break;
+            --pos;
+        }
+        if (pos >= count)
+            continue;
+        uint32_t last = n < count ? n : count - 1;
+        for (uint32_t a = last; a > pos; --a)
+            ids[a] = ids[a - 1];
+        ids[pos] = i;
+        if (n < count)
+            ++n;
+    }
+    for (uint32_t a = n; a < count; ++a)
+        ids[a] = 0;
+    return n;
+}
+
+/* One pass of the OnIdle loop with K hemicubes (Main.cpp:1137-1303), float
+ * state as the reference's Patch fields:
+ *  - select K shooters (ModelContainer.cpp:259-299 list rule, or the POWER
+ *    argmax / top-K); unfilled slots are skipped (Main.cpp:1157,1253,1289);
+ *  - snapshot every shooter's radiosity before any update (Main.cpp:1161);
+ *  - per shooter in slot order: its hemicube, the run-form ProcessHemicube
+ *    sums accumulated in float (Kernel_ProcessHemicube.h:23-69,
+ *    Main.cpp:1251-1269), then every receiver updated (Main.cpp:1272-1275);
+ *  - emitter update in slot order; lastEnergy is the last shooter's
+ *    radiosity before its subtraction (Main.cpp:1286-1295).
+ * item: TEX_RES scratch, ff: npatch scratch, ids: K entries (out).
+ * last_energy3 gets lastEnergy.  Returns the number of shooters. */
+uint32_t orc_pass_k(uint32_t W, uint32_t mode, uint32_t select_rule, uint32_t K,
+                    const float *tri, const uint32_t *tri_patch, uint32_t ntri,
+                    const float *frames, const float *refl, const float *area, uint32_t npatch,
+                    const float *dff, float *unshot, float *shot, uint32_t *item, float *ff,
+                    uint32_t *ids, float *last_energy3)
+{
+    uint32_t n;
+    if (select_rule == 0)
+        n = orc_select_reference(unshot, npatch, K, ids);
+    else if (K == 1) {
+        ids[0] = orc_select_argmax_power(unshot, area, npatch);
+        n = 1;
+    } else
+        n = orc_select_topk_power(unshot, area, npatch, K, ids);
+    float *snap = (float *)malloc(sizeof(float) * 3 * (n ? n : 1));
+    for (uint32_t s = 0; s < n; ++s)
+        for (int c = 0; c < 3; ++c)
+            snap[3 * s + c] = unshot[3 * ids[s] + c];
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint32_t i = ids[s];
+        orc_render_hemicube(W, tri, tri_patch, ntri, frames + 9 * (size_t)i, item, NULL);
+        memset(ff, 0, sizeof(float) * npatch);
+        orc_process_hemicube_runs(W, 1, 4, item, dff, npatch, ff, NULL);
+        for (uint32_t j = 0; j < npatch; ++j) {
+            float f = ff[j];
+            if (f == 0.0f)
+                continue;
+            for (int c = 0; c < 3; ++c) {
+                if (mode == 0)
+                    unshot[3 * j + c] += snap[3 * s + c] * f * 0.3f * refl[3 * i + c];
+                else
+                    unshot[3 * j + c] += refl[3 * j + c] * snap[3 * s + c] * f *
+                                         (area[i] / area[j]);
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+        last_energy3[c] = 0.0f;
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint32_t i = ids[s];
+        for (int c = 0; c < 3; ++c) {
+            last_energy3[c] = unshot[3 * i + c];
+            shot[3 * i + c] += snap[3 * s + c];
+            unshot[3 * i + c] -= snap[3 * s + c];
+        }
+    }
+    free(snap);
+    return n;
 }
 
 uint32_t orc_gather_jacobi(uint32_t n, uint32_t R, const float *M, const float *refl,

@@ -75,6 +75,19 @@ uint64_t orc_process_hemicube_runs(uint32_t W, uint32_t K, uint32_t workitems_x,
                                    const uint32_t *item, const float *dff_cross,
                                    uint32_t npatch, float *ff, uint64_t *unknown);
 
+/* The same kernel's output itself: per-run triplets (hemicube, patch id,
+ * float sum of ΔFF along the run) for every run of a nonzero id, each
+ * work-item (row y, span gx) reserving 4 slots at a time and padding its
+ * last block with (0, 0, 0) triplets (Kernel_ProcessHemicube.h:52-69).
+ * Written in work-item order (row, then span), one of the orders the
+ * reference's atomics may produce; ids are item - 1 whatever their range.
+ * Returns the write index (a multiple of 4); entries at or beyond cap are
+ * not written. */
+uint64_t orc_process_hemicube_triplets(uint32_t W, uint32_t K, uint32_t workitems_x,
+                                       const uint32_t *item, const float *dff_cross,
+                                       uint64_t cap, uint32_t *hemis, uint32_t *ids,
+                                       float *energies);
+
 /* Direct per-pixel histogram of the same item buffer: hits[K*npatch] and
  * ff64[K*npatch] (FP64 twin).  Both are overwritten. */
 void orc_hemicube_histogram(uint32_t W, uint32_t K, const uint32_t *item,
@@ -117,6 +130,23 @@ uint32_t orc_plain_pass_mt(uint32_t W, uint32_t mode, uint32_t select_rule,
                            const float *frames, const float *refl, const float *area,
                            uint32_t npatch, const float *dff, float *unshot, float *shot,
                            uint32_t *item, float *ff, int nthreads);
+
+/* POWER top-K: patches with A (r+g+b) > 0, by that key descending, equal
+ * keys by index ascending; count ids (unfilled 0), returns the number filled. */
+uint32_t orc_select_topk_power(const float *unshot, const float *area, uint32_t npatch,
+                               uint32_t count, uint32_t *ids);
+
+/* One OnIdle pass with K hemicubes (Main.cpp:1137-1303) on float state:
+ * select, snapshot, per shooter render + run-form ProcessHemicube + receiver
+ * update, then the emitter updates.  last_energy3 = lastEnergy, the last
+ * shooter's radiosity before its subtraction (Main.cpp:1288-1292).
+ * item: TEX_RES scratch, ff: npatch scratch, ids: K (out).  Returns the
+ * number of shooters. */
+uint32_t orc_pass_k(uint32_t W, uint32_t mode, uint32_t select_rule, uint32_t K,
+                    const float *tri, const uint32_t *tri_patch, uint32_t ntri,
+                    const float *frames, const float *refl, const float *area, uint32_t npatch,
+                    const float *dff, float *unshot, float *shot, uint32_t *item, float *ff,
+                    uint32_t *ids, float *last_energy3);
 
 /* FP64 Jacobi for the gather B = E + diag(rho) M B (no reference
  * counterpart: the fixed point the shooting loop of Main.cpp:1251-1295

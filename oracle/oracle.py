@@ -41,6 +41,11 @@ _SIG = {
     "orc_plain_pass": (_u32, [_u32, _u32, _u32, _PF, _PU, _u32, _PF, _PF, _PF, _u32, _PF,
                               _PF, _PF, _PU, _PF]),
     "orc_gather_jacobi": (_u32, [_u32, _u32, _PF, _PF, _PF, _PD, C.c_double, _u32, _PD]),
+    "orc_select_topk_power": (_u32, [_PF, _PF, _u32, _u32, _PU]),
+    "orc_process_hemicube_triplets": (C.c_uint64, [_u32, _u32, _u32, _PU, _PF, C.c_uint64, _PU,
+                                                   _PU, _PF]),
+    "orc_pass_k": (_u32, [_u32, _u32, _u32, _u32, _PF, _PU, _u32, _PF, _PF, _PF, _u32, _PF,
+                          _PF, _PF, _PU, _PF, _PU, _PF]),
 }
 
 _lib = None
@@ -234,3 +239,37 @@ def gather_jacobi(M, refl, area, E, tol=1e-13, max_it=100000):
                                  _f(np.ascontiguousarray(area, np.float32)), _d(E), tol, max_it,
                                  _d(B))
     return B, it
+
+
+def select_topk_power(unshot, area, count):
+    u = np.ascontiguousarray(unshot, np.float32)
+    a = np.ascontiguousarray(area, np.float32)
+    ids = np.zeros(count, np.uint32)
+    n = lib().orc_select_topk_power(_f(u), _f(a), u.shape[0], count, _u(ids))
+    return ids, n
+
+
+def pass_k(W, mode, rule, K, sc, dff, unshot, shot):
+    """One K-hemicube pass of the OnIdle loop on float state (updated in
+    place); returns (shooter ids, lastEnergy as 3 floats)."""
+    item = np.zeros(geometry(W)[2], np.uint32)
+    ff = np.zeros(sc.npatch, np.float32)
+    ids = np.zeros(K, np.uint32)
+    le = np.zeros(3, np.float32)
+    n = lib().orc_pass_k(W, mode, rule, K, _f(sc.tri), _u(sc.tri_patch), sc.ntri, _f(sc.frame),
+                         _f(sc.refl), _f(sc.area), sc.npatch, _f(dff), _f(unshot), _f(shot),
+                         _u(item), _f(ff), _u(ids), _f(le))
+    return ids[:n], le
+
+
+def process_triplets(W, K, item, dff, workitems_x=4):
+    """The ProcessHemicube kernel's triplets in work-item order, padding
+    included: (hemicubes, ids, energies)."""
+    item = np.ascontiguousarray(item, np.uint32)
+    cap = int(item.size) + 4 * K * geometry(W)[1] * workitems_x
+    h = np.zeros(cap, np.uint32)
+    i = np.zeros(cap, np.uint32)
+    e = np.zeros(cap, np.float32)
+    n = lib().orc_process_hemicube_triplets(W, K, workitems_x, _u(item), _f(dff), cap, _u(h),
+                                            _u(i), _f(e))
+    return h[:n].copy(), i[:n].copy(), e[:n].copy()

@@ -166,3 +166,70 @@ def smooth_shade(val, nbr):
         lb = val[q] + val[n[5]] + val[n[6]] + val[n[7]]
         out[q] = [lb * quarter, rb * quarter, rt * quarter, lt * quarter]
     return out
+
+
+# ---------------------------------------------------------------- checkpoints
+# The portable solver-state file that succeeds the reference's .rr dump of
+# raw Patch structs (Main.cpp:1460-1647, LoadingModel.cpp:4-26), written here
+# independently of the library from its documented layout:
+#   "RADCKPT1" | u32 version (1) | u32 npatch | u32 ntri | u32 mode |
+#   u64 scene hash | f32 unshot[3][npatch] | f32 shot[3][npatch] |
+#   f32 emission[3][npatch] | u64 FNV-1a of every byte before it
+# little-endian, one plane per channel.  The scene hash is FNV-1a over ntri
+# and npatch (u32 each), the triangles (ntri x 9 f32, triangle after
+# triangle), the triangle patch ids (u32), the patch frames (npatch x 9 f32),
+# the reflectances (npatch x 3 f32) and the areas (f32).
+
+FNV_OFFSET = 0xCBF29CE484222325
+FNV_PRIME = 0x100000001B3
+_MASK64 = (1 << 64) - 1
+
+
+def fnv1a64(data, h=FNV_OFFSET):
+    """64-bit FNV-1a (Fowler/Noll/Vo) of a bytes-like object."""
+    for b in bytes(data):
+        h = ((h ^ b) * FNV_PRIME) & _MASK64
+    return h
+
+
+def _le(a, dtype):
+    return np.ascontiguousarray(a, dtype=np.dtype(dtype).newbyteorder("<")).tobytes()
+
+
+def scene_hash(tri, tri_patch, frame, refl, area):
+    tri = np.asarray(tri, f32).reshape(-1, 9)
+    ntri, npatch = tri.shape[0], np.asarray(area).shape[0]
+    h = fnv1a64(_le([ntri, npatch], np.uint32))
+    for part in (_le(tri, f32), _le(tri_patch, np.uint32), _le(frame, f32), _le(refl, f32),
+                 _le(area, f32)):
+        h = fnv1a64(part, h)
+    return h
+
+
+def ckpt_write(path, sc, mode, unshot, shot, emission, corrupt=None):
+    """Writes a checkpoint of state (unshot, shot, emission: npatch x 3) for
+    scene sc (tri, tri_patch, frame, refl, area)."""
+    n = sc.area.shape[0]
+    head = (b"RADCKPT1" + _le([1, n, sc.tri.shape[0], mode], np.uint32)
+            + _le([scene_hash(sc.tri, sc.tri_patch, sc.frame, sc.refl, sc.area)], np.uint64))
+    body = b"".join(_le(np.asarray(a, f32).reshape(n, 3).T, f32)
+                    for a in (unshot, shot, emission))
+    blob = head + body
+    blob += _le([fnv1a64(blob)], np.uint64)
+    with open(path, "wb") as f:
+        f.write(blob)
+
+
+def ckpt_read(path):
+    """Parses a checkpoint; returns a dict of its fields and whether its
+    checksum holds."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"RADCKPT1", "magic"
+    ver, n, ntri, mode = np.frombuffer(raw[8:24], "<u4")
+    (h,) = np.frombuffer(raw[24:32], "<u8")
+    body = np.frombuffer(raw[32:32 + 36 * int(n)], "<f4").reshape(9, int(n))
+    (cs,) = np.frombuffer(raw[32 + 36 * int(n):], "<u8")
+    return {"version": int(ver), "npatch": int(n), "ntri": int(ntri), "mode": int(mode),
+            "scene_hash": int(h), "unshot": body[0:3].T.copy(), "shot": body[3:6].T.copy(),
+            "emission": body[6:9].T.copy(), "size": len(raw),
+            "checksum_ok": int(cs) == fnv1a64(raw[:32 + 36 * int(n)])}

@@ -38,6 +38,11 @@ class rad_stats(C.Structure):
                 ("seconds", C.c_double), ("last_energy", C.c_float), ("reserved", C.c_uint32)]
 
 
+class rad_cycle_stats(C.Structure):
+    _fields_ = [("passes", C.c_uint64), ("shooters", C.c_uint64),
+                ("last_energy", C.c_float * 3), ("stopped", C.c_uint32)]
+
+
 class rad_scene(C.Structure):
     _fields_ = [("ntri", C.c_uint32), ("npatch", C.c_uint32), ("nquad", C.c_uint32),
                 ("reserved", C.c_uint32),
@@ -88,9 +93,13 @@ SIGNATURES = {
     "rad_render_itembuf": (C.c_int, [_P, _PU, _u32, _PU, _PF]),
     "rad_process_hemicube": (C.c_int, [_P, _u32, _PU, _PF]),
     "rad_process_itembuf": (C.c_int, [_P, _PU, _u32, _PU, _PF]),
+    "rad_process_runs": (C.c_int, [_P, _u32, _u32, _PU, _PU, _PF, _PU]),
     "rad_shoot": (C.c_int, [_P, _PU, _u32]),
     "rad_step": (C.c_int, [_P, _u32]),
     "rad_run": (C.c_int, [_P, C.c_uint64, C.c_double, _u32, C.POINTER(rad_stats)]),
+    "rad_run_cycle": (C.c_int, [_P, _u32, C.c_double, C.POINTER(rad_cycle_stats)]),
+    "rad_shot_count": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "rad_set_deterministic": (C.c_int, [_P, C.c_int]),
     "rad_residual_fp64": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "rad_finish": (C.c_int, [_P]),
     "rad_face_mvps": (C.c_int, [_P, _u32, _PF]),
@@ -113,6 +122,7 @@ SIGNATURES = {
     "rad_scene_faces": (C.c_int, [_PF, _u32, _PF, _PF, _PF, C.c_double, C.c_int,
                                   C.POINTER(C.POINTER(rad_scene))]),
     "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
+    "rad_debug_counters": (C.c_int, [_P, _PU, _PU]),
     "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
     "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
     "rad_pack_begin": (C.c_int, [_P, C.c_double]),
@@ -256,10 +266,10 @@ class Solver:
     """One rad_ctx: scene, state and the shooting passes on the current GPU."""
 
     def __init__(self, hemicube=16, hemicubes=1, mode=RAD_MODE_PHYSICAL, select=RAD_SELECT_POWER,
-                 shoots=500, small_tri_px=0, tile_px=0, graph_passes=0):
+                 shoots=500, small_tri_px=0, tile_px=0, graph_passes=0, workitems_x=4):
         L = lib()
-        cfg = rad_config(hemicube, hemicubes, shoots, 4, mode, select, small_tri_px, tile_px,
-                         graph_passes)
+        cfg = rad_config(hemicube, hemicubes, shoots, workitems_x, mode, select, small_tri_px,
+                         tile_px, graph_passes)
         self._ctx = C.c_void_p()
         rc = L.rad_create(C.byref(cfg), C.byref(self._ctx))
         if rc != RAD_OK:
@@ -403,6 +413,20 @@ class Solver:
         self._chk(lib().rad_process_hemicube(self._ctx, k, None, fptr(ff)), "rad_process_hemicube")
         return ff.reshape(k, self.npatch)
 
+    def process_runs(self, k):
+        """The run-form triplets (hemicubes, ids, energies) of the last k
+        rendered hemicubes, padding included (rad_process_runs)."""
+        n = C.c_uint32()
+        rc = lib().rad_process_runs(self._ctx, k, 0, None, None, None, C.byref(n))
+        if rc not in (RAD_OK, -5):
+            self._chk(rc, "rad_process_runs")
+        h = np.zeros(n.value, np.uint32)
+        i = np.zeros(n.value, np.uint32)
+        e = np.zeros(n.value, np.float32)
+        self._chk(lib().rad_process_runs(self._ctx, k, n.value, uptr(h), uptr(i), fptr(e),
+                                         C.byref(n)), "rad_process_runs")
+        return h, i, e
+
     def process_itembuf(self, item, k):
         it = np.ascontiguousarray(item, np.uint32)
         hits = np.zeros(k * self.npatch, np.uint32)
@@ -426,6 +450,24 @@ class Solver:
         self._chk(lib().rad_run(self._ctx, max_passes, target, check_every, C.byref(st)), "rad_run")
         return st
 
+    def run_cycle(self, passes, stop_energy=0.1):
+        """One OnIdle cycle with the reference's per-pass stop test
+        (rad_run_cycle); returns the rad_cycle_stats."""
+        st = rad_cycle_stats()
+        self._chk(lib().rad_run_cycle(self._ctx, passes, stop_energy, C.byref(st)),
+                  "rad_run_cycle")
+        return st
+
+    def set_deterministic(self, on=True):
+        """Fixed-point F accumulation: bit-identical passes run to run."""
+        self._chk(lib().rad_set_deterministic(self._ctx, int(bool(on))), "rad_set_deterministic")
+
+    def shot_count(self):
+        """(passes that shot, shooters shot) counted on the device."""
+        p, n = C.c_uint64(), C.c_uint64()
+        self._chk(lib().rad_shot_count(self._ctx, C.byref(p), C.byref(n)), "rad_shot_count")
+        return p.value, n.value
+
     def residual(self):
         r = C.c_double()
         self._chk(lib().rad_residual_fp64(self._ctx, C.byref(r)), "rad_residual_fp64")
@@ -443,6 +485,17 @@ class Solver:
         self._chk(lib().rad_debug_bins(self._ctx, uptr(out), C.byref(nt), C.byref(tp)),
                   "rad_debug_bins")
         return out.reshape(self.K, nt.value), tp.value
+
+    def debug_counters(self):
+        """The rasterizer's pass counters of the last pass (dict) and the
+        spill, bin-entry and clip-queue capacities."""
+        out = np.zeros(8, np.uint32)
+        caps = np.zeros(3, np.uint32)
+        self._chk(lib().rad_debug_counters(self._ctx, uptr(out), uptr(caps)), "rad_debug_counters")
+        names = ["unused0", "unused1", "overflow", "entries", "unknown", "spill", "clip", "sticky"]
+        d = {k: int(v) for k, v in zip(names, out)}
+        d.update(spill_cap=int(caps[0]), entry_cap=int(caps[1]), clip_cap=int(caps[2]))
+        return d
 
     def kernel_times(self, enable=-1):
         out = (C.c_double * 6)()

@@ -83,6 +83,29 @@ def build(verbose=False):
     return OUT
 
 
+# Host code under AddressSanitizer + UndefinedBehaviorSanitizer (CPU only,
+# g++, no device code): the scene builders, the OBJ reader and the drop-in
+# layer linked into the drop-in test driver; the device entry points come from
+# libradhip.so.  Built by the sanitizer test, not by build().
+ASAN_SOURCES = [os.path.join(CSRC, f) for f in ("rad_scene.cpp", "rad_obj.cpp", "rad_dropin.cpp")]
+ASAN_EXE = os.path.join(BUILD, "asan", "dropin_check_asan")
+
+
+def build_asan(verbose=False):
+    os.makedirs(os.path.dirname(ASAN_EXE), exist_ok=True)
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-Wall", "-ffp-contract=off",
+           "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+           "-fno-omit-frame-pointer", "-I", os.path.join(ROOT, "include")]
+    cmd += ASAN_SOURCES + [os.path.join(ROOT, "tests", "dropin_check.cpp"),
+                           "-o", ASAN_EXE, "-L", HERE, "-lradhip", f"-Wl,-rpath,{HERE}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"sanitizer build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {ASAN_EXE}")
+    return ASAN_EXE
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

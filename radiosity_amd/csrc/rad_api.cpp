@@ -41,19 +41,31 @@ struct rad_ctx {
     float *shot = nullptr;         // npatch*3
     float *ff = nullptr;           // K*npatch
     uint32_t *hits = nullptr;      // K*npatch
-    // every F row entry is zero at this point of the stream (set by a clear
-    // or by the shoot kernels, which zero what they read; false after any
-    // kernel that accumulates into F); lets a pass skip clearing the rows
-    bool ff_zero = false;
+    // F rows [0, ff_dirty) may hold nonzero entries at this point of the
+    // stream, rows from ff_dirty on are zero: a clear or a shoot (the shoot
+    // kernels zero every entry they read) leaves 0, a kernel that accumulates
+    // into k rows raises it to at least k.  Lets a pass skip clearing F.
+    uint32_t ff_dirty = 0;
 
     // per pass
     float *dff = nullptr;          // RES
-    uint64_t *keys = nullptr;      // K*RES
+    uint32_t *item_ids = nullptr;  // K*RES: the item buffer of the last render (patch + 1)
+    float *item_depth = nullptr;   // K*RES: its depths (allocated when first asked for)
+    unsigned long long *ffq = nullptr; // K*rowlen fixed-point F rows (deterministic mode)
+    bool det = false;              // deterministic F rows (rad_set_deterministic)
+    // run-form ProcessHemicube (rad_process_runs): per-work-item slot counts
+    // and offsets, and the device triplet stream
+    uint32_t *runs_slots = nullptr, *runs_offs = nullptr;
+    uint32_t runs_items = 0;
+    uint32_t *trip_h = nullptr, *trip_i = nullptr;
+    float *trip_e = nullptr;
+    uint32_t trip_cap = 0;
     float *mvp = nullptr;          // K*5*16
     float *snap = nullptr;         // K*3
     uint32_t *shooters = nullptr;  // K
     uint32_t *nshoot = nullptr;    // 1
     uint32_t *counters = nullptr;  // kCntN
+    uint32_t *ctl = nullptr;       // kCtlN: stop test, pass and shooter counts
     uint64_t *sel = nullptr;       // 1 (+ pad)
     double *resid = nullptr;       // [0,1] selection residual, [4,5] rad_residual_fp64
     double *parts = nullptr;       // per-block partials of the selection (3 words per block)
@@ -61,7 +73,7 @@ struct rad_ctx {
     uint64_t *cand = nullptr;      // top-K candidates (two halves)
     uint32_t small_px = 16;        // in-lane rasterization limit (pixels of a tile)
     int sel_blocks = 1024;
-    bool rendered = false;        // keys hold the item buffer of the last render
+    bool rendered = false;        // item_ids hold the item buffer of the last render
     uint32_t rendered_k = 0;
 
     // F row length: npatch, or the largest scene of a pack
@@ -174,27 +186,38 @@ void record(rad_ctx *c, size_t idx)
 }
 
 // Renders the hemicubes of the shooters currently in c->shooters/nshoot
-// (k slots): keys != null writes the item buffer, otherwise the tiles are
-// reduced into F rows (and hit counts when hits != null).
-int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint64_t *keys, uint32_t *hits,
-                const uint32_t *shooters = nullptr, const float *frames = nullptr)
+// (k slots): ids != null writes the item buffer (and its depths when depth
+// != null), otherwise the tiles are reduced into F rows (and hit counts when
+// hits != null); in the deterministic mode through the fixed-point rows and
+// a fold into F.  The F rows that
+// may be dirty are cleared first (all of them, not only the k this pass
+// writes: a render of fewer rows must not leave an earlier writer's rows
+// behind for a later pass that skips its clear).
+int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *depth,
+                uint32_t *hits, const uint32_t *shooters = nullptr, const float *frames = nullptr)
 {
-    const bool keep_ff = c->ff_zero && !hits;
+    uint32_t rows = c->ff_dirty;
+    if (hits && rows < k)
+        rows = k;
     launch_prepare_clear(k, shooters ? shooters : c->shooters, c->nshoot,
                          frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->bb.count,
-                         c->bb.ntiles, keep_ff ? nullptr : c->ff, hits, c->rowlen, c->counters,
-                         c->bb.ent_cap, c->st);
-    c->ff_zero = true;
+                         c->bb.ntiles, c->ff_dirty ? c->ff : nullptr, hits, rows, c->rowlen,
+                         c->counters, c->bb.ent_cap, c->ctl, c->st);
+    c->ff_dirty = 0;
     record(c, evbase + 2);
     launch_raster_setup(c->g, k, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
                         c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->st);
     record(c, evbase + 3);
     launch_bin(k, c->nshoot, c->bb, c->counters, c->st);
     record(c, evbase + 4);
-    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, keys, c->dff, c->rowlen, c->pk.slot_pbeg,
-                keys ? nullptr : c->ff, hits, c->counters, c->st);
-    if (!keys)
-        c->ff_zero = false;
+    const bool fq = !ids && c->det;
+    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, ids, depth, c->dff, c->rowlen,
+                c->pk.slot_pbeg, ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits,
+                c->counters, c->st);
+    if (fq)
+        launch_fold(c->ffq, c->ff, (size_t)k * c->rowlen, c->st);
+    if (!ids)
+        c->ff_dirty = k;
     record(c, evbase + 5);
     return check_launch(c, "render kernels");
 }
@@ -249,15 +272,15 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
         return rc;
     }
     record(c, evbase + 1);
-    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr)) != RAD_OK)
+    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr, nullptr)) != RAD_OK)
         return rc;
     if (c->pk.nscene)
         launch_pack_shoot(c->pk, c->cfg.mode, c->snap, c->ff, c->rowlen, c->refl, c->area,
                           c->npatch, c->unshot, c->shot, c->st);
     else
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl,
-                     c->area, c->npatch, c->unshot, c->shot, c->st);
-    c->ff_zero = true; // the shoot kernels zero the entries they read
+                     c->area, c->npatch, c->unshot, c->shot, c->ctl, c->st);
+    c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
     c->rendered = false;
     return check_launch(c, "shoot");
@@ -347,15 +370,17 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         delete c;
         return RAD_E_HIP;
     }
-    if ((rc = dalloc(c, &c->dff, c->g.RES)) || (rc = dalloc(c, &c->keys, (size_t)c->K * c->g.RES)) ||
+    if ((rc = dalloc(c, &c->dff, c->g.RES)) ||
+        (rc = dalloc(c, &c->item_ids, (size_t)c->K * c->g.RES)) ||
         (rc = dalloc(c, &c->mvp, (size_t)c->K * kShooterStride)) || (rc = dalloc(c, &c->snap, (size_t)c->K * 3)) ||
         (rc = dalloc(c, &c->shooters, c->K)) || (rc = dalloc(c, &c->nshoot, 1)) ||
         (rc = dalloc(c, &c->counters, kCntN)) || (rc = dalloc(c, &c->sel, 4)) ||
-        (rc = dalloc(c, &c->resid, 8))) {
+        (rc = dalloc(c, &c->resid, 8)) || (rc = dalloc(c, &c->ctl, kCtlN))) {
         rad_destroy(c);
         return rc;
     }
     if (hipMemsetAsync(c->counters, 0, kCntN * sizeof(uint32_t), c->st) != hipSuccess ||
+        hipMemsetAsync(c->ctl, 0, kCtlN * sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->nshoot, 0, sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->shooters, 0, c->K * sizeof(uint32_t), c->st) != hipSuccess) {
         rad_destroy(c);
@@ -396,8 +421,10 @@ void rad_destroy(rad_ctx *c)
         (void)hipStreamSynchronize(c->st);
     drop_graph(c);
     void *ptrs[] = {c->tri,  c->tri_patch, c->tsph, c->frames, c->refl,     c->area,   c->emis,
-                    c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->keys,
-                    c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->sel,
+                    c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->item_ids,
+                    c->item_depth, c->ffq, c->runs_slots, c->runs_offs, c->trip_h, c->trip_i,
+                    c->trip_e,
+                    c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->ctl, c->sel,
                     c->resid, c->bb.recs, c->bb.count, c->bb.start, c->bb.fill,
                     c->bb.list, c->bb.blk_nrec, c->bb.bbox, c->bb.clipq, c->cand, c->parts, c->bb.order,
                     c->fhi,  c->flo,      c->pk.sc_tbeg, c->pk.sc_pbeg, c->pk.sc_slots,
@@ -498,6 +525,11 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
         (rc = dalloc(c, &c->unshot, (size_t)npatch * 3)) || (rc = dalloc(c, &c->shot, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->ff, K * rowlen)) || (rc = dalloc(c, &c->hits, K * rowlen)))
         return rc;
+    if (c->det) {
+        if ((rc = dalloc(c, &c->ffq, K * rowlen)))
+            return rc;
+        HIPCHK(c, hipMemset(c->ffq, 0, K * rowlen * 8));
+    }
     // records: one region of kRecsPerBlock per (setup block, shooter) plus a
     // spill region; bin entries: three per record slot
     c->bb.nblk = nblk ? nblk : 1;
@@ -519,7 +551,7 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
         (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
         return rc;
     HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
-    c->ff_zero = true;
+    c->ff_dirty = 0;
     HIPCHK(c, hipMemset(c->hits, 0, K * rowlen * 4));
     c->ntri = ntri;
     c->npatch = npatch;
@@ -712,46 +744,37 @@ int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_
     int rc;
     if ((rc = need_plain(c)) || (rc = upload_shooters(c, shooters, k)))
         return rc;
-    if ((rc = render_pass(c, k, 0, c->keys, nullptr)))
+    if (out_depth && !c->item_depth && (rc = dalloc(c, &c->item_depth, (size_t)c->K * c->g.RES)))
+        return rc;
+    if ((rc = render_pass(c, k, 0, c->item_ids, out_depth ? c->item_depth : nullptr, nullptr)))
         return rc;
     c->rendered = true;
     c->rendered_k = k;
     size_t n = (size_t)k * c->g.RES;
-    if (out_ids || out_depth) {
-        uint32_t *dids = nullptr;
-        float *dz = nullptr;
-        if (out_ids && (rc = dalloc(c, &dids, n)))
-            return rc;
-        if (out_depth && (rc = dalloc(c, &dz, n))) {
-            dfree(dids);
-            return rc;
-        }
-        launch_split_keys(c->keys, dids, dz, n, c->st);
-        if (out_ids)
-            (void)hipMemcpyAsync(out_ids, dids, n * 4, hipMemcpyDeviceToHost, c->st);
-        if (out_depth)
-            (void)hipMemcpyAsync(out_depth, dz, n * 4, hipMemcpyDeviceToHost, c->st);
-        hipError_t e = hipStreamSynchronize(c->st);
-        dfree(dids);
-        dfree(dz);
-        if (e != hipSuccess)
-            return fail(c, RAD_E_HIP, std::string("render copy-out: ") + hipGetErrorString(e));
-    }
+    if (out_ids)
+        HIPCHK(c, hipMemcpyAsync(out_ids, c->item_ids, n * 4, hipMemcpyDeviceToHost, c->st));
+    if (out_depth)
+        HIPCHK(c, hipMemcpyAsync(out_depth, c->item_depth, n * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return check_overflow(c);
 }
 
 static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
 {
-    size_t nf = (size_t)k * c->npatch;
-    HIPCHK(c, hipMemsetAsync(c->ff, 0, nf * 4, c->st));
-    HIPCHK(c, hipMemsetAsync(c->hits, 0, nf * 4, c->st));
+    // F rows [0, max(k, ff_dirty)) and hit rows [0, k) start from zero
+    uint32_t rows = c->ff_dirty > k ? c->ff_dirty : k;
+    HIPCHK(c, hipMemsetAsync(c->ff, 0, (size_t)rows * c->rowlen * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(c->hits, 0, (size_t)k * c->npatch * 4, c->st));
     HIPCHK(c, hipMemsetAsync(c->counters, 0, kCntN * 4, c->st));
-    launch_reduce(c->g, k, c->nshoot, c->keys, c->dff, c->npatch, c->ff, c->hits, c->counters, c->st);
-    c->ff_zero = false;
+    launch_reduce(c->g, k, c->nshoot, c->item_ids, c->dff, c->npatch, c->ff,
+                  c->det ? c->ffq : nullptr, c->hits, c->counters, c->st);
+    if (c->det)
+        launch_fold(c->ffq, c->ff, (size_t)k * c->npatch, c->st);
+    c->ff_dirty = k;
     int rc;
     if ((rc = check_launch(c, "reduce")))
         return rc;
+    size_t nf = (size_t)k * c->npatch;
     if (out_hits)
         HIPCHK(c, hipMemcpyAsync(out_hits, c->hits, nf * 4, hipMemcpyDeviceToHost, c->st));
     if (out_ff)
@@ -783,19 +806,72 @@ int rad_process_itembuf(rad_ctx *c, const uint32_t *itembuf, uint32_t k, uint32_
     if (k < 1 || k > c->K)
         return fail(c, RAD_E_ARG, "process_itembuf: k must be in [1, hemicubes]");
     size_t n = (size_t)k * c->g.RES;
-    uint32_t *dids = nullptr;
-    if ((rc = dalloc(c, &dids, n)))
-        return rc;
-    (void)hipMemcpyAsync(dids, itembuf, n * 4, hipMemcpyHostToDevice, c->st);
-    launch_ids_to_keys(dids, c->keys, n, c->st);
+    HIPCHK(c, hipMemcpyAsync(c->item_ids, itembuf, n * 4, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->nshoot, &k, 4, hipMemcpyHostToDevice, c->st));
-    hipError_t e = hipStreamSynchronize(c->st);
-    dfree(dids);
-    if (e != hipSuccess)
-        return fail(c, RAD_E_HIP, std::string("process_itembuf upload: ") + hipGetErrorString(e));
+    // k must outlive the async copy
+    HIPCHK(c, hipStreamSynchronize(c->st));
     c->rendered = true;
     c->rendered_k = k;
     return reduce_out(c, k, out_hits, out_ff);
+}
+
+int rad_process_runs(rad_ctx *c, uint32_t k, uint32_t cap, uint32_t *out_hemicubes,
+                     uint32_t *out_ids, float *out_energies, uint32_t *out_count)
+{
+    if (!c || !out_count)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_plain(c)))
+        return rc;
+    if (!c->rendered || k != c->rendered_k)
+        return fail(c, RAD_E_STATE, "process_runs: k must match the last render");
+    if (cap && (!out_hemicubes || !out_ids || !out_energies))
+        return fail(c, RAD_E_ARG, "process_runs: null output arrays");
+    const uint32_t wx = c->cfg.ocl_workitems_x;
+    const uint32_t nx = wx < c->g.TW ? wx : c->g.TW; // Config.cpp:43-45
+    const uint64_t items64 = (uint64_t)k * c->g.TH * nx;
+    if (items64 >= (1ull << 31))
+        return fail(c, RAD_E_LIMIT, "process_runs: too many work-items");
+    const uint32_t items = (uint32_t)items64;
+    if (items > c->runs_items) {
+        if ((rc = dalloc(c, &c->runs_slots, items)) || (rc = dalloc(c, &c->runs_offs, items + 1))) {
+            c->runs_items = 0;
+            return rc;
+        }
+        c->runs_items = items;
+    }
+    launch_runs_count(c->g, k, nx, c->item_ids, c->runs_slots, c->runs_offs, c->st);
+    if ((rc = check_launch(c, "process_runs count")))
+        return rc;
+    uint32_t total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->runs_offs + items, 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    *out_count = total;
+    if (total > cap)
+        return fail(c, RAD_E_LIMIT, "process_runs: " + std::to_string(total) +
+                                        " triplets, output arrays hold " + std::to_string(cap));
+    if (total > c->trip_cap) {
+        if ((rc = dalloc(c, &c->trip_h, total)) || (rc = dalloc(c, &c->trip_i, total)) ||
+            (rc = dalloc(c, &c->trip_e, total))) {
+            c->trip_cap = 0;
+            return rc;
+        }
+        c->trip_cap = total;
+    }
+    launch_runs_write(c->g, k, nx, c->item_ids, c->dff, c->runs_offs, total, c->trip_h, c->trip_i,
+                      c->trip_e, c->st);
+    if ((rc = check_launch(c, "process_runs write")))
+        return rc;
+    if (total) {
+        HIPCHK(c, hipMemcpyAsync(out_hemicubes, c->trip_h, (size_t)total * 4,
+                                 hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(out_ids, c->trip_i, (size_t)total * 4, hipMemcpyDeviceToHost,
+                                 c->st));
+        HIPCHK(c, hipMemcpyAsync(out_energies, c->trip_e, (size_t)total * 4,
+                                 hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return RAD_OK;
 }
 
 int rad_shoot(rad_ctx *c, const uint32_t *shooters, uint32_t k)
@@ -860,12 +936,12 @@ static int step_impl(rad_ctx *c, uint32_t passes)
         // captured passes assume zero F rows at their start (the first one
         // skips clearing them); every replay ends with a shoot, which leaves
         // them zero again
-        if (!c->ff_zero) {
-            launch_clear(c->K, c->bb.count, c->bb.ntiles, c->ff, nullptr, c->rowlen, c->counters,
-                         c->bb.ent_cap, c->st);
+        if (c->ff_dirty) {
+            launch_clear(c->K, c->bb.count, c->bb.ntiles, c->ff, nullptr, c->ff_dirty, c->rowlen,
+                         c->counters, c->bb.ent_cap, c->st);
             if ((rc = check_launch(c, "clear")))
                 return rc;
-            c->ff_zero = true;
+            c->ff_dirty = 0;
         }
         if (!c->gexec || c->gpasses != G)
             if ((rc = capture_graph(c, G)))
@@ -933,6 +1009,50 @@ int rad_residual_fp64(rad_ctx *c, double *out)
     return RAD_OK;
 }
 
+} // extern "C"
+
+namespace {
+
+// The control block as the host sees it after a synchronisation.
+struct CtlView {
+    uint64_t shooters = 0, passes = 0;
+    float last[3] = {0, 0, 0};
+    bool stopped = false;
+};
+
+int read_ctl(rad_ctx *c, CtlView &v)
+{
+    uint32_t w[kCtlN];
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(w, c->ctl, sizeof(w), hipMemcpyDeviceToHost));
+    v.stopped = w[kCtlStop] != 0;
+    memcpy(v.last, &w[kCtlLastE], 12);
+    v.shooters = (uint64_t)w[kCtlShotLo] | ((uint64_t)w[kCtlShotLo + 1] << 32);
+    v.passes = (uint64_t)w[kCtlPassLo] | ((uint64_t)w[kCtlPassLo + 1] << 32);
+    return RAD_OK;
+}
+
+// Arms (threshold > 0) or disarms (0) the stop test; clears the stop flag.
+int set_stop(rad_ctx *c, double threshold)
+{
+    uint32_t w[4] = {0u, 0u, 0u, 0u}; // words kCtlStop .. kCtlThresh + 1
+    static_assert(kCtlStop == 0 && kCtlThresh == 2, "control block layout");
+    memcpy(&w[kCtlThresh], &threshold, 8);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->ctl, w, sizeof(w), hipMemcpyHostToDevice));
+    return RAD_OK;
+}
+
+float length3(const float *v)
+{
+    // Vector3f::f_Length (Vector.h:351-354): the sum of squares in float
+    return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+} // namespace
+
+extern "C" {
+
 int rad_run(rad_ctx *c, uint64_t max_passes, double target, uint32_t check_every, rad_stats *out)
 {
     if (!c)
@@ -942,6 +1062,9 @@ int rad_run(rad_ctx *c, uint64_t max_passes, double target, uint32_t check_every
         return rc;
     if (check_every == 0)
         check_every = c->cfg.shoots ? c->cfg.shoots : 500;
+    CtlView v0;
+    if ((rc = read_ctl(c, v0)))
+        return rc;
     auto t0 = std::chrono::steady_clock::now();
     uint64_t done = 0;
     double r = 0.0;
@@ -959,21 +1082,82 @@ int rad_run(rad_ctx *c, uint64_t max_passes, double target, uint32_t check_every
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     auto t1 = std::chrono::steady_clock::now();
+    CtlView v1;
+    if ((rc = read_ctl(c, v1)))
+        return rc;
     if (out) {
         out->passes = done;
-        out->shooters = done * c->K;
+        out->shooters = v1.shooters - v0.shooters;
         out->residual = r;
         out->seconds = std::chrono::duration<double>(t1 - t0).count();
-        float sn[3] = {0, 0, 0};
-        if (done) {
-            uint32_t n = 0;
-            (void)hipMemcpy(&n, c->nshoot, 4, hipMemcpyDeviceToHost);
-            if (n)
-                (void)hipMemcpy(sn, c->snap + 3 * (n - 1), 12, hipMemcpyDeviceToHost);
-        }
-        out->last_energy = std::sqrt(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]);
+        out->last_energy = done ? length3(v1.last) : 0.0f;
         out->reserved = 0;
     }
+    return RAD_OK;
+}
+
+int rad_run_cycle(rad_ctx *c, uint32_t passes, double stop_energy, rad_cycle_stats *out)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_plain(c)))
+        return rc;
+    if (!(stop_energy >= 0.0) || !std::isfinite(stop_energy))
+        return fail(c, RAD_E_ARG, "run_cycle: stop_energy must be finite and >= 0");
+    CtlView v0, v1;
+    if ((rc = read_ctl(c, v0)) || (rc = set_stop(c, stop_energy)))
+        return rc;
+    rc = step_impl(c, passes);
+    int rc2 = read_ctl(c, v1);
+    int rc3 = set_stop(c, 0.0);
+    if (rc || (rc = rc2) || (rc = rc3) || (rc = check_overflow(c)))
+        return rc;
+    if (out) {
+        out->passes = v1.passes - v0.passes;
+        out->shooters = v1.shooters - v0.shooters;
+        for (int k = 0; k < 3; ++k)
+            out->last_energy[k] = out->passes ? v1.last[k] : 0.0f;
+        out->stopped = v1.stopped ? 1u : 0u;
+    }
+    return RAD_OK;
+}
+
+int rad_set_deterministic(rad_ctx *c, int on)
+{
+    if (!c)
+        return RAD_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (!on) {
+        dfree(c->ffq);
+        c->ffq = nullptr;
+        c->det = false;
+        return RAD_OK;
+    }
+    if (!c->ffq && c->rowlen) {
+        size_t n = (size_t)c->K * c->rowlen;
+        int rc;
+        if ((rc = dalloc(c, &c->ffq, n)))
+            return rc;
+        HIPCHK(c, hipMemset(c->ffq, 0, n * 8));
+    }
+    drop_graph(c); // captured passes name the F accumulation buffers
+    c->det = true;
+    return RAD_OK;
+}
+
+int rad_shot_count(rad_ctx *c, uint64_t *passes, uint64_t *shooters)
+{
+    if (!c)
+        return RAD_E_ARG;
+    CtlView v;
+    int rc;
+    if ((rc = read_ctl(c, v)))
+        return rc;
+    if (passes)
+        *passes = v.passes;
+    if (shooters)
+        *shooters = v.shooters;
     return RAD_OK;
 }
 
@@ -1054,10 +1238,10 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
             for (uint32_t si = 0; si < 5 && rc == RAD_OK && r.ncand < RAD_TUNE_MAX; ++si) {
                 if ((rc = set_tiles(c, tiles[ti], smalls[si])))
                     break;
-                rc = render_pass(c, c->K, 0, nullptr, nullptr); // warm
+                rc = render_pass(c, c->K, 0, nullptr, nullptr, nullptr); // warm
                 (void)hipEventRecord(e0, c->st);
                 for (uint32_t p = 0; p < passes && rc == RAD_OK; ++p)
-                    rc = render_pass(c, c->K, 0, nullptr, nullptr);
+                    rc = render_pass(c, c->K, 0, nullptr, nullptr, nullptr);
                 (void)hipEventRecord(e1, c->st);
                 if (rc)
                     break;
@@ -1094,6 +1278,21 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
         return rc2;
     if (out)
         *out = r;
+    return RAD_OK;
+}
+
+int rad_debug_counters(rad_ctx *c, uint32_t *out8, uint32_t *caps)
+{
+    if (!c || !out8)
+        return RAD_E_ARG;
+    static_assert(kCntN == 8, "eight pass counters");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(out8, c->counters, kCntN * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (caps) {
+        caps[0] = c->bb.spill_cap;
+        caps[1] = c->bb.ent_cap;
+        caps[2] = c->bb.clip_cap;
+    }
     return RAD_OK;
 }
 
@@ -1227,7 +1426,7 @@ int rad_dense_build(rad_ctx *c, uint32_t layout)
     for (uint32_t base = 0; base < c->npatch; base += c->K) {
         uint32_t k = c->npatch - base < c->K ? c->npatch - base : c->K;
         launch_iota_shooters(base, k, c->shooters, c->nshoot, c->st);
-        if ((rc = render_pass(c, k, 0, nullptr, nullptr)))
+        if ((rc = render_pass(c, k, 0, nullptr, nullptr, nullptr)))
             break;
         if (layout == RAD_DENSE_ROWS)
             launch_dense_store(c->ff, k, base, c->npatch, c->npad, c->fhi, c->flo, c->st);
@@ -1914,19 +2113,12 @@ int rad_render_view(rad_ctx *c, const float *frame, uint32_t *out_ids)
     HIPCHK(c, hipMemcpyAsync(c->viewfr, frame, 36, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->view_sh, &zero, 4, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->nshoot, &one, 4, hipMemcpyHostToDevice, c->st));
-    if ((rc = render_pass(c, 1, 0, c->keys, nullptr, c->view_sh, c->viewfr)))
+    if ((rc = render_pass(c, 1, 0, c->item_ids, nullptr, nullptr, c->view_sh, c->viewfr)))
         return rc;
-    c->rendered = false; // the keys do not hold a shooter's hemicube
-    uint32_t *dids = nullptr;
-    if ((rc = dalloc(c, &dids, c->g.RES)))
-        return rc;
-    launch_split_keys(c->keys, dids, nullptr, c->g.RES, c->st);
-    hipError_t e = hipMemcpyAsync(out_ids, dids, (size_t)c->g.RES * 4, hipMemcpyDeviceToHost, c->st);
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(c->st);
-    dfree(dids);
-    if (e != hipSuccess)
-        return fail(c, RAD_E_HIP, std::string("render_view: ") + hipGetErrorString(e));
+    c->rendered = false; // the item buffer does not hold a shooter's hemicube
+    HIPCHK(c, hipMemcpyAsync(out_ids, c->item_ids, (size_t)c->g.RES * 4, hipMemcpyDeviceToHost,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return check_overflow(c);
 }
 

@@ -179,7 +179,7 @@ int main(int argc, const char **argv)
     float energy = 0.0f;
     while (cycles < max_cycles) {
         energy = shooter.shootCycle();
-        if (!lastError().empty() && energy == 0.0f) {
+        if (shooter.failed()) {
             fprintf(stderr, "radhip: %s\n", lastError().c_str());
             return 1;
         }

@@ -647,18 +647,27 @@ Shooter::Shooter(ModelContainer *s) : scene(s) {}
 float Shooter::shootCycle()
 {
     rad_ctx *c = scene ? scene->context() : nullptr;
-    if (!c)
-        return 0.0f;
+    if (!c) {
+        error = true;
+        setErr("Shooter::shootCycle: the scene has no device context");
+        return -1.0f;
+    }
+    if (stopped)
+        return lastEnergy;
     scene->pushState();
-    rad_stats st{};
-    // a target below any residual: exactly SHOOTS_PER_CYCLE passes
-    if (libErr(c, rad_run(c, Config::SHOOTS_PER_CYCLE(), -1.0, Config::SHOOTS_PER_CYCLE(), &st),
-               "rad_run") != RAD_OK)
-        return 0.0f;
+    rad_cycle_stats st{};
+    // the stop test of Main.cpp:1298 runs on the device after every pass
+    if (libErr(c, rad_run_cycle(c, Config::SHOOTS_PER_CYCLE(), 0.1, &st), "rad_run_cycle") !=
+        RAD_OK) {
+        error = true;
+        return -1.0f;
+    }
     scene->pullState();
-    lastEnergy = st.last_energy;
+    lastEnergy = Vector3f(st.last_energy[0], st.last_energy[1], st.last_energy[2]).f_Length();
+    stopped = st.stopped != 0;
     ++cycles;
     shot += st.shooters;
+    npasses += st.passes;
     return lastEnergy;
 }
 
@@ -676,23 +685,20 @@ unsigned int ProcessHemicube(ModelContainer *scene, const uint32_t *itembuf,
                              float *p_energies)
 {
     rad_ctx *c = scene ? scene->context() : nullptr;
-    if (!c || !itembuf)
+    if (!c || !itembuf || !p_hemicubes || !p_ids || !p_energies) {
+        setErr("ProcessHemicube: no device context or null arrays");
         return 0;
-    size_t np = scene->getPatchesCount();
-    std::vector<uint32_t> hits(n_hemicubes * np);
-    std::vector<float> ff(n_hemicubes * np);
-    if (libErr(c, rad_process_itembuf(c, itembuf, n_hemicubes, hits.data(), ff.data()),
+    }
+    // the item buffer goes to the device (the reduction into F rows there is
+    // not read), then the kernel's triplet stream comes back
+    if (libErr(c, rad_process_itembuf(c, itembuf, n_hemicubes, nullptr, nullptr),
                "rad_process_itembuf") != RAD_OK)
         return 0;
-    unsigned n = 0;
-    for (unsigned h = 0; h < n_hemicubes; ++h)
-        for (size_t j = 0; j < np; ++j)
-            if (hits[h * np + j]) {
-                p_hemicubes[n] = h;
-                p_ids[n] = (uint32_t)j;
-                p_energies[n] = ff[h * np + j];
-                ++n;
-            }
+    uint32_t n = 0;
+    const uint32_t cap = Config::PATCHVIEW_TEX_RES() * n_hemicubes; // Main.cpp:605-607
+    if (libErr(c, rad_process_runs(c, n_hemicubes, cap, p_hemicubes, p_ids, p_energies, &n),
+               "rad_process_runs") != RAD_OK)
+        return 0;
     return n;
 }
 

@@ -51,6 +51,18 @@ enum {
     kCntN = 8
 };
 
+// Control block of a context (u32 words on the device), written by the
+// shoot kernels and read by the pass gate in k_prepare_clear:
+enum {
+    kCtlStop = 0,      // the stop test fired: later passes are no-ops (nshoot = 0)
+    kCtlThresh = 2,    // [2..3] stop threshold on |lastEnergy| (double; 0: off)
+    kCtlLastE = 4,     // [4..6] unshot RGB of the pass's last shooter before its
+                       // emitter update (lastEnergy, Main.cpp:1290-1292), float bits
+    kCtlShotLo = 8,    // [8..9] shooters shot (u64, low word first)
+    kCtlPassLo = 10,   // [10..11] passes that shot at least one shooter (u64)
+    kCtlN = 16
+};
+
 // Bins of the sort-middle rasterizer: one per (shooter slot, tile).
 struct BinBuffers {
     RasterRec *recs;       // nblk*K regions of kRecsPerBlock, then spill_cap
@@ -113,15 +125,21 @@ void launch_select_reference(const float *unshot, uint32_t npatch, uint32_t k,
 void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
                     const uint32_t *nshoot, const float *frames, const float *unshot,
                     float *snap, float *mvp, hipStream_t st);
-void launch_prepare_clear(uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
+// Prepare + clear of a pass: ff_rows rows of rowlen entries of F (and hits)
+// are zeroed when ff (hits) is non-null; ctl (may be null) gates a stopped
+// context.
+void launch_prepare_clear(uint32_t K, const uint32_t *shooters, uint32_t *nshoot,
                           const float *frames, const float *unshot, float *snap, float *mvp,
                           uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
-                          uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st);
+                          uint32_t ff_rows, uint32_t rowlen, uint32_t *counters, uint32_t ent_cap,
+                          const uint32_t *ctl, hipStream_t st);
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
-                  uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st);
+                  uint32_t ff_rows, uint32_t rowlen, uint32_t *counters, uint32_t ent_cap,
+                  hipStream_t st);
 // Binned render of the nshoot (<= K) hemicubes in three launches:
-// setup (records + bin counts), bin (scan + fill), tile (LDS depth test; keys
-// != null writes the item buffer, ff != null reduces into F rows and hits).
+// setup (records + bin counts), bin (scan + fill), tile (LDS depth test;
+// out_ids != null writes the item buffer, ff or ffq != null reduces into F
+// rows and hits).
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
@@ -130,19 +148,27 @@ void launch_tri_sphere(const float *tri, uint32_t ntri, float4 *sph, hipStream_t
 void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
                 hipStream_t st);
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
-                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t rowlen,
-                 const uint32_t *slot_pbeg, float *ff, uint32_t *hits, uint32_t *counters,
-                 hipStream_t st);
-void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot,
-                   const uint64_t *keys, const float *dff, uint32_t npatch, float *ff,
+                 uint32_t small_px, uint32_t *out_ids, float *out_depth, const float *dff,
+                 uint32_t rowlen, const uint32_t *slot_pbeg, float *ff, unsigned long long *ffq,
+                 uint32_t *hits, uint32_t *counters, hipStream_t st);
+// Deterministic mode: fixed-point F entries (2^-52 units) into the float rows.
+void launch_fold(unsigned long long *ffq, float *ff, size_t n, hipStream_t st);
+// ProcessHemicube over an id plane (K x RES u32): ff (or ffq) and hits +=.
+void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot, const uint32_t *ids,
+                   const float *dff, uint32_t npatch, float *ff, unsigned long long *ffq,
                    uint32_t *hits, uint32_t *counters, hipStream_t st);
+// The run-form triplets of ProcessHemicube: slot counts per work-item and
+// their exclusive scan (offs: nitems + 1 entries, offs[nitems] = total), then
+// the triplets at those offsets (entries >= cap skipped).
+void launch_runs_count(const Geometry &g, uint32_t k, uint32_t nx, const uint32_t *item,
+                       uint32_t *slots, uint32_t *offs, hipStream_t st);
+void launch_runs_write(const Geometry &g, uint32_t k, uint32_t nx, const uint32_t *item,
+                       const float *dff, const uint32_t *offs, uint32_t cap, uint32_t *th,
+                       uint32_t *tid, float *te, hipStream_t st);
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
                   const uint32_t *nshoot, const float *snap, float *ff,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
-                  float *shot, hipStream_t st);
-void launch_ids_to_keys(const uint32_t *ids, uint64_t *keys, size_t n, hipStream_t st);
-void launch_split_keys(const uint64_t *keys, uint32_t *ids, float *depth, size_t n,
-                       hipStream_t st);
+                  float *shot, uint32_t *ctl, hipStream_t st);
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);

@@ -182,9 +182,9 @@ __device__ static void prepare_item(uint32_t t, uint32_t K, const uint32_t *__re
                                     const uint32_t *__restrict__ nshoot,
                                     const float *__restrict__ frames,
                                     const float *__restrict__ unshot, float *__restrict__ snap,
-                                    float *__restrict__ mvp)
+                                    float *__restrict__ mvp, const uint32_t *__restrict__ ctl)
 {
-    if (t >= K * 5)
+    if (t >= K * 5 || (ctl && ctl[kCtlStop]))
         return;
     uint32_t s = t / 5, face = t % 5;
     if (s >= *nshoot)
@@ -227,7 +227,7 @@ __global__ void k_prepare(uint32_t K, const uint32_t *__restrict__ shooters,
                           float *__restrict__ mvp)
 {
     prepare_item(blockIdx.x * blockDim.x + threadIdx.x, K, shooters, nshoot, frames, unshot, snap,
-                 mvp);
+                 mvp, nullptr);
 }
 
 void launch_prepare(const Geometry &g, uint32_t K, const uint32_t *shooters,
@@ -279,30 +279,39 @@ __global__ void k_clear(uint32_t *__restrict__ bin_count, size_t nbins, float *_
 // k_prepare and k_clear of a pass in one launch (the first np workgroups
 // prepare, the others clear): both are small, so one launch saves a kernel's
 // start-up and drain.
+// Pass gate (the loop condition of Main.cpp:1137): once the stop test of a
+// previous shoot has fired (ctl[kCtlStop]), the pass selects nobody: nshoot
+// is set to 0 for the later kernels of the pass, and the prepare items of
+// this launch return at once (they read the flag themselves rather than the
+// nshoot this launch writes).
 __global__ __launch_bounds__(256) void k_prepare_clear(
     uint32_t np, uint32_t K, const uint32_t *__restrict__ shooters,
-    const uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
+    uint32_t *__restrict__ nshoot, const float *__restrict__ frames,
     const float *__restrict__ unshot, float *__restrict__ snap, float *__restrict__ mvp,
     uint32_t *__restrict__ bin_count, size_t nbins, float *__restrict__ ff,
-    uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters, uint32_t ent_cap)
+    uint32_t *__restrict__ hits, size_t nff, uint32_t *__restrict__ counters, uint32_t ent_cap,
+    const uint32_t *__restrict__ ctl)
 {
     if (blockIdx.x < np) {
         prepare_item(blockIdx.x * blockDim.x + threadIdx.x, K, shooters, nshoot, frames, unshot,
-                     snap, mvp);
+                     snap, mvp, ctl);
         return;
     }
+    if (ctl && blockIdx.x == np && threadIdx.x == 0 && ctl[kCtlStop])
+        *nshoot = 0u;
     clear_range((size_t)(blockIdx.x - np) * blockDim.x + threadIdx.x,
                 (size_t)(gridDim.x - np) * blockDim.x, bin_count, nbins, ff, hits, nff, counters,
                 ent_cap);
 }
 
-void launch_prepare_clear(uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
+void launch_prepare_clear(uint32_t K, const uint32_t *shooters, uint32_t *nshoot,
                           const float *frames, const float *unshot, float *snap, float *mvp,
                           uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
-                          uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st)
+                          uint32_t ff_rows, uint32_t rowlen, uint32_t *counters, uint32_t ent_cap,
+                          const uint32_t *ctl, hipStream_t st)
 {
     const uint32_t np = (K * 5 + 255) / 256;
-    size_t nbins = (size_t)K * ntiles, nff = ff || hits ? (size_t)K * npatch : 0;
+    size_t nbins = (size_t)K * ntiles, nff = ff || hits ? (size_t)ff_rows * rowlen : 0;
     size_t n = nbins > nff ? nbins : nff;
     size_t blocks = (n + 255) / 256;
     if (blocks > 8192)
@@ -311,14 +320,15 @@ void launch_prepare_clear(uint32_t K, const uint32_t *shooters, const uint32_t *
         blocks = 1;
     k_prepare_clear<<<(unsigned)(np + blocks), 256, 0, st>>>(np, K, shooters, nshoot, frames,
                                                             unshot, snap, mvp, bin_count, nbins,
-                                                            ff, hits, nff, counters, ent_cap);
+                                                            ff, hits, nff, counters, ent_cap, ctl);
 }
 
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
-                  uint32_t npatch, uint32_t *counters, uint32_t ent_cap, hipStream_t st)
+                  uint32_t ff_rows, uint32_t rowlen, uint32_t *counters, uint32_t ent_cap,
+                  hipStream_t st)
 {
     // ff null: the F rows are known to be zero (the last shoot cleared them)
-    size_t nbins = (size_t)K * ntiles, nff = ff || hits ? (size_t)K * npatch : 0;
+    size_t nbins = (size_t)K * ntiles, nff = ff || hits ? (size_t)ff_rows * rowlen : 0;
     size_t n = nbins > nff ? nbins : nff;
     size_t blocks = (n + 255) / 256;
     if (blocks > 8192)
@@ -337,8 +347,9 @@ void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, u
 //  k_bin_scan      exclusive scan of the bin counts
 //  k_bin_fill      record indices into their bins
 //  k_tile_raster   one workgroup per tile: depth test in LDS with 64-bit
-//                  atomicMin, then either writes the tile's keys (item buffer
-//                  out) or reduces it into form factors (ProcessHemicube fused)
+//                  atomicMin, then either writes the tile's ids and depths
+//                  (item buffer out) or reduces it into form factors
+//                  (ProcessHemicube fused)
 struct CV4 {
     float c[4];
 };
@@ -1212,8 +1223,18 @@ __device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c, bool
 // texels make runs long).  Every lane of the wavefront must call it.
 // F row entries are id - 1 - pbase for ids in [pbase + 1, pbase + rowlen]
 // (pbase: first patch of the shooter's scene in a packed launch, else 0).
+// Deterministic mode: a run's sum in fixed point, units of 2^-52 (exact for
+// sums from 2^-29 up, well below 1e-7 relative under that; F < 4096), added
+// with 64-bit integer atomics, so the total does not depend on their order.
+__device__ static inline unsigned long long fixed52(float f)
+{
+    return __float2ull_rn(f * 4503599627370496.0f);
+}
+
+// ffq non-null: the runs go to the fixed-point rows instead of ff.
 __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t pbase,
                                           uint32_t rowlen, float *__restrict__ ff,
+                                          unsigned long long *__restrict__ ffq,
                                           uint32_t *__restrict__ hits,
                                           uint32_t *__restrict__ counters)
 {
@@ -1222,7 +1243,10 @@ __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uin
         uint32_t p = id - 1 - pbase;
         if (p < rowlen) {
             size_t o = (size_t)hi * rowlen + p;
-            atomicAdd(&ff[o], f);
+            if (ffq)
+                atomicAdd(&ffq[o], fixed52(f));
+            else
+                atomicAdd(&ff[o], f);
             if (hits)
                 atomicAdd(&hits[o], c);
         } else {
@@ -1362,17 +1386,19 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
 // atomicMin, so the drawing order does not matter; the atomic returns nothing
 // and is issued without first reading the texel, so no fragment waits on an
 // LDS read.  Then the tile either goes
-// to the key buffer (keys != null) or is reduced into F rows and hit counts
-// (ff != null).
+// to the item buffer (out_ids != null; out_depth != null adds the depth
+// plane) or is reduced into F rows and hit counts (ff or, in the
+// deterministic mode, its fixed-point rows ffq != null).
 __global__ __launch_bounds__(256) void k_tile_raster(
     uint32_t W, uint32_t TW, uint32_t TH, uint32_t RES, const uint32_t *__restrict__ nshoot,
     const RasterRec *__restrict__ recs, const uint64_t *__restrict__ bbox,
     const uint32_t *__restrict__ bin_count, const uint32_t *__restrict__ bin_start,
     const uint32_t *__restrict__ list, uint32_t ent_cap, uint32_t ntx, uint32_t ntiles,
-    uint32_t tshift, uint32_t small_px, uint64_t *__restrict__ keys, const float *__restrict__ dff,
+    uint32_t tshift, uint32_t small_px, uint32_t *__restrict__ out_ids,
+    float *__restrict__ out_depth, const float *__restrict__ dff,
     uint32_t rowlen, const uint32_t *__restrict__ slot_pbeg, float *__restrict__ ff,
-    uint32_t *__restrict__ hits, uint32_t *__restrict__ counters,
-    const uint32_t *__restrict__ order)
+    unsigned long long *__restrict__ ffq, uint32_t *__restrict__ hits,
+    uint32_t *__restrict__ counters, const uint32_t *__restrict__ order)
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     __shared__ uint32_t perm[256];
@@ -1440,12 +1466,15 @@ __global__ __launch_bounds__(256) void k_tile_raster(
         bool inside = p < TT && tx < (int32_t)TW && ty < (int32_t)TH;
         uint64_t key = inside ? tk[p] : kClearKey;
         uint32_t texel = (uint32_t)ty * TW + (uint32_t)tx;
-        if (keys && inside)
-            keys[(size_t)s * RES + texel] = key;
-        if (ff) {
+        if (out_ids && inside) {
+            out_ids[(size_t)s * RES + texel] = (uint32_t)key;
+            if (out_depth)
+                out_depth[(size_t)s * RES + texel] = __uint_as_float((uint32_t)(key >> 32));
+        }
+        if (ff || ffq) {
             uint32_t id = (uint32_t)key;
             float f = id ? dff[texel] : 0.0f;
-            reduce_wave(id, f, s, slot_pbeg ? slot_pbeg[s] : 0u, rowlen, ff, hits, counters);
+            reduce_wave(id, f, s, slot_pbeg ? slot_pbeg[s] : 0u, rowlen, ff, ffq, hits, counters);
         }
     }
 }
@@ -1497,36 +1526,72 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
 }
 
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
-                 uint32_t small_px, uint64_t *keys, const float *dff, uint32_t rowlen,
-                 const uint32_t *slot_pbeg, float *ff, uint32_t *hits, uint32_t *counters,
-                 hipStream_t st)
+                 uint32_t small_px, uint32_t *out_ids, float *out_depth, const float *dff,
+                 uint32_t rowlen, const uint32_t *slot_pbeg, float *ff, unsigned long long *ffq,
+                 uint32_t *hits, uint32_t *counters, hipStream_t st)
 {
     k_tile_raster<<<bb.ntiles * K, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox,
                                                  bb.count, bb.start, bb.list, bb.ent_cap, bb.ntx,
-                                                 bb.ntiles, bb.tshift, small_px, keys, dff, rowlen,
-                                                 slot_pbeg, ff, hits, counters, bb.order);
+                                                 bb.ntiles, bb.tshift, small_px, out_ids, out_depth,
+                                                 dff, rowlen, slot_pbeg, ff, ffq, hits, counters,
+                                                 bb.order);
+}
+
+// Deterministic mode: n fixed-point entries (units of 2^-52) into the float
+// rows, which are zero where the entries are nonzero; the fixed-point
+// entries are left zero.
+__global__ __launch_bounds__(256) void k_fold(unsigned long long *__restrict__ ffq,
+                                              float *__restrict__ ff, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned long long q = ffq[i];
+        if (q) {
+            ff[i] = (float)((double)q * 0x1p-52);
+            ffq[i] = 0ull;
+        }
+    }
+}
+
+void launch_fold(unsigned long long *ffq, float *ff, size_t n, hipStream_t st)
+{
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    if (blocks)
+        k_fold<<<(unsigned)blocks, 256, 0, st>>>(ffq, ff, n);
 }
 
 // ----------------------------------------------------------- ProcessHemicube
-// A workgroup takes kReduceChunk consecutive texels of one hemicube, one
-// texel per lane per step.  Equal ids in a wavefront's 64 consecutive texels
-// form runs (the span walk of Kernel_ProcessHemicube.h:36-49) summed with a
-// segmented scan; each run's ΔFF sum and pixel count go into an LDS hash
-// table keyed by id, which also merges the runs of the chunk's other rows;
-// the table is then flushed with one global atomic per distinct id (the host
-// accumulation of Main.cpp:1257-1269 fused in).  A run that finds no free
-// slot adds to the global entry directly.
+// A workgroup takes kReduceChunk consecutive texels of one hemicube's id
+// plane (4 bytes per texel), one texel per lane per step.  Equal ids in a
+// wavefront's 64 consecutive texels form runs (the span walk of
+// Kernel_ProcessHemicube.h:36-49) summed with a segmented scan; each run's
+// ΔFF sum and pixel count go into an LDS hash table keyed by id, which also
+// merges the runs of the chunk's other rows; the table is then flushed with
+// one global atomic per distinct id (the host accumulation of
+// Main.cpp:1257-1269 fused in).  A run that finds no free slot adds to the
+// global entry directly.  Acc is float, or the 2^-52 fixed point of the
+// deterministic mode (LDS and global sums in 64-bit integers).
 constexpr uint32_t kReduceChunk = 4096, kReduceSlots = 2048, kReduceBatch = 8;
 
+template <typename Acc> __device__ static inline Acc reduce_acc(float f);
+template <> __device__ inline float reduce_acc<float>(float f) { return f; }
+template <> __device__ inline unsigned long long reduce_acc<unsigned long long>(float f)
+{
+    return fixed52(f);
+}
+
+template <typename Acc>
 __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__restrict__ nshoot,
-                                                const uint64_t *__restrict__ keys,
+                                                const uint32_t *__restrict__ ids_in,
                                                 const float *__restrict__ dff, uint32_t npatch,
-                                                float *__restrict__ ff,
+                                                Acc *__restrict__ ff,
                                                 uint32_t *__restrict__ hits,
                                                 uint32_t *__restrict__ counters)
 {
     __shared__ uint32_t hkey[kReduceSlots], hcnt[kReduceSlots];
-    __shared__ float hsum[kReduceSlots];
+    __shared__ Acc hsum[kReduceSlots];
     const uint32_t chunks = (RES + kReduceChunk - 1) / kReduceChunk;
     const uint32_t hi = blockIdx.x / chunks, c0 = (blockIdx.x - hi * chunks) * kReduceChunk;
     if (hi >= *nshoot)
@@ -1534,12 +1599,12 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
     for (uint32_t q = threadIdx.x; q < kReduceSlots; q += blockDim.x) {
         hkey[q] = 0u;
         hcnt[q] = 0u;
-        hsum[q] = 0.0f;
+        hsum[q] = Acc(0);
     }
     __syncthreads();
-    const uint64_t *kp = keys + (size_t)hi * RES;
+    const uint32_t *ip = ids_in + (size_t)hi * RES;
     const uint32_t c1 = min(RES, c0 + kReduceChunk);
-    // kReduceBatch texels per lane are loaded together (keys and weights,
+    // kReduceBatch texels per lane are loaded together (ids and weights,
     // independent of each other), then reduced one after the other
     for (uint32_t base = c0; base < c1; base += blockDim.x * kReduceBatch) {
         uint32_t ids[kReduceBatch];
@@ -1547,7 +1612,7 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
 #pragma unroll
         for (uint32_t b = 0; b < kReduceBatch; ++b) {
             const uint32_t t = base + b * blockDim.x + threadIdx.x;
-            ids[b] = t < c1 ? (uint32_t)kp[t] : 0u;
+            ids[b] = t < c1 ? __builtin_nontemporal_load(ip + t) : 0u;
             fs[b] = t < c1 ? dff[t] : 0.0f;
         }
 #pragma unroll
@@ -1557,6 +1622,7 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
             float f = id ? fs[b] : 0.0f;
             if (wave_runs(id, f, c, true)) {
                 uint32_t h = (id * 2654435761u) >> 21; // kReduceSlots = 2^11
+                const Acc a = reduce_acc<Acc>(f);
                 // Main.cpp:1259-1262: an unknown patch id is counted (once per
                 // run) and skipped
                 bool done = id - 1 >= npatch;
@@ -1567,13 +1633,13 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
                     if (k == 0u)
                         k = atomicCAS(&hkey[h], 0u, id);
                     if (k == 0u || k == id) {
-                        atomicAdd(&hsum[h], f);
+                        atomicAdd(&hsum[h], a);
                         atomicAdd(&hcnt[h], c);
                         done = true;
                     }
                 }
                 if (!done) { // no free slot: straight to the global entry
-                    atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], f);
+                    atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], a);
                     if (hits)
                         atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], c);
                 }
@@ -1591,15 +1657,192 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
     }
 }
 
-void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot, const uint64_t *keys,
-                   const float *dff, uint32_t npatch, float *ff, uint32_t *hits,
-                   uint32_t *counters, hipStream_t st)
+void launch_reduce(const Geometry &g, uint32_t K, const uint32_t *nshoot, const uint32_t *ids,
+                   const float *dff, uint32_t npatch, float *ff, unsigned long long *ffq,
+                   uint32_t *hits, uint32_t *counters, hipStream_t st)
 {
     const uint32_t chunks = (g.RES + kReduceChunk - 1) / kReduceChunk;
-    k_reduce<<<K * chunks, 256, 0, st>>>(g.RES, nshoot, keys, dff, npatch, ff, hits, counters);
+    if (ffq)
+        k_reduce<unsigned long long><<<K * chunks, 256, 0, st>>>(g.RES, nshoot, ids, dff, npatch,
+                                                                 ffq, hits, counters);
+    else
+        k_reduce<float><<<K * chunks, 256, 0, st>>>(g.RES, nshoot, ids, dff, npatch, ff, hits,
+                                                    counters);
+}
+
+// The ProcessHemicube kernel's own output (Kernel_ProcessHemicube.h:23-69):
+// one thread per work-item of the reference (row y of the k stacked
+// hemicubes, span gx of TW / nx texels, the last TW - nx * span texels of a
+// row left out as there), walking its span texel by texel.  Pass 1 counts the
+// slots a work-item reserves (4 per allocation block, runs of id 0 skipped);
+// an exclusive scan turns them into offsets; pass 2 writes the triplets
+// (hemicube, id - 1, float ΔFF sum of the run in texel order) and the zero
+// padding of the last block at those offsets, so the stream comes out in
+// work-item order.  Slots at or beyond cap are not written.
+__global__ __launch_bounds__(256) void k_runs(uint32_t TW, uint32_t TH, uint32_t nx, uint32_t span,
+                                              uint32_t nitems, const uint32_t *__restrict__ item,
+                                              const float *__restrict__ dff,
+                                              uint32_t *__restrict__ slots,
+                                              const uint32_t *__restrict__ offs, uint32_t cap,
+                                              uint32_t *__restrict__ th, uint32_t *__restrict__ tid,
+                                              float *__restrict__ te)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nitems)
+        return;
+    const uint32_t y = t / nx, gx = t - y * nx;
+    uint32_t x0 = min(gx * span, TW - 1);
+    const uint32_t x1 = min(x0 + span, TW);
+    const uint32_t *row = item + (size_t)y * TW;
+    const float *frow = dff + (size_t)(y % TH) * TW;
+    // next: the first slot of the work-item's next allocation block
+    uint32_t nruns = 0, next = offs ? offs[t] : 0u, w = 0, space = 0;
+    while (x0 < x1) {
+        const uint32_t id = row[x0];
+        float e = 0.0f;
+        while (x0 < x1 && row[x0] == id) {
+            if (offs)
+                e += frow[x0];
+            ++x0;
+        }
+        if (id == 0u)
+            continue;
+        ++nruns;
+        if (!offs)
+            continue;
+        if (!space) {
+            w = next;
+            next += 4u;
+            space = 3;
+        } else {
+            ++w;
+            --space;
+        }
+        if (w < cap) {
+            th[w] = y / TH;
+            tid[w] = id - 1u;
+            te[w] = e;
+        }
+    }
+    if (!offs) {
+        slots[t] = (nruns + 3u) & ~3u;
+        return;
+    }
+    for (; space > 0; --space) {
+        ++w;
+        if (w < cap) {
+            th[w] = 0u;
+            tid[w] = 0u;
+            te[w] = 0.0f;
+        }
+    }
+}
+
+// Exclusive scan of n counts by one workgroup of 1024 threads in chunks of
+// 1024 x 8 through LDS: out[i] = sum of in[0..i), out[n] = the total.
+__global__ __launch_bounds__(1024) void k_scan_excl(const uint32_t *__restrict__ in,
+                                                    uint32_t *__restrict__ out, uint32_t n)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0)
+        carry = 0u;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < n; c0 += 1024u * 8u) {
+        uint32_t x[8], sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t i = c0 + tid * 8u + k;
+            x[k] = i < n ? in[i] : 0u;
+            sum += x[k];
+        }
+        uint32_t inc = sum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d)
+                inc += o;
+        }
+        if (lane == 63)
+            wsum[w] = inc;
+        const uint32_t base = carry;
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t y = lane < 16 ? wsum[lane] : 0u;
+            uint32_t z = y;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const uint32_t o = __shfl_up(z, d, 64);
+                if (lane >= (uint32_t)d)
+                    z += o;
+            }
+            if (lane < 16)
+                wsum[lane] = z - y;
+            if (lane == 15)
+                carry = base + z;
+        }
+        __syncthreads();
+        uint32_t run = base + wsum[w] + inc - sum;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t i = c0 + tid * 8u + k;
+            if (i < n)
+                out[i] = run;
+            run += x[k];
+        }
+        __syncthreads(); // wsum and carry are reused by the next chunk
+    }
+    if (tid == 0)
+        out[n] = carry;
+}
+
+void launch_runs_count(const Geometry &g, uint32_t k, uint32_t nx, const uint32_t *item,
+                       uint32_t *slots, uint32_t *offs, hipStream_t st)
+{
+    const uint32_t nitems = k * g.TH * nx;
+    k_runs<<<(nitems + 255) / 256, 256, 0, st>>>(g.TW, g.TH, nx, g.TW / nx, nitems, item, nullptr,
+                                                 slots, nullptr, 0u, nullptr, nullptr, nullptr);
+    k_scan_excl<<<1, 1024, 0, st>>>(slots, offs, nitems);
+}
+
+void launch_runs_write(const Geometry &g, uint32_t k, uint32_t nx, const uint32_t *item,
+                       const float *dff, const uint32_t *offs, uint32_t cap, uint32_t *th,
+                       uint32_t *tid, float *te, hipStream_t st)
+{
+    const uint32_t nitems = k * g.TH * nx;
+    k_runs<<<(nitems + 255) / 256, 256, 0, st>>>(g.TW, g.TH, nx, g.TW / nx, nitems, item, dff,
+                                                 nullptr, offs, cap, th, tid, te);
 }
 
 // ------------------------------------------------------------------ shooting
+// Stop test of Main.cpp:1288-1300, run by the lane of the pass's last
+// shooter j with its unshot value u before the emitter update (that is
+// lastEnergy: the shooter's radiosity including what it received during the
+// pass, Main.cpp:1290): u goes to ctl[kCtlLastE..+2]; when a threshold is
+// set and |u| (Vector3f::f_Length in float, compared with the double
+// threshold as `lastEnergy.f_Length() < 0.1` does) is below it,
+// ctl[kCtlStop] makes the later passes no-ops.
+__device__ static inline void stop_test(uint32_t *__restrict__ ctl, float u0, float u1, float u2)
+{
+    ctl[kCtlLastE + 0] = __float_as_uint(u0);
+    ctl[kCtlLastE + 1] = __float_as_uint(u1);
+    ctl[kCtlLastE + 2] = __float_as_uint(u2);
+    const double th = *(const double *)(ctl + kCtlThresh);
+    const float len = sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+    if (th > 0.0 && (double)len < th)
+        ctl[kCtlStop] = 1u;
+}
+
+// Passes and shooters counted by the first thread of a shoot launch.
+__device__ static inline void count_pass(uint32_t *__restrict__ ctl, uint32_t n)
+{
+    if (n == 0)
+        return;
+    *(unsigned long long *)(ctl + kCtlPassLo) += 1ull;
+    *(unsigned long long *)(ctl + kCtlShotLo) += n;
+}
+
 // Main.cpp:1251-1295 for all receivers at once: one thread per receiver j
 // accumulates the contributions of the pass's shooters, then the shooters
 // move their snapshot from unshot to shot.  Every nonzero F entry it reads
@@ -1611,7 +1854,7 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
                                                const float *__restrict__ refl,
                                                const float *__restrict__ area, uint32_t npatch,
                                                float *__restrict__ unshot,
-                                               float *__restrict__ shot)
+                                               float *__restrict__ shot, uint32_t *__restrict__ ctl)
 {
     __shared__ uint32_t sid[RAD_MAX_K];
     __shared__ float sc[RAD_MAX_K * 4];
@@ -1627,6 +1870,8 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
         sc[4 * s + 3] = area[i];
     }
     __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        count_pass(ctl, n);
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= npatch)
         return;
@@ -1669,6 +1914,8 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
         }
     }
     // Main.cpp:1287-1295
+    if (n && sid[n - 1] == j)
+        stop_test(ctl, u0, u1, u2);
     for (uint32_t s = 0; s < n; ++s) {
         if (sid[s] == j) {
             float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
@@ -1687,11 +1934,11 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
 
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
                   const float *snap, float *ff, const float *refl, const float *area,
-                  uint32_t npatch, float *unshot, float *shot, hipStream_t st)
+                  uint32_t npatch, float *unshot, float *shot, uint32_t *ctl, hipStream_t st)
 {
     (void)K;
     k_shoot<<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, refl, area,
-                                                  npatch, unshot, shot);
+                                                  npatch, unshot, shot, ctl);
 }
 
 // ----------------------------------------------------------------- selection
@@ -1842,42 +2089,6 @@ void launch_residual(const float *unshot, const float *area, const float *emis, 
     k_select_final<<<1, 256, 0, st>>>(RAD_SELECT_POWER, (uint32_t)nblocks, pkeys, partials,
                                       nullptr, nullptr, resid);
 }
-
-// ------------------------------------------------------------------ helpers
-__global__ void k_ids_to_keys(const uint32_t *__restrict__ ids, uint64_t *__restrict__ keys,
-                              size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        keys[i] = kClearKey | ids[i];
-}
-
-__global__ void k_split_keys(const uint64_t *__restrict__ keys, uint32_t *__restrict__ ids,
-                             float *__restrict__ depth, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        uint64_t k = keys[i];
-        if (ids)
-            ids[i] = (uint32_t)k;
-        if (depth)
-            depth[i] = __uint_as_float((uint32_t)(k >> 32));
-    }
-}
-
-void launch_ids_to_keys(const uint32_t *ids, uint64_t *keys, size_t n, hipStream_t st)
-{
-    size_t b = (n + 255) / 256;
-    k_ids_to_keys<<<(unsigned)(b > 4096 ? 4096 : (b ? b : 1)), 256, 0, st>>>(ids, keys, n);
-}
-
-void launch_split_keys(const uint64_t *keys, uint32_t *ids, float *depth, size_t n,
-                       hipStream_t st)
-{
-    size_t b = (n + 255) / 256;
-    k_split_keys<<<(unsigned)(b > 4096 ? 4096 : (b ? b : 1)), 256, 0, st>>>(keys, ids, depth, n);
-}
-
 
 // ------------------------------------------------------------ top-K (POWER)
 // Each block sorts a chunk of kTopkChunk packed keys descending in LDS

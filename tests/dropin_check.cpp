@@ -3,9 +3,10 @@
 //
 //   dropin_check config W K WX SHOOTS AREA     Config::freeze arithmetic (CPU)
 //   dropin_check patches AREA                  load + updateData arrays (CPU)
+//   dropin_check scenes OBJPATH                scene builders and the OBJ reader (CPU)
 //   dropin_check dff W K                       precomputeHemicubeFormFactors
 //   dropin_check select AREA K SEED            list rule on seeded radiosities
-//   dropin_check shoot AREA W K SHOOTS CYCLES  Shooter cycles, final fields
+//   dropin_check shoot AREA W K SHOOTS CYCLES  Shooter cycles (until finished()), final fields
 //   dropin_check process AREA W                ProcessHemicube triplets
 //   dropin_check resume AREA W PATH            shoot, saveToFile, loadFromFile
 //   dropin_check obj PATH AREA                 WaveFrontModel patches and links (CPU)
@@ -21,6 +22,7 @@
 #include <vector>
 
 #include "radhip_dropin.hpp"
+#include "radhip_scene.h"
 
 using namespace radhip;
 
@@ -51,6 +53,48 @@ int main(int argc, char **argv)
                Config::PATCHVIEW_TEX_H(), Config::PATCHVIEW_TEX_RES(), Config::OCL_WORKITEMS_X(),
                Config::OCL_WORKITEMS_Y(), Config::SHOOTS_PER_CYCLE(), Config::HEMICUBES_CNT(),
                Config::MAX_PATCH_AREA(), lastError().empty() ? 0 : 1);
+        return 0;
+    }
+    if (mode == "scenes" && argc == 3) {
+        // the host scene builders and the OBJ reader (CPU only): seeded box
+        // rooms, the Cornell box both ways, the Cornell faces through
+        // rad_scene_faces, and an OBJ file
+        unsigned long long sum = 0;
+        rad_scene *sc = nullptr;
+        for (unsigned seed = 1; seed <= 3; ++seed) {
+            if (rad_scene_boxes(20 * seed, 6.0f, 3.0f, 6.0f, 3000 * seed, 2, seed, &sc) != RAD_OK)
+                return 1;
+            sum += sc->ntri + sc->npatch;
+            rad_scene_free(sc);
+        }
+        for (int tp = 0; tp < 2; ++tp) {
+            if (rad_scene_cornell(0.3127, tp, tp, 1, &sc) != RAD_OK)
+                return 1;
+            sum += sc->ntri + sc->npatch + sc->nquad;
+            rad_scene_free(sc);
+        }
+        uint32_t nf = 0;
+        rad_cornell_faces(nullptr, nullptr, nullptr, nullptr, &nf);
+        std::vector<float> q((size_t)nf * 12), c((size_t)nf * 3), e((size_t)nf * 3),
+            sh((size_t)nf * 3);
+        rad_cornell_faces(q.data(), c.data(), e.data(), sh.data(), &nf);
+        for (int m = 0; m < 2; ++m) {
+            if (rad_scene_faces(q.data(), nf, c.data(), e.data(), sh.data(), 0.05, m, &sc) != RAD_OK)
+                return 1;
+            sum += sc->ntri + sc->npatch + sc->nquad;
+            rad_scene_free(sc);
+        }
+        float *oq = nullptr;
+        uint32_t on = 0;
+        int orc = rad_obj_parse(argv[2], &oq, &on);
+        if (oq) {
+            if (on && rad_scene_faces(oq, on, nullptr, nullptr, nullptr, 0.1, 1, &sc) == RAD_OK) {
+                sum += sc->ntri;
+                rad_scene_free(sc);
+            }
+            rad_obj_free(oq);
+        }
+        printf("{\"sum\": %llu, \"obj_faces\": %u, \"obj_rc\": %d}\n", sum, on, orc);
         return 0;
     }
     if (mode == "patches" && argc == 3) {
@@ -225,18 +269,24 @@ int main(int argc, char **argv)
         }
         if (mode == "shoot") {
             Shooter sh(&mc);
-            unsigned cycles = (unsigned)atoi(argv[6]);
+            unsigned cycles = (unsigned)atoi(argv[6]), ran = 0;
             float e = 0;
-            for (unsigned c = 0; c < cycles; ++c)
+            for (unsigned c = 0; c < cycles && !sh.finished(); ++c, ++ran) {
                 e = sh.shootCycle();
+                if (sh.failed()) {
+                    fprintf(stderr, "%s\n", lastError().c_str());
+                    return 1;
+                }
+            }
             std::vector<double> rad, ill;
             for (unsigned i = 0; i < n; ++i) {
                 rad.insert(rad.end(), {ps[i]->radiosity.x, ps[i]->radiosity.y, ps[i]->radiosity.z});
                 ill.insert(ill.end(),
                            {ps[i]->illumination.x, ps[i]->illumination.y, ps[i]->illumination.z});
             }
-            printf("{\"shooters\": %llu, \"last_energy\": %.9g, \"residual\": %.9g, ",
-                   sh.shooters(), e, sh.residual());
+            printf("{\"shooters\": %llu, \"passes\": %llu, \"cycles\": %u, \"finished\": %d, "
+                   "\"last_energy\": %.9g, \"residual\": %.9g, ",
+                   sh.shooters(), sh.passes(), ran, sh.finished() ? 1 : 0, e, sh.residual());
             vec("radiosity", rad);
             vec("illumination", ill, true);
             printf("}\n");
@@ -257,8 +307,9 @@ int main(int argc, char **argv)
             fprintf(stderr, "%s\n", rad_last_error(c));
             return 1;
         }
-        std::vector<uint32_t> hc(n), idv(n);
-        std::vector<float> en(n);
+        // output arrays as the reference allocates them (Main.cpp:605-607)
+        std::vector<uint32_t> hc(res), idv(res);
+        std::vector<float> en(res);
         unsigned m = ProcessHemicube(&mc, item.data(), 1, hc.data(), idv.data(), en.data());
         std::vector<double> ids(idv.begin(), idv.begin() + m), e(en.begin(), en.begin() + m),
             h(hc.begin(), hc.begin() + m), items(item.begin(), item.end());

@@ -145,12 +145,18 @@ def test_dropin_process_hemicube_triplets(rad):
     light = d["light"]
     item = np.array(d["item"], np.uint32)
     assert np.array_equal(item, O.render_hemicube(W, sc.tri, sc.tri_patch, sc.frame[light]))
+    # the kernel's own stream: per-run triplets in blocks of 4 with zero
+    # padding, bit for bit the oracle's (work-item order)
+    h, i, e = O.process_triplets(W, 1, item, O.dff_cross(W))
+    assert d["count"] == len(h) and d["count"] % 4 == 0
+    assert np.array_equal(np.array(d["hemicubes"], np.uint32), h)
+    assert np.array_equal(np.array(d["ids"], np.uint32), i)
+    assert np.array_equal(np.array(d["energies"], np.float32).view(np.uint32), e.view(np.uint32))
+    # the host accumulation of Main.cpp:1257-1266 over it gives the F row
     hits, ff = O.histogram(W, 1, item, O.dff_cross(W), sc.npatch)
-    seen = np.nonzero(hits[0])[0]
-    assert d["count"] == len(seen)
-    assert np.array_equal(np.array(d["ids"], np.uint32), seen)
-    assert np.all(np.array(d["hemicubes"]) == 0)
-    assert np.allclose(d["energies"], ff[0, seen], rtol=1e-4, atol=1e-8)
+    acc = np.zeros(sc.npatch, np.float64)
+    np.add.at(acc, i, e.astype(np.float64))
+    assert np.allclose(acc, ff[0], rtol=1e-5, atol=1e-9)
 
 
 @pytest.mark.gpu

@@ -72,3 +72,51 @@ def test_dropin_save_and_load(tmp_path):
     assert p.returncode == 0, p.stderr
     d = json.loads(p.stdout.strip().splitlines()[-1])
     assert d["saved"] == 1 and d["loaded"] == 1 and d["same"] == d["n"] > 0
+
+
+def test_oracle_written_checkpoint_loads(rad, scene, tmp_path):
+    """A file written by the independent writer of oracle/scene_io.py from
+    the documented layout loads, and the device state equals its planes bit
+    for bit."""
+    from oracle import scene_io as S
+    rng = np.random.default_rng(9)
+    n = scene.npatch
+    u = rng.uniform(0, 3, (n, 3)).astype(np.float32)
+    sh = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    path = str(tmp_path / "oracle.rck")
+    S.ckpt_write(path, scene, 1, u, sh, scene.emission)
+    a = rad.Solver(hemicube=16, hemicubes=1)
+    a.upload(scene)
+    a.checkpoint_load(path)
+    gu, gs = a.state()
+    assert np.array_equal(gu.view(np.uint32), u.view(np.uint32))
+    assert np.array_equal(gs.view(np.uint32), sh.view(np.uint32))
+    want = (scene.area[:, None] * u.astype(np.float64)).sum() / \
+        (scene.area[:, None] * scene.emission.astype(np.float64)).sum()
+    assert a.residual() == pytest.approx(want, rel=1e-12)
+    # the same file with a wrong mode word is refused
+    S.ckpt_write(path, scene, 0, u, sh, scene.emission)
+    with pytest.raises(rad.RadError, match="mode"):
+        a.checkpoint_load(path)
+
+
+def test_device_checkpoint_parsed_by_oracle(rad, scene, tmp_path):
+    """A file the library writes, read by the independent reader: header
+    fields, the scene hash recomputed from the scene arrays, the checksum and
+    the three planes."""
+    from oracle import scene_io as S
+    path = str(tmp_path / "device.rck")
+    a = rad.Solver(hemicube=32, hemicubes=4)
+    a.upload(scene)
+    a.step(6)
+    a.checkpoint_save(path)
+    gu, gs = a.state()
+    d = S.ckpt_read(path)
+    assert d["checksum_ok"] and d["version"] == 1 and d["mode"] == 1
+    assert d["npatch"] == scene.npatch and d["ntri"] == scene.ntri
+    assert d["size"] == 8 + 16 + 8 + 36 * scene.npatch + 8
+    assert d["scene_hash"] == S.scene_hash(scene.tri, scene.tri_patch, scene.frame, scene.refl,
+                                           scene.area)
+    assert np.array_equal(d["unshot"].view(np.uint32), gu.view(np.uint32))
+    assert np.array_equal(d["shot"].view(np.uint32), gs.view(np.uint32))
+    assert np.array_equal(d["emission"], scene.emission)

@@ -588,3 +588,23 @@ def test_itembuf_bit_exact_random_soup(rad, W):
     shooters = [int(x) for x in rng.choice(small, 4, replace=False)]
     say(f"itembuf random soup W={W}: {m} triangles, shooters {shooters}")
     _render_case(rad, sc, W, shooters)
+
+
+@pytest.mark.parametrize("wx", [4, 3, 1])
+def test_process_runs_equals_oracle_triplets(rad, boxes_small, wx):
+    """rad_process_runs: the ProcessHemicube kernel's triplet stream for K
+    hemicubes, bit for bit the oracle's restatement (Kernel_ProcessHemicube.h
+    :23-69) at oclWorkitemsX 4, 3 (spans leave the last texels of a row out,
+    as the reference's integer span does) and 1."""
+    W, K = 64, 4
+    sc = boxes_small
+    s = rad.Solver(hemicube=W, hemicubes=K, workitems_x=wx)
+    s.upload(sc)
+    sh = [0, 17, sc.npatch // 2, sc.npatch - 1]
+    item = s.render(sh)
+    h, i, e = s.process_runs(K)
+    ho, io, eo = O.process_triplets(W, K, item, O.dff_cross(W), workitems_x=wx)
+    say(f"runs wx={wx}: {len(h)} triplets")
+    assert len(h) == len(ho) and len(h) % 4 == 0
+    assert np.array_equal(h, ho) and np.array_equal(i, io)
+    assert np.array_equal(e.view(np.uint32), eo.view(np.uint32))

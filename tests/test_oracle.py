@@ -565,3 +565,48 @@ def test_mt_render_equals_single_thread(rad, threads):
     assert np.array_equal(u1.view(np.uint32), u2.view(np.uint32))
     assert np.array_equal(s1.view(np.uint32), s2.view(np.uint32))
 
+
+
+@pytest.mark.parametrize("wx", [4, 3, 1])
+def test_triplet_stream_from_run_boundaries(wx):
+    """orc_process_hemicube_triplets against run boundaries found with
+    numpy (np.diff of each span): one triplet per run of a nonzero id, its
+    ΔFF summed in texel order (a float32 cumulative sum), each work-item's
+    triplets padded to a multiple of 4 with zeros, work-items in order; the
+    triplets accumulated per id give the histogram's form factors."""
+    W, K = 16, 3
+    TW, TH, RES = O.geometry(W)
+    rng = np.random.default_rng(wx)
+    item = np.repeat(rng.integers(0, 6, size=K * RES), rng.integers(1, 9, size=K * RES))
+    item = item[:K * RES].astype(np.uint32)
+    dff = O.dff_cross(W)
+    h, i, e = O.process_triplets(W, K, item, dff, workitems_x=wx)
+    nx = min(wx, TW)
+    span = TW // nx
+    wh, wi, we = [], [], []
+    for y in range(TH * K):
+        for g in range(nx):
+            x0 = min(g * span, TW - 1)
+            x1 = min(x0 + span, TW)
+            seg = item[y * TW + x0:y * TW + x1]
+            cuts = np.r_[0, np.flatnonzero(np.diff(seg)) + 1, len(seg)]
+            n0 = len(wh)
+            for a, b in zip(cuts[:-1], cuts[1:]):
+                if seg[a] == 0:
+                    continue
+                wh.append(y // TH)
+                wi.append(int(seg[a]) - 1)
+                we.append(np.cumsum(dff[(y % TH) * TW + x0 + a:(y % TH) * TW + x0 + b],
+                                    dtype=np.float32)[-1])
+            while (len(wh) - n0) % 4:
+                wh.append(0)
+                wi.append(0)
+                we.append(np.float32(0))
+    assert np.array_equal(h, np.array(wh, np.uint32))
+    assert np.array_equal(i, np.array(wi, np.uint32))
+    assert np.array_equal(e.view(np.uint32), np.array(we, np.float32).view(np.uint32))
+    hits, ff64 = O.histogram(W, K, item, dff, 6)
+    acc = np.zeros((K, 6))
+    np.add.at(acc, (h, i), e.astype(np.float64))
+    if wx in (4, 1):  # spans cover every texel
+        assert np.allclose(acc, ff64, rtol=1e-5, atol=1e-12)

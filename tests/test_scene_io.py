@@ -230,3 +230,33 @@ def test_smooth_shade_linear_field_interior():
             for k, (dc, dr) in enumerate([(-0.5, -0.5), (0.5, -0.5), (0.5, 0.5), (-0.5, 0.5)]):
                 x, y = c + dc, r + dr
                 assert np.allclose(out[i, k], [3 + 2 * x + 4 * y, 1 + x, 2 + y], atol=0), (i, k)
+
+
+def test_fnv1a64_known_answers():
+    """The published FNV-1a 64-bit test vectors pin the checksum and scene
+    hash of the checkpoint format."""
+    from oracle import scene_io as S
+    assert S.fnv1a64(b"") == 0xCBF29CE484222325
+    assert S.fnv1a64(b"a") == 0xAF63DC4C8601EC8C
+    assert S.fnv1a64(b"foobar") == 0x85944171F73967E8
+
+
+def test_checkpoint_writer_reader_round_trip(rad, tmp_path):
+    from oracle import scene_io as S
+    sc = rad.Scene.cornell(0.5, triangle_patches=True, physical=True, seed=1)
+    rng = np.random.default_rng(3)
+    u = rng.random((sc.npatch, 3), dtype=np.float32)
+    sh = rng.random((sc.npatch, 3), dtype=np.float32)
+    path = str(tmp_path / "c.rck")
+    S.ckpt_write(path, sc, 1, u, sh, sc.emission)
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"RADCKPT1" and len(raw) == 40 + 36 * sc.npatch
+    # the first unshot plane (red of every patch) follows the 32-byte header
+    assert np.array_equal(np.frombuffer(raw[32:32 + 4 * sc.npatch], "<f4"), u[:, 0])
+    d = S.ckpt_read(path)
+    assert d["checksum_ok"] and d["ntri"] == sc.ntri and d["npatch"] == sc.npatch
+    assert np.array_equal(d["unshot"], u) and np.array_equal(d["shot"], sh)
+    bad = bytearray(raw)
+    bad[40] ^= 1
+    open(path, "wb").write(bytes(bad))
+    assert not S.ckpt_read(path)["checksum_ok"]
