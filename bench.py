@@ -50,39 +50,62 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(W, K):
-    """HBM bytes per launch from the committed counter summary of the default
-    workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
-    if (W, K) != (256, 64):
-        return {}
-    path = os.path.join(ROOT, "profiles", "room200k_W256_K64_hbm_counters.md")
+def _profile_line(name, key):
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return json.loads(line.split(":", 1)[1])
     except OSError:
         pass
     return {}
 
 
+def measured_traffic(W, K):
+    """HBM bytes per launch from the committed counter summary of the default
+    workload (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
+    if (W, K) != (256, 64):
+        return {}
+    return _profile_line("room200k_W256_K64_hbm_counters.md", "traffic_bytes_per_launch")
+
+
+def measured_valu(W, K):
+    """Wavefront VALU instructions per launch (SQ_INSTS_VALU) from the
+    committed counter summary of the default workload."""
+    if (W, K) != (256, 64):
+        return {}
+    return _profile_line("room200k_W256_K64_valu_counters.md", "valu_insts_per_launch")
+
+
+# MI355X: 256 CUs x 4 SIMDs x 2.4 GHz, a wavefront VALU instruction every 4
+# cycles per SIMD (64 lanes over 16-wide SIMDs)
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
 def stop_rule_leg(s, sc, chunk, cap):
-    """The reference's own stop rule (Main.cpp:1298): passes until |dB| of
-    the last shooter of a pass falls below 0.1, checked every `chunk`
-    passes, from the initial state."""
+    """The reference's own stop rule (Main.cpp:1137, 1288-1300): OnIdle
+    cycles of `chunk` passes with the stop test after every pass (on the
+    device: lastEnergy.f_Length() < 0.1, the last shooter's radiosity before
+    its emitter update), from the initial state."""
+    import numpy as np
     s.upload(sc)
-    ms = 0.0
-    passes = 0
+    passes = shooters = 0
     st = None
+    t0 = time.perf_counter()
     while passes < cap:
-        st = s.run(chunk, 0.0, check_every=chunk)
-        ms += st.seconds * 1e3
+        st = s.run_cycle(chunk, 0.1)
         passes += st.passes
-        if st.last_energy < 0.1:
+        shooters += st.shooters
+        if st.stopped:
             break
-    out = {"ms": ms, "passes": passes, "last_energy": float(st.last_energy),
-           "residual": st.residual, "stopped": bool(st.last_energy < 0.1), "check_every": chunk}
-    log(f"reference stop rule (|dB| < 0.1): {'after' if out['stopped'] else 'not within'} "
-        f"{passes} passes, {ms:.0f} ms, residual {st.residual:.3e}")
+    ms = (time.perf_counter() - t0) * 1e3
+    le = np.array(st.last_energy, np.float32)
+    out = {"ms": ms, "passes": passes, "shooters": shooters,
+           "last_energy": float(np.sqrt(np.float32((le * le).sum()))),
+           "residual": s.residual(), "stopped": bool(st.stopped), "cycle_passes": chunk}
+    log(f"reference stop rule (lastEnergy < 0.1): {'after' if out['stopped'] else 'not within'} "
+        f"{passes} passes, {ms:.0f} ms, residual {out['residual']:.3e}")
     return out
 
 
@@ -103,6 +126,13 @@ def gather_leg(R, s, sc, args):
     # 1e-3 answer against the converged fixed point of the stored matrix
     B7, st7 = s.gather(target=1e-7, inner=args.gather_inner, max_outer=20)
     s.dense_release()
+    # against the exact FP64 solution of the FP32 form factors: every
+    # hemicube rendered again and streamed into an FP64 residual on the device
+    t3 = time.perf_counter()
+    res64, q = s.gather_check_fp64(B)
+    chk = {"fp64_residual_rel_l1": res64, "contraction_q": q,
+           "error_bound_rel_l1": res64 / (1.0 - q) if q < 1 else None,
+           "seconds": time.perf_counter() - t3}
     a = sc.area.astype(np.float64)[None, :]
     diff = float((a * np.abs(B - shoot_B)).sum() / (a * np.abs(shoot_B)).sum())
     B7 = B7.astype(np.float64)
@@ -119,12 +149,14 @@ def gather_leg(R, s, sc, args):
            "matrix_bytes": 4 * npad * npad,
            "rel_l1_vs_batched_shooting": diff,
            "rel_l1_vs_converged": err7, "max_rel_vs_converged": errmax,
-           "converged_residual": st7.residual}
+           "converged_residual": st7.residual, "fp64_check": chk}
     log(f"gather bf16 to {args.target}: {out['ms'] / 1e3:.2f} s (build {out['build_ms'] / 1e3:.2f}"
         f" s, solve {out['solve_ms']:.0f} ms, {st.outer} steps, {st.sweeps} sweeps), residual "
         f"{st.residual:.2e}; sweep {st.sweep_ms:.2f} ms ({sweep_gbs:.0f} GB/s), residual product "
         f"{st.resid_ms:.2f} ms ({resid_gbs:.0f} GB/s); vs shooting {diff:.2e}; vs the solve "
-        f"converged to {st7.residual:.1e}: {err7:.2e} rel L1, {errmax:.2e} max")
+        f"converged to {st7.residual:.1e}: {err7:.2e} rel L1, {errmax:.2e} max; against the "
+        f"FP64 problem of the FP32 rows: residual {res64:.2e}, q {q:.3f}, error bound "
+        f"{chk['error_bound_rel_l1']}")
     return out
 
 
@@ -398,6 +430,8 @@ def main():
     ap.add_argument("--small-tri-px", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the K=1 leg (profiling)")
     ap.add_argument("--no-gather", action="store_true", help="skip the dense BF16 gather leg")
+    ap.add_argument("--no-reduce", action="store_true",
+                    help="skip the standalone ProcessHemicube timing")
     ap.add_argument("--gather-inner", type=int, default=4)
     ap.add_argument("--graph-passes", type=int, default=8,
                     help="passes per captured HIP graph in the batched leg (0: plain launches)")
@@ -450,13 +484,17 @@ def main():
         log(f"autotune K={K}: tile {t}, in-lane limit {sm} px ({tms:.3f} ms per render pass)")
     s.step(args.warmup)
     s.finish()
+    c0 = s.shot_count()
     t0 = time.perf_counter()
     s.step(args.steps)
     s.finish()
     dt = time.perf_counter() - t0
-    batched_sps = args.steps * K / dt
+    c1 = s.shot_count()
+    shot_timed = c1[1] - c0[1]                 # shooters the device shot
+    batched_sps = shot_timed / dt
     ms_per_step = dt * 1e3 / args.steps
-    log(f"batched K={K}: {ms_per_step:.3f} ms/pass, {batched_sps:.0f} shooters/s")
+    log(f"batched K={K}: {ms_per_step:.3f} ms/pass, {batched_sps:.0f} shooters/s "
+        f"({shot_timed} shooters in {c1[0] - c0[0]} passes, counted on the device)")
     # per-kernel device times over the same number of passes (HIP events on
     # the library stream)
     s.kernel_times(1)
@@ -466,19 +504,56 @@ def main():
     kt = dict(zip(names, kt))
     log("kernel ms/pass: " + json.dumps({k: round(v, 4) for k, v in kt.items()}))
 
+    # --------------------------------------- standalone ProcessHemicube
+    # the reference-shaped reduction (rad_process_hemicube) over 64 item
+    # buffers of this scene, timed with HIP events: per hemicube 4 B of ids
+    # per texel + 4 B F and 4 B hit count per patch, the ΔFF table once
+    reduce_leg = None
+    if not args.no_reduce:
+        import numpy as np
+        r = R.Solver(hemicube=W, hemicubes=K, tile_px=32, small_tri_px=64)
+        r.upload(sc)
+        r.render(np.random.default_rng(12).choice(sc.npatch, K, replace=False))
+        r.process(K)
+        r.kernel_times(1)
+        ms = []
+        for _ in range(5):
+            r.process(K)
+            ms.append(r.reduce_kernel_ms())
+        r.kernel_times(0)
+        rms = float(np.mean(ms))
+        rb = K * (4.0 * 3 * W * W + 8.0 * sc.npatch) + 4.0 * 3 * W * W
+        reduce_leg = {"bound": "hbm", "avg_launch_ms": round(rms, 5), "alg_bytes_per_launch": rb,
+                      "achieved_gbs": round(rb / (rms * 1e-3) / 1e9, 1),
+                      "frac": round(rb / (rms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"standalone ProcessHemicube, {K} item buffers: {rms * 1e3:.1f} us, "
+            f"{reduce_leg['achieved_gbs']:.0f} GB/s")
+        del r
+
     # ----------------------------------------------------------- plain K=1
+    # the single-shooter drop-in, launched eagerly and replayed from captured
+    # graphs of 8 passes; the faster of the two is the plain baseline
     p = plain_sps = pdt = pkt = None
+    plain_modes = {}
     if not args.no_plain:
-        p = R.Solver(hemicube=W, hemicubes=1, small_tri_px=args.small_tri_px)
-        p.upload(sc)
-        p.step(20)
-        p.finish()
-        t0 = time.perf_counter()
-        p.step(args.plain_passes)
-        p.finish()
-        pdt = time.perf_counter() - t0
-        plain_sps = args.plain_passes / pdt
-        log(f"plain K=1: {pdt * 1e3 / args.plain_passes:.4f} ms/pass, {plain_sps:.0f} shooters/s")
+        for gp in (0, 8):
+            q = R.Solver(hemicube=W, hemicubes=1, small_tri_px=args.small_tri_px, graph_passes=gp)
+            q.upload(sc)
+            q.step(24)
+            q.finish()
+            c0 = q.shot_count()
+            t0 = time.perf_counter()
+            q.step(args.plain_passes)
+            q.finish()
+            qdt = time.perf_counter() - t0
+            nshot = q.shot_count()[1] - c0[1]
+            plain_modes["graph8" if gp else "eager"] = {
+                "shooters_per_s": round(nshot / qdt, 2),
+                "ms_per_pass": round(qdt * 1e3 / args.plain_passes, 5), "graph_passes": gp}
+            log(f"plain K=1 {'graph' if gp else 'eager'}: {qdt * 1e3 / args.plain_passes:.4f} "
+                f"ms/pass, {nshot / qdt:.0f} shooters/s")
+            if plain_sps is None or nshot / qdt > plain_sps:
+                p, plain_sps, pdt = q, nshot / qdt, qdt
         p.kernel_times(1)
         p.step(200)
         p.finish()
@@ -550,6 +625,31 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
             "alg_bytes_per_launch": alg, "avg_launch_ms": round(kt[dom], 5),
             "traffic": measured_traffic(W, K).get(dom)}
+    # the rasterizing kernels are bound by instruction issue, not HBM: their
+    # wavefront VALU instructions per launch (counter run of this workload)
+    # over the launch time, against 614 G/s
+    valu = measured_valu(W, K)
+    roof["issue"] = {k: {"valu_insts_per_launch": valu[k], "avg_launch_ms": round(kt[k], 5),
+                         "achieved_ginst_s": round(valu[k] / (kt[k] * 1e-3) / 1e9, 1),
+                         "peak_ginst_s": VALU_PEAK_GINST,
+                         "frac": round(valu[k] / (kt[k] * 1e-3) / 1e9 / VALU_PEAK_GINST, 4)}
+                     for k in ("tile_raster_reduce", "raster_setup") if k in valu}
+    sh_gbs = per_launch["shoot"] / (kt["shoot"] * 1e-3) / 1e9
+    roof["others"] = {"shoot": {"bound": "hbm", "achieved_gbs": round(sh_gbs, 1),
+                                "frac": round(sh_gbs / HBM_PEAK_GBS, 4),
+                                "alg_bytes_per_launch": per_launch["shoot"],
+                                "avg_launch_ms": round(kt["shoot"], 5)}}
+    if reduce_leg:
+        roof["others"]["process_hemicube"] = reduce_leg
+    if configs and "dense32k_R192" in configs:
+        d = configs["dense32k_R192"]
+        roof["others"]["gather_gemm_32k_R192"] = {
+            "bound": "mfma", "achieved_tflops": round(d["sweep_mfma_tflops"], 1),
+            "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+            "frac": round(d["sweep_mfma_tflops"] / MFMA_BF16_PEAK_TFLOPS, 4),
+            "hbm_gbs": round(d["sweep_hbm_gbs"], 1),
+            "hbm_frac": round(d["sweep_hbm_gbs"] / HBM_PEAK_GBS, 4),
+            "sweep_ms": d["sweep_kernel_ms"]}
 
     # --------------------------------------------------------- CPU baseline
     cpu = cpu_mt = raster = None
@@ -623,7 +723,7 @@ def main():
         "plain": None if p is None else {
             "shooters_per_s": round(plain_sps, 2),
             "ms_per_pass": round(pdt * 1e3 / args.plain_passes, 5),
-            "kernel_ms": {k: round(v, 5) for k, v in pkt.items()}},
+            "modes": plain_modes, "kernel_ms": {k: round(v, 5) for k, v in pkt.items()}},
         "batched": {"shooters_per_s": round(batched_sps, 2), "autotune": tune,
                     "graph_passes": args.graph_passes,
                     "kernel_ms": {k: round(v, 5) for k, v in kt.items()}},

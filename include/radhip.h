@@ -210,6 +210,11 @@ int rad_face_mvps(rad_ctx *ctx, uint32_t patch, float *out80);
  * number of timed passes. */
 int rad_kernel_times(rad_ctx *ctx, int enable, double *out6, uint64_t *npasses);
 
+/* Device time (ms, HIP events) of the last standalone ProcessHemicube
+ * launch (rad_process_hemicube / rad_process_itembuf) made while timing was
+ * enabled with rad_kernel_times; 0 before one. */
+int rad_reduce_kernel_ms(rad_ctx *ctx, double *ms);
+
 /* Diagnostics (no counterpart in the reference): copies the per-tile record
  * counts of the last pass (hemicubes * tiles entries, row-major tiles of
  * tile_px) into out and returns the tile count per hemicube in *ntiles. */
@@ -261,8 +266,11 @@ int rad_checkpoint_load(rad_ctx *ctx, const char *path);
  * [scene_tri_beg[s], scene_tri_beg[s+1]) and patches [scene_patch_beg[s],
  * scene_patch_beg[s+1]) (nscene + 1 offsets each, first 0, last ntri /
  * npatch); tri_patch holds packed patch indices and each triangle's patch
- * lies in its own scene.  scene_slots[s] (1..K, summing to at most K) is the
- * number of shooters the scene gets per pass while it is active.  The pack
+ * lies in its own scene.  scene_slots[s] (1..K) is the number of shooters
+ * the scene asks for per pass while it is active; when the active scenes ask
+ * for more than K in all, the K slots are shared out by water-filling (each
+ * scene min(quota, L) for the largest level L that fits, the rest one each in
+ * an order that rotates every pass).  The pack
  * replaces the context's scene; rad_set_state / rad_get_state / rad_step /
  * rad_finish / rad_residual_fp64 work on the packed arrays, and the calls
  * that take explicit shooters, rad_select, rad_autotune and the dense gather
@@ -271,6 +279,13 @@ int rad_pack_upload(rad_ctx *ctx, uint32_t nscene, const uint32_t *scene_tri_beg
                     const uint32_t *scene_patch_beg, const uint32_t *scene_slots,
                     const float *tri_xyz, const uint32_t *tri_patch, uint32_t ntri, uint32_t npatch,
                     const float *patch_frame, const float *refl_rgb, const float *area);
+/* Per-scene hemicube resolution of the loaded pack (scene_w[s]: even, 2 up
+ * to the context's hemicube side, which sizes the screen tiles): each slot
+ * renders and reduces its scene's hemicubes at the scene's own side with
+ * that side's ΔFF table, all slots still in one launch per kernel.  Without
+ * this call every scene uses the context's side. */
+int rad_pack_hemicubes(rad_ctx *ctx, const uint32_t *scene_w);
+
 /* Starts a solve: per-scene normalisation sum_A (r+g+b)(E) from the emission
  * of rad_set_state; a scene is done once its residual <= target. */
 int rad_pack_begin(rad_ctx *ctx, double target);
@@ -343,6 +358,20 @@ typedef struct rad_gather_stats {
  * not yet gathered (as shot + unshot after shooting).  Blocks until done. */
 int rad_gather_solve(rad_ctx *ctx, const float *rhs, uint32_t R, const rad_gather_opts *opts,
                      float *out, rad_gather_stats *stats);
+
+/* FP64 check of a gather solution (no counterpart in the reference).  B and
+ * E: R <= 16 columns of npatch (column r contiguous, reflectance channel
+ * r % 3; E NULL: the scene's emission, R = 3).  Every patch's hemicube is
+ * rendered again (FP32 rows, K per pass, never stored) and streamed into the
+ * FP64 residual r = E + diag(rho) M B - B of the shooting factors
+ * M_ji = F_ij A_i / A_j (the matrix RAD_DENSE_SHOOT rounds to BF16):
+ * out2[0] = sum_j A_j sum_r |r_rj| / sum_j A_j sum_r |B_rj|, out2[1] =
+ * q = max over rows i and channels c of sum_j rho_jc F_ij.  In the
+ * area-weighted norm x -> diag(rho) M x contracts by q, so when q < 1 the
+ * relative error of B against the exact FP64 solution of the FP32 form
+ * factors is at most out2[0] / (1 - q). */
+int rad_gather_check_fp64(rad_ctx *ctx, const float *B, const float *E_or_null, uint32_t R,
+                          double *out2);
 
 /* ---------------------------------------------------------------------------
  * Scene ingest on the device and vertex-colour export

@@ -104,6 +104,7 @@ SIGNATURES = {
     "rad_finish": (C.c_int, [_P]),
     "rad_face_mvps": (C.c_int, [_P, _u32, _PF]),
     "rad_kernel_times": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "rad_reduce_kernel_ms": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "rad_scene_cornell": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_uint64,
                                     C.POINTER(C.POINTER(rad_scene))]),
     "rad_scene_boxes": (C.c_int, [_u32, C.c_float, C.c_float, C.c_float, _u32, _u32,
@@ -126,6 +127,7 @@ SIGNATURES = {
     "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
     "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
     "rad_pack_begin": (C.c_int, [_P, C.c_double]),
+    "rad_pack_hemicubes": (C.c_int, [_P, _PU]),
     "rad_pack_status": (C.c_int, [_P, _PU, C.POINTER(C.c_double), _PU, _PU, _PU]),
     "rad_checkpoint_save": (C.c_int, [_P, C.c_char_p]),
     "rad_checkpoint_load": (C.c_int, [_P, C.c_char_p]),
@@ -134,6 +136,7 @@ SIGNATURES = {
     "rad_dense_download": (C.c_int, [_P, _u32, _u32, _PF]),
     "rad_dense_apply": (C.c_int, [_P, _PF, _u32, C.c_int, _PF]),
     "rad_dense_release": (C.c_int, [_P]),
+    "rad_gather_check_fp64": (C.c_int, [_P, _PF, _PF, _u32, C.POINTER(C.c_double)]),
     "rad_gather_solve": (C.c_int, [_P, _PF, _u32, C.POINTER(rad_gather_opts), _PF,
                                    C.POINTER(rad_gather_stats)]),
 }
@@ -503,6 +506,13 @@ class Solver:
         self._chk(lib().rad_kernel_times(self._ctx, enable, out, C.byref(n)), "rad_kernel_times")
         return list(out), n.value
 
+    def reduce_kernel_ms(self):
+        """Device ms of the last standalone ProcessHemicube launched while
+        kernel timing was enabled."""
+        ms = C.c_double()
+        self._chk(lib().rad_reduce_kernel_ms(self._ctx, C.byref(ms)), "rad_reduce_kernel_ms")
+        return ms.value
+
     def checkpoint_save(self, path):
         self._chk(lib().rad_checkpoint_save(self._ctx, os.fsencode(path)), "rad_checkpoint_save")
 
@@ -540,6 +550,11 @@ class Solver:
         self.pack_offsets = (tb, pb)
         em = cat("emission")
         self.set_state(em, cat("shot0"), em)
+
+    def pack_hemicubes(self, scene_w):
+        """Per-scene hemicube sides of the loaded pack (rad_pack_hemicubes)."""
+        w = np.ascontiguousarray(scene_w, np.uint32)
+        self._chk(lib().rad_pack_hemicubes(self._ctx, uptr(w)), "rad_pack_hemicubes")
 
     def pack_begin(self, target):
         self._chk(lib().rad_pack_begin(self._ctx, target), "rad_pack_begin")
@@ -580,6 +595,17 @@ class Solver:
         self._chk(lib().rad_dense_apply(self._ctx, fptr(x), x.shape[0], int(split), fptr(y)),
                   "rad_dense_apply")
         return y
+
+    def gather_check_fp64(self, B, E=None):
+        """(FP64 residual, contraction q) of a solution B (R x npatch) against
+        freshly rendered FP32 rows in the shooting layout; the relative
+        area-weighted error bound is residual / (1 - q)."""
+        B = np.ascontiguousarray(np.atleast_2d(B), np.float32)
+        Ep = None if E is None else np.ascontiguousarray(np.atleast_2d(E), np.float32)
+        out = (C.c_double * 2)()
+        self._chk(lib().rad_gather_check_fp64(self._ctx, fptr(B), fptr(Ep), B.shape[0], out),
+                  "rad_gather_check_fp64")
+        return out[0], out[1]
 
     def dense_release(self):
         self._chk(lib().rad_dense_release(self._ctx), "rad_dense_release")

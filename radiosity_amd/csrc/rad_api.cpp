@@ -6,6 +6,7 @@
 // -> reduce -> shoot.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -104,6 +105,7 @@ struct rad_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev;
     double t_acc[6] = {0, 0, 0, 0, 0, 0};
+    double t_reduce = 0.0;          // device ms of the last standalone ProcessHemicube
     std::string pending_error;
     uint64_t t_passes = 0;
 };
@@ -206,14 +208,15 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
     c->ff_dirty = 0;
     record(c, evbase + 2);
     launch_raster_setup(c->g, k, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
-                        c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->st);
+                        c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->pk.slot_w, c->st);
     record(c, evbase + 3);
     launch_bin(k, c->nshoot, c->bb, c->counters, c->st);
     record(c, evbase + 4);
     const bool fq = !ids && c->det;
-    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, ids, depth, c->dff, c->rowlen,
-                c->pk.slot_pbeg, ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits,
-                c->counters, c->st);
+    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, ids, depth,
+                c->pk.slot_w ? c->pk.dffs : c->dff, c->rowlen, c->pk.slot_pbeg,
+                ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits, c->counters,
+                c->pk.slot_w, c->pk.slot_dffo, c->st);
     if (fq)
         launch_fold(c->ffq, c->ff, (size_t)k * c->rowlen, c->st);
     if (!ids)
@@ -306,13 +309,15 @@ int set_tiles(rad_ctx *c, uint32_t T, uint32_t small_px)
     c->bb.nty = nty;
     c->bb.ntiles = ntx * nty;
     c->small_px = small_px;
+    // bin lists: records of at most a quarter of the in-lane limit in front
+    c->bb.split_px = small_px / 4 > 4 ? small_px / 4 : 4;
     c->cfg.tile_px = T;
     c->cfg.small_tri_px = small_px;
     drop_graph(c);
     size_t nbins = (size_t)c->K * c->bb.ntiles;
     int rc;
     if ((rc = dalloc(c, &c->bb.count, nbins)) || (rc = dalloc(c, &c->bb.start, nbins)) ||
-        (rc = dalloc(c, &c->bb.fill, nbins)) || (rc = dalloc(c, &c->bb.order, nbins)))
+        (rc = dalloc(c, &c->bb.fill, 2 * nbins)) || (rc = dalloc(c, &c->bb.order, nbins)))
         return rc;
     return RAD_OK;
 }
@@ -431,7 +436,8 @@ void rad_destroy(rad_ctx *c)
                     c->pk.sc_slot_beg, c->pk.sc_nsel, c->pk.sc_done, c->pk.sc_den, c->pk.sc_res,
                     c->pk.sc_shot,
                     c->pk.cand, c->pk.patch_scene, c->pk.pass, c->pk.slot_tbeg, c->pk.slot_tcnt,
-                    c->pk.slot_pbeg, c->patch_quad, c->quad_pbeg, c->quad_nbr, c->qval,
+                    c->pk.slot_pbeg, c->pk.sc_w, c->pk.sc_dffo, c->pk.slot_w, c->pk.slot_dffo,
+                    c->pk.dffs, c->patch_quad, c->quad_pbeg, c->quad_nbr, c->qval,
                     c->viewfr, c->view_sh};
     for (void *p : ptrs)
         dfree(p);
@@ -477,7 +483,7 @@ void pack_free(rad_ctx *c)
     PackBuffers &p = c->pk;
     void *ptrs[] = {p.sc_tbeg, p.sc_pbeg, p.sc_slots, p.sc_slot_beg, p.sc_nsel, p.sc_done, p.sc_den,
                     p.sc_res,  p.sc_shot, p.cand,    p.patch_scene, p.pass,     p.slot_tbeg,
-                    p.slot_tcnt, p.slot_pbeg};
+                    p.slot_tcnt, p.slot_pbeg, p.sc_w, p.sc_dffo, p.slot_w, p.slot_dffo, p.dffs};
     for (void *q : ptrs)
         dfree(q);
     c->pk = PackBuffers{};
@@ -759,6 +765,10 @@ int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_
     return check_overflow(c);
 }
 
+static int ensure_events(rad_ctx *c, size_t n);
+
+// Standalone ProcessHemicube of the item buffer in item_ids (k hemicubes),
+// timed with HIP events when timing is on (rad_reduce_kernel_ms).
 static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
 {
     // F rows [0, max(k, ff_dirty)) and hit rows [0, k) start from zero
@@ -766,12 +776,16 @@ static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
     HIPCHK(c, hipMemsetAsync(c->ff, 0, (size_t)rows * c->rowlen * 4, c->st));
     HIPCHK(c, hipMemsetAsync(c->hits, 0, (size_t)k * c->npatch * 4, c->st));
     HIPCHK(c, hipMemsetAsync(c->counters, 0, kCntN * 4, c->st));
+    int rc;
+    if (c->timing && (rc = ensure_events(c, 2)))
+        return rc;
+    record(c, 0);
     launch_reduce(c->g, k, c->nshoot, c->item_ids, c->dff, c->npatch, c->ff,
-                  c->det ? c->ffq : nullptr, c->hits, c->counters, c->st);
+                  c->det ? c->ffq : nullptr, out_hits ? c->hits : nullptr, c->counters, c->st);
+    record(c, 1);
     if (c->det)
         launch_fold(c->ffq, c->ff, (size_t)k * c->npatch, c->st);
     c->ff_dirty = k;
-    int rc;
     if ((rc = check_launch(c, "reduce")))
         return rc;
     size_t nf = (size_t)k * c->npatch;
@@ -780,6 +794,11 @@ static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
     if (out_ff)
         HIPCHK(c, hipMemcpyAsync(out_ff, c->ff, nf * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (c->timing) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+        c->t_reduce = ms;
+    }
     return RAD_OK;
 }
 
@@ -1196,6 +1215,14 @@ int rad_kernel_times(rad_ctx *c, int enable, double *out6, uint64_t *npasses)
             c->t_acc[k] = 0;
         c->t_passes = 0;
     }
+    return RAD_OK;
+}
+
+int rad_reduce_kernel_ms(rad_ctx *c, double *ms)
+{
+    if (!c || !ms)
+        return RAD_E_ARG;
+    *ms = c->t_reduce;
     return RAD_OK;
 }
 
@@ -1635,6 +1662,99 @@ int rad_gather_solve(rad_ctx *c, const float *rhs, uint32_t R, const rad_gather_
     return RAD_OK;
 }
 
+int rad_gather_check_fp64(rad_ctx *c, const float *B, const float *E, uint32_t R, double *out2)
+{
+    if (!c || !B || !out2)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_plain(c)))
+        return rc;
+    if (R == 0 || R > 16 || (!E && R != 3))
+        return fail(c, RAD_E_ARG, "gather_check: R must be 1..16 (3 without E)");
+    const uint32_t N = c->npatch;
+    const size_t n = (size_t)R * N;
+    std::vector<float> eh;
+    if (!E) {
+        std::vector<float> em((size_t)N * 3);
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, hipMemcpy(em.data(), c->emis, em.size() * 4, hipMemcpyDeviceToHost));
+        eh.resize(n);
+        for (uint32_t i = 0; i < N; ++i)
+            for (uint32_t ch = 0; ch < 3; ++ch)
+                eh[(size_t)ch * N + i] = em[(size_t)i * 3 + ch];
+        E = eh.data();
+    }
+    const uint32_t nblk = 1024;
+    float *dB = nullptr, *dE = nullptr;
+    double *acc = nullptr, *parts = nullptr;
+    unsigned long long *q = nullptr;
+    auto cleanup = [&]() {
+        dfree(dB);
+        dfree(dE);
+        dfree(acc);
+        dfree(parts);
+        dfree(q);
+    };
+    if ((rc = dalloc(c, &dB, n)) || (rc = dalloc(c, &dE, n)) || (rc = dalloc(c, &acc, n)) ||
+        (rc = dalloc(c, &parts, 2 * nblk)) || (rc = dalloc(c, &q, 1))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(dB, B, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(dE, E, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemset(acc, 0, n * 8);
+    if (e == hipSuccess)
+        e = hipMemset(q, 0, 8);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(c, RAD_E_HIP, std::string("gather_check: ") + hipGetErrorString(e));
+    }
+    bool timing = c->timing;
+    c->timing = false;
+    // every patch's hemicube once, K per pass, accumulated as it is rendered
+    for (uint32_t base = 0; base < N && rc == RAD_OK; base += c->K) {
+        uint32_t k = N - base < c->K ? N - base : c->K;
+        launch_iota_shooters(base, k, c->shooters, c->nshoot, c->st);
+        if ((rc = render_pass(c, k, 0, nullptr, nullptr, nullptr)))
+            break;
+        launch_check_accum(c->ff, k, base, c->rowlen, N, R, dB, c->area, acc, c->st);
+        launch_check_rowsum(c->ff, k, c->rowlen, N, c->refl, q, c->st);
+        rc = check_launch(c, "gather_check pass");
+    }
+    c->timing = timing;
+    c->rendered = false;
+    if (rc == RAD_OK) {
+        launch_check_final(R, N, acc, dE, dB, c->refl, c->area, parts, nblk, c->st);
+        rc = check_launch(c, "gather_check final");
+    }
+    std::vector<double> h(2 * nblk);
+    unsigned long long qb = 0;
+    if (rc == RAD_OK) {
+        e = hipMemcpyAsync(h.data(), parts, h.size() * 8, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&qb, q, 8, hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess)
+            rc = fail(c, RAD_E_HIP, std::string("gather_check: ") + hipGetErrorString(e));
+    }
+    cleanup();
+    if (rc)
+        return rc;
+    if ((rc = check_overflow(c)))
+        return rc;
+    double num = 0.0, den = 0.0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        num += h[b];
+        den += h[nblk + b];
+    }
+    out2[0] = den > 0 ? num / den : 0.0;
+    memcpy(&out2[1], &qb, 8);
+    return RAD_OK;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -1656,7 +1776,6 @@ int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
     if (scene_tri_beg[0] != 0 || scene_tri_beg[nscene] != ntri || scene_patch_beg[0] != 0 ||
         scene_patch_beg[nscene] != npatch)
         return fail(c, RAD_E_ARG, "pack: offsets must start at 0 and end at ntri / npatch");
-    uint64_t slots = 0;
     uint32_t nblk = 1, rowlen = 1, kmax = 1;
     std::vector<uint32_t> patch_scene(npatch);
     for (uint32_t s = 0; s < nscene; ++s) {
@@ -1671,15 +1790,12 @@ int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
                                               " names a patch outside its scene");
         if (scene_slots[s] == 0 || scene_slots[s] > c->K)
             return fail(c, RAD_E_ARG, "pack: shooters per scene must be 1..K");
-        slots += scene_slots[s];
         nblk = std::max(nblk, (t1 - t0 + 255) / 256);
         rowlen = std::max(rowlen, p1 - p0);
         kmax = std::max(kmax, scene_slots[s]);
         for (uint32_t p = p0; p < p1; ++p)
             patch_scene[p] = s;
     }
-    if (slots > c->K)
-        return fail(c, RAD_E_ARG, "pack: shooters per pass of all scenes exceed K");
     if ((rc = upload_impl(c, tri_xyz, tri_patch, ntri, npatch, frames, refl, area, nblk, rowlen)))
         return rc;
     PackBuffers &p = c->pk;
@@ -1729,6 +1845,63 @@ int rad_pack_upload(rad_ctx *c, uint32_t nscene, const uint32_t *scene_tri_beg,
     p.K = c->K;
     p.shooters = c->shooters;
     p.nshoot = c->nshoot;
+    return RAD_OK;
+}
+
+int rad_pack_hemicubes(rad_ctx *c, const uint32_t *scene_w)
+{
+    if (!c || !scene_w)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    PackBuffers &p = c->pk;
+    if (!p.nscene)
+        return fail(c, RAD_E_STATE, "no scene pack (rad_pack_upload)");
+    const uint32_t S = p.nscene;
+    std::vector<uint32_t> ws;
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint32_t w = scene_w[s];
+        if (w < 2 || (w & 1) || w > c->g.W)
+            return fail(c, RAD_E_ARG, "pack_hemicubes: scene " + std::to_string(s) +
+                                          ": hemicube side must be even, 2 .. the context's");
+        ws.push_back(w);
+    }
+    std::sort(ws.begin(), ws.end());
+    ws.erase(std::unique(ws.begin(), ws.end()), ws.end());
+    // one ΔFF table per distinct side, concatenated
+    std::vector<uint32_t> off(ws.size());
+    size_t total = 0;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        off[i] = (uint32_t)total;
+        total += (size_t)3 * ws[i] * ws[i];
+    }
+    std::vector<uint32_t> dffo(S), slot_fill(c->K, c->g.W);
+    for (uint32_t s = 0; s < S; ++s)
+        dffo[s] = off[std::lower_bound(ws.begin(), ws.end(), scene_w[s]) - ws.begin()];
+    if ((rc = dalloc(c, &p.dffs, total)) || (rc = dalloc(c, &p.sc_w, S)) ||
+        (rc = dalloc(c, &p.sc_dffo, S)) || (rc = dalloc(c, &p.slot_w, c->K)) ||
+        (rc = dalloc(c, &p.slot_dffo, c->K))) {
+        dfree(p.dffs);
+        dfree(p.sc_w);
+        dfree(p.sc_dffo);
+        dfree(p.slot_w);
+        dfree(p.slot_dffo);
+        p.dffs = nullptr;
+        p.sc_w = p.sc_dffo = p.slot_w = p.slot_dffo = nullptr;
+        return rc;
+    }
+    for (size_t i = 0; i < ws.size(); ++i) {
+        Geometry gw{ws[i], 2 * ws[i], ws[i] + ws[i] / 2, 3 * ws[i] * ws[i]};
+        launch_dff(gw, p.dffs + off[i], c->st);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(p.sc_w, scene_w, S * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(p.sc_dffo, dffo.data(), S * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(p.slot_w, slot_fill.data(), c->K * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(p.slot_dffo, 0, c->K * 4, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    drop_graph(c); // captured passes must read the per-slot sides
     return RAD_OK;
 }
 

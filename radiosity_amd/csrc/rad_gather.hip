@@ -195,11 +195,13 @@ template <int RT> __device__ __forceinline__ uint4 gemm_aload(const uint16_t *p)
 // Wave w owns rows i0 .. i0+31 (two 16-row A fragments) and all RT column
 // tiles of 16.  Per 64-wide k chunk the X slice (RT*16 rows x 64 bf16) is
 // staged in LDS and shared by the waves; the A fragments go straight
-// from HBM to registers (each F element is used by one wave only), with the
-// next chunk's loads issued before the current chunk's MFMAs.
+// from HBM to registers (each F element is used by one wave only).  PF
+// chunks of A fragments are in flight ahead of the one being multiplied
+// (PF = 2: two register sets, the loop unrolled by two chunks), so a load
+// has PF chunks of MFMA work to arrive in.
 // SPLIT accumulates Fhi Xhi + Fhi Xlo + Flo Xhi (about 16 significant bits
 // of F and X) for the refinement residual.
-template <int RT, bool SPLIT, int WAVES>
+template <int RT, bool SPLIT, int WAVES, int PF>
 __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict__ fhi,
                                               const uint16_t *__restrict__ flo,
                                               const uint16_t *__restrict__ xhi,
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
                                               uint32_t kspan, uint32_t nslice,
                                               float *__restrict__ yp)
 {
+    static_assert(PF == 1 || PF == 2, "one or two chunks of A in flight");
     constexpr int NB = SPLIT ? 2 : 1;
     constexpr int LROW = 72;               // bf16 per LDS row: 64 + 8 pad (144 B)
     constexpr int UNITS = RT * 16 * 8;     // 16-byte units of one X chunk
@@ -244,11 +247,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     // lo): named variables, which stay in VGPRs where an array was kept in
     // scratch memory
     uint4 xh0, xh1, xh2, xl0, xl1, xl2;
-    uint4 ar[NB][2][2];
+    uint4 ar[NB][2][2], br[NB][2][2], ac[NB][2][2];
 
-    // X chunk kc into registers (staged to LDS at the top of the next
-    // iteration) and the A fragments of chunk kc.  The matrix is streamed once per product: with several column tiles
-    // non-temporal loads keep it from evicting X out of L2.
+    // X chunk kc into registers (staged to LDS at the top of the chunk's
+    // iteration) and the A fragments of chunk kc.  The matrix is streamed
+    // once per product: with several column tiles non-temporal loads keep it
+    // from evicting X out of L2.
 #define RAD_GEMM_XUNIT(u_, h_, l_, kc_)                                                \
     if (FULL || tid + (u_) * NT < UNITS) {                                             \
         const int idx = tid + (u_) * NT;                                               \
@@ -267,68 +271,100 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
             RAD_GEMM_XUNIT(2, xh2, xl2, kc_)                                           \
         }                                                                              \
     }
-#define RAD_GEMM_LOAD_A(kc_)                                                           \
+#define RAD_GEMM_LOAD_A(dst_, kc_)                                                     \
     _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                                   \
     {                                                                                  \
-        ar[0][0][ks] = gemm_aload<RT>(fhi + arow0 + (kc_) + ks * 32);                  \
-        ar[0][1][ks] = gemm_aload<RT>(fhi + arow1 + (kc_) + ks * 32);                  \
+        dst_[0][0][ks] = gemm_aload<RT>(fhi + arow0 + (kc_) + ks * 32);                \
+        dst_[0][1][ks] = gemm_aload<RT>(fhi + arow1 + (kc_) + ks * 32);                \
         if (SPLIT) {                                                                   \
-            ar[NB - 1][0][ks] = gemm_aload<RT>(flo + arow0 + (kc_) + ks * 32);         \
-            ar[NB - 1][1][ks] = gemm_aload<RT>(flo + arow1 + (kc_) + ks * 32);         \
+            dst_[NB - 1][0][ks] = gemm_aload<RT>(flo + arow0 + (kc_) + ks * 32);       \
+            dst_[NB - 1][1][ks] = gemm_aload<RT>(flo + arow1 + (kc_) + ks * 32);       \
+        }                                                                              \
+    }
+    // X of the chunk to LDS buffer cb_ (after the barrier that frees it)
+#define RAD_GEMM_STAGE_X(cb_)                                                          \
+    {                                                                                  \
+        if (DB == 1)                                                                   \
+            __syncthreads();                                                           \
+        _Pragma("unroll") for (int u = 0; u < UPT; ++u)                                \
+        {                                                                              \
+            int idx = tid + u * NT;                                                    \
+            if (FULL || idx < UNITS) {                                                 \
+                const uint4 h = u == 0 ? xh0 : u == 1 ? xh1 : xh2;                     \
+                *(uint4 *)&lds[cb_][0][idx >> 3][(idx & 7) * 8] = h;                   \
+                if (SPLIT) {                                                           \
+                    const uint4 l = u == 0 ? xl0 : u == 1 ? xl1 : xl2;                 \
+                    *(uint4 *)&lds[cb_][NB - 1][idx >> 3][(idx & 7) * 8] = l;          \
+                }                                                                      \
+            }                                                                          \
+        }                                                                              \
+        __syncthreads();                                                               \
+    }
+#define RAD_GEMM_TAKE(src_)                                                            \
+    _Pragma("unroll") for (int b = 0; b < NB; ++b)                                     \
+        _Pragma("unroll") for (int f = 0; f < 2; ++f)                                  \
+            _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                           \
+                ac[b][f][ks] = src_[b][f][ks];
+#define RAD_GEMM_MFMA(cb_)                                                             \
+    _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                                   \
+    {                                                                                  \
+        _Pragma("unroll") for (int t = 0; t < RT; ++t)                                 \
+        {                                                                              \
+            bf16x8 bh = *(const bf16x8 *)&lds[cb_][0][t * 16 + m][ks * 32 + 8 * kg];   \
+            _Pragma("unroll") for (int f = 0; f < 2; ++f)                              \
+            {                                                                          \
+                bf16x8 ah = __builtin_bit_cast(bf16x8, ac[0][f][ks]);                  \
+                acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[f][t], 0, 0, 0); \
+                if (SPLIT) {                                                           \
+                    bf16x8 bl = *(const bf16x8 *)&lds[cb_][NB - 1][t * 16 + m][ks * 32 + 8 * kg]; \
+                    bf16x8 al = __builtin_bit_cast(bf16x8, ac[NB - 1][f][ks]);         \
+                    acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[f][t], 0, 0, 0); \
+                    acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[f][t], 0, 0, 0); \
+                }                                                                      \
+            }                                                                          \
         }                                                                              \
     }
 
     RAD_GEMM_LOAD_X(kbeg)
-    RAD_GEMM_LOAD_A(kbeg)
-    for (uint32_t kc = kbeg, ci = 0; kc < kend; kc += 64, ++ci) {
-        const int cb = DB == 2 ? (int)(ci & 1u) : 0;
-        // one buffer: every wave must be done with the previous chunk; two:
-        // the buffer written now was last read two chunks ago, before the
-        // barrier of the previous chunk
-        if (DB == 1)
-            __syncthreads();
-#pragma unroll
-        for (int u = 0; u < UPT; ++u) {
-            int idx = tid + u * NT;
-            if (FULL || idx < UNITS) {
-                const uint4 h = u == 0 ? xh0 : u == 1 ? xh1 : xh2;
-                *(uint4 *)&lds[cb][0][idx >> 3][(idx & 7) * 8] = h;
-                if (SPLIT) {
-                    const uint4 l = u == 0 ? xl0 : u == 1 ? xl1 : xl2;
-                    *(uint4 *)&lds[cb][NB - 1][idx >> 3][(idx & 7) * 8] = l;
-                }
+    RAD_GEMM_LOAD_A(ar, kbeg)
+    if constexpr (PF == 1) {
+        for (uint32_t kc = kbeg, ci = 0; kc < kend; kc += 64, ++ci) {
+            const int cb = DB == 2 ? (int)(ci & 1u) : 0;
+            RAD_GEMM_STAGE_X(cb)
+            RAD_GEMM_TAKE(ar)
+            if (kc + 64 < kend) {
+                RAD_GEMM_LOAD_X(kc + 64)
+                RAD_GEMM_LOAD_A(ar, kc + 64)
             }
+            RAD_GEMM_MFMA(cb)
         }
-        __syncthreads();
-        uint4 ac[NB][2][2];
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int f = 0; f < 2; ++f)
-#pragma unroll
-                for (int ks = 0; ks < 2; ++ks)
-                    ac[b][f][ks] = ar[b][f][ks];
-        if (kc + 64 < kend) {
-            RAD_GEMM_LOAD_X(kc + 64)
-            RAD_GEMM_LOAD_A(kc + 64)
-        }
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-#pragma unroll
-            for (int t = 0; t < RT; ++t) {
-                bf16x8 bh = *(const bf16x8 *)&lds[cb][0][t * 16 + m][ks * 32 + 8 * kg];
-#pragma unroll
-                for (int f = 0; f < 2; ++f) {
-                    bf16x8 ah = __builtin_bit_cast(bf16x8, ac[0][f][ks]);
-                    acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[f][t], 0, 0, 0);
-                    if (SPLIT) {
-                        bf16x8 bl = *(const bf16x8 *)&lds[cb][NB - 1][t * 16 + m][ks * 32 + 8 * kg];
-                        bf16x8 al = __builtin_bit_cast(bf16x8, ac[NB - 1][f][ks]);
-                        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[f][t], 0, 0, 0);
-                        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[f][t], 0, 0, 0);
-                    }
-                }
-            }
+    } else {
+        if (kbeg + 64 < kend)
+            RAD_GEMM_LOAD_A(br, kbeg + 64)
+        // chunks in pairs: the first of a pair in ar and LDS buffer 0, the
+        // second in br and buffer DB - 1; each chunk's X is loaded one chunk
+        // ahead, its A two chunks ahead
+        // (scheduling barriers keep the two halves apart: merged, their LDS
+        // reads were hoisted together and the registers spilled)
+        for (uint32_t kc = kbeg; kc < kend; kc += 128) {
+            RAD_GEMM_STAGE_X(0)
+            RAD_GEMM_TAKE(ar)
+            if (kc + 64 < kend)
+                RAD_GEMM_LOAD_X(kc + 64)
+            if (kc + 128 < kend)
+                RAD_GEMM_LOAD_A(ar, kc + 128)
+            RAD_GEMM_MFMA(0)
+            __builtin_amdgcn_sched_barrier(0);
+            if (kc + 64 >= kend)
+                break;
+            RAD_GEMM_STAGE_X(DB - 1)
+            RAD_GEMM_TAKE(br)
+            if (kc + 128 < kend)
+                RAD_GEMM_LOAD_X(kc + 128)
+            if (kc + 192 < kend)
+                RAD_GEMM_LOAD_A(br, kc + 192)
+            RAD_GEMM_MFMA(DB - 1)
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     // lane holds rows 4*kg .. 4*kg+3 of column m of each 16x16 tile
@@ -342,6 +378,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
 #undef RAD_GEMM_LOAD_X
 #undef RAD_GEMM_XUNIT
 #undef RAD_GEMM_LOAD_A
+#undef RAD_GEMM_STAGE_X
+#undef RAD_GEMM_TAKE
+#undef RAD_GEMM_MFMA
 
 // ------------------------------------------------------- vector updates
 
@@ -465,6 +504,132 @@ __global__ void k_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S
     }
 }
 
+// ---------------------------------------------------------- FP64 checks
+// The FP64 residual of a gather solution against freshly rendered FP32
+// hemicube rows (shooting layout M_ji = F_ij A_i / A_j), streamed one pass
+// of k rows at a time:
+//  k_check_accum   acc[r][j] += sum_s F[s][j] * A_i B[r][i]  (i = row0 + s),
+//                  one lane per receiver j, FP64
+//  k_check_rowsum  q = max over rows and channels of sum_j rho_jc F_ij
+//                  (one workgroup per row; positive doubles ordered as u64)
+//  k_check_final   r = E + rho acc / A - B, partial sums of A|r| and A|B|
+constexpr uint32_t kCheckMaxR = 16;
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_check_accum(const float *__restrict__ ff, uint32_t k,
+                                                     uint32_t row0, uint32_t rowlen, uint32_t n,
+                                                     uint32_t R, const float *__restrict__ B,
+                                                     const float *__restrict__ area,
+                                                     double *__restrict__ acc)
+{
+    __shared__ double w[kCheckMaxR][RAD_MAX_K];
+    for (uint32_t e = threadIdx.x; e < R * k; e += blockDim.x) {
+        const uint32_t r = e / k, s = e - r * k, i = row0 + s;
+        w[r][s] = (double)area[i] * (double)B[(size_t)r * n + i];
+    }
+    __syncthreads();
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    double a[kCheckMaxR];
+    for (uint32_t r = 0; r < R; ++r)
+        a[r] = 0.0;
+    for (uint32_t s = 0; s < k; ++s) {
+        const float f = ff[(size_t)s * rowlen + j];
+        if (f == 0.0f)
+            continue;
+        for (uint32_t r = 0; r < R; ++r)
+            a[r] += (double)f * w[r][s];
+    }
+    for (uint32_t r = 0; r < R; ++r)
+        acc[(size_t)r * n + j] += a[r];
+}
+
+__global__ __launch_bounds__(1024) void k_check_rowsum(const float *__restrict__ ff, uint32_t rowlen,
+                                                       uint32_t n, const float *__restrict__ refl,
+                                                       unsigned long long *__restrict__ qbits)
+{
+    __shared__ double part[3][16];
+    const uint32_t s = blockIdx.x;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+        const double f = (double)ff[(size_t)s * rowlen + j];
+        c0 += f * (double)refl[(size_t)j * 3 + 0];
+        c1 += f * (double)refl[(size_t)j * 3 + 1];
+        c2 += f * (double)refl[(size_t)j * 3 + 2];
+    }
+    c0 = wave_sum_d(c0);
+    c1 = wave_sum_d(c1);
+    c2 = wave_sum_d(c2);
+    const uint32_t wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][wv] = c0;
+        part[1][wv] = c1;
+        part[2][wv] = c2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            double t = 0.0;
+            for (uint32_t q = 0; q < (blockDim.x >> 6); ++q)
+                t += part[c][q];
+            m = t > m ? t : m;
+        }
+        atomicMax(qbits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_check_final(uint32_t R, uint32_t n,
+                                                     const double *__restrict__ acc,
+                                                     const float *__restrict__ E,
+                                                     const float *__restrict__ B,
+                                                     const float *__restrict__ refl,
+                                                     const float *__restrict__ area,
+                                                     double *__restrict__ parts)
+{
+    double num = 0.0, den = 0.0;
+    const size_t total = (size_t)R * n;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(e / n), j = (uint32_t)(e - (size_t)r * n);
+        const double a = (double)area[j];
+        const double res = (double)E[e] + (double)refl[(size_t)j * 3 + r % 3] * acc[e] / a -
+                           (double)B[e];
+        num += a * fabs(res);
+        den += a * fabs((double)B[e]);
+    }
+    block_sum_store(num, parts);
+    __syncthreads(); // block_sum_store's LDS is reused
+    block_sum_store(den, parts + gridDim.x);
+}
+
+void launch_check_accum(const float *ff, uint32_t k, uint32_t row0, uint32_t rowlen, uint32_t n,
+                        uint32_t R, const float *B, const float *area, double *acc,
+                        hipStream_t st)
+{
+    k_check_accum<<<(n + 255) / 256, 256, 0, st>>>(ff, k, row0, rowlen, n, R, B, area, acc);
+}
+
+void launch_check_rowsum(const float *ff, uint32_t k, uint32_t rowlen, uint32_t n,
+                         const float *refl, unsigned long long *qbits, hipStream_t st)
+{
+    k_check_rowsum<<<k, 1024, 0, st>>>(ff, rowlen, n, refl, qbits);
+}
+
+void launch_check_final(uint32_t R, uint32_t n, const double *acc, const float *E, const float *B,
+                        const float *refl, const float *area, double *parts, uint32_t nblocks,
+                        hipStream_t st)
+{
+    k_check_final<<<nblocks, 256, 0, st>>>(R, n, acc, E, B, refl, area, parts);
+}
+
 // --------------------------------------------------------------- launchers
 
 void launch_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, uint32_t *nshoot,
@@ -518,17 +683,24 @@ uint32_t gemm_tiles(uint32_t R)
 
 // Wide products use 8 waves (256 rows) per workgroup so every X chunk staged
 // in LDS serves twice the rows; narrow ones 4 waves (more workgroups).
+#ifndef RAD_GEMM_PF
+#define RAD_GEMM_PF 1
+#endif
 template <int RT>
 static void gemm_rt(bool split, const uint16_t *fhi, const uint16_t *flo, const uint16_t *xhi,
                     const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp, hipStream_t st)
 {
     constexpr int WV = RT >= 4 ? 8 : 4;
+    // two chunks of A in flight for the single-half sweep; the split product
+    // already holds both halves of A and X (PF = 2 would cost occupancy)
+    constexpr int PFS = RT >= 4 ? RAD_GEMM_PF : 1;
     const uint32_t nwg = (npad / (WV * 32)) * S;
     uint32_t kspan = npad / S;
     if (split)
-        k_gemm<RT, true, WV><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
+        k_gemm<RT, true, WV, 1><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
     else
-        k_gemm<RT, false, WV><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
+        k_gemm<RT, false, WV, PFS><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S,
+                                                            yp);
 }
 
 void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,

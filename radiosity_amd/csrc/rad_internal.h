@@ -68,7 +68,8 @@ struct BinBuffers {
     RasterRec *recs;       // nblk*K regions of kRecsPerBlock, then spill_cap
     uint64_t *bbox;        // texel bbox per record slot, read by the bin fill
     uint32_t *blk_nrec;    // records per (shooter, setup block)
-    uint32_t *count, *start, *fill, *list;
+    uint32_t *count, *start, *list;
+    uint32_t *fill;        // 2 x bins: front and back fill counters of each bin list
     uint32_t *order;       // bins by descending size class (tile launch order)
     uint32_t nblk;         // setup blocks per shooter and face group (256 triangles each)
     uint32_t nz;           // face groups: 5 (one face per block, small K) or 1
@@ -78,6 +79,7 @@ struct BinBuffers {
     int clip_blocks;
     uint32_t ntx, nty, ntiles, tshift;
     uint32_t setup_group;  // shooter slots one setup workgroup walks
+    uint32_t split_px;     // records covering more texels of a tile go to the back of its list
 };
 
 struct Geometry {
@@ -106,6 +108,11 @@ struct PackBuffers {
     uint32_t *patch_scene = nullptr;                 // npatch
     uint32_t *pass = nullptr;                        // passes selected so far
     uint32_t *slot_tbeg = nullptr, *slot_tcnt = nullptr, *slot_pbeg = nullptr; // K
+    // scenes at their own hemicube resolution (rad_pack_hemicubes; null: all
+    // at the context's): per scene W and offset of its ΔFF table in dffs,
+    // the same per slot of the pass
+    uint32_t *sc_w = nullptr, *sc_dffo = nullptr, *slot_w = nullptr, *slot_dffo = nullptr;
+    float *dffs = nullptr;
     uint32_t *shooters = nullptr, *nshoot = nullptr; // the context's own
 };
 
@@ -143,14 +150,16 @@ void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, u
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt, hipStream_t st);
+                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt,
+                         const uint32_t *slot_w, hipStream_t st);
 void launch_tri_sphere(const float *tri, uint32_t ntri, float4 *sph, hipStream_t st);
 void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
                 hipStream_t st);
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
                  uint32_t small_px, uint32_t *out_ids, float *out_depth, const float *dff,
                  uint32_t rowlen, const uint32_t *slot_pbeg, float *ff, unsigned long long *ffq,
-                 uint32_t *hits, uint32_t *counters, hipStream_t st);
+                 uint32_t *hits, uint32_t *counters, const uint32_t *slot_w,
+                 const uint32_t *slot_dffo, hipStream_t st);
 // Deterministic mode: fixed-point F entries (2^-52 units) into the float rows.
 void launch_fold(unsigned long long *ffq, float *ff, size_t n, hipStream_t st);
 // ProcessHemicube over an id plane (K x RES u32): ff (or ffq) and hits +=.
@@ -200,6 +209,16 @@ void launch_gather_update(int kind, const float *yp, uint32_t S, uint32_t rp, ui
                           uint32_t npad, uint32_t n, const float *E, const float *refl,
                           const float *area, float *rv, float *B, uint16_t *xhi, uint16_t *xlo,
                           double *parts, uint32_t nblocks, hipStream_t st);
+// FP64 residual check of a gather solution against rendered FP32 rows
+// (shooting layout), one pass of k rows at a time; R <= kCheckMaxR (16).
+void launch_check_accum(const float *ff, uint32_t k, uint32_t row0, uint32_t rowlen, uint32_t n,
+                        uint32_t R, const float *B, const float *area, double *acc,
+                        hipStream_t st);
+void launch_check_rowsum(const float *ff, uint32_t k, uint32_t rowlen, uint32_t n,
+                         const float *refl, unsigned long long *qbits, hipStream_t st);
+void launch_check_final(uint32_t R, uint32_t n, const double *acc, const float *E, const float *B,
+                        const float *refl, const float *area, double *parts, uint32_t nblocks,
+                        hipStream_t st);
 void launch_gather_out(uint32_t R, uint32_t npad, uint32_t n, const float *B, const float *rv,
                        float *out, hipStream_t st);
 void launch_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *src, float *dst,

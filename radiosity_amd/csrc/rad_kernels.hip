@@ -668,7 +668,8 @@ __device__ static inline void setup_face(const float *mm, float4 scis, const flo
 __global__ __launch_bounds__(64) void k_setup_clipped(uint32_t W, const float *__restrict__ tri,
                                                        const uint32_t *__restrict__ tri_patch,
                                                        uint32_t ntri, const float *__restrict__ mvp,
-                                                       BinCtx bc)
+                                                       BinCtx bc,
+                                                       const uint32_t *__restrict__ slot_w)
 {
     __shared__ CV4 lp[2][kClipMax][64];
     uint32_t n = min(bc.counters[kCntClip], bc.clip_cap);
@@ -682,8 +683,9 @@ __global__ __launch_bounds__(64) void k_setup_clipped(uint32_t W, const float *_
                 vx[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
         CV4 poly[3];
         transform3(mvp + (size_t)s * kShooterStride + face * 16, vx, poly);
-        FaceView fv = face_view((int32_t)W, face);
-        emit_clipped(poly, 0.5f * (float)W, tri_patch[t] + 1u, fv, s, face, bc, lp, threadIdx.x);
+        const uint32_t Ws = slot_w ? slot_w[s] : W;
+        FaceView fv = face_view((int32_t)Ws, face);
+        emit_clipped(poly, 0.5f * (float)Ws, tri_patch[t] + 1u, fv, s, face, bc, lp, threadIdx.x);
     }
 }
 
@@ -738,6 +740,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                                                       BinCtx bc, uint32_t *__restrict__ blk_nrec,
                                                       const uint32_t *__restrict__ slot_tbeg,
                                                       const uint32_t *__restrict__ slot_tcnt,
+                                                      const uint32_t *__restrict__ slot_w,
                                                       uint32_t group, uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[];
@@ -749,8 +752,9 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     __shared__ float4 lscis[5];
     __shared__ float lmvp[4][80]; // the current shooters' five face MVPs
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
-    if (threadIdx.x < 5)
-        lscis[threadIdx.x] = face_scissor_clip((int32_t)W, (int)threadIdx.x);
+    // hemicube resolution of the slots being drawn (per slot in a pack of
+    // scenes at several resolutions; slots drawn together share a scene)
+    uint32_t lw = 0u;
     const uint32_t tl0 = blockIdx.x * blockDim.x;
     // gridDim.z == 5 splits the faces over blocks (small K), else one block
     // walks all five faces
@@ -788,6 +792,12 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
             continue;
         }
         __syncthreads(); // the previous slots' phase 2 and flush are done with LDS
+        const uint32_t Ws = slot_w ? slot_w[s] : W;
+        if (Ws != lw) {
+            lw = Ws;
+            if (threadIdx.x < 5)
+                lscis[threadIdx.x] = face_scissor_clip((int32_t)Ws, (int)threadIdx.x);
+        }
         if (tb != ltb || tc != ltc) {
             ltb = tb;
             ltc = tc;
@@ -843,7 +853,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
             b2.region = rbof(s + sh) * kRecsPerBlock;
             b2.lnrec = &lnrec2[sh];
             b2.lcnt = lcnt + sh * bc.ntiles;
-            setup_face(lmvp[sh] + face * 16, lscis[face], wv, W, s + sh, face, lid[tl],
+            setup_face(lmvp[sh] + face * 16, lscis[face], wv, Ws, s + sh, face, lid[tl],
                        tb + tl0 + tl, b2);
         }
         __syncthreads();
@@ -969,7 +979,8 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
             const uint32_t i = c0 + k * 1024u + tid;
             lc[k * 1024u + tid] = v[k];
             if (i < n) {
-                fill[i] = 0u;
+                fill[i] = 0u;     // front (small records) counter of the bin
+                fill[n + i] = 0u; // back (larger records) counter
                 atomicAdd(&bucket[size_class(v[k])], 1u);
             }
         }
@@ -1043,6 +1054,13 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
 // Same grid as k_raster_setup: block (blk, s) puts the records of its region
 // into their bins, reserving each bin's range with one global atomic per
 // touched tile.  The extra column blk == nblk handles the spill region.
+// Each bin's list is filled from both ends: a record whose texel bbox covers
+// at most split_px texels of the tile from the front (counter fill[b]),
+// larger ones from the back (counter fill[nbins + b], down from count[b]).
+// The tile kernel takes a bin's records 256 at a time, one per lane, and a
+// lane draws a small record itself: lists ordered this way give a
+// wavefront records of similar size, so its lanes finish together.  The
+// depth test is order-independent, so the item buffer does not change.
 __device__ static inline void unpack_bbox(uint64_t v, uint32_t tshift, uint32_t &g0,
                                           uint32_t &g1, uint32_t &h0, uint32_t &h1)
 {
@@ -1052,21 +1070,37 @@ __device__ static inline void unpack_bbox(uint64_t v, uint32_t tshift, uint32_t 
     h1 = (uint32_t)(v >> 48) >> tshift;
 }
 
+// 1 when the record (texel bbox v) covers more than split_px texels of tile
+// (gx, gy): its place is at the back of that bin's list.
+__device__ static inline uint32_t bin_side(uint64_t v, uint32_t gx, uint32_t gy, uint32_t tshift,
+                                           uint32_t split_px)
+{
+    const uint32_t T = 1u << tshift;
+    const uint32_t x0 = max((uint32_t)(v & 0xFFFFu), gx << tshift);
+    const uint32_t y0 = max((uint32_t)((v >> 16) & 0xFFFFu), gy << tshift);
+    const uint32_t x1 = min((uint32_t)((v >> 32) & 0xFFFFu), (gx << tshift) + T - 1);
+    const uint32_t y1 = min((uint32_t)(v >> 48), (gy << tshift) + T - 1);
+    return (x1 - x0 + 1) * (y1 - y0 + 1) > split_px ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ recs,
                                                   const uint64_t *__restrict__ bbox,
                                                   const uint32_t *__restrict__ blk_nrec,
                                                   const uint32_t *__restrict__ counters,
                                                   uint32_t nblk, uint32_t spill_base,
                                                   uint32_t spill_cap,
+                                                  const uint32_t *__restrict__ count,
                                                   const uint32_t *__restrict__ start,
                                                   uint32_t *__restrict__ fill,
                                                   uint32_t *__restrict__ list, uint32_t ent_cap,
                                                   uint32_t ntx, uint32_t ntiles, uint32_t tshift,
                                                   const uint32_t *__restrict__ nshoot,
-                                                  uint32_t group, uint32_t nslot)
+                                                  uint32_t group, uint32_t nslot, uint32_t nbins,
+                                                  uint32_t split_px)
 {
     extern __shared__ uint32_t dyn_lds[];
-    uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + ntiles; // dynamic LDS, 2 x ntiles
+    // dynamic LDS, 4 x ntiles: per tile front / back counts, then bases
+    uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + 2 * ntiles;
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
     if (blockIdx.x == nblk) {
         if (blockIdx.z != 0)
@@ -1075,20 +1109,23 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             const uint32_t s = blockIdx.y * group + g;
             if (s >= ns)
                 break;
-            uint32_t *fl = fill + (size_t)s * ntiles;
-            const uint32_t *stt = start + (size_t)s * ntiles;
+            const size_t sb = (size_t)s * ntiles;
             // spill records of this shooter: one global atomic each (rare)
             uint32_t nsp = min(counters[kCntSpill], spill_cap);
             for (uint32_t q = threadIdx.x; q < nsp; q += blockDim.x) {
                 uint32_t r = spill_base + q;
                 if ((recs[r].sf >> 3) != s)
                     continue;
+                const uint64_t bv = bbox[r];
                 uint32_t g0, g1, h0, h1;
-                unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
+                unpack_bbox(bv, tshift, g0, g1, h0, h1);
                 for (uint32_t gy = h0; gy <= h1; ++gy)
                     for (uint32_t gx = g0; gx <= g1; ++gx) {
-                        uint32_t b = gy * ntx + gx;
-                        uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
+                        const size_t b = sb + gy * ntx + gx;
+                        const uint32_t pos =
+                            bin_side(bv, gx, gy, tshift, split_px)
+                                ? start[b] + count[b] - 1u - atomicAdd(&fill[nbins + b], 1u)
+                                : start[b] + atomicAdd(&fill[b], 1u);
                         if (pos < ent_cap)
                             list[pos] = r;
                     }
@@ -1119,8 +1156,8 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
         const uint32_t s = blockIdx.y * group + g;
         if (s >= ns)
             break;
-        uint32_t *fl = fill + (size_t)s * ntiles;
-        const uint32_t *stt = start + (size_t)s * ntiles;
+        uint32_t *fl = fill + (size_t)s * ntiles, *flb = fl + nbins;
+        const uint32_t *stt = start + (size_t)s * ntiles, *cnt = count + (size_t)s * ntiles;
         const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
         const uint32_t region = blk * kRecsPerBlock, n = nrec;
         uint64_t cur[kFillPer];
@@ -1132,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
         if (n == 0)
             continue;
         __syncthreads(); // the previous slot is done with lcnt / lbase
-        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x)
+        for (uint32_t i = threadIdx.x; i < 2 * ntiles; i += blockDim.x)
             lcnt[i] = 0u;
         __syncthreads();
 #pragma unroll
@@ -1143,13 +1180,18 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx)
-                    atomicAdd(&lcnt[gy * ntx + gx], 1u);
+                    atomicAdd(&lcnt[bin_side(cur[k], gx, gy, tshift, split_px) * ntiles +
+                                    gy * ntx + gx],
+                              1u);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
-            uint32_t c = lcnt[i];
-            lbase[i] = c ? stt[i] + atomicAdd(&fl[i], c) : 0u;
+            const uint32_t cf = lcnt[i], cb = lcnt[ntiles + i];
+            lbase[i] = cf ? stt[i] + atomicAdd(&fl[i], cf) : 0u;
+            // the back range grows down from the end of the bin
+            lbase[ntiles + i] = cb ? stt[i] + cnt[i] - atomicAdd(&flb[i], cb) - cb : 0u;
             lcnt[i] = 0u;
+            lcnt[ntiles + i] = 0u;
         }
         __syncthreads();
 #pragma unroll
@@ -1161,7 +1203,8 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx) {
-                    uint32_t b = gy * ntx + gx;
+                    const uint32_t b =
+                        bin_side(cur[k], gx, gy, tshift, split_px) * ntiles + gy * ntx + gx;
                     uint32_t pos = lbase[b] + atomicAdd(&lcnt[b], 1u);
                     if (pos < ent_cap)
                         list[pos] = region + q;
@@ -1398,7 +1441,8 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     float *__restrict__ out_depth, const float *__restrict__ dff,
     uint32_t rowlen, const uint32_t *__restrict__ slot_pbeg, float *__restrict__ ff,
     unsigned long long *__restrict__ ffq, uint32_t *__restrict__ hits,
-    uint32_t *__restrict__ counters, const uint32_t *__restrict__ order)
+    uint32_t *__restrict__ counters, const uint32_t *__restrict__ order,
+    const uint32_t *__restrict__ slot_w, const uint32_t *__restrict__ slot_dffo)
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     __shared__ uint32_t perm[256];
@@ -1409,12 +1453,23 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     if (s >= *nshoot)
         return;
     const uint32_t tile = bin - s * ntiles;
+    const size_t b = (size_t)s * ntiles + tile;
+    const uint32_t n = bin_count[b], st0 = bin_start[b];
+    // an empty bin adds nothing to the F rows (its texels are background)
+    if (n == 0 && !out_ids)
+        return;
+    // a slot of a pack may render at its scene's own resolution; tiles are
+    // laid out for the largest, so its slab uses the first ones
+    if (slot_w) {
+        W = slot_w[s];
+        TW = 2 * W;
+        TH = W + W / 2;
+        dff += slot_dffo[s];
+    }
     const uint32_t T = 1u << tshift, TT = T * T;
     const int32_t ox = (int32_t)((tile % ntx) << tshift), oy = (int32_t)((tile / ntx) << tshift);
     for (uint32_t p = threadIdx.x; p < TT; p += blockDim.x)
         tk[p] = kClearKey;
-    const size_t b = (size_t)s * ntiles + tile;
-    const uint32_t n = bin_count[b], st0 = bin_start[b];
     const int32_t Wi = (int32_t)W;
     const uint32_t me = threadIdx.x;
     for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
@@ -1482,7 +1537,8 @@ __global__ __launch_bounds__(256) void k_tile_raster(
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt, hipStream_t st)
+                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt,
+                         const uint32_t *slot_w, hipStream_t st)
 {
     BinCtx bc;
     bc.recs = bb.recs;
@@ -1504,10 +1560,10 @@ void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const 
         group = K;
     dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
     k_raster_setup<<<grid, 256, 4 * bb.ntiles * sizeof(uint32_t), st>>>(
-        g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec,
-                                         slot_tbeg, slot_tcnt, group, K);
+        g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec, slot_tbeg, slot_tcnt,
+        slot_w, group, K);
     bc.lnrec = nullptr;
-    k_setup_clipped<<<bb.clip_blocks, 64, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc);
+    k_setup_clipped<<<bb.clip_blocks, 64, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc, slot_w);
 }
 
 void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
@@ -1519,22 +1575,23 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
     if (group > K)
         group = K;
     dim3 grid(bb.nblk + 1, (K + group - 1) / group, bb.nz);
-    k_bin_fill<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
-        bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base,
-                                     bb.spill_cap, bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx,
-                                     bb.ntiles, bb.tshift, nshoot, group, K);
+    k_bin_fill<<<grid, 256, 4 * bb.ntiles * sizeof(uint32_t), st>>>(
+        bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap, bb.count,
+        bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K, nb,
+        bb.split_px);
 }
 
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
                  uint32_t small_px, uint32_t *out_ids, float *out_depth, const float *dff,
                  uint32_t rowlen, const uint32_t *slot_pbeg, float *ff, unsigned long long *ffq,
-                 uint32_t *hits, uint32_t *counters, hipStream_t st)
+                 uint32_t *hits, uint32_t *counters, const uint32_t *slot_w,
+                 const uint32_t *slot_dffo, hipStream_t st)
 {
     k_tile_raster<<<bb.ntiles * K, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox,
                                                  bb.count, bb.start, bb.list, bb.ent_cap, bb.ntx,
                                                  bb.ntiles, bb.tshift, small_px, out_ids, out_depth,
                                                  dff, rowlen, slot_pbeg, ff, ffq, hits, counters,
-                                                 bb.order);
+                                                 bb.order, slot_w, slot_dffo);
 }
 
 // Deterministic mode: n fixed-point entries (units of 2^-52) into the float

@@ -86,51 +86,137 @@ __global__ __launch_bounds__(256) void k_pack_select(
     }
     if (threadIdx.x == 0) {
         sc_res[sc] = res;
-        sc_nsel[sc] = nsel;
-        sc_shot[sc] += nsel;
+        sc_nsel[sc] = nsel; // the slot kernel may give the scene fewer
     }
 }
 
-// Slots of the pass: scenes in order, each its sc_nsel shooters; the slot
-// arrays carry the scene's triangle range and first patch.  One workgroup.
-__global__ __launch_bounds__(1024) void k_pack_slots(
-    uint32_t nscene, const uint32_t *__restrict__ sc_tbeg, const uint32_t *__restrict__ sc_pbeg,
-    const uint32_t *__restrict__ sc_nsel, const uint32_t *__restrict__ cand, uint32_t kmax,
-    uint32_t K, uint32_t *__restrict__ sc_slot_beg, uint32_t *__restrict__ shooters,
-    uint32_t *__restrict__ slot_tbeg, uint32_t *__restrict__ slot_tcnt,
-    uint32_t *__restrict__ slot_pbeg, uint32_t *__restrict__ nshoot, uint32_t *__restrict__ pass)
+// Slots of the pass, one workgroup.  Each active scene found sc_nsel <=
+// its quota of shooters; when they all fit in the K slots every scene keeps
+// them, otherwise the slots are shared out by water-filling: every scene
+// gets min(nsel, L) for the largest level L whose total fits, and the slots
+// left over go one each to scenes that wanted more, starting at a scene that
+// moves on every pass (so none is passed over for long).  sc_nsel becomes
+// the number the scene shoots, sc_shot counts them.  Slots are laid out
+// scene after scene; the slot arrays carry the scene's triangle range, first
+// patch, and, for scenes at their own resolution, hemicube side and ΔFF
+// table offset.
+__device__ static inline uint32_t block_sum_u32(uint32_t v, uint32_t *red)
 {
-    __shared__ uint32_t part[1024];
-    // each thread owns a contiguous run of scenes
-    const uint32_t per = (nscene + blockDim.x - 1) / blockDim.x;
-    const uint32_t a = threadIdx.x * per, b = a + per < nscene ? a + per : nscene;
-    uint32_t mine = 0;
-    for (uint32_t s = a; s < b; ++s)
-        mine += sc_nsel[s];
-    part[threadIdx.x] = mine;
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < (blockDim.x >> 6); ++w)
+        t += red[w];
+    return t;
+}
+
+// Inclusive scan of one value per thread through part[] (blockDim.x <= 1024).
+__device__ static inline uint32_t block_scan_u32(uint32_t v, uint32_t *part)
+{
+    __syncthreads();
+    part[threadIdx.x] = v;
     __syncthreads();
     for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
-        uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        uint32_t o = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
         __syncthreads();
-        part[threadIdx.x] += v;
+        part[threadIdx.x] += o;
         __syncthreads();
     }
-    uint32_t off = part[threadIdx.x] - mine;
+    return part[threadIdx.x];
+}
+
+__global__ __launch_bounds__(1024) void k_pack_slots(
+    uint32_t nscene, const uint32_t *__restrict__ sc_tbeg, const uint32_t *__restrict__ sc_pbeg,
+    uint32_t *__restrict__ sc_nsel, uint32_t *__restrict__ sc_shot,
+    const uint32_t *__restrict__ cand, uint32_t kmax, uint32_t K,
+    uint32_t *__restrict__ sc_slot_beg, uint32_t *__restrict__ shooters,
+    uint32_t *__restrict__ slot_tbeg, uint32_t *__restrict__ slot_tcnt,
+    uint32_t *__restrict__ slot_pbeg, const uint32_t *__restrict__ sc_w,
+    const uint32_t *__restrict__ sc_dffo, uint32_t *__restrict__ slot_w,
+    uint32_t *__restrict__ slot_dffo, uint32_t *__restrict__ nshoot, uint32_t *__restrict__ pass)
+{
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t red[16];
+    // each thread owns a contiguous run of scenes
+    const uint32_t per = (nscene + blockDim.x - 1) / blockDim.x;
+    const uint32_t a = threadIdx.x * per < nscene ? threadIdx.x * per : nscene;
+    const uint32_t b = a + per < nscene ? a + per : nscene;
+    auto level_total = [&](uint32_t L) {
+        uint32_t t = 0;
+        for (uint32_t s = a; s < b; ++s)
+            t += min(sc_nsel[s], L);
+        return block_sum_u32(t, red);
+    };
+    // largest L in [0, kmax] with sum_s min(nsel_s, L) <= K (binary search;
+    // the total only grows with L)
+    uint32_t lo = 0, hi = kmax;
+    if (level_total(kmax) <= K) {
+        lo = kmax;
+    } else {
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (level_total(mid) <= K)
+                lo = mid;
+            else
+                hi = mid;
+        }
+    }
+    const uint32_t L = lo;
+    const uint32_t left = K - level_total(L);
+    // the scenes wanting more than L, ranked from a start scene that moves
+    // on every pass; the first `left` of them get one slot more
+    const uint32_t start = nscene ? *pass % nscene : 0u;
+    uint32_t want_mine = 0, r0_mine = 0;
     for (uint32_t s = a; s < b; ++s) {
+        want_mine += sc_nsel[s] > L ? 1u : 0u;
+        r0_mine += (s < start && sc_nsel[s] > L) ? 1u : 0u;
+    }
+    const uint32_t want_before = block_scan_u32(want_mine, part) - want_mine;
+    const uint32_t nwant = part[blockDim.x - 1];
+    const uint32_t r0 = block_sum_u32(r0_mine, red);
+    auto count_of = [&](uint32_t s, uint32_t &rank) {
+        uint32_t n = min(sc_nsel[s], L);
+        if (sc_nsel[s] > L) {
+            const uint32_t rot = (rank + nwant - r0) % nwant;
+            n += rot < left ? 1u : 0u;
+            ++rank;
+        }
+        return n;
+    };
+    uint32_t mine = 0;
+    {
+        uint32_t rank = want_before;
+        for (uint32_t s = a; s < b; ++s)
+            mine += count_of(s, rank);
+    }
+    const uint32_t incl = block_scan_u32(mine, part);
+    const uint32_t total = part[blockDim.x - 1];
+    uint32_t off = incl - mine, rank = want_before;
+    for (uint32_t s = a; s < b; ++s) {
+        const uint32_t n = count_of(s, rank);
         sc_slot_beg[s] = off;
-        for (uint32_t r = 0; r < sc_nsel[s]; ++r) {
-            uint32_t slot = off + r;
+        sc_nsel[s] = n;
+        sc_shot[s] += n;
+        for (uint32_t r = 0; r < n; ++r) {
+            const uint32_t slot = off + r;
             if (slot < K) {
                 shooters[slot] = cand[(size_t)s * kmax + r];
                 slot_tbeg[slot] = sc_tbeg[s];
                 slot_tcnt[slot] = sc_tbeg[s + 1] - sc_tbeg[s];
                 slot_pbeg[slot] = sc_pbeg[s];
+                if (sc_w) {
+                    slot_w[slot] = sc_w[s];
+                    slot_dffo[slot] = sc_dffo[s];
+                }
             }
         }
-        off += sc_nsel[s];
+        off += n;
     }
-    if (threadIdx.x == blockDim.x - 1) {
-        uint32_t total = part[blockDim.x - 1];
+    if (threadIdx.x == 0) {
         *nshoot = total < K ? total : K;
         *pass += 1u;
     }
@@ -199,9 +285,10 @@ void launch_pack_select(const PackBuffers &pk, const float *unshot, const float 
     k_pack_select<<<pk.nscene, 256, 0, st>>>(pk.sc_pbeg, pk.sc_slots, pk.sc_den, unshot, area,
                                              pk.pass, pk.target, pk.kmax, pk.sc_done, pk.sc_res,
                                              pk.sc_nsel, pk.sc_shot, pk.cand);
-    k_pack_slots<<<1, 1024, 0, st>>>(pk.nscene, pk.sc_tbeg, pk.sc_pbeg, pk.sc_nsel, pk.cand,
-                                     pk.kmax, pk.K, pk.sc_slot_beg, pk.shooters, pk.slot_tbeg,
-                                     pk.slot_tcnt, pk.slot_pbeg, pk.nshoot, pk.pass);
+    k_pack_slots<<<1, 1024, 0, st>>>(pk.nscene, pk.sc_tbeg, pk.sc_pbeg, pk.sc_nsel, pk.sc_shot,
+                                     pk.cand, pk.kmax, pk.K, pk.sc_slot_beg, pk.shooters,
+                                     pk.slot_tbeg, pk.slot_tcnt, pk.slot_pbeg, pk.sc_w, pk.sc_dffo,
+                                     pk.slot_w, pk.slot_dffo, pk.nshoot, pk.pass);
 }
 
 void launch_pack_shoot(const PackBuffers &pk, uint32_t mode, const float *snap, float *ff,

@@ -152,3 +152,34 @@ def test_gather_without_matrix_is_an_error(rad, cornell):
     s.upload(cornell)
     with pytest.raises(rad.RadError):
         s.gather()
+
+
+def test_gather_check_fp64_matches_host(rad, cornell):
+    """rad_gather_check_fp64 (rows rendered again and streamed on the
+    device, FP64 accumulation) against the same residual and contraction
+    computed on the host from FP32 rows, for a solution of the gather and
+    for a perturbed one."""
+    sc = cornell
+    s = rad.Solver(hemicube=32, hemicubes=16)
+    s.upload(sc)
+    s.dense_build(rad.RAD_DENSE_SHOOT)
+    B, st = s.gather(target=1e-6)
+    n = sc.npatch
+    F = np.concatenate([s.ff_rows(np.arange(b, min(n, b + 16), dtype=np.uint32))
+                        for b in range(0, n, 16)]).astype(np.float64)
+    A = sc.area.astype(np.float64)
+    E = sc.emission.T.astype(np.float64)
+    rho = sc.refl.T.astype(np.float64)
+    q_host = float(np.max(F @ sc.refl.astype(np.float64)))
+    for scale in (0.0, 1e-3):
+        Bp = (B * (1.0 + scale * np.sin(np.arange(n)))[None, :]).astype(np.float32)
+        Bd = Bp.astype(np.float64)
+        r = E + rho * ((F.T * A[None, :]) @ Bd.T).T / A[None, :] - Bd
+        want = float((A[None, :] * np.abs(r)).sum() / (A[None, :] * np.abs(Bd)).sum())
+        res, q = s.gather_check_fp64(Bp)
+        say(f"perturbation {scale}: residual {res:.3e} (host {want:.3e}), q {q:.4f}")
+        # the rows here and on the device come from separate renders whose
+        # float atomics round differently (~1e-7 of an entry)
+        assert res == pytest.approx(want, rel=1e-3, abs=3e-7)
+        assert q == pytest.approx(q_host, rel=1e-6)
+    assert q < 1.0
