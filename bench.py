@@ -349,12 +349,10 @@ def profile_leg(R, name):
         raise SystemExit(f"unknown profile leg {name}")
 
 
-def pack_leg(R, args):
+def _pack_scenes(R, args):
     """256 independent box rooms, 500-20,000 triangle patches (log-uniform,
-    seed 5), hemicube 64 / 128 / 256 by size, solved to the residual target
-    by scene packs: every kernel of a pass runs once per pack, the packs
-    (one per hemicube class, each up to 256 shooter slots) run on their own
-    streams side by side."""
+    seed 5), hemicube 64 / 128 / 256 by size, shooters wanted per pass
+    ceil(N / 2000) (1..16)."""
     import math
     import numpy as np
     rng = np.random.default_rng(5)
@@ -367,32 +365,25 @@ def pack_leg(R, args):
         scenes.append(sc)
         Ws.append(64 if sc.npatch < 1500 else (128 if sc.npatch < 6000 else 256))
         ks.append(int(min(16, max(1, math.ceil(sc.npatch / 2000)))))
-    packs = []   # (Solver, scene indices)
-    for W in (256, 128, 64):
-        idx = [i for i in range(len(scenes)) if Ws[i] == W]
-        cur, tot = [], 0
-        for i in idx + [None]:
-            if i is None or tot + ks[i] > 256:
-                if cur:
-                    s = R.Solver(hemicube=W, hemicubes=256, graph_passes=8)
-                    s.pack_upload([scenes[j] for j in cur], [ks[j] for j in cur])
-                    packs.append((s, cur))
-                cur, tot = [], 0
-            if i is not None:
-                cur.append(i)
-                tot += ks[i]
+    return scenes, Ws, ks
+
+
+def _run_packs(packs, nscenes, target, cap_s):
+    import numpy as np
     for s, _ in packs:
-        s.pack_begin(args.target)
+        s.pack_begin(target)
     t0 = time.perf_counter()
-    done_t = np.full(len(scenes), np.nan)
-    shooters = np.zeros(len(scenes), np.int64)
+    done_t = np.full(nscenes, np.nan)
+    shooters = np.zeros(nscenes, np.int64)
     live = list(packs)
-    while live and time.perf_counter() - t0 < args.pack_cap_s:
+    npass = 0
+    while live and time.perf_counter() - t0 < cap_s:
         for s, _ in live:
             s.step(16)
         now_live = []
         for s, idx in live:
             done, res, passes, rem = s.pack_status()
+            npass = max(npass, passes)
             now = time.perf_counter() - t0
             for k, i in enumerate(idx):
                 if done[k] != 0xFFFFFFFF and np.isnan(done_t[i]):
@@ -403,15 +394,54 @@ def pack_leg(R, args):
         live = now_live
     wall = time.perf_counter() - t0
     finished = ~np.isnan(done_t)
+    return {"wall_s": wall, "finished": int(finished.sum()), "passes": int(npass),
+            "shooters": int(shooters.sum()), "shooters_per_s": float(shooters.sum() / wall),
+            "completion_s_p50": float(np.percentile(done_t[finished], 50)) if finished.any() else None,
+            "completion_s_p99": float(np.percentile(done_t[finished], 99)) if finished.any() else None}
+
+
+def pack_leg(R, args):
+    """The 256 scenes solved to the residual target in ONE pack: every
+    kernel of a pass is one launch for all scenes, each scene rendering at
+    its own hemicube side (rad_pack_hemicubes), K = 256 slots per pass shared
+    out by water-filling over the scenes' quotas.  For comparison the earlier
+    arrangement is timed too (--pack-multi): one pack per hemicube class and
+    256-slot budget, side by side on their own streams."""
+    scenes, Ws, ks = _pack_scenes(R, args)
+    s = R.Solver(hemicube=256, hemicubes=256, graph_passes=8)
+    s.pack_upload(scenes, ks)
+    s.pack_hemicubes(Ws)
     out = {"scenes": len(scenes), "patches_total": int(sum(sc.npatch for sc in scenes)),
-           "packs": len(packs), "hemicube_classes": {str(W): Ws.count(W) for W in (64, 128, 256)},
-           "wall_s": wall, "finished": int(finished.sum()),
-           "shooters": int(shooters.sum()), "shooters_per_s": float(shooters.sum() / wall),
-           "completion_s_p50": float(np.percentile(done_t[finished], 50)) if finished.any() else None,
-           "completion_s_p99": float(np.percentile(done_t[finished], 99)) if finished.any() else None}
-    log(f"pack: {out['scenes']} scenes ({out['patches_total']} patches) in {len(packs)} packs: "
-        f"{out['finished']} done in {wall:.2f} s, {out['shooters_per_s']:.0f} shooters/s, "
-        f"completion p50 {out['completion_s_p50']:.2f} s p99 {out['completion_s_p99']:.2f} s")
+           "packs": 1, "launch": "one launch per kernel per pass for all scenes",
+           "slots_per_pass": 256, "quota_total": int(sum(ks)),
+           "hemicube_classes": {str(W): Ws.count(W) for W in (64, 128, 256)}}
+    out.update(_run_packs([(s, list(range(len(scenes))))], len(scenes), args.target,
+                          args.pack_cap_s))
+    del s
+    log(f"pack: {out['scenes']} scenes ({out['patches_total']} patches) in one pack: "
+        f"{out['finished']} done in {out['wall_s']:.2f} s ({out['passes']} passes), "
+        f"{out['shooters_per_s']:.0f} shooters/s, completion p50 {out['completion_s_p50']:.2f} s "
+        f"p99 {out['completion_s_p99']:.2f} s")
+    if args.pack_multi:
+        packs = []   # (Solver, scene indices)
+        for W in (256, 128, 64):
+            idx = [i for i in range(len(scenes)) if Ws[i] == W]
+            cur, tot = [], 0
+            for i in idx + [None]:
+                if i is None or tot + ks[i] > 256:
+                    if cur:
+                        q = R.Solver(hemicube=W, hemicubes=256, graph_passes=8)
+                        q.pack_upload([scenes[j] for j in cur], [ks[j] for j in cur])
+                        packs.append((q, cur))
+                    cur, tot = [], 0
+                if i is not None:
+                    cur.append(i)
+                    tot += ks[i]
+        m = _run_packs(packs, len(scenes), args.target, args.pack_cap_s)
+        m["packs"] = len(packs)
+        out["multi_pack"] = m
+        log(f"pack, {len(packs)} packs by class: {m['finished']} done in {m['wall_s']:.2f} s, "
+            f"{m['shooters_per_s']:.0f} shooters/s")
     return out
 
 
@@ -438,6 +468,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--pack-scenes", type=int, default=256)
     ap.add_argument("--pack-cap-s", type=float, default=120.0)
+    ap.add_argument("--pack-multi", action="store_true",
+                    help="also time the scenes as one pack per hemicube class")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the Cornell (GPU vs CPU) and dense 32k x 192 legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -68,8 +68,7 @@ struct BinBuffers {
     RasterRec *recs;       // nblk*K regions of kRecsPerBlock, then spill_cap
     uint64_t *bbox;        // texel bbox per record slot, read by the bin fill
     uint32_t *blk_nrec;    // records per (shooter, setup block)
-    uint32_t *count, *start, *list;
-    uint32_t *fill;        // 2 x bins: front and back fill counters of each bin list
+    uint32_t *count, *start, *fill, *list;
     uint32_t *order;       // bins by descending size class (tile launch order)
     uint32_t nblk;         // setup blocks per shooter and face group (256 triangles each)
     uint32_t nz;           // face groups: 5 (one face per block, small K) or 1
@@ -79,7 +78,6 @@ struct BinBuffers {
     int clip_blocks;
     uint32_t ntx, nty, ntiles, tshift;
     uint32_t setup_group;  // shooter slots one setup workgroup walks
-    uint32_t split_px;     // records covering more texels of a tile go to the back of its list
 };
 
 struct Geometry {

@@ -979,8 +979,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
             const uint32_t i = c0 + k * 1024u + tid;
             lc[k * 1024u + tid] = v[k];
             if (i < n) {
-                fill[i] = 0u;     // front (small records) counter of the bin
-                fill[n + i] = 0u; // back (larger records) counter
+                fill[i] = 0u;
                 atomicAdd(&bucket[size_class(v[k])], 1u);
             }
         }
@@ -1054,13 +1053,6 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
 // Same grid as k_raster_setup: block (blk, s) puts the records of its region
 // into their bins, reserving each bin's range with one global atomic per
 // touched tile.  The extra column blk == nblk handles the spill region.
-// Each bin's list is filled from both ends: a record whose texel bbox covers
-// at most split_px texels of the tile from the front (counter fill[b]),
-// larger ones from the back (counter fill[nbins + b], down from count[b]).
-// The tile kernel takes a bin's records 256 at a time, one per lane, and a
-// lane draws a small record itself: lists ordered this way give a
-// wavefront records of similar size, so its lanes finish together.  The
-// depth test is order-independent, so the item buffer does not change.
 __device__ static inline void unpack_bbox(uint64_t v, uint32_t tshift, uint32_t &g0,
                                           uint32_t &g1, uint32_t &h0, uint32_t &h1)
 {
@@ -1070,37 +1062,21 @@ __device__ static inline void unpack_bbox(uint64_t v, uint32_t tshift, uint32_t 
     h1 = (uint32_t)(v >> 48) >> tshift;
 }
 
-// 1 when the record (texel bbox v) covers more than split_px texels of tile
-// (gx, gy): its place is at the back of that bin's list.
-__device__ static inline uint32_t bin_side(uint64_t v, uint32_t gx, uint32_t gy, uint32_t tshift,
-                                           uint32_t split_px)
-{
-    const uint32_t T = 1u << tshift;
-    const uint32_t x0 = max((uint32_t)(v & 0xFFFFu), gx << tshift);
-    const uint32_t y0 = max((uint32_t)((v >> 16) & 0xFFFFu), gy << tshift);
-    const uint32_t x1 = min((uint32_t)((v >> 32) & 0xFFFFu), (gx << tshift) + T - 1);
-    const uint32_t y1 = min((uint32_t)(v >> 48), (gy << tshift) + T - 1);
-    return (x1 - x0 + 1) * (y1 - y0 + 1) > split_px ? 1u : 0u;
-}
-
 __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ recs,
                                                   const uint64_t *__restrict__ bbox,
                                                   const uint32_t *__restrict__ blk_nrec,
                                                   const uint32_t *__restrict__ counters,
                                                   uint32_t nblk, uint32_t spill_base,
                                                   uint32_t spill_cap,
-                                                  const uint32_t *__restrict__ count,
                                                   const uint32_t *__restrict__ start,
                                                   uint32_t *__restrict__ fill,
                                                   uint32_t *__restrict__ list, uint32_t ent_cap,
                                                   uint32_t ntx, uint32_t ntiles, uint32_t tshift,
                                                   const uint32_t *__restrict__ nshoot,
-                                                  uint32_t group, uint32_t nslot, uint32_t nbins,
-                                                  uint32_t split_px)
+                                                  uint32_t group, uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[];
-    // dynamic LDS, 4 x ntiles: per tile front / back counts, then bases
-    uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + 2 * ntiles;
+    uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + ntiles; // dynamic LDS, 2 x ntiles
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
     if (blockIdx.x == nblk) {
         if (blockIdx.z != 0)
@@ -1109,23 +1085,20 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             const uint32_t s = blockIdx.y * group + g;
             if (s >= ns)
                 break;
-            const size_t sb = (size_t)s * ntiles;
+            uint32_t *fl = fill + (size_t)s * ntiles;
+            const uint32_t *stt = start + (size_t)s * ntiles;
             // spill records of this shooter: one global atomic each (rare)
             uint32_t nsp = min(counters[kCntSpill], spill_cap);
             for (uint32_t q = threadIdx.x; q < nsp; q += blockDim.x) {
                 uint32_t r = spill_base + q;
                 if ((recs[r].sf >> 3) != s)
                     continue;
-                const uint64_t bv = bbox[r];
                 uint32_t g0, g1, h0, h1;
-                unpack_bbox(bv, tshift, g0, g1, h0, h1);
+                unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
                 for (uint32_t gy = h0; gy <= h1; ++gy)
                     for (uint32_t gx = g0; gx <= g1; ++gx) {
-                        const size_t b = sb + gy * ntx + gx;
-                        const uint32_t pos =
-                            bin_side(bv, gx, gy, tshift, split_px)
-                                ? start[b] + count[b] - 1u - atomicAdd(&fill[nbins + b], 1u)
-                                : start[b] + atomicAdd(&fill[b], 1u);
+                        uint32_t b = gy * ntx + gx;
+                        uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
                         if (pos < ent_cap)
                             list[pos] = r;
                     }
@@ -1156,8 +1129,8 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
         const uint32_t s = blockIdx.y * group + g;
         if (s >= ns)
             break;
-        uint32_t *fl = fill + (size_t)s * ntiles, *flb = fl + nbins;
-        const uint32_t *stt = start + (size_t)s * ntiles, *cnt = count + (size_t)s * ntiles;
+        uint32_t *fl = fill + (size_t)s * ntiles;
+        const uint32_t *stt = start + (size_t)s * ntiles;
         const uint32_t blk = (s * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
         const uint32_t region = blk * kRecsPerBlock, n = nrec;
         uint64_t cur[kFillPer];
@@ -1169,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
         if (n == 0)
             continue;
         __syncthreads(); // the previous slot is done with lcnt / lbase
-        for (uint32_t i = threadIdx.x; i < 2 * ntiles; i += blockDim.x)
+        for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x)
             lcnt[i] = 0u;
         __syncthreads();
 #pragma unroll
@@ -1180,18 +1153,13 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx)
-                    atomicAdd(&lcnt[bin_side(cur[k], gx, gy, tshift, split_px) * ntiles +
-                                    gy * ntx + gx],
-                              1u);
+                    atomicAdd(&lcnt[gy * ntx + gx], 1u);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < ntiles; i += blockDim.x) {
-            const uint32_t cf = lcnt[i], cb = lcnt[ntiles + i];
-            lbase[i] = cf ? stt[i] + atomicAdd(&fl[i], cf) : 0u;
-            // the back range grows down from the end of the bin
-            lbase[ntiles + i] = cb ? stt[i] + cnt[i] - atomicAdd(&flb[i], cb) - cb : 0u;
+            uint32_t c = lcnt[i];
+            lbase[i] = c ? stt[i] + atomicAdd(&fl[i], c) : 0u;
             lcnt[i] = 0u;
-            lcnt[ntiles + i] = 0u;
         }
         __syncthreads();
 #pragma unroll
@@ -1203,8 +1171,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
             for (uint32_t gy = h0; gy <= h1; ++gy)
                 for (uint32_t gx = g0; gx <= g1; ++gx) {
-                    const uint32_t b =
-                        bin_side(cur[k], gx, gy, tshift, split_px) * ntiles + gy * ntx + gx;
+                    uint32_t b = gy * ntx + gx;
                     uint32_t pos = lbase[b] + atomicAdd(&lcnt[b], 1u);
                     if (pos < ent_cap)
                         list[pos] = region + q;
@@ -1274,7 +1241,8 @@ __device__ static inline unsigned long long fixed52(float f)
     return __float2ull_rn(f * 4503599627370496.0f);
 }
 
-// ffq non-null: the runs go to the fixed-point rows instead of ff.
+// FQ: the runs go to the fixed-point rows ffq instead of ff.
+template <bool FQ>
 __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t pbase,
                                           uint32_t rowlen, float *__restrict__ ff,
                                           unsigned long long *__restrict__ ffq,
@@ -1286,7 +1254,7 @@ __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uin
         uint32_t p = id - 1 - pbase;
         if (p < rowlen) {
             size_t o = (size_t)hi * rowlen + p;
-            if (ffq)
+            if constexpr (FQ)
                 atomicAdd(&ffq[o], fixed52(f));
             else
                 atomicAdd(&ff[o], f);
@@ -1432,6 +1400,7 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
 // to the item buffer (out_ids != null; out_depth != null adds the depth
 // plane) or is reduced into F rows and hit counts (ff or, in the
 // deterministic mode, its fixed-point rows ffq != null).
+template <bool FQ>
 __global__ __launch_bounds__(256) void k_tile_raster(
     uint32_t W, uint32_t TW, uint32_t TH, uint32_t RES, const uint32_t *__restrict__ nshoot,
     const RasterRec *__restrict__ recs, const uint64_t *__restrict__ bbox,
@@ -1529,7 +1498,8 @@ __global__ __launch_bounds__(256) void k_tile_raster(
         if (ff || ffq) {
             uint32_t id = (uint32_t)key;
             float f = id ? dff[texel] : 0.0f;
-            reduce_wave(id, f, s, slot_pbeg ? slot_pbeg[s] : 0u, rowlen, ff, ffq, hits, counters);
+            reduce_wave<FQ>(id, f, s, slot_pbeg ? slot_pbeg[s] : 0u, rowlen, ff, ffq, hits,
+                            counters);
         }
     }
 }
@@ -1575,10 +1545,9 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
     if (group > K)
         group = K;
     dim3 grid(bb.nblk + 1, (K + group - 1) / group, bb.nz);
-    k_bin_fill<<<grid, 256, 4 * bb.ntiles * sizeof(uint32_t), st>>>(
-        bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap, bb.count,
-        bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K, nb,
-        bb.split_px);
+    k_bin_fill<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
+        bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap, bb.start,
+        bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
 }
 
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
@@ -1587,11 +1556,16 @@ void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const Bi
                  uint32_t *hits, uint32_t *counters, const uint32_t *slot_w,
                  const uint32_t *slot_dffo, hipStream_t st)
 {
-    k_tile_raster<<<bb.ntiles * K, 256, 0, st>>>(g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox,
-                                                 bb.count, bb.start, bb.list, bb.ent_cap, bb.ntx,
-                                                 bb.ntiles, bb.tshift, small_px, out_ids, out_depth,
-                                                 dff, rowlen, slot_pbeg, ff, ffq, hits, counters,
-                                                 bb.order, slot_w, slot_dffo);
+    if (ffq)
+        k_tile_raster<true><<<bb.ntiles * K, 256, 0, st>>>(
+            g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, bb.list,
+            bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
+            slot_pbeg, ff, ffq, hits, counters, bb.order, slot_w, slot_dffo);
+    else
+        k_tile_raster<false><<<bb.ntiles * K, 256, 0, st>>>(
+            g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, bb.list,
+            bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
+            slot_pbeg, ff, ffq, hits, counters, bb.order, slot_w, slot_dffo);
 }
 
 // Deterministic mode: n fixed-point entries (units of 2^-52) into the float
