@@ -136,6 +136,20 @@ int rad_process_itembuf(rad_ctx *ctx, const uint32_t *itembuf, uint32_t k,
 int rad_process_runs(rad_ctx *ctx, uint32_t k, uint32_t cap, uint32_t *out_hemicubes,
                      uint32_t *out_ids, float *out_energies, uint32_t *out_count);
 
+/* No counterpart in the reference: the rad_process_runs triplet stream for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  All pointers are device memory: the k item buffers (k*TEX_RES
+ * uint32, as rad_process_itembuf takes them), the three triplet arrays of cap
+ * entries, and one uint32 that receives the write index (entries at or past
+ * cap are not written; the index still counts them).  Its work arrays live in
+ * a fixed 4 MiB array per GPU, so k * TEX_H * min(ocl_workitems_x, TEX_W)
+ * may be at most 524287 (RAD_E_LIMIT beyond).  The call queues its kernels on
+ * the context's stream and returns without waiting or allocating; calls that
+ * overlap on one GPU share that array and race. */
+int rad_process_runs_static(rad_ctx *ctx, const uint32_t *d_itembuf, uint32_t k, uint32_t cap,
+                            uint32_t *d_hemicubes, uint32_t *d_ids, float *d_energies,
+                            uint32_t *d_count);
+
 /* One pass of the OnIdle shoot loop (Main.cpp:1137-1295) for the k given
  * shooters: snapshot, render, reduce, shoot, emitter update. */
 int rad_shoot(rad_ctx *ctx, const uint32_t *shooters, uint32_t k);

@@ -94,6 +94,7 @@ SIGNATURES = {
     "rad_process_hemicube": (C.c_int, [_P, _u32, _PU, _PF]),
     "rad_process_itembuf": (C.c_int, [_P, _PU, _u32, _PU, _PF]),
     "rad_process_runs": (C.c_int, [_P, _u32, _u32, _PU, _PU, _PF, _PU]),
+    "rad_process_runs_static": (C.c_int, [_P, _P, _u32, _u32, _P, _P, _P, _P]),
     "rad_shoot": (C.c_int, [_P, _PU, _u32]),
     "rad_step": (C.c_int, [_P, _u32]),
     "rad_run": (C.c_int, [_P, C.c_uint64, C.c_double, _u32, C.POINTER(rad_stats)]),

@@ -36,6 +36,7 @@ SOURCES = [
     ("rad_gather.hip", True),
     ("rad_pack.hip", True),
     ("rad_ingest.hip", True),
+    ("rad_static.hip", True),
     ("rad_api.cpp", True),
     ("rad_scene.cpp", False),
     ("rad_obj.cpp", False),

@@ -320,7 +320,8 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         rad_destroy(c);
         return rc;
     }
-    if ((rc = dalloc(c, &c->parts, (size_t)c->sel_blocks * 3))) {
+    if ((rc = dalloc(c, &c->parts, (size_t)c->sel_blocks * 3)) ||
+        (rc = rad::static_scratch_init(c))) {
         rad_destroy(c);
         return rc;
     }

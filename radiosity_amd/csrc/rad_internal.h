@@ -235,3 +235,11 @@ void launch_vertex_colors(uint32_t nquad, const uint32_t *quad_pbeg, const uint3
                           const float *shot, float *val, float *out, hipStream_t st);
 
 } // namespace rad
+
+struct rad_ctx;
+namespace rad {
+// rad_static.hip: the device address of the fixed scratch array of
+// rad_process_runs_static on the context's (current) device, looked up once
+// per device at context creation.
+int static_scratch_init(rad_ctx *c);
+} // namespace rad
