@@ -677,6 +677,7 @@ int rad_render_itembuf(rad_ctx *c, const uint32_t *shooters, uint32_t k, uint32_
 }
 
 static int ensure_events(rad_ctx *c, size_t n);
+static int add_stage_times(rad_ctx *c, uint32_t passes);
 
 // Standalone ProcessHemicube of the item buffer in item_ids (k hemicubes),
 // timed with HIP events when timing is on (rad_reduce_kernel_ms).
@@ -811,11 +812,13 @@ int rad_shoot(rad_ctx *c, const uint32_t *shooters, uint32_t k)
     int rc;
     if ((rc = need_plain(c)) || (rc = upload_shooters(c, shooters, k)))
         return rc;
+    if (c->timing && (rc = ensure_events(c, 7)))
+        return rc;
     uint32_t saveK = c->K;
     c->K = k;
     rc = one_pass(c, false, 0);
     c->K = saveK;
-    if (rc)
+    if (rc || (c->timing && (rc = add_stage_times(c, 1))))
         return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return check_overflow(c);
@@ -856,6 +859,22 @@ static int capture_graph(rad_ctx *c, uint32_t n)
     return RAD_OK;
 }
 
+// With timing on: waits for the last `passes` passes and adds their stage
+// times (select, prepare + clear, setup, bins, tiles, shoot) to the totals.
+static int add_stage_times(rad_ctx *c, uint32_t passes)
+{
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (uint32_t p = 0; p < passes; ++p) {
+        for (int k = 0; k < 6; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, c->ev[(size_t)p * 7 + k], c->ev[(size_t)p * 7 + k + 1]);
+            c->t_acc[k] += ms;
+        }
+    }
+    c->t_passes += passes;
+    return RAD_OK;
+}
+
 static int step_impl(rad_ctx *c, uint32_t passes)
 {
     int rc;
@@ -883,22 +902,7 @@ static int step_impl(rad_ctx *c, uint32_t passes)
     for (uint32_t p = 0; p < passes; ++p)
         if ((rc = one_pass(c, true, (size_t)p * 7)))
             return rc;
-    if (c->timing) {
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        for (uint32_t p = 0; p < passes; ++p) {
-            float ms[6];
-            for (int k = 0; k < 6; ++k) {
-                ms[k] = 0.f;
-                (void)hipEventElapsedTime(&ms[k], c->ev[(size_t)p * 7 + k],
-                                          c->ev[(size_t)p * 7 + k + 1]);
-            }
-            // stages: select, prepare+clear, setup, tiles, reduce, shoot
-            for (int k = 0; k < 6; ++k)
-                c->t_acc[k] += ms[k];
-        }
-        c->t_passes += passes;
-    }
-    return RAD_OK;
+    return c->timing ? add_stage_times(c, passes) : RAD_OK;
 }
 
 int rad_step(rad_ctx *c, uint32_t passes)

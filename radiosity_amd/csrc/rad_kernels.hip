@@ -1211,12 +1211,12 @@ __device__ static inline void seg_step(float &f, uint32_t &c, bool take, bool wi
     }
 }
 
-__device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c, bool with_c)
+// The segmented scan itself, for the wavefront's mask hm of segment starts
+// (lane 0 must be one): f (and c) become the sums over the lanes from the
+// segment's start up to this lane.
+__device__ static inline void seg_scan(uint64_t hm, float &f, uint32_t &c, bool with_c)
 {
     const int lane = (int)(threadIdx.x & 63);
-    const uint32_t pid = dpp_u<0x138, 0xF>(id); // wave_shr:1
-    const bool head = lane == 0 || pid != id;
-    const uint64_t hm = __ballot(head);
     const int ss = 63 - __clzll(hm & ((2ull << lane) - 1ull));
     seg_step<0x111, 0xF>(f, c, lane - 1 >= ss, with_c); // row_shr:1
     seg_step<0x112, 0xF>(f, c, lane - 2 >= ss, with_c); // row_shr:2
@@ -1224,13 +1224,19 @@ __device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c, bool
     seg_step<0x118, 0xF>(f, c, lane - 8 >= ss, with_c); // row_shr:8
     seg_step<0x142, 0xA>(f, c, (lane & ~15) - 1 >= ss, with_c); // row_bcast:15
     seg_step<0x143, 0xC>(f, c, 31 >= ss, with_c);               // row_bcast:31
+}
+
+__device__ static inline bool wave_runs(uint32_t id, float &f, uint32_t &c, bool with_c)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t pid = dpp_u<0x138, 0xF>(id); // wave_shr:1
+    const bool head = lane == 0 || pid != id;
+    const uint64_t hm = __ballot(head);
+    seg_scan(hm, f, c, with_c);
     const bool tail = lane == 63 || ((hm >> (lane + 1)) & 1ull);
     return tail && id != 0;
 }
 
-// Wave-level ProcessHemicube on 64 consecutive texels of one shooter's
-// hemicube hi (any grouping of equal ids gives the same sums; consecutive
-// texels make runs long).  Every lane of the wavefront must call it.
 // F row entries are id - 1 - pbase for ids in [pbase + 1, pbase + rowlen]
 // (pbase: first patch of the shooter's scene in a packed launch, else 0).
 // Deterministic mode: a run's sum in fixed point, units of 2^-52 (exact for
@@ -1241,7 +1247,11 @@ __device__ static inline unsigned long long fixed52(float f)
     return __float2ull_rn(f * 4503599627370496.0f);
 }
 
-// FQ: the runs go to the fixed-point rows ffq instead of ff.
+// Wave-level ProcessHemicube on 64 consecutive texels of one shooter's
+// hemicube hi (any grouping of equal ids gives the same sums; consecutive
+// texels make runs long): each run's sum is added by its last lane.  FQ: the
+// runs go to the fixed-point rows ffq instead of ff.  Every lane of the
+// wavefront must call it.
 template <bool FQ>
 __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uint32_t pbase,
                                           uint32_t rowlen, float *__restrict__ ff,
@@ -1251,9 +1261,9 @@ __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uin
 {
     uint32_t c = id ? 1u : 0u;
     if (wave_runs(id, f, c, hits != nullptr)) {
-        uint32_t p = id - 1 - pbase;
+        const uint32_t p = id - 1 - pbase;
         if (p < rowlen) {
-            size_t o = (size_t)hi * rowlen + p;
+            const size_t o = (size_t)hi * rowlen + p;
             if constexpr (FQ)
                 atomicAdd(&ffq[o], fixed52(f));
             else
@@ -1400,8 +1410,10 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
 // to the item buffer (out_ids != null; out_depth != null adds the depth
 // plane) or is reduced into F rows and hit counts (ff or, in the
 // deterministic mode, its fixed-point rows ffq != null).
+constexpr uint32_t kTileThreads = 256; // k_tile_raster's workgroup size
+
 template <bool FQ>
-__global__ __launch_bounds__(256) void k_tile_raster(
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tile_raster(
     uint32_t W, uint32_t TW, uint32_t TH, uint32_t RES, const uint32_t *__restrict__ nshoot,
     const RasterRec *__restrict__ recs, const uint64_t *__restrict__ bbox,
     const uint32_t *__restrict__ bin_count, const uint32_t *__restrict__ bin_start,
@@ -1422,8 +1434,7 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     if (s >= *nshoot)
         return;
     const uint32_t tile = bin - s * ntiles;
-    const size_t b = (size_t)s * ntiles + tile;
-    const uint32_t n = bin_count[b], st0 = bin_start[b];
+    const uint32_t n = bin_count[bin], st0 = bin_start[bin];
     // an empty bin adds nothing to the F rows (its texels are background)
     if (n == 0 && !out_ids)
         return;
@@ -1437,25 +1448,33 @@ __global__ __launch_bounds__(256) void k_tile_raster(
     }
     const uint32_t T = 1u << tshift, TT = T * T;
     const int32_t ox = (int32_t)((tile % ntx) << tshift), oy = (int32_t)((tile / ntx) << tshift);
-    for (uint32_t p = threadIdx.x; p < TT; p += blockDim.x)
-        tk[p] = kClearKey;
     const int32_t Wi = (int32_t)W;
     const uint32_t me = threadIdx.x;
+    // A lane's entry of the first chunk (list index, bbox and record, the
+    // record loaded with the bbox, not after the size test) is fetched while
+    // the tile clears; each later chunk's list entries are fetched during the
+    // chunk before it.
+    constexpr uint32_t kNoRec = ~0u;
+    uint32_t ri = me < n && st0 + me < ent_cap ? list[st0 + me] : kNoRec;
+    uint64_t bv = 0;
+    if (ri != kNoRec)
+        bv = bbox[ri];
+    for (uint32_t p = threadIdx.x; p < TT; p += blockDim.x)
+        tk[p] = kClearKey;
     for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
         if (me == 0)
             nbig = 0u;
+        const uint32_t inext = c0 + blockDim.x + me;
+        const uint32_t rnext = inext < n && st0 + inext < ent_cap ? list[st0 + inext] : kNoRec;
         __syncthreads();
-        const uint32_t i = c0 + me;
-        if (i < n && st0 + i < ent_cap) {
-            const uint32_t ri = list[st0 + i];
-            const uint64_t bv = bbox[ri];
+        if (ri != kNoRec) {
             const int32_t ax0 = max((int32_t)(bv & 0xFFFFu), ox);
             const int32_t ay0 = max((int32_t)((bv >> 16) & 0xFFFFu), oy);
             const int32_t ax1 = min((int32_t)((bv >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
             const int32_t ay1 = min((int32_t)(bv >> 48), oy + (int32_t)T - 1);
             const uint32_t area = (uint32_t)(ax1 - ax0 + 1) * (uint32_t)(ay1 - ay0 + 1);
             if (area <= small_px) {
-                RasterRec r = recs[ri];
+                const RasterRec r = recs[ri];
                 FaceView fv = face_view(Wi, (int)(r.sf & 7u));
                 draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
             } else {
@@ -1469,37 +1488,56 @@ __global__ __launch_bounds__(256) void k_tile_raster(
         // blocks; wave w takes queue entries w, w + 4, ...
         const uint32_t nall = nbig;
         for (uint32_t q = me >> 6; q < nall; q += blockDim.x >> 6) {
-            RasterRec r = recs[perm[q]];
+            const RasterRec rb = recs[perm[q]];
             const uint64_t bb = pbb[q];
             const int32_t ax0 = max((int32_t)(bb & 0xFFFFu), ox);
             const int32_t ax1 = min((int32_t)((bb >> 32) & 0xFFFFu), ox + (int32_t)T - 1);
             const int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
             const int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
-            FaceView fv = face_view(Wi, (int)(r.sf & 7u));
+            FaceView fv = face_view(Wi, (int)(rb.sf & 7u));
             Edges e;
-            make_edges(r.x, r.y, e);
-            draw_wave(r, e, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
+            make_edges(rb.x, rb.y, e);
+            draw_wave(rb, e, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
         }
+        ri = rnext;
+        if (ri != kNoRec)
+            bv = bbox[ri];
         __syncthreads();
     }
+    // The resolve's ΔFF values are all loaded before the last barrier, so no
+    // load in the resolve waits behind the F-row atomics of an earlier
+    // texel group (the memory counter a load waits on counts those too).
+    constexpr uint32_t kPer = kMaxTile * kMaxTile / kTileThreads;
+    const bool reduce = ff || ffq;
+    float fd[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t p = k * kTileThreads + threadIdx.x;
+        const int32_t tx = ox + (int32_t)(p & (T - 1)), ty = oy + (int32_t)(p >> tshift);
+        fd[k] = reduce && p < TT && tx < (int32_t)TW && ty < (int32_t)TH
+                    ? dff[(uint32_t)ty * TW + (uint32_t)tx]
+                    : 0.0f;
+    }
+    const uint32_t pbase = slot_pbeg ? slot_pbeg[s] : 0u;
     __syncthreads();
     // resolve: whole waves walk the tile's texels in row-major order
-    for (uint32_t p0 = 0; p0 < TT; p0 += blockDim.x) {
-        uint32_t p = p0 + threadIdx.x;
-        int32_t tx = ox + (int32_t)(p & (T - 1)), ty = oy + (int32_t)(p >> tshift);
-        bool inside = p < TT && tx < (int32_t)TW && ty < (int32_t)TH;
-        uint64_t key = inside ? tk[p] : kClearKey;
-        uint32_t texel = (uint32_t)ty * TW + (uint32_t)tx;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        if (k * kTileThreads >= TT)
+            break;
+        const uint32_t p = k * kTileThreads + threadIdx.x;
+        const int32_t tx = ox + (int32_t)(p & (T - 1)), ty = oy + (int32_t)(p >> tshift);
+        const bool inside = p < TT && tx < (int32_t)TW && ty < (int32_t)TH;
+        const uint64_t key = inside ? tk[p] : kClearKey;
+        const uint32_t texel = (uint32_t)ty * TW + (uint32_t)tx;
         if (out_ids && inside) {
             out_ids[(size_t)s * RES + texel] = (uint32_t)key;
             if (out_depth)
                 out_depth[(size_t)s * RES + texel] = __uint_as_float((uint32_t)(key >> 32));
         }
-        if (ff || ffq) {
-            uint32_t id = (uint32_t)key;
-            float f = id ? dff[texel] : 0.0f;
-            reduce_wave<FQ>(id, f, s, slot_pbeg ? slot_pbeg[s] : 0u, rowlen, ff, ffq, hits,
-                            counters);
+        if (reduce) {
+            const uint32_t id = (uint32_t)key;
+            reduce_wave<FQ>(id, id ? fd[k] : 0.0f, s, pbase, rowlen, ff, ffq, hits, counters);
         }
     }
 }
@@ -1557,12 +1595,12 @@ void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const Bi
                  const uint32_t *slot_dffo, hipStream_t st)
 {
     if (ffq)
-        k_tile_raster<true><<<bb.ntiles * K, 256, 0, st>>>(
+        k_tile_raster<true><<<bb.ntiles * K, kTileThreads, 0, st>>>(
             g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, bb.list,
             bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
             slot_pbeg, ff, ffq, hits, counters, bb.order, slot_w, slot_dffo);
     else
-        k_tile_raster<false><<<bb.ntiles * K, 256, 0, st>>>(
+        k_tile_raster<false><<<bb.ntiles * K, kTileThreads, 0, st>>>(
             g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, bb.list,
             bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
             slot_pbeg, ff, ffq, hits, counters, bb.order, slot_w, slot_dffo);
