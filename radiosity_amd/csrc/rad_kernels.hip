@@ -1276,6 +1276,47 @@ __device__ static inline void reduce_wave(uint32_t id, float f, uint32_t hi, uin
     }
 }
 
+constexpr uint32_t kTileThreads = 256; // k_tile_raster's workgroup size
+constexpr uint32_t kTileTable = 256;   // its LDS table of F entries (one slot per thread)
+static_assert(kTileTable == kTileThreads, "the table is cleared and flushed one slot per thread");
+
+// reduce_wave for the tile kernel: the run sums go to the workgroup's LDS
+// table (hk: F-row index + 1, 0 = free; hs, hc: the sums; open addressing,
+// 16 probes) and only a run that finds no slot goes straight to the global
+// row, so that a tile adds each patch it sees to the row once rather than
+// once per run.  Every lane of the wavefront must call it.
+__device__ static inline void reduce_wave_lds(uint32_t id, float f, uint32_t pbase,
+                                              uint32_t rowlen, uint32_t *hk, float *hs,
+                                              uint32_t *hc, bool with_c, float *__restrict__ frow,
+                                              uint32_t *__restrict__ hrow,
+                                              uint32_t *__restrict__ counters)
+{
+    uint32_t c = id ? 1u : 0u;
+    if (!wave_runs(id, f, c, with_c))
+        return;
+    const uint32_t p = id - 1 - pbase;
+    if (p >= rowlen) {
+        atomicAdd(&counters[kCntUnknown], 1u);
+        return;
+    }
+    const uint32_t key = p + 1;
+    uint32_t h = (key * 2654435761u) >> 24; // kTileTable = 2^8
+    for (int probe = 0; probe < 16; ++probe, h = (h + 1) & (kTileTable - 1)) {
+        uint32_t k = hk[h];
+        if (k == 0u)
+            k = atomicCAS(&hk[h], 0u, key);
+        if (k == 0u || k == key) {
+            atomicAdd(&hs[h], f);
+            if (with_c)
+                atomicAdd(&hc[h], c);
+            return;
+        }
+    }
+    atomicAdd(&frow[p], f);
+    if (with_c)
+        atomicAdd(&hrow[p], c);
+}
+
 __device__ static inline void draw_span(const RasterRec &r, const Edges &e, int32_t vpx,
                                         int32_t vpy, int32_t ax0, int32_t ay0, uint32_t w,
                                         uint32_t area, uint32_t p0, uint32_t step, int32_t ox,
@@ -1410,7 +1451,6 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
 // to the item buffer (out_ids != null; out_depth != null adds the depth
 // plane) or is reduced into F rows and hit counts (ff or, in the
 // deterministic mode, its fixed-point rows ffq != null).
-constexpr uint32_t kTileThreads = 256; // k_tile_raster's workgroup size
 
 template <bool FQ>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tile_raster(
@@ -1426,8 +1466,20 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const uint32_t *__restrict__ slot_w, const uint32_t *__restrict__ slot_dffo)
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
-    __shared__ uint32_t perm[256];
-    __shared__ uint64_t pbb[256];
+    // the chunk queues, then (resolve) the table of F entries
+    __shared__ union {
+        struct {
+            uint32_t perm[kTileThreads];
+            uint64_t pbb[kTileThreads];
+        } q;
+        struct {
+            uint32_t key[kTileTable];
+            float sum[kTileTable];
+            uint32_t cnt[kTileTable];
+        } t;
+    } sh;
+    uint32_t *const perm = sh.q.perm;
+    uint64_t *const pbb = sh.q.pbb;
     __shared__ uint32_t nbig;
     const uint32_t bin = order[blockIdx.x];
     const uint32_t s = bin / ntiles;
@@ -1519,7 +1571,16 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                     : 0.0f;
     }
     const uint32_t pbase = slot_pbeg ? slot_pbeg[s] : 0u;
+    // float rows: the runs go through the LDS table (the queues are done)
+    const bool table = !FQ && ff != nullptr;
+    if (table) {
+        sh.t.key[threadIdx.x] = 0u;
+        sh.t.sum[threadIdx.x] = 0.0f;
+        sh.t.cnt[threadIdx.x] = 0u;
+    }
     __syncthreads();
+    float *const frow = ff ? ff + (size_t)s * rowlen : nullptr;
+    uint32_t *const hrow = hits ? hits + (size_t)s * rowlen : nullptr;
     // resolve: whole waves walk the tile's texels in row-major order
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
@@ -1535,9 +1596,22 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             if (out_depth)
                 out_depth[(size_t)s * RES + texel] = __uint_as_float((uint32_t)(key >> 32));
         }
-        if (reduce) {
+        if (table) {
+            const uint32_t id = (uint32_t)key;
+            reduce_wave_lds(id, id ? fd[k] : 0.0f, pbase, rowlen, sh.t.key, sh.t.sum, sh.t.cnt,
+                            hits != nullptr, frow, hrow, counters);
+        } else if (reduce) {
             const uint32_t id = (uint32_t)key;
             reduce_wave<FQ>(id, id ? fd[k] : 0.0f, s, pbase, rowlen, ff, ffq, hits, counters);
+        }
+    }
+    if (table) {
+        __syncthreads();
+        const uint32_t key = sh.t.key[threadIdx.x];
+        if (key) {
+            atomicAdd(&frow[key - 1], sh.t.sum[threadIdx.x]);
+            if (hrow)
+                atomicAdd(&hrow[key - 1], sh.t.cnt[threadIdx.x]);
         }
     }
 }
