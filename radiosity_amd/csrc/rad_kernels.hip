@@ -1280,25 +1280,15 @@ constexpr uint32_t kTileThreads = 256; // k_tile_raster's workgroup size
 constexpr uint32_t kTileTable = 256;   // its LDS table of F entries (one slot per thread)
 static_assert(kTileTable == kTileThreads, "the table is cleared and flushed one slot per thread");
 
-// reduce_wave for the tile kernel: the run sums go to the workgroup's LDS
-// table (hk: F-row index + 1, 0 = free; hs, hc: the sums; open addressing,
-// 16 probes) and only a run that finds no slot goes straight to the global
-// row, so that a tile adds each patch it sees to the row once rather than
-// once per run.  Every lane of the wavefront must call it.
-__device__ static inline void reduce_wave_lds(uint32_t id, float f, uint32_t pbase,
-                                              uint32_t rowlen, uint32_t *hk, float *hs,
-                                              uint32_t *hc, bool with_c, float *__restrict__ frow,
-                                              uint32_t *__restrict__ hrow,
-                                              uint32_t *__restrict__ counters)
+// The tile kernel's F sums go through a per-workgroup LDS table (hk: F-row
+// index + 1, 0 = free; hs, hc: the sums of f and of the texel counts; open
+// addressing, 16 probes), which is added to the global row once at the end,
+// so that a tile adds each patch it sees to the row once rather than once
+// per run.  A run that finds no free slot goes straight to the global row.
+__device__ static inline void table_add(uint32_t p, float f, uint32_t c, bool with_c,
+                                        uint32_t *hk, float *hs, uint32_t *hc,
+                                        float *__restrict__ frow, uint32_t *__restrict__ hrow)
 {
-    uint32_t c = id ? 1u : 0u;
-    if (!wave_runs(id, f, c, with_c))
-        return;
-    const uint32_t p = id - 1 - pbase;
-    if (p >= rowlen) {
-        atomicAdd(&counters[kCntUnknown], 1u);
-        return;
-    }
     const uint32_t key = p + 1;
     uint32_t h = (key * 2654435761u) >> 24; // kTileTable = 2^8
     for (int probe = 0; probe < 16; ++probe, h = (h + 1) & (kTileTable - 1)) {
@@ -1315,6 +1305,94 @@ __device__ static inline void reduce_wave_lds(uint32_t id, float f, uint32_t pba
     atomicAdd(&frow[p], f);
     if (with_c)
         atomicAdd(&hrow[p], c);
+}
+
+// One run (id, sums f and c) of a shooter's hemicube: id 0 (background) adds
+// nothing; F entries are id - 1 - pbase (pbase: first patch of the shooter's
+// scene in a packed launch, else 0); an id beyond the row is counted (once
+// per run, Main.cpp:1259-1262) and skipped.
+__device__ static inline void run_add(uint32_t id, float f, uint32_t c, uint32_t pbase,
+                                      uint32_t rowlen, bool with_c, uint32_t *hk, float *hs,
+                                      uint32_t *hc, float *__restrict__ frow,
+                                      uint32_t *__restrict__ hrow, uint32_t *__restrict__ counters)
+{
+    if (!id)
+        return;
+    const uint32_t p = id - 1 - pbase;
+    if (p >= rowlen) {
+        atomicAdd(&counters[kCntUnknown], 1u);
+        return;
+    }
+    table_add(p, f, c, with_c, hk, hs, hc, frow, hrow);
+}
+
+// ProcessHemicube on 256 consecutive texels of one shooter's hemicube, 4
+// consecutive texels per lane (ids q, ΔFF f, f = 0 where q = 0).  A run of
+// equal ids inside a lane is summed in the lane; a run that reaches a lane's
+// last texel (its tail run) may go on into the next lanes, so the tail runs
+// are summed across lanes by the segmented scan (a segment starts at a lane
+// whose tail run starts inside it, or whose first id differs from the
+// previous lane's last).  Every run is added once: a lane adds its middle
+// runs, its first run unless that is its tail run (plus the scanned sum the
+// previous lane carries into it), and its tail run's scanned sum when the
+// next lane does not go on with it.  Any grouping of equal ids gives the
+// same F sums.  Every lane of the wavefront must call it.
+__device__ static inline void reduce_quad(const uint32_t (&q)[4], const float (&f)[4],
+                                          uint32_t pbase, uint32_t rowlen, bool with_c,
+                                          uint32_t *hk, float *hs, uint32_t *hc,
+                                          float *__restrict__ frow, uint32_t *__restrict__ hrow,
+                                          uint32_t *__restrict__ counters)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const bool b1 = q[1] != q[0], b2 = q[2] != q[1], b3 = q[3] != q[2];
+    const bool whole = !(b1 || b2 || b3);
+    // first run [0, first boundary), tail run [last boundary, 4)
+    float fs = f[0];
+    uint32_t fc = 1;
+    if (!b1) {
+        fs += f[1];
+        ++fc;
+        if (!b2) {
+            fs += f[2];
+            ++fc;
+            if (!b3) {
+                fs += f[3];
+                ++fc;
+            }
+        }
+    }
+    float ts = f[3];
+    uint32_t tc = 1;
+    if (!b3) {
+        ts += f[2];
+        ++tc;
+        if (!b2) {
+            ts += f[1];
+            ++tc;
+            if (!b1) {
+                ts += f[0];
+                ++tc;
+            }
+        }
+    }
+    // middle runs: [1, 2) or [1, 3), and [2, 3)
+    if (b1 && (b2 || b3))
+        run_add(q[1], b2 ? f[1] : f[1] + f[2], b2 ? 1u : 2u, pbase, rowlen, with_c, hk, hs, hc,
+                frow, hrow, counters);
+    if (b2 && b3)
+        run_add(q[2], f[2], 1u, pbase, rowlen, with_c, hk, hs, hc, frow, hrow, counters);
+    const uint32_t prev3 = dpp_u<0x138, 0xF>(q[3]); // wave_shr:1: lane - 1's last id
+    const uint32_t next0 = dpp_u<0x130, 0xF>(q[0]); // wave_shl:1: lane + 1's first id
+    const bool cont_prev = lane > 0 && prev3 == q[0];
+    const bool cont_next = lane < 63 && next0 == q[3];
+    seg_scan(__ballot(!whole || !cont_prev), ts, tc, with_c);
+    const float sp = dpp_f<0x138, 0xF>(ts); // the previous lane's scanned tail sum
+    const uint32_t cp = with_c ? dpp_u<0x138, 0xF>(tc) : 0u;
+    if (!whole)
+        run_add(q[0], cont_prev ? fs + sp : fs, cont_prev ? fc + cp : fc, pbase, rowlen, with_c,
+                hk, hs, hc, frow, hrow, counters);
+    if (!cont_next)
+        run_add(q[3], ts, tc, pbase, rowlen, with_c, hk, hs, hc, frow, hrow, counters);
 }
 
 __device__ static inline void draw_span(const RasterRec &r, const Edges &e, int32_t vpx,
@@ -1556,32 +1634,70 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             bv = bbox[ri];
         __syncthreads();
     }
-    // The resolve's ΔFF values are all loaded before the last barrier, so no
-    // load in the resolve waits behind the F-row atomics of an earlier
-    // texel group (the memory counter a load waits on counts those too).
+    // Resolve.  Float F rows (the shooting pass) go through the LDS table in
+    // the 4-texels-per-lane layout: wavefront w takes texels [256 w, 256 w +
+    // 256) of the tile in row-major order (4 waves cover a 32 x 32 tile); TW
+    // and the tile side are multiples of 4, so a lane's texels lie in one row
+    // and are all inside the hemicube or all outside, and their ΔFF values
+    // (16-byte aligned: RES, and a pack slot's table offset, are multiples of
+    // 4) are one load.  The item buffer and the fixed-point rows walk the
+    // tile one texel per lane.  Every ΔFF load is issued before the last
+    // barrier.
+    static_assert(kMaxTile * kMaxTile == 4 * kTileThreads, "one table resolve step per tile");
     constexpr uint32_t kPer = kMaxTile * kMaxTile / kTileThreads;
-    const bool reduce = ff || ffq;
-    float fd[kPer];
-#pragma unroll
-    for (uint32_t k = 0; k < kPer; ++k) {
-        const uint32_t p = k * kTileThreads + threadIdx.x;
-        const int32_t tx = ox + (int32_t)(p & (T - 1)), ty = oy + (int32_t)(p >> tshift);
-        fd[k] = reduce && p < TT && tx < (int32_t)TW && ty < (int32_t)TH
-                    ? dff[(uint32_t)ty * TW + (uint32_t)tx]
-                    : 0.0f;
-    }
-    const uint32_t pbase = slot_pbeg ? slot_pbeg[s] : 0u;
-    // float rows: the runs go through the LDS table (the queues are done)
     const bool table = !FQ && ff != nullptr;
+    const bool reduce = ff || ffq;
+    const uint32_t rp = (threadIdx.x >> 6) * 256u + (threadIdx.x & 63) * 4u;
+    const int32_t rx = ox + (int32_t)(rp & (T - 1)), ry = oy + (int32_t)(rp >> tshift);
+    const bool rin = rp < TT && rx < (int32_t)TW && ry < (int32_t)TH;
+    float4 d4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float fd[kPer];
     if (table) {
+        if (rin)
+            d4 = *reinterpret_cast<const float4 *>(dff + (uint32_t)ry * TW + (uint32_t)rx);
+        // the queues are done: their LDS becomes the table
         sh.t.key[threadIdx.x] = 0u;
         sh.t.sum[threadIdx.x] = 0.0f;
         sh.t.cnt[threadIdx.x] = 0u;
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t p = k * kTileThreads + threadIdx.x;
+            const int32_t tx = ox + (int32_t)(p & (T - 1)), ty = oy + (int32_t)(p >> tshift);
+            fd[k] = reduce && p < TT && tx < (int32_t)TW && ty < (int32_t)TH
+                        ? dff[(uint32_t)ty * TW + (uint32_t)tx]
+                        : 0.0f;
+        }
     }
+    const uint32_t pbase = slot_pbeg ? slot_pbeg[s] : 0u;
     __syncthreads();
-    float *const frow = ff ? ff + (size_t)s * rowlen : nullptr;
-    uint32_t *const hrow = hits ? hits + (size_t)s * rowlen : nullptr;
-    // resolve: whole waves walk the tile's texels in row-major order
+    if (table) {
+        float *const frow = ff + (size_t)s * rowlen;
+        uint32_t *const hrow = hits ? hits + (size_t)s * rowlen : nullptr;
+        if ((threadIdx.x >> 6) * 256u < TT) { // whole wavefronts
+            uint32_t q4[4] = {0u, 0u, 0u, 0u};
+            float f4[4] = {0.f, 0.f, 0.f, 0.f};
+            if (rin) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    q4[j] = (uint32_t)tk[rp + j];
+                f4[0] = q4[0] ? d4.x : 0.f;
+                f4[1] = q4[1] ? d4.y : 0.f;
+                f4[2] = q4[2] ? d4.z : 0.f;
+                f4[3] = q4[3] ? d4.w : 0.f;
+            }
+            reduce_quad(q4, f4, pbase, rowlen, hrow != nullptr, sh.t.key, sh.t.sum, sh.t.cnt,
+                        frow, hrow, counters);
+        }
+        __syncthreads();
+        const uint32_t key = sh.t.key[threadIdx.x];
+        if (key) {
+            atomicAdd(&frow[key - 1], sh.t.sum[threadIdx.x]);
+            if (hrow)
+                atomicAdd(&hrow[key - 1], sh.t.cnt[threadIdx.x]);
+        }
+        return;
+    }
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
         if (k * kTileThreads >= TT)
@@ -1596,22 +1712,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             if (out_depth)
                 out_depth[(size_t)s * RES + texel] = __uint_as_float((uint32_t)(key >> 32));
         }
-        if (table) {
-            const uint32_t id = (uint32_t)key;
-            reduce_wave_lds(id, id ? fd[k] : 0.0f, pbase, rowlen, sh.t.key, sh.t.sum, sh.t.cnt,
-                            hits != nullptr, frow, hrow, counters);
-        } else if (reduce) {
+        if (reduce) {
             const uint32_t id = (uint32_t)key;
             reduce_wave<FQ>(id, id ? fd[k] : 0.0f, s, pbase, rowlen, ff, ffq, hits, counters);
-        }
-    }
-    if (table) {
-        __syncthreads();
-        const uint32_t key = sh.t.key[threadIdx.x];
-        if (key) {
-            atomicAdd(&frow[key - 1], sh.t.sum[threadIdx.x]);
-            if (hrow)
-                atomicAdd(&hrow[key - 1], sh.t.cnt[threadIdx.x]);
         }
     }
 }
