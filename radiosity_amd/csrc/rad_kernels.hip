@@ -1337,11 +1337,10 @@ __device__ static inline void run_add(uint32_t id, float f, uint32_t c, uint32_t
 // previous lane carries into it), and its tail run's scanned sum when the
 // next lane does not go on with it.  Any grouping of equal ids gives the
 // same F sums.  Every lane of the wavefront must call it.
-__device__ static inline void reduce_quad(const uint32_t (&q)[4], const float (&f)[4],
-                                          uint32_t pbase, uint32_t rowlen, bool with_c,
-                                          uint32_t *hk, float *hs, uint32_t *hc,
-                                          float *__restrict__ frow, uint32_t *__restrict__ hrow,
-                                          uint32_t *__restrict__ counters)
+// emit(id, f, c) adds one run.
+template <typename Emit>
+__device__ static inline void reduce_quad(const uint32_t (&q)[4], const float (&f)[4], bool with_c,
+                                          Emit &&emit)
 {
     const int lane = (int)(threadIdx.x & 63);
     const bool b1 = q[1] != q[0], b2 = q[2] != q[1], b3 = q[3] != q[2];
@@ -1377,10 +1376,9 @@ __device__ static inline void reduce_quad(const uint32_t (&q)[4], const float (&
     }
     // middle runs: [1, 2) or [1, 3), and [2, 3)
     if (b1 && (b2 || b3))
-        run_add(q[1], b2 ? f[1] : f[1] + f[2], b2 ? 1u : 2u, pbase, rowlen, with_c, hk, hs, hc,
-                frow, hrow, counters);
+        emit(q[1], b2 ? f[1] : f[1] + f[2], b2 ? 1u : 2u);
     if (b2 && b3)
-        run_add(q[2], f[2], 1u, pbase, rowlen, with_c, hk, hs, hc, frow, hrow, counters);
+        emit(q[2], f[2], 1u);
     const uint32_t prev3 = dpp_u<0x138, 0xF>(q[3]); // wave_shr:1: lane - 1's last id
     const uint32_t next0 = dpp_u<0x130, 0xF>(q[0]); // wave_shl:1: lane + 1's first id
     const bool cont_prev = lane > 0 && prev3 == q[0];
@@ -1389,10 +1387,9 @@ __device__ static inline void reduce_quad(const uint32_t (&q)[4], const float (&
     const float sp = dpp_f<0x138, 0xF>(ts); // the previous lane's scanned tail sum
     const uint32_t cp = with_c ? dpp_u<0x138, 0xF>(tc) : 0u;
     if (!whole)
-        run_add(q[0], cont_prev ? fs + sp : fs, cont_prev ? fc + cp : fc, pbase, rowlen, with_c,
-                hk, hs, hc, frow, hrow, counters);
+        emit(q[0], cont_prev ? fs + sp : fs, cont_prev ? fc + cp : fc);
     if (!cont_next)
-        run_add(q[3], ts, tc, pbase, rowlen, with_c, hk, hs, hc, frow, hrow, counters);
+        emit(q[3], ts, tc);
 }
 
 __device__ static inline void draw_span(const RasterRec &r, const Edges &e, int32_t vpx,
@@ -1686,8 +1683,11 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                 f4[2] = q4[2] ? d4.z : 0.f;
                 f4[3] = q4[3] ? d4.w : 0.f;
             }
-            reduce_quad(q4, f4, pbase, rowlen, hrow != nullptr, sh.t.key, sh.t.sum, sh.t.cnt,
-                        frow, hrow, counters);
+            const bool with_c = hrow != nullptr;
+            reduce_quad(q4, f4, with_c, [&](uint32_t id, float f, uint32_t c) {
+                run_add(id, f, c, pbase, rowlen, with_c, sh.t.key, sh.t.sum, sh.t.cnt, frow, hrow,
+                        counters);
+            });
         }
         __syncthreads();
         const uint32_t key = sh.t.key[threadIdx.x];
@@ -1810,16 +1810,19 @@ void launch_fold(unsigned long long *ffq, float *ff, size_t n, hipStream_t st)
 
 // ----------------------------------------------------------- ProcessHemicube
 // A workgroup takes kReduceChunk consecutive texels of one hemicube's id
-// plane (4 bytes per texel), one texel per lane per step.  Equal ids in a
-// wavefront's 64 consecutive texels form runs (the span walk of
-// Kernel_ProcessHemicube.h:36-49) summed with a segmented scan; each run's
+// plane (4 bytes per texel), 4 consecutive texels per lane per step.  Equal
+// ids in a wavefront's 256 consecutive texels form runs (the span walk of
+// Kernel_ProcessHemicube.h:36-49) summed in the lane and across lanes with
+// a segmented scan (reduce_quad); each run's
 // ΔFF sum and pixel count go into an LDS hash table keyed by id, which also
 // merges the runs of the chunk's other rows; the table is then flushed with
 // one global atomic per distinct id (the host accumulation of
 // Main.cpp:1257-1269 fused in).  A run that finds no free slot adds to the
 // global entry directly.  Acc is float, or the 2^-52 fixed point of the
 // deterministic mode (LDS and global sums in 64-bit integers).
-constexpr uint32_t kReduceChunk = 4096, kReduceSlots = 2048, kReduceBatch = 8;
+constexpr uint32_t kReduceChunk = 4096, kReduceSlotsLog2 = 11, kReduceSlots = 1u << kReduceSlotsLog2;
+static_assert(kReduceChunk % 1024 == 0, "k_reduce steps of 1024 texels");
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <typename Acc> __device__ static inline Acc reduce_acc(float f);
 template <> __device__ inline float reduce_acc<float>(float f) { return f; }
@@ -1850,47 +1853,54 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t RES, const uint32_t *__
     __syncthreads();
     const uint32_t *ip = ids_in + (size_t)hi * RES;
     const uint32_t c1 = min(RES, c0 + kReduceChunk);
-    // kReduceBatch texels per lane are loaded together (ids and weights,
-    // independent of each other), then reduced one after the other
-    for (uint32_t base = c0; base < c1; base += blockDim.x * kReduceBatch) {
-        uint32_t ids[kReduceBatch];
-        float fs[kReduceBatch];
-#pragma unroll
-        for (uint32_t b = 0; b < kReduceBatch; ++b) {
-            const uint32_t t = base + b * blockDim.x + threadIdx.x;
-            ids[b] = t < c1 ? __builtin_nontemporal_load(ip + t) : 0u;
-            fs[b] = t < c1 ? dff[t] : 0.0f;
+    const bool with_c = hits != nullptr;
+    // one run: into the LDS table (keyed by id), else straight to the row;
+    // Main.cpp:1259-1262: an unknown patch id is counted (once per run) and
+    // skipped
+    auto emit = [&](uint32_t id, float f, uint32_t c) {
+        if (!id)
+            return;
+        if (id - 1 >= npatch) {
+            atomicAdd(&counters[kCntUnknown], 1u);
+            return;
         }
-#pragma unroll
-        for (uint32_t b = 0; b < kReduceBatch; ++b) {
-            const uint32_t id = ids[b];
-            uint32_t c = id ? 1u : 0u;
-            float f = id ? fs[b] : 0.0f;
-            if (wave_runs(id, f, c, true)) {
-                uint32_t h = (id * 2654435761u) >> 21; // kReduceSlots = 2^11
-                const Acc a = reduce_acc<Acc>(f);
-                // Main.cpp:1259-1262: an unknown patch id is counted (once per
-                // run) and skipped
-                bool done = id - 1 >= npatch;
-                if (done)
-                    atomicAdd(&counters[kCntUnknown], 1u);
-                for (int probe = 0; probe < 16 && !done; ++probe, h = (h + 1) & (kReduceSlots - 1)) {
-                    uint32_t k = hkey[h];
-                    if (k == 0u)
-                        k = atomicCAS(&hkey[h], 0u, id);
-                    if (k == 0u || k == id) {
-                        atomicAdd(&hsum[h], a);
-                        atomicAdd(&hcnt[h], c);
-                        done = true;
-                    }
-                }
-                if (!done) { // no free slot: straight to the global entry
-                    atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], a);
-                    if (hits)
-                        atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], c);
-                }
+        const Acc a = reduce_acc<Acc>(f);
+        uint32_t h = (id * 2654435761u) >> (32 - kReduceSlotsLog2);
+        for (int probe = 0; probe < 16; ++probe, h = (h + 1) & (kReduceSlots - 1)) {
+            uint32_t k = hkey[h];
+            if (k == 0u)
+                k = atomicCAS(&hkey[h], 0u, id);
+            if (k == 0u || k == id) {
+                atomicAdd(&hsum[h], a);
+                if (with_c)
+                    atomicAdd(&hcnt[h], c);
+                return;
             }
         }
+        atomicAdd(&ff[(size_t)hi * npatch + (id - 1)], a);
+        if (with_c)
+            atomicAdd(&hits[(size_t)hi * npatch + (id - 1)], c);
+    };
+    // A step covers 1024 texels, 4 consecutive ones per lane (RES and the
+    // chunk size are multiples of 4, so the 16-byte loads are aligned and a
+    // lane's texels are all in the chunk or all out); the chunk's ids and
+    // weights are all loaded before the first step is reduced.
+    constexpr uint32_t kSteps = kReduceChunk / (4 * 256);
+    u32x4 iv[kSteps];
+    float4 fv[kSteps];
+#pragma unroll
+    for (uint32_t j = 0; j < kSteps; ++j) {
+        const uint32_t t = c0 + j * 1024u + threadIdx.x * 4u;
+        iv[j] = t < c1 ? __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(ip + t))
+                       : u32x4{0u, 0u, 0u, 0u};
+        fv[j] = t < c1 ? *reinterpret_cast<const float4 *>(dff + t) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kSteps; ++j) {
+        const uint32_t q[4] = {iv[j].x, iv[j].y, iv[j].z, iv[j].w};
+        const float f[4] = {q[0] ? fv[j].x : 0.f, q[1] ? fv[j].y : 0.f, q[2] ? fv[j].z : 0.f,
+                            q[3] ? fv[j].w : 0.f};
+        reduce_quad(q, f, with_c, emit);
     }
     __syncthreads();
     for (uint32_t q = threadIdx.x; q < kReduceSlots; q += blockDim.x) {
