@@ -1453,6 +1453,58 @@ __device__ static inline void draw_wave(const RasterRec &r, const Edges &e, int3
     }
 }
 
+// draw_wave for a record whose snapped vertices span at most 96 pixels in x
+// and in y (kWave32Span subpixels): every pixel a lane visits, the 8 x 8
+// blocks' overshoot past the bbox included, lies within 16 pixels of the
+// vertices' box, so |A (px - x_k)| + |B (py - y_k)| < 2 * 2^15 * 2^15 = 2^31
+// and the edge functions step in int32 (E + bias - 1 >= 0, exactly the
+// 64-bit test).  Depth is frag_key's expression.
+constexpr int32_t kWave32Span = 96 * kSub;
+
+__device__ static inline void draw_wave32(const RasterRec &r, int32_t vpx, int32_t vpy,
+                                          int32_t ax0, int32_t ax1, int32_t ay0, int32_t ay1,
+                                          int32_t ox, int32_t oy, uint32_t tshift, uint64_t *tk)
+{
+    const int32_t lane = (int32_t)(threadIdx.x & 63), lx = lane & 7, ly = lane >> 3;
+    const int32_t px0 = (ax0 - vpx + lx) * kSub + kSub / 2, py0 = (ay0 - vpy + ly) * kSub + kSub / 2;
+    int32_t er[3], sx[3], sy[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int q = k == 2 ? 0 : k + 1;
+        const int32_t A = r.y[k] - r.y[q], B = r.x[q] - r.x[k];
+        const int32_t bias = (A > 0 || (A == 0 && B < 0)) ? 1 : 0;
+        er[k] = A * (px0 - r.x[k]) + B * (py0 - r.y[k]) + bias - 1;
+        sx[k] = A * (8 * kSub);
+        sy[k] = B * (8 * kSub);
+    }
+    for (int32_t by = 0; ay0 + by <= ay1; by += 8) {
+        const int32_t ty = ay0 + by + ly;
+        const int32_t py = py0 + by * kSub;
+        const float zr = fmaf(r.dzdy, (float)(py - r.y[0]), r.z0);
+        int32_t e0 = er[0], e1 = er[1], e2 = er[2];
+        for (int32_t bx = 0; ax0 + bx <= ax1; bx += 8) {
+            const int32_t tx = ax0 + bx + lx;
+            if (tx <= ax1 && ty <= ay1 && (e0 | e1 | e2) >= 0) {
+                const int32_t px = px0 + bx * kSub;
+                float z = fmaf(r.dzdx, (float)(px - r.x[0]), zr);
+                if (!(z > 0.0f))
+                    z = 0.0f;
+                if (z < 1.0f) {
+                    const uint64_t key = ((uint64_t)__float_as_uint(z) << 32) | r.id;
+                    const uint32_t pp = (uint32_t)((ty - oy) << tshift) + (uint32_t)(tx - ox);
+                    atomicMin((unsigned long long *)&tk[pp], (unsigned long long)key);
+                }
+            }
+            e0 += sx[0];
+            e1 += sx[1];
+            e2 += sx[2];
+        }
+        er[0] += sy[0];
+        er[1] += sy[1];
+        er[2] += sy[2];
+    }
+}
+
 // One lane draws a record's pixels [ax0,ax1] x [ay0,ay1] (texels).  When the
 // snapped vertices span less than 64 pixels every edge value inside the bbox
 // fits in int32, so the edge functions are stepped with 32-bit adds; the
@@ -1622,9 +1674,15 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             const int32_t ay0 = max((int32_t)((bb >> 16) & 0xFFFFu), oy);
             const int32_t ay1 = min((int32_t)(bb >> 48), oy + (int32_t)T - 1);
             FaceView fv = face_view(Wi, (int)(rb.sf & 7u));
-            Edges e;
-            make_edges(rb.x, rb.y, e);
-            draw_wave(rb, e, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
+            const int32_t xspan = max(rb.x[0], max(rb.x[1], rb.x[2])) - min(rb.x[0], min(rb.x[1], rb.x[2]));
+            const int32_t yspan = max(rb.y[0], max(rb.y[1], rb.y[2])) - min(rb.y[0], min(rb.y[1], rb.y[2]));
+            if (xspan <= kWave32Span && yspan <= kWave32Span) {
+                draw_wave32(rb, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
+            } else {
+                Edges e;
+                make_edges(rb.x, rb.y, e);
+                draw_wave(rb, e, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
+            }
         }
         ri = rnext;
         if (ri != kNoRec)
