@@ -610,3 +610,72 @@ def test_triplet_stream_from_run_boundaries(wx):
     np.add.at(acc, (h, i), e.astype(np.float64))
     if wx in (4, 1):  # spans cover every texel
         assert np.allclose(acc, ff64, rtol=1e-5, atol=1e-12)
+
+
+# ------------------------------------------------- top-K and the K-shooter pass
+
+@pytest.mark.parametrize("seed", range(3))
+def test_select_topk_power_is_a_stable_sort(seed):
+    # POWER rule: key = area * (r + g + b) in float; patches without positive
+    # power are never taken; equal keys keep index order (numpy lexsort)
+    rng = np.random.default_rng(seed)
+    n = 500
+    u = rng.uniform(0.0, 1.0, size=(n, 3)).astype(np.float32)
+    u[rng.random(n) < 0.3] = 0.0
+    u[rng.random(n) < 0.05] *= -1.0
+    a = rng.choice([0.5, 1.0, 2.0], n).astype(np.float32)
+    u[10:20] = u[5]
+    a[10:20] = a[5]
+    key = a * ((u[:, 0] + u[:, 1]) + u[:, 2])
+    pos = np.nonzero(key > 0)[0]
+    order = pos[np.lexsort((pos, -key[pos]))]
+    for count in (1, 7, 64, 600):
+        ids, m = O.select_topk_power(u, a, count)
+        want = order[:count]
+        assert m == len(want)
+        assert np.array_equal(ids[:m], want)
+        assert not ids[m:].any()
+
+
+@pytest.mark.parametrize("mode,rule", [(0, 0), (1, 1)])
+def test_pass_k_single_shooter_equals_plain_pass(rad, mode, rule):
+    # orc_pass_k at K = 1 and orc_plain_pass are separate code: same shooter,
+    # same float state, and lastEnergy = the shooter's unshot before its
+    # snapshot is taken away
+    sc = rad.Scene.cornell(0.3127, triangle_patches=False, physical=bool(mode))
+    W = 32
+    dff = O.dff_cross(W)
+    u1, s1 = sc.emission.copy(), sc.shot0.copy()
+    u2, s2 = sc.emission.copy(), sc.shot0.copy()
+    for _ in range(4):
+        i = O.plain_pass(W, mode, rule, sc, dff, u1, s1)
+        snap = u2.copy()
+        ids, le = O.pass_k(W, mode, rule, 1, sc, dff, u2, s2)
+        assert list(ids) == [i]
+        assert np.array_equal(u1, u2) and np.array_equal(s1, s2)
+        assert np.array_equal(u2[i], le - snap[i])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pass_k_against_fp64_shoot(rad, mode):
+    # K shooters of one pass: F rows from the oracle's render and FP64
+    # histogram, applied by orc_shoot in FP64 (pinned by the hand-computed
+    # two-patch case and energy conservation) from the same start state
+    sc = rad.Scene.cornell(0.3127, triangle_patches=False, physical=bool(mode))
+    W, K = 32, 4
+    dff = O.dff_cross(W)
+    u, s = sc.emission.copy(), sc.shot0.copy()
+    O.pass_k(W, mode, mode, K, sc, dff, u, s)  # a first pass spreads the energy
+    u64, s64 = u.astype(np.float64), s.astype(np.float64)
+    ids, le = O.pass_k(W, mode, mode, K, sc, dff, u, s)
+    assert len(ids) == K
+    item = np.concatenate([O.render_hemicube(W, sc.tri, sc.tri_patch, sc.frame[i]) for i in ids])
+    _, ff64 = O.histogram(W, K, item, dff, sc.npatch)
+    last_before = u64[ids[-1]].copy()
+    O.shoot(mode, ids, ff64, sc.refl, sc.area, u64, s64)
+    scale = np.abs(u64).max()
+    assert np.abs(u - u64).max() <= 1e-5 * scale
+    assert np.abs(s - s64).max() <= 1e-5 * np.abs(s64).max()
+    # lastEnergy: the last shooter after the pass's contributions, before its
+    # own snapshot is taken away
+    assert np.allclose(le, u64[ids[-1]] + last_before, rtol=1e-5, atol=1e-6 * scale)
