@@ -2199,33 +2199,41 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
         inv_aj = 1.0f / area[j];
     }
     // F entries are read 8 shooters at a time (independent loads in flight),
-    // then added in shooter order
-    for (uint32_t s0 = 0; s0 < n; s0 += 8) {
-        float fb[8];
+    // then added in shooter order.  The nonzero entries of a group of 64
+    // shooters are set back to zero after the group's last read (a load
+    // issued after a store waits for it), which leaves the F rows zero for
+    // the next pass (its clear skips them).
+    for (uint32_t g0 = 0; g0 < n; g0 += 64) {
+        const uint32_t g1 = min(n, g0 + 64);
+        uint64_t nz = 0;
+        for (uint32_t s0 = g0; s0 < g1; s0 += 8) {
+            float fb[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            fb[q] = s0 + q < n ? ff[(size_t)(s0 + q) * npatch + j] : 0.0f;
+            for (int q = 0; q < 8; ++q)
+                fb[q] = s0 + q < g1 ? ff[(size_t)(s0 + q) * npatch + j] : 0.0f;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const uint32_t s = s0 + q;
-            const float f = fb[q];
-            if (f == 0.0f)
-                continue;
-            // leave the F rows zero for the next pass (its clear skips them)
-            ff[(size_t)s * npatch + j] = 0.0f;
-            float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
-            if (mode == RAD_MODE_REFERENCE) {
-                // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
-                u0 += d0 * f * 0.3f * sc[4 * s + 0];
-                u1 += d1 * f * 0.3f * sc[4 * s + 1];
-                u2 += d2 * f * 0.3f * sc[4 * s + 2];
-            } else {
-                float g = f * (sc[4 * s + 3] * inv_aj);
-                u0 += r0 * d0 * g;
-                u1 += r1 * d1 * g;
-                u2 += r2 * d2 * g;
+            for (int q = 0; q < 8; ++q) {
+                const uint32_t s = s0 + q;
+                const float f = fb[q];
+                if (f == 0.0f)
+                    continue;
+                nz |= 1ull << (s - g0);
+                float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
+                if (mode == RAD_MODE_REFERENCE) {
+                    // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
+                    u0 += d0 * f * 0.3f * sc[4 * s + 0];
+                    u1 += d1 * f * 0.3f * sc[4 * s + 1];
+                    u2 += d2 * f * 0.3f * sc[4 * s + 2];
+                } else {
+                    float g = f * (sc[4 * s + 3] * inv_aj);
+                    u0 += r0 * d0 * g;
+                    u1 += r1 * d1 * g;
+                    u2 += r2 * d2 * g;
+                }
             }
         }
+        for (uint64_t m = nz; m; m &= m - 1)
+            ff[(size_t)(g0 + (uint32_t)__ffsll((long long)m) - 1) * npatch + j] = 0.0f;
     }
     // Main.cpp:1287-1295
     if (n && sid[n - 1] == j)
