@@ -1062,6 +1062,38 @@ __device__ static inline void unpack_bbox(uint64_t v, uint32_t tshift, uint32_t 
     h1 = (uint32_t)(v >> 48) >> tshift;
 }
 
+// The spill records of shooter slots [s0, s1) into their tiles' lists: one
+// global atomic per entry (records go to the spill region only when a setup
+// block's region is full, which is rare).
+__device__ static void bin_spill(const RasterRec *__restrict__ recs,
+                                 const uint64_t *__restrict__ bbox,
+                                 const uint32_t *__restrict__ counters, uint32_t spill_base,
+                                 uint32_t spill_cap, const uint32_t *__restrict__ start,
+                                 uint32_t *__restrict__ fill, uint32_t *__restrict__ list,
+                                 uint32_t ent_cap, uint32_t ntx, uint32_t ntiles, uint32_t tshift,
+                                 uint32_t s0, uint32_t s1)
+{
+    const uint32_t nsp = min(counters[kCntSpill], spill_cap);
+    for (uint32_t s = s0; s < s1; ++s) {
+        uint32_t *fl = fill + (size_t)s * ntiles;
+        const uint32_t *stt = start + (size_t)s * ntiles;
+        for (uint32_t q = threadIdx.x; q < nsp; q += blockDim.x) {
+            const uint32_t r = spill_base + q;
+            if ((recs[r].sf >> 3) != s)
+                continue;
+            uint32_t g0, g1, h0, h1;
+            unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
+            for (uint32_t gy = h0; gy <= h1; ++gy)
+                for (uint32_t gx = g0; gx <= g1; ++gx) {
+                    const uint32_t b = gy * ntx + gx;
+                    const uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
+                    if (pos < ent_cap)
+                        list[pos] = r;
+                }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ recs,
                                                   const uint64_t *__restrict__ bbox,
                                                   const uint32_t *__restrict__ blk_nrec,
@@ -1079,31 +1111,9 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
     uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + ntiles; // dynamic LDS, 2 x ntiles
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
     if (blockIdx.x == nblk) {
-        if (blockIdx.z != 0)
-            return;
-        for (uint32_t g = 0; g < group; ++g) {
-            const uint32_t s = blockIdx.y * group + g;
-            if (s >= ns)
-                break;
-            uint32_t *fl = fill + (size_t)s * ntiles;
-            const uint32_t *stt = start + (size_t)s * ntiles;
-            // spill records of this shooter: one global atomic each (rare)
-            uint32_t nsp = min(counters[kCntSpill], spill_cap);
-            for (uint32_t q = threadIdx.x; q < nsp; q += blockDim.x) {
-                uint32_t r = spill_base + q;
-                if ((recs[r].sf >> 3) != s)
-                    continue;
-                uint32_t g0, g1, h0, h1;
-                unpack_bbox(bbox[r], tshift, g0, g1, h0, h1);
-                for (uint32_t gy = h0; gy <= h1; ++gy)
-                    for (uint32_t gx = g0; gx <= g1; ++gx) {
-                        uint32_t b = gy * ntx + gx;
-                        uint32_t pos = stt[b] + atomicAdd(&fl[b], 1u);
-                        if (pos < ent_cap)
-                            list[pos] = r;
-                    }
-            }
-        }
+        if (blockIdx.z == 0)
+            bin_spill(recs, bbox, counters, spill_base, spill_cap, start, fill, list, ent_cap, ntx,
+                      ntiles, tshift, blockIdx.y * group, min(ns, blockIdx.y * group + group));
         return;
     }
     // The workgroup walks `group` consecutive shooter slots, like the setup.
@@ -1777,6 +1787,136 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     }
 }
 
+// k_bin_fill for up to 256 * RPT tiles, software-pipelined over the
+// workgroup's shooter slots: the list space of slot g is reserved (one
+// returning global atomic per touched tile) and, while those atomics are in
+// flight, the records of the previous slot are placed; two LDS sets of
+// counts and bases alternate between the slots.  Slots without records are
+// skipped.  The placement is k_bin_fill's: a record's entries go to its
+// tiles' lists at base + rank, rank from an LDS atomic in lane order.
+template <int RPT>
+__global__ __launch_bounds__(256) void k_bin_fill_pipe(const RasterRec *__restrict__ recs,
+                                                       const uint64_t *__restrict__ bbox,
+                                                       const uint32_t *__restrict__ blk_nrec,
+                                                       const uint32_t *__restrict__ counters,
+                                                       uint32_t nblk, uint32_t spill_base,
+                                                       uint32_t spill_cap,
+                                                       const uint32_t *__restrict__ start,
+                                                       uint32_t *__restrict__ fill,
+                                                       uint32_t *__restrict__ list,
+                                                       uint32_t ent_cap, uint32_t ntx,
+                                                       uint32_t ntiles, uint32_t tshift,
+                                                       const uint32_t *__restrict__ nshoot,
+                                                       uint32_t group, uint32_t nslot)
+{
+    extern __shared__ uint32_t dyn_lds[]; // 4 x ntiles: counts and bases, twice
+    constexpr uint32_t kFillPer = kRecsPerBlock / 256;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
+    const uint32_t sbeg = blockIdx.y * group, send = min(ns, sbeg + group);
+    if (blockIdx.x == nblk) { // the last column: the spill records
+        if (blockIdx.z == 0)
+            bin_spill(recs, bbox, counters, spill_base, spill_cap, start, fill, list, ent_cap, ntx,
+                      ntiles, tshift, sbeg, send);
+        return;
+    }
+    auto blk_of = [&](uint32_t sl) { return (sl * gridDim.z + blockIdx.z) * nblk + blockIdx.x; };
+    uint64_t bv[kFillPer];
+    uint32_t nrec = 0;
+    auto prefetch = [&](uint32_t sl) {
+        const uint32_t blk = blk_of(sl);
+        nrec = blk_nrec[blk];
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k) {
+            const uint32_t q = tid + 256u * k;
+            bv[k] = q < nrec ? bbox[(size_t)blk * kRecsPerBlock + q] : 0ull;
+        }
+    };
+    // the records of one slot into the lists (bases lb, ranks from lc)
+    auto place = [&](const uint64_t (&v)[kFillPer], uint32_t n, uint32_t region, uint32_t *lc,
+                     const uint32_t *lb) {
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k) {
+            const uint32_t q = tid + 256u * k;
+            if (q >= n)
+                break;
+            uint32_t g0, g1, h0, h1;
+            unpack_bbox(v[k], tshift, g0, g1, h0, h1);
+            for (uint32_t gy = h0; gy <= h1; ++gy)
+                for (uint32_t gx = g0; gx <= g1; ++gx) {
+                    const uint32_t b = gy * ntx + gx;
+                    const uint32_t pos = lb[b] + atomicAdd(&lc[b], 1u);
+                    if (pos < ent_cap)
+                        list[pos] = region + q;
+                }
+        }
+    };
+    uint64_t pv[kFillPer];
+    uint32_t pn = 0, pregion = 0, it = 0;
+    if (sbeg < send)
+        prefetch(sbeg);
+    for (uint32_t sl = sbeg; sl < send; ++sl) {
+        const uint32_t n = nrec, region = blk_of(sl) * kRecsPerBlock;
+        uint64_t cur[kFillPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k)
+            cur[k] = bv[k];
+        if (sl + 1 < send)
+            prefetch(sl + 1);
+        if (n == 0)
+            continue;
+        uint32_t *lc = dyn_lds + (it & 1u) * 2u * ntiles, *lb = lc + ntiles;
+        uint32_t *pc = dyn_lds + ((it & 1u) ^ 1u) * 2u * ntiles, *pb = pc + ntiles;
+        __syncthreads(); // the slot two back is placed; the previous slot's bases are written
+        for (uint32_t i = tid; i < ntiles; i += 256u)
+            lc[i] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k) {
+            if (tid + 256u * k >= n)
+                break;
+            uint32_t g0, g1, h0, h1;
+            unpack_bbox(cur[k], tshift, g0, g1, h0, h1);
+            for (uint32_t gy = h0; gy <= h1; ++gy)
+                for (uint32_t gx = g0; gx <= g1; ++gx)
+                    atomicAdd(&lc[gy * ntx + gx], 1u);
+        }
+        __syncthreads();
+        // reserve this slot's list space ...
+        uint32_t *fl = fill + (size_t)sl * ntiles;
+        const uint32_t *stt = start + (size_t)sl * ntiles;
+        uint32_t rb[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const uint32_t i = tid + 256u * (uint32_t)r;
+            const uint32_t c = i < ntiles ? lc[i] : 0u;
+            rb[r] = c ? stt[i] + atomicAdd(&fl[i], c) : 0u;
+            if (i < ntiles)
+                lc[i] = 0u;
+        }
+        // ... while the previous slot's records are placed
+        if (it)
+            place(pv, pn, pregion, pc, pb);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const uint32_t i = tid + 256u * (uint32_t)r;
+            if (i < ntiles)
+                lb[i] = rb[r];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFillPer; ++k)
+            pv[k] = cur[k];
+        pn = n;
+        pregion = region;
+        ++it;
+    }
+    if (it) {
+        __syncthreads();
+        uint32_t *lc = dyn_lds + ((it - 1u) & 1u) * 2u * ntiles;
+        place(pv, pn, pregion, lc, lc + ntiles);
+    }
+}
+
 void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
                          const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
                          const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
@@ -1818,9 +1958,21 @@ void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32
     if (group > K)
         group = K;
     dim3 grid(bb.nblk + 1, (K + group - 1) / group, bb.nz);
-    k_bin_fill<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
-        bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap, bb.start,
-        bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
+    if (bb.ntiles > 1024) {
+        k_bin_fill<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
+            bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap,
+            bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
+        return;
+    }
+    const size_t lds = 4 * bb.ntiles * sizeof(uint32_t);
+    if (bb.ntiles <= 256)
+        k_bin_fill_pipe<1><<<grid, 256, lds, st>>>(
+            bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap,
+            bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
+    else
+        k_bin_fill_pipe<4><<<grid, 256, lds, st>>>(
+            bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap,
+            bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
 }
 
 void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,

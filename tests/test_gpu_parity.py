@@ -95,6 +95,23 @@ def test_itembuf_bit_exact_boxes(rad, boxes_small):
     _render_case(rad, boxes_small, 128, sh)
 
 
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_itembuf_bit_exact_w256_every_tile_side(rad, boxes_small, tile):
+    # W=256 gives 3,072, 768 and 192 tiles per hemicube: the three ways the
+    # bin fill runs (one tile per thread, four per thread, and the plain fill
+    # beyond 1,024 tiles)
+    say(f"itembuf boxes W=256 tile {tile}")
+    rng = np.random.default_rng(5)
+    shooters = [int(x) for x in rng.choice(boxes_small.npatch, 3, replace=False)]
+    s = rad.Solver(hemicube=256, hemicubes=len(shooters), tile_px=tile, small_tri_px=64)
+    s.upload(boxes_small)
+    item = s.render(shooters)
+    res = s.tex_res
+    for k, p in enumerate(shooters):
+        it_o = O.render_hemicube(256, boxes_small.tri, boxes_small.tri_patch, boxes_small.frame[p])
+        assert np.array_equal(item[k * res:(k + 1) * res], it_o)
+
+
 def test_itembuf_tuning_knobs_invariant(rad, boxes_small):
     # tile size and the in-lane pixel limit are tuning knobs: the item buffer
     # must not depend on them
