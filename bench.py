@@ -515,6 +515,7 @@ def main():
                 "candidates": {f"{a}x{b}": round(v, 5) for (a, b), v in sorted(cands.items())}}
         log(f"autotune K={K}: tile {t}, in-lane limit {sm} px ({tms:.3f} ms per render pass)")
     s.step(args.warmup)
+    s.prepare()             # capture the pass graph before the clock starts
     s.finish()
     c0 = s.shot_count()
     t0 = time.perf_counter()

@@ -158,6 +158,13 @@ int rad_shoot(rad_ctx *ctx, const uint32_t *shooters, uint32_t k);
  * device without host synchronisation. */
 int rad_step(rad_ctx *ctx, uint32_t passes);
 
+/* No counterpart in the reference: captures and uploads the graph of
+ * graph_passes passes that rad_step replays, and runs no pass, so that the
+ * first rad_step after it pays no capture.  Nothing to do when graph_passes
+ * is 0 or kernel timing is on.  A new scene, tile size or pack drops the
+ * graph again. */
+int rad_prepare(rad_ctx *ctx);
+
 /* Runs passes until the FP64 residual <= residual_target or max_passes;
  * the residual is checked every check_every passes (the residual rule; the
  * reference's own per-pass stop test is rad_run_cycle). */
@@ -240,7 +247,9 @@ int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32
  * [3] bin entries, [4] item-buffer ids beyond npatch, [5] records in the
  * spill region, [6] triangle-faces queued for clipping, [7] nonzero if an
  * earlier pass overflowed.  Capacities: out_caps (optional, 3 entries) gets
- * the spill capacity, the bin-entry capacity and the clip-queue capacity. */
+ * the spill capacity, the bin-entry capacity and the clip-queue capacity.
+ * A pass of 16 or more slots renders as two halves, each with these
+ * capacities: [3], [5] and [6] are then the fuller half's, [4] the sum. */
 int rad_debug_counters(rad_ctx *ctx, uint32_t *out8, uint32_t *out_caps_or_null);
 
 /* Autotuner of the rasterizer: times `passes` render passes of K shooters

@@ -97,6 +97,7 @@ SIGNATURES = {
     "rad_process_runs_static": (C.c_int, [_P, _P, _u32, _u32, _P, _P, _P, _P]),
     "rad_shoot": (C.c_int, [_P, _PU, _u32]),
     "rad_step": (C.c_int, [_P, _u32]),
+    "rad_prepare": (C.c_int, [_P]),
     "rad_run": (C.c_int, [_P, C.c_uint64, C.c_double, _u32, C.POINTER(rad_stats)]),
     "rad_run_cycle": (C.c_int, [_P, _u32, C.c_double, C.POINTER(rad_cycle_stats)]),
     "rad_shot_count": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -445,6 +446,10 @@ class Solver:
 
     def step(self, passes=1):
         self._chk(lib().rad_step(self._ctx, passes), "rad_step")
+
+    def prepare(self):
+        """Captures the pass graph now (rad_prepare); runs no pass."""
+        self._chk(lib().rad_prepare(self._ctx), "rad_prepare")
 
     def finish(self):
         self._chk(lib().rad_finish(self._ctx), "rad_finish")

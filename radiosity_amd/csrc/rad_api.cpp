@@ -119,16 +119,36 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
                          c->counters, c->bb.ent_cap, c->ctl, c->st);
     c->ff_dirty = 0;
     record(c, evbase + 2);
-    launch_raster_setup(c->g, k, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp, c->nshoot, c->bb,
-                        c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt, c->pk.slot_w, c->st);
-    record(c, evbase + 3);
-    launch_bin(k, c->nshoot, c->bb, c->counters, c->st);
-    record(c, evbase + 4);
     const bool fq = !ids && c->det;
-    launch_tile(c->g, k, c->nshoot, c->bb, c->small_px, ids, depth,
-                c->pk.slot_w ? c->pk.dffs : c->dff, c->rowlen, c->pk.slot_pbeg,
-                ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits, c->counters,
-                c->pk.slot_w, c->pk.slot_dffo, c->st);
+    // slots [s0, s1) with counter block h on stream st
+    auto part = [&](uint32_t s0, uint32_t s1, uint32_t h, hipStream_t st, bool timed) {
+        launch_raster_setup(c->g, s0, s1, h, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp,
+                            c->nshoot, c->bb, c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt,
+                            c->pk.slot_w, st);
+        if (timed)
+            record(c, evbase + 3);
+        launch_bin(s0, s1, h, c->nshoot, c->bb, c->counters, st);
+        if (timed)
+            record(c, evbase + 4);
+        launch_tile(c->g, s0, s1, h, c->nshoot, c->bb, c->small_px, ids, depth,
+                    c->pk.slot_w ? c->pk.dffs : c->dff, c->rowlen, c->pk.slot_pbeg,
+                    ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits, c->counters,
+                    c->pk.slot_w, c->pk.slot_dffo, st);
+    };
+    // A pass of 16 or more slots renders as two halves side by side: the
+    // halves share no bins, records, list entries or counters, so the item
+    // buffers and F rows are those of one launch over all slots.  Timed
+    // passes stay whole so that the stage events bracket every kernel.
+    if (k >= kSplitMin && !c->timing) {
+        HIPCHK(c, hipEventRecord(c->ev_fork, c->st));
+        HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+        part(0, k / 2, 0, c->st, false);
+        part(k / 2, k, 1, c->st2, false);
+        HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
+        HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
+    } else {
+        part(0, k, 0, c->st, true);
+    }
     if (fq)
         launch_fold(c->ffq, c->ff, (size_t)k * c->rowlen, c->st);
     if (!ids)
@@ -142,18 +162,21 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
 // pass lost triangles.
 int check_overflow(rad_ctx *c)
 {
-    uint32_t cnt[kCntN];
+    uint32_t cnt[kCntN * kCntBlocks];
     hipError_t e = hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost);
     if (e != hipSuccess)
         return fail(c, RAD_E_HIP, std::string("counters: ") + hipGetErrorString(e));
-    if (cnt[kCntOverflow] || cnt[kCntEntries] > c->bb.ent_cap || cnt[kCntSticky]) {
-        uint32_t zero = 0;
-        (void)hipMemcpy(c->counters + kCntSticky, &zero, 4, hipMemcpyHostToDevice);
-        return fail(c, RAD_E_LIMIT, "rasterizer bins overflowed (records " +
-                                        std::to_string(cnt[kCntSpill]) + " spilled of " +
-                                        std::to_string(c->bb.spill_cap) + ", entries " +
-                                        std::to_string(cnt[kCntEntries]) + " of " +
-                                        std::to_string(c->bb.ent_cap) + ")");
+    for (int h = 0; h < kCntBlocks; ++h) {
+        const uint32_t *b = cnt + h * kCntN;
+        if (b[kCntOverflow] || b[kCntEntries] > c->bb.ent_cap || cnt[kCntSticky]) {
+            uint32_t zero = 0;
+            (void)hipMemcpy(c->counters + kCntSticky, &zero, 4, hipMemcpyHostToDevice);
+            return fail(c, RAD_E_LIMIT, "rasterizer bins overflowed (records " +
+                                            std::to_string(b[kCntSpill]) + " spilled of " +
+                                            std::to_string(c->bb.spill_cap) + ", entries " +
+                                            std::to_string(b[kCntEntries]) + " of " +
+                                            std::to_string(c->bb.ent_cap) + ")");
+        }
     }
     return RAD_OK;
 }
@@ -281,20 +304,26 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess)
         e = hipStreamCreate(&c->st);
+    if (e == hipSuccess)
+        e = hipStreamCreate(&c->st2);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete c;
+        rad_destroy(c);
         return RAD_E_HIP;
     }
     if ((rc = dalloc(c, &c->dff, c->g.RES)) ||
         (rc = dalloc(c, &c->item_ids, (size_t)c->K * c->g.RES)) ||
         (rc = dalloc(c, &c->mvp, (size_t)c->K * kShooterStride)) || (rc = dalloc(c, &c->snap, (size_t)c->K * 3)) ||
         (rc = dalloc(c, &c->shooters, c->K)) || (rc = dalloc(c, &c->nshoot, 1)) ||
-        (rc = dalloc(c, &c->counters, kCntN)) || (rc = dalloc(c, &c->sel, 4)) ||
+        (rc = dalloc(c, &c->counters, kCntN * kCntBlocks)) || (rc = dalloc(c, &c->sel, 4)) ||
         (rc = dalloc(c, &c->resid, 8)) || (rc = dalloc(c, &c->ctl, kCtlN))) {
         rad_destroy(c);
         return rc;
     }
-    if (hipMemsetAsync(c->counters, 0, kCntN * sizeof(uint32_t), c->st) != hipSuccess ||
+    if (hipMemsetAsync(c->counters, 0, kCntN * kCntBlocks * sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->ctl, 0, kCtlN * sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->nshoot, 0, sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->shooters, 0, c->K * sizeof(uint32_t), c->st) != hipSuccess) {
@@ -335,6 +364,8 @@ void rad_destroy(rad_ctx *c)
         return;
     if (c->st)
         (void)hipStreamSynchronize(c->st);
+    if (c->st2)
+        (void)hipStreamSynchronize(c->st2);
     drop_graph(c);
     void *ptrs[] = {c->tri,  c->tri_patch, c->tsph, c->frames, c->refl,     c->area,   c->emis,
                     c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->item_ids,
@@ -354,6 +385,11 @@ void rad_destroy(rad_ctx *c)
         dfree(p);
     for (hipEvent_t e : c->ev)
         (void)hipEventDestroy(e);
+    for (hipEvent_t e : {c->ev_fork, c->ev_join})
+        if (e)
+            (void)hipEventDestroy(e);
+    if (c->st2)
+        (void)hipStreamDestroy(c->st2);
     if (c->st)
         (void)hipStreamDestroy(c->st);
     delete c;
@@ -453,7 +489,7 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
     c->bb.nz = K <= 4 ? 5u : 1u;
     uint64_t regions = (uint64_t)c->bb.nblk * K * c->bb.nz * kRecsPerBlock;
     uint64_t spill = regions / 8 + 65536;
-    if (regions + spill > (1ull << 31))
+    if (regions + kCntBlocks * spill > (1ull << 31))
         return fail(c, RAD_E_LIMIT, "scene too large for the rasterizer record regions");
     c->bb.spill_base = (uint32_t)regions;
     c->bb.spill_cap = (uint32_t)spill;
@@ -461,10 +497,13 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
     c->bb.clip_blocks = 512;
     uint64_t went = 2 * (regions + spill);
     c->bb.ent_cap = (uint32_t)(went > 0xF0000000ull ? 0xF0000000ull : went);
-    if ((rc = dalloc(c, &c->bb.recs, regions + spill)) || (rc = dalloc(c, &c->bb.bbox, regions + spill)) ||
-        (rc = dalloc(c, &c->bb.list, c->bb.ent_cap)) ||
+    // each half of a split pass has its own spill region, list area and
+    // clipping queue
+    const uint64_t nrec = regions + kCntBlocks * spill;
+    if ((rc = dalloc(c, &c->bb.recs, nrec)) || (rc = dalloc(c, &c->bb.bbox, nrec)) ||
+        (rc = dalloc(c, &c->bb.list, (size_t)kCntBlocks * c->bb.ent_cap)) ||
         (rc = dalloc(c, &c->bb.blk_nrec, (size_t)c->bb.nblk * K * c->bb.nz)) ||
-        (rc = dalloc(c, &c->bb.clipq, c->bb.clip_cap)) ||
+        (rc = dalloc(c, &c->bb.clipq, (size_t)kCntBlocks * c->bb.clip_cap)) ||
         (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
         return rc;
     HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
@@ -687,7 +726,7 @@ static int reduce_out(rad_ctx *c, uint32_t k, uint32_t *out_hits, float *out_ff)
     uint32_t rows = c->ff_dirty > k ? c->ff_dirty : k;
     HIPCHK(c, hipMemsetAsync(c->ff, 0, (size_t)rows * c->rowlen * 4, c->st));
     HIPCHK(c, hipMemsetAsync(c->hits, 0, (size_t)k * c->npatch * 4, c->st));
-    HIPCHK(c, hipMemsetAsync(c->counters, 0, kCntN * 4, c->st));
+    HIPCHK(c, hipMemsetAsync(c->counters, 0, kCntN * kCntBlocks * 4, c->st));
     int rc;
     if (c->timing && (rc = ensure_events(c, 2)))
         return rc;
@@ -914,6 +953,33 @@ int rad_step(rad_ctx *c, uint32_t passes)
         return rc;
     if ((rc = step_impl(c, passes)))
         return rc;
+    return RAD_OK;
+}
+
+int rad_prepare(rad_ctx *c)
+{
+    if (!c)
+        return RAD_E_ARG;
+    int rc;
+    if ((rc = need_scene(c)))
+        return rc;
+    const uint32_t G = c->cfg.graph_passes;
+    if (c->timing || !G)
+        return RAD_OK;
+    if (!c->gexec || c->gpasses != G) {
+        // captured as rad_step captures, from zero F rows (rad_step clears
+        // dirty rows before it replays); the capture runs nothing, so the
+        // record of dirty rows and of the last render stays as it was
+        const uint32_t dirty = c->ff_dirty;
+        const bool rendered = c->rendered;
+        c->ff_dirty = 0;
+        rc = capture_graph(c, G);
+        c->ff_dirty = dirty;
+        c->rendered = rendered;
+        if (rc)
+            return rc;
+    }
+    HIPCHK(c, hipGraphUpload(c->gexec, c->st));
     return RAD_OK;
 }
 
@@ -1229,7 +1295,22 @@ int rad_debug_counters(rad_ctx *c, uint32_t *out8, uint32_t *caps)
         return RAD_E_ARG;
     static_assert(kCntN == 8, "eight pass counters");
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(out8, c->counters, kCntN * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t cnt[kCntN * kCntBlocks];
+    HIPCHK(c, hipMemcpy(cnt, c->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    // the halves of a split pass: flags OR, unknown ids add up, and the
+    // fill levels are the fuller half's (each half has the capacities)
+    for (int i = 0; i < kCntN; ++i) {
+        out8[i] = 0;
+        for (int h = 0; h < kCntBlocks; ++h) {
+            const uint32_t v = cnt[h * kCntN + i];
+            if (i == kCntOverflow || i == kCntSticky)
+                out8[i] |= v;
+            else if (i == kCntUnknown || i == kCntRec)
+                out8[i] += v;
+            else
+                out8[i] = out8[i] > v ? out8[i] : v;
+        }
+    }
     if (caps) {
         caps[0] = c->bb.spill_cap;
         caps[1] = c->bb.ent_cap;

@@ -21,6 +21,10 @@ struct rad_ctx {
     uint32_t K = 1;
     int device = 0;
     hipStream_t st = nullptr;
+    // second stream of a split pass (its halves render side by side), with
+    // the events that fork it from st and join it back
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
 
     // scene
@@ -59,7 +63,7 @@ struct rad_ctx {
     float *snap = nullptr;         // K*3
     uint32_t *shooters = nullptr;  // K
     uint32_t *nshoot = nullptr;    // 1
-    uint32_t *counters = nullptr;  // kCntN
+    uint32_t *counters = nullptr;  // kCntBlocks blocks of kCntN
     uint32_t *ctl = nullptr;       // kCtlN: stop test, pass and shooter counts
     uint64_t *sel = nullptr;       // 1 (+ pad)
     double *resid = nullptr;       // [0,1] selection residual, [4,5] rad_residual_fp64

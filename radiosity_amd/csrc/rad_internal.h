@@ -50,6 +50,11 @@ enum {
     kCntSticky = 7,   // some earlier pass overflowed (cleared by the host only)
     kCntN = 8
 };
+// A pass may render as two halves on two streams (slots [0, k/2) and
+// [k/2, k)), each with its own block of kCntN counters, spill region, clip
+// queue and list area; the sticky flag lives in the first block.
+constexpr int kCntBlocks = 2;
+constexpr uint32_t kSplitMin = 16; // smallest pass rendered as two halves
 
 // Control block of a context (u32 words on the device), written by the
 // shoot kernels and read by the pass gate in k_prepare_clear:
@@ -65,15 +70,16 @@ enum {
 
 // Bins of the sort-middle rasterizer: one per (shooter slot, tile).
 struct BinBuffers {
-    RasterRec *recs;       // nblk*K regions of kRecsPerBlock, then spill_cap
+    RasterRec *recs;       // nblk*K regions of kRecsPerBlock, then kCntBlocks spill regions of spill_cap
     uint64_t *bbox;        // texel bbox per record slot, read by the bin fill
     uint32_t *blk_nrec;    // records per (shooter, setup block)
-    uint32_t *count, *start, *fill, *list;
+    uint32_t *count, *start, *fill;
+    uint32_t *list;        // kCntBlocks list areas of ent_cap entries
     uint32_t *order;       // bins by descending size class (tile launch order)
     uint32_t nblk;         // setup blocks per shooter and face group (256 triangles each)
     uint32_t nz;           // face groups: 5 (one face per block, small K) or 1
     uint32_t spill_base, spill_cap, ent_cap;
-    uint2 *clipq;          // clipping jobs of a pass
+    uint2 *clipq;          // kCntBlocks queues of clip_cap clipping jobs
     uint32_t clip_cap;
     int clip_blocks;
     uint32_t ntx, nty, ntiles, tshift;
@@ -141,23 +147,25 @@ void launch_prepare_clear(uint32_t K, const uint32_t *shooters, uint32_t *nshoot
 void launch_clear(uint32_t K, uint32_t *bin_count, uint32_t ntiles, float *ff, uint32_t *hits,
                   uint32_t ff_rows, uint32_t rowlen, uint32_t *counters, uint32_t ent_cap,
                   hipStream_t st);
-// Binned render of the nshoot (<= K) hemicubes in three launches:
-// setup (records + bin counts), bin (scan + fill), tile (LDS depth test;
-// out_ids != null writes the item buffer, ff or ffq != null reduces into F
-// rows and hits).
-void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
-                         const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
-                         const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt,
-                         const uint32_t *slot_w, hipStream_t st);
+// Binned render of the shooter slots [s0, s1) (of which those below nshoot
+// are drawn) in three launches: setup (records + bin counts), bin (scan +
+// fill), tile (LDS depth test; out_ids != null writes the item buffer, ff or
+// ffq != null reduces into F rows and hits).  h selects the counter block,
+// spill region, clip queue and list area (0, or 1 for the second half of a
+// split pass).
+void launch_raster_setup(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h,
+                         const float *tri, const float4 *tsph, const uint32_t *tri_patch,
+                         uint32_t ntri, const float *mvp, const uint32_t *nshoot,
+                         const BinBuffers &bb, uint32_t *counters, const uint32_t *slot_tbeg,
+                         const uint32_t *slot_tcnt, const uint32_t *slot_w, hipStream_t st);
 void launch_tri_sphere(const float *tri, uint32_t ntri, float4 *sph, hipStream_t st);
-void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                hipStream_t st);
-void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
-                 uint32_t small_px, uint32_t *out_ids, float *out_depth, const float *dff,
-                 uint32_t rowlen, const uint32_t *slot_pbeg, float *ff, unsigned long long *ffq,
-                 uint32_t *hits, uint32_t *counters, const uint32_t *slot_w,
-                 const uint32_t *slot_dffo, hipStream_t st);
+void launch_bin(uint32_t s0, uint32_t s1, uint32_t h, const uint32_t *nshoot,
+                const BinBuffers &bb, uint32_t *counters, hipStream_t st);
+void launch_tile(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h, const uint32_t *nshoot,
+                 const BinBuffers &bb, uint32_t small_px, uint32_t *out_ids, float *out_depth,
+                 const float *dff, uint32_t rowlen, const uint32_t *slot_pbeg, float *ff,
+                 unsigned long long *ffq, uint32_t *hits, uint32_t *counters,
+                 const uint32_t *slot_w, const uint32_t *slot_dffo, hipStream_t st);
 // Deterministic mode: fixed-point F entries (2^-52 units) into the float rows.
 void launch_fold(unsigned long long *ffq, float *ff, size_t n, hipStream_t st);
 // ProcessHemicube over an id plane (K x RES u32): ff (or ffq) and hits +=.

@@ -251,8 +251,13 @@ __device__ static void clear_range(size_t i, size_t stride, uint32_t *__restrict
                                    uint32_t *__restrict__ counters, uint32_t ent_cap)
 {
     if (i == 0) {
-        bool bad = counters[kCntOverflow] != 0 || counters[kCntEntries] > ent_cap;
-        for (int k = 0; k < kCntN; ++k)
+        // every block of pass counters (one per half-pass), the sticky flag
+        // in the first
+        bool bad = false;
+        for (int h = 0; h < kCntBlocks; ++h)
+            bad = bad || counters[h * kCntN + kCntOverflow] != 0 ||
+                  counters[h * kCntN + kCntEntries] > ent_cap;
+        for (int k = 0; k < kCntN * kCntBlocks; ++k)
             if (k != kCntSticky)
                 counters[k] = 0;
         if (bad)
@@ -741,7 +746,8 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
                                                       const uint32_t *__restrict__ slot_tbeg,
                                                       const uint32_t *__restrict__ slot_tcnt,
                                                       const uint32_t *__restrict__ slot_w,
-                                                      uint32_t group, uint32_t nslot)
+                                                      uint32_t group, uint32_t slot0,
+                                                      uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[];
     uint32_t *lcnt = dyn_lds; // 4 x bc.ntiles per-tile counts (dynamic LDS)
@@ -772,7 +778,7 @@ __global__ __launch_bounds__(256) void k_raster_setup(uint32_t W, const float *_
     float vx[3][3];
     float4 sph = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (uint32_t g = 0; g < group;) {
-        const uint32_t s = blockIdx.y * group + g;
+        const uint32_t s = (slot0 + blockIdx.y * group) + g;
         if (s >= ns)
             break;
         const uint32_t tb = slot_tbeg ? slot_tbeg[s] : 0u, tc = slot_tcnt ? slot_tcnt[s] : ntri;
@@ -943,7 +949,8 @@ __device__ static uint32_t cull_faces(const float *__restrict__ sm, const float 
 // Also orders the bins for the tile kernel by descending size class
 // (floor(log2(count)) buckets, a counting sort): the largest bins start
 // first, so the heavy horizon tiles overlap the light ones instead of
-// forming the tail of the launch.
+// forming the tail of the launch.  The arrays given start at bin bin0 of the
+// pass (a half-pass scans its own bins); the order holds pass bin numbers.
 constexpr uint32_t kScanPer = 8, kScanChunk = 1024 * kScanPer;
 
 __device__ static inline uint32_t size_class(uint32_t c)
@@ -955,7 +962,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
                                                    uint32_t *__restrict__ start,
                                                    uint32_t *__restrict__ fill, uint32_t n,
                                                    uint32_t *__restrict__ counters,
-                                                   uint32_t *__restrict__ order)
+                                                   uint32_t *__restrict__ order, uint32_t bin0)
 {
     __shared__ uint32_t lc[kScanChunk];
     __shared__ uint32_t wsum[16];
@@ -1047,7 +1054,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const uint32_t *__restrict__ 
     const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = tid * per, b1 = min(n, b0 + per);
     for (uint32_t i = b0; i < b1; ++i)
-        order[atomicAdd(&boff[size_class(cnt[i])], 1u)] = i;
+        order[atomicAdd(&boff[size_class(cnt[i])], 1u)] = bin0 + i;
 }
 
 // Same grid as k_raster_setup: block (blk, s) puts the records of its region
@@ -1105,7 +1112,8 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
                                                   uint32_t *__restrict__ list, uint32_t ent_cap,
                                                   uint32_t ntx, uint32_t ntiles, uint32_t tshift,
                                                   const uint32_t *__restrict__ nshoot,
-                                                  uint32_t group, uint32_t nslot)
+                                                  uint32_t group, uint32_t slot0,
+                                                  uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[];
     uint32_t *lcnt = dyn_lds, *lbase = dyn_lds + ntiles; // dynamic LDS, 2 x ntiles
@@ -1113,7 +1121,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
     if (blockIdx.x == nblk) {
         if (blockIdx.z == 0)
             bin_spill(recs, bbox, counters, spill_base, spill_cap, start, fill, list, ent_cap, ntx,
-                      ntiles, tshift, blockIdx.y * group, min(ns, blockIdx.y * group + group));
+                      ntiles, tshift, (slot0 + blockIdx.y * group), min(ns, (slot0 + blockIdx.y * group) + group));
         return;
     }
     // The workgroup walks `group` consecutive shooter slots, like the setup.
@@ -1133,10 +1141,10 @@ __global__ __launch_bounds__(256) void k_bin_fill(const RasterRec *__restrict__ 
             bv[k] = q < nrec ? bbox[(size_t)blk * kRecsPerBlock + q] : 0ull;
         }
     };
-    if (blockIdx.y * group < ns)
-        prefetch(blockIdx.y * group);
+    if ((slot0 + blockIdx.y * group) < ns)
+        prefetch((slot0 + blockIdx.y * group));
     for (uint32_t g = 0; g < group; ++g) {
-        const uint32_t s = blockIdx.y * group + g;
+        const uint32_t s = (slot0 + blockIdx.y * group) + g;
         if (s >= ns)
             break;
         uint32_t *fl = fill + (size_t)s * ntiles;
@@ -1807,13 +1815,14 @@ __global__ __launch_bounds__(256) void k_bin_fill_pipe(const RasterRec *__restri
                                                        uint32_t ent_cap, uint32_t ntx,
                                                        uint32_t ntiles, uint32_t tshift,
                                                        const uint32_t *__restrict__ nshoot,
-                                                       uint32_t group, uint32_t nslot)
+                                                       uint32_t group, uint32_t slot0,
+                                                       uint32_t nslot)
 {
     extern __shared__ uint32_t dyn_lds[]; // 4 x ntiles: counts and bases, twice
     constexpr uint32_t kFillPer = kRecsPerBlock / 256;
     const uint32_t tid = threadIdx.x;
     const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
-    const uint32_t sbeg = blockIdx.y * group, send = min(ns, sbeg + group);
+    const uint32_t sbeg = slot0 + blockIdx.y * group, send = min(ns, sbeg + group);
     if (blockIdx.x == nblk) { // the last column: the spill records
         if (blockIdx.z == 0)
             bin_spill(recs, bbox, counters, spill_base, spill_cap, start, fill, list, ent_cap, ntx,
@@ -1917,80 +1926,88 @@ __global__ __launch_bounds__(256) void k_bin_fill_pipe(const RasterRec *__restri
     }
 }
 
-void launch_raster_setup(const Geometry &g, uint32_t K, const float *tri, const float4 *tsph,
-                         const uint32_t *tri_patch, uint32_t ntri, const float *mvp,
-                         const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                         const uint32_t *slot_tbeg, const uint32_t *slot_tcnt,
-                         const uint32_t *slot_w, hipStream_t st)
+void launch_raster_setup(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h,
+                         const float *tri, const float4 *tsph, const uint32_t *tri_patch,
+                         uint32_t ntri, const float *mvp, const uint32_t *nshoot,
+                         const BinBuffers &bb, uint32_t *counters, const uint32_t *slot_tbeg,
+                         const uint32_t *slot_tcnt, const uint32_t *slot_w, hipStream_t st)
 {
     BinCtx bc;
     bc.recs = bb.recs;
     bc.bbox = bb.bbox;
     bc.bin_count = bb.count;
-    bc.counters = counters;
+    bc.counters = counters + h * kCntN;
     bc.lnrec = nullptr;
     bc.lcnt = nullptr;
     bc.region = 0;
-    bc.spill_base = bb.spill_base;
+    bc.spill_base = bb.spill_base + h * bb.spill_cap;
     bc.spill_cap = bb.spill_cap;
-    bc.clipq = bb.clipq;
+    bc.clipq = bb.clipq + (size_t)h * bb.clip_cap;
     bc.clip_cap = bb.clip_cap;
     bc.ntx = bb.ntx;
     bc.ntiles = bb.ntiles;
     bc.tshift = bb.tshift;
+    const uint32_t n = s1 - s0;
     uint32_t group = bb.setup_group ? bb.setup_group : 1u;
-    if (group > K)
-        group = K;
-    dim3 grid(bb.nblk, (K + group - 1) / group, bb.nz);
+    if (group > n)
+        group = n;
+    dim3 grid(bb.nblk, (n + group - 1) / group, bb.nz);
     k_raster_setup<<<grid, 256, 4 * bb.ntiles * sizeof(uint32_t), st>>>(
         g.W, tri, tsph, tri_patch, ntri, mvp, nshoot, bc, bb.blk_nrec, slot_tbeg, slot_tcnt,
-        slot_w, group, K);
+        slot_w, group, s0, s1);
     bc.lnrec = nullptr;
     k_setup_clipped<<<bb.clip_blocks, 64, 0, st>>>(g.W, tri, tri_patch, ntri, mvp, bc, slot_w);
 }
 
-void launch_bin(uint32_t K, const uint32_t *nshoot, const BinBuffers &bb, uint32_t *counters,
-                hipStream_t st)
+void launch_bin(uint32_t s0, uint32_t s1, uint32_t h, const uint32_t *nshoot,
+                const BinBuffers &bb, uint32_t *counters, hipStream_t st)
 {
-    uint32_t nb = K * bb.ntiles;
-    k_bin_scan<<<1, 1024, 0, st>>>(bb.count, bb.start, bb.fill, nb, counters, bb.order);
+    const uint32_t n = s1 - s0, b0 = s0 * bb.ntiles, nb = n * bb.ntiles;
+    uint32_t *ctr = counters + h * kCntN, *list = bb.list + (size_t)h * bb.ent_cap;
+    const uint32_t spill_base = bb.spill_base + h * bb.spill_cap;
+    k_bin_scan<<<1, 1024, 0, st>>>(bb.count + b0, bb.start + b0, bb.fill + b0, nb, ctr,
+                                   bb.order + b0, b0);
     uint32_t group = bb.setup_group ? bb.setup_group : 1u;
-    if (group > K)
-        group = K;
-    dim3 grid(bb.nblk + 1, (K + group - 1) / group, bb.nz);
+    if (group > n)
+        group = n;
+    dim3 grid(bb.nblk + 1, (n + group - 1) / group, bb.nz);
     if (bb.ntiles > 1024) {
         k_bin_fill<<<grid, 256, 2 * bb.ntiles * sizeof(uint32_t), st>>>(
-            bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap,
-            bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
+            bb.recs, bb.bbox, bb.blk_nrec, ctr, bb.nblk, spill_base, bb.spill_cap, bb.start,
+            bb.fill, list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, s0, s1);
         return;
     }
     const size_t lds = 4 * bb.ntiles * sizeof(uint32_t);
     if (bb.ntiles <= 256)
         k_bin_fill_pipe<1><<<grid, 256, lds, st>>>(
-            bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap,
-            bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
+            bb.recs, bb.bbox, bb.blk_nrec, ctr, bb.nblk, spill_base, bb.spill_cap, bb.start,
+            bb.fill, list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, s0, s1);
     else
         k_bin_fill_pipe<4><<<grid, 256, lds, st>>>(
-            bb.recs, bb.bbox, bb.blk_nrec, counters, bb.nblk, bb.spill_base, bb.spill_cap,
-            bb.start, bb.fill, bb.list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, K);
+            bb.recs, bb.bbox, bb.blk_nrec, ctr, bb.nblk, spill_base, bb.spill_cap, bb.start,
+            bb.fill, list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, s0, s1);
 }
 
-void launch_tile(const Geometry &g, uint32_t K, const uint32_t *nshoot, const BinBuffers &bb,
-                 uint32_t small_px, uint32_t *out_ids, float *out_depth, const float *dff,
-                 uint32_t rowlen, const uint32_t *slot_pbeg, float *ff, unsigned long long *ffq,
-                 uint32_t *hits, uint32_t *counters, const uint32_t *slot_w,
-                 const uint32_t *slot_dffo, hipStream_t st)
+void launch_tile(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h, const uint32_t *nshoot,
+                 const BinBuffers &bb, uint32_t small_px, uint32_t *out_ids, float *out_depth,
+                 const float *dff, uint32_t rowlen, const uint32_t *slot_pbeg, float *ff,
+                 unsigned long long *ffq, uint32_t *hits, uint32_t *counters,
+                 const uint32_t *slot_w, const uint32_t *slot_dffo, hipStream_t st)
 {
+    const uint32_t nb = (s1 - s0) * bb.ntiles;
+    const uint32_t *order = bb.order + (size_t)s0 * bb.ntiles;
+    const uint32_t *list = bb.list + (size_t)h * bb.ent_cap;
+    uint32_t *ctr = counters + h * kCntN;
     if (ffq)
-        k_tile_raster<true><<<bb.ntiles * K, kTileThreads, 0, st>>>(
-            g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, bb.list,
+        k_tile_raster<true><<<nb, kTileThreads, 0, st>>>(
+            g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, list,
             bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
-            slot_pbeg, ff, ffq, hits, counters, bb.order, slot_w, slot_dffo);
+            slot_pbeg, ff, ffq, hits, ctr, order, slot_w, slot_dffo);
     else
-        k_tile_raster<false><<<bb.ntiles * K, kTileThreads, 0, st>>>(
-            g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, bb.list,
+        k_tile_raster<false><<<nb, kTileThreads, 0, st>>>(
+            g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, list,
             bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
-            slot_pbeg, ff, ffq, hits, counters, bb.order, slot_w, slot_dffo);
+            slot_pbeg, ff, ffq, hits, ctr, order, slot_w, slot_dffo);
 }
 
 // Deterministic mode: n fixed-point entries (units of 2^-52) into the float

@@ -431,6 +431,52 @@ def test_graph_replay_matches_eager(rad, cornell_t):
         assert abs(e.residual() - g.residual()) <= 1e-4 * e.residual()
 
 
+def test_prepare_then_step_equals_step(rad, boxes_small):
+    """rad_prepare captures the pass graph without running a pass: with F
+    rows left dirty by a standalone reduction, eager passes after it, then
+    replays, give the state of the same calls without it, bit for bit (the
+    deterministic mode fixes the F sums)."""
+    sc = boxes_small
+    outs = []
+    for prep in (False, True):
+        say(f"prepare={prep}: 2 passes, reduction, 3 eager passes, 8 replayed")
+        s = rad.Solver(hemicube=32, hemicubes=4, graph_passes=4)
+        s.set_deterministic(True)
+        s.upload(sc)
+        s.step(2)
+        s.render([0, 9, 77, 130])
+        s.process(4)                      # leaves F rows [0, 4) dirty
+        if prep:
+            s.prepare()
+        s.step(3)
+        s.step(8)
+        s.finish()
+        u, sh = s.state()
+        outs.append((u.view(np.uint32).copy(), sh.view(np.uint32).copy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("k", [15, 19])
+def test_split_pass_item_buffers_bit_exact(rad, boxes_small, k):
+    """A pass of 16 or more slots renders as two halves on two streams (here
+    9 + 10 slots); 15 slots render whole.  Every item buffer bit-exact against
+    the oracle, and no counter block overflows."""
+    W = 64
+    sc = boxes_small
+    sh = [int(x) for x in np.random.default_rng(k).choice(sc.npatch, k, replace=False)]
+    say(f"split pass: W={W} k={k}")
+    s = rad.Solver(hemicube=W, hemicubes=24, tile_px=16, small_tri_px=32)
+    s.upload(sc)
+    item = s.render(sh)
+    c = s.debug_counters()
+    assert c["overflow"] == 0 and c["sticky"] == 0 and c["unknown"] == 0
+    res = s.tex_res
+    for j, p in enumerate(sh):
+        it_o = O.render_hemicube(W, sc.tri, sc.tri_patch, sc.frame[p])
+        assert np.array_equal(item[j * res:(j + 1) * res], it_o), (j, p)
+
+
 def test_autotune_picks_a_candidate_and_keeps_parity(rad, boxes_small):
     s = rad.Solver(hemicube=64, hemicubes=4)
     s.upload(boxes_small)
