@@ -120,20 +120,20 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
     c->ff_dirty = 0;
     record(c, evbase + 2);
     const bool fq = !ids && c->det;
-    // slots [s0, s1) with counter block h on stream st
-    auto part = [&](uint32_t s0, uint32_t s1, uint32_t h, hipStream_t st, bool timed) {
-        launch_raster_setup(c->g, s0, s1, h, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp,
-                            c->nshoot, c->bb, c->counters, c->pk.slot_tbeg, c->pk.slot_tcnt,
-                            c->pk.slot_w, st);
-        if (timed)
-            record(c, evbase + 3);
-        launch_bin(s0, s1, h, c->nshoot, c->bb, c->counters, st);
-        if (timed)
-            record(c, evbase + 4);
-        launch_tile(c->g, s0, s1, h, c->nshoot, c->bb, c->small_px, ids, depth,
-                    c->pk.slot_w ? c->pk.dffs : c->dff, c->rowlen, c->pk.slot_pbeg,
-                    ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits, c->counters,
-                    c->pk.slot_w, c->pk.slot_dffo, st);
+    // stage j (0 setup, 1 bins, 2 tiles) of slots [s0, s1) with counter
+    // block h on stream st
+    auto stage = [&](int j, uint32_t s0, uint32_t s1, uint32_t h, hipStream_t st) {
+        if (j == 0)
+            launch_raster_setup(c->g, s0, s1, h, c->tri, c->tsph, c->tri_patch, c->ntri, c->mvp,
+                                c->nshoot, c->bb, c->counters, c->pk.slot_tbeg,
+                                c->pk.slot_tcnt, c->pk.slot_w, st);
+        else if (j == 1)
+            launch_bin(s0, s1, h, c->nshoot, c->bb, c->counters, st);
+        else
+            launch_tile(c->g, s0, s1, h, c->nshoot, c->bb, c->small_px, ids, depth,
+                        c->pk.slot_w ? c->pk.dffs : c->dff, c->rowlen, c->pk.slot_pbeg,
+                        ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits, c->counters,
+                        c->pk.slot_w, c->pk.slot_dffo, st);
     };
     // A pass of 16 or more slots renders as two halves side by side: the
     // halves share no bins, records, list entries or counters, so the item
@@ -142,12 +142,18 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
     if (k >= kSplitMin && !c->timing) {
         HIPCHK(c, hipEventRecord(c->ev_fork, c->st));
         HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
-        part(0, k / 2, 0, c->st, false);
-        part(k / 2, k, 1, c->st2, false);
+        for (int j = 0; j < 3; ++j) {
+            stage(j, 0, k / 2, 0, c->st);
+            stage(j, k / 2, k, 1, c->st2);
+        }
         HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
         HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
     } else {
-        part(0, k, 0, c->st, true);
+        for (int j = 0; j < 3; ++j) {
+            stage(j, 0, k, 0, c->st);
+            if (j < 2)
+                record(c, evbase + 3 + j);
+        }
     }
     if (fq)
         launch_fold(c->ffq, c->ff, (size_t)k * c->rowlen, c->st);
