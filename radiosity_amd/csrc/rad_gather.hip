@@ -31,6 +31,22 @@ __device__ __forceinline__ uint16_t bf16_rne(float f)
 
 __device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Element (r, c) of a dense npad x npad half (fhi or flo).  The halves are
+// stored in 4 KB tiles of 32 rows x 64 columns, tile (r / 32, c / 64) at
+// element ((r / 32) * (npad / 64) + c / 64) * 2048, so a GEMM wavefront's
+// 32 rows of one 64-column chunk are one contiguous 4 KB block and its row
+// tile one contiguous stream.  Inside a tile the 16-byte units of 8 columns
+// are in the order of the MFMA A fragments: unit (f * 2 + ks) * 64 + lane
+// for rows 16 f .. 16 f + 15 and columns 32 ks .. 32 ks + 31 of the tile,
+// lane = (column / 8 % 4) * 16 + row % 16.  Eight consecutive columns from a
+// multiple of 8 are one unit.
+__device__ __forceinline__ size_t dense_off(uint32_t r, uint32_t c, uint32_t npad)
+{
+    const uint32_t rr = r & 31u, cc = c & 63u;
+    const uint32_t unit = ((rr >> 4) * 2u + (cc >> 5)) * 64u + ((cc >> 3) & 3u) * 16u + (rr & 15u);
+    return ((size_t)(r >> 5) * (npad >> 6) + (c >> 6)) * 2048u + unit * 8u + (cc & 7u);
+}
+
 // ---------------------------------------------------------------- F build
 
 // Slots [0,k) of a pass render the hemicubes of patches row0 .. row0+k-1.
@@ -53,7 +69,7 @@ __global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t
     if (s >= k)
         return;
     const float *src = ff + (size_t)s * npatch;
-    size_t row = (size_t)(row0 + s) * npad;
+    const uint32_t row = row0 + s;
     for (uint32_t j0 = (blockIdx.x * blockDim.x + threadIdx.x) * 8; j0 < npad;
          j0 += gridDim.x * blockDim.x * 8) {
         uint16_t h[8], l[8];
@@ -73,9 +89,10 @@ __global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t
         vl.y = l[2] | ((uint32_t)l[3] << 16);
         vl.z = l[4] | ((uint32_t)l[5] << 16);
         vl.w = l[6] | ((uint32_t)l[7] << 16);
-        *(uint4 *)(fhi + row + j0) = vh;
+        const size_t o = dense_off(row, j0, npad);
+        *(uint4 *)(fhi + o) = vh;
         if (flo)
-            *(uint4 *)(flo + row + j0) = vl;
+            *(uint4 *)(flo + o) = vl;
     }
 }
 
@@ -115,7 +132,7 @@ __global__ __launch_bounds__(256) void k_dense_store_t(const float *__restrict__
             const uint32_t jj = j0 + r;
             if (jj >= npatch || s8 >= nc)
                 continue;
-            const size_t o = (size_t)jj * npad + row0 + c0 + s8;
+            const uint32_t col = row0 + c0 + s8;
             uint32_t w[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e)
@@ -130,12 +147,14 @@ __global__ __launch_bounds__(256) void k_dense_store_t(const float *__restrict__
                 vl.y = (w[2] >> 16) | (w[3] & 0xFFFF0000u);
                 vl.z = (w[4] >> 16) | (w[5] & 0xFFFF0000u);
                 vl.w = (w[6] >> 16) | (w[7] & 0xFFFF0000u);
+                const size_t o = dense_off(jj, col, npad);
                 *(uint4 *)(fhi + o) = vh;
                 *(uint4 *)(flo + o) = vl;
             } else {
                 for (uint32_t e = 0; e < 8 && s8 + e < nc; ++e) {
-                    fhi[o + e] = (uint16_t)(w[e] & 0xFFFFu);
-                    flo[o + e] = (uint16_t)(w[e] >> 16);
+                    const size_t o = dense_off(jj, col + e, npad);
+                    fhi[o] = (uint16_t)(w[e] & 0xFFFFu);
+                    flo[o] = (uint16_t)(w[e] >> 16);
                 }
             }
         }
@@ -154,7 +173,7 @@ __global__ void k_dense_from_f32(const float *__restrict__ src, uint32_t rows, u
         size_t i = e / npatch, j = e - i * npatch;
         float f = src[e];
         uint16_t h = bf16_rne(f);
-        size_t o = (row0 + i) * (size_t)npad + j;
+        const size_t o = dense_off((uint32_t)(row0 + i), (uint32_t)j, npad);
         fhi[o] = h;
         if (flo)
             flo[o] = bf16_rne(f - bf16_val(h));
@@ -169,7 +188,7 @@ __global__ void k_dense_to_f32(const uint16_t *__restrict__ fhi, const uint16_t 
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x) {
         size_t i = e / npatch, j = e - i * npatch;
-        size_t o = (row0 + i) * (size_t)npad + j;
+        const size_t o = dense_off((uint32_t)(row0 + i), (uint32_t)j, npad);
         float f = bf16_val(fhi[o]);
         if (flo)
             f += bf16_val(flo[o]);
@@ -230,6 +249,10 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     // keeps that slice of X in its own L2
     const uint32_t slice = blockIdx.x % nslice, rtile = blockIdx.x / nslice;
     const uint32_t i0 = rtile * (WAVES * 32) + wave * 32;
+    // the last row tile may be partial (npad is a multiple of 512, a tile
+    // of 12 wavefronts holds 384 rows): its waves beyond npad load row 0,
+    // stage their share of X and store nothing
+    const bool live = i0 < npad;
     const uint32_t kbeg = slice * kspan, kend = kbeg + kspan;
 
     f32x4 acc[2][RT];
@@ -239,8 +262,9 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
         for (int t = 0; t < RT; ++t)
             acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const size_t arow0 = (size_t)(i0 + m) * npad + 8 * kg;
-    const size_t arow1 = arow0 + (size_t)16 * npad;
+    // the wave's row tile: chunk kc is the 4 KB block kc / 64 of it, the
+    // lane's 16-byte unit of fragment (f, ks) at (f * 2 + ks) * 512 + 8 lane
+    const size_t abase = (size_t)((live ? i0 : 0u) >> 5) * (npad >> 6) * 2048u + (size_t)lane * 8u;
 
     static_assert(UPT <= 3, "three staging units per thread at most");
     // staging registers of the X chunk units tid, tid + NT, tid + 2 NT (hi,
@@ -274,11 +298,12 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
 #define RAD_GEMM_LOAD_A(dst_, kc_)                                                     \
     _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                                   \
     {                                                                                  \
-        dst_[0][0][ks] = gemm_aload<RT>(fhi + arow0 + (kc_) + ks * 32);                \
-        dst_[0][1][ks] = gemm_aload<RT>(fhi + arow1 + (kc_) + ks * 32);                \
+        const size_t a0_ = abase + (size_t)((kc_) >> 6) * 2048u + ks * 512;            \
+        dst_[0][0][ks] = gemm_aload<RT>(fhi + a0_);                                    \
+        dst_[0][1][ks] = gemm_aload<RT>(fhi + a0_ + 1024);                             \
         if (SPLIT) {                                                                   \
-            dst_[NB - 1][0][ks] = gemm_aload<RT>(flo + arow0 + (kc_) + ks * 32);       \
-            dst_[NB - 1][1][ks] = gemm_aload<RT>(flo + arow1 + (kc_) + ks * 32);       \
+            dst_[NB - 1][0][ks] = gemm_aload<RT>(flo + a0_);                           \
+            dst_[NB - 1][1][ks] = gemm_aload<RT>(flo + a0_ + 1024);                    \
         }                                                                              \
     }
     // X of the chunk to LDS buffer cb_ (after the barrier that frees it)
@@ -369,6 +394,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     }
     // lane holds rows 4*kg .. 4*kg+3 of column m of each 16x16 tile
     float *out = yp + (size_t)slice * (RT * 16) * npad;
+    if (!live)
+        return;
 #pragma unroll
     for (int f = 0; f < 2; ++f)
 #pragma unroll
@@ -686,21 +713,29 @@ uint32_t gemm_tiles(uint32_t R)
 #ifndef RAD_GEMM_PF
 #define RAD_GEMM_PF 1
 #endif
+#ifndef RAD_GEMM_WVS
+#define RAD_GEMM_WVS 12
+#endif
 template <int RT>
 static void gemm_rt(bool split, const uint16_t *fhi, const uint16_t *flo, const uint16_t *xhi,
                     const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp, hipStream_t st)
 {
     constexpr int WV = RT >= 4 ? 8 : 4;
+    // the widest single-half sweep: 12 wavefronts (3 per SIMD at 162
+    // VGPRs), so each CU has 12 chunks of A in flight instead of 8
+    constexpr int WVS = RT >= 12 ? RAD_GEMM_WVS : WV;
     // two chunks of A in flight for the single-half sweep; the split product
     // already holds both halves of A and X (PF = 2 would cost occupancy)
     constexpr int PFS = RT >= 4 ? RAD_GEMM_PF : 1;
-    const uint32_t nwg = (npad / (WV * 32)) * S;
     uint32_t kspan = npad / S;
-    if (split)
+    if (split) {
+        const uint32_t nwg = (npad / (WV * 32)) * S;
         k_gemm<RT, true, WV, 1><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
-    else
-        k_gemm<RT, false, WV, PFS><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S,
-                                                            yp);
+    } else {
+        const uint32_t nwg = ((npad + WVS * 32 - 1) / (WVS * 32)) * S;
+        k_gemm<RT, false, WVS, PFS><<<nwg, WVS * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan,
+                                                              S, yp);
+    }
 }
 
 void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
