@@ -1394,6 +1394,7 @@ int need_dense(rad_ctx *c)
 // Work buffers of one gather call (R columns, rp = 16 * tiles).
 struct GatherBufs {
     uint32_t rt = 1, rp = 16, S = 1, nblk = 1024;
+    GemmSplit ks1{}, ks2{}; // k slices of the single-half and two-half products
     float *yp = nullptr, *E = nullptr, *B = nullptr, *rv = nullptr, *out = nullptr;
     uint16_t *xhi = nullptr, *xlo = nullptr;
     double *parts = nullptr;
@@ -1410,9 +1411,14 @@ int gather_alloc(rad_ctx *c, GatherBufs &g, uint32_t R)
     g.rt = gemm_tiles(R);
     g.rp = g.rt * 16;
     g.S = gather_split(c->npad, R);
+    int ncu = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    g.ks1 = gemm_split(g.rt, false, c->npad, g.S, ncu > 0 ? (uint32_t)ncu : 1u);
+    g.ks2 = gemm_split(g.rt, true, c->npad, g.S, ncu > 0 ? (uint32_t)ncu : 1u);
+    const uint32_t planes = std::max(g.S, std::max(g.ks1.S2, g.ks2.S2));
     size_t plane = (size_t)g.rp * c->npad;
     int rc;
-    if ((rc = dalloc(c, &g.yp, plane * g.S)) || (rc = dalloc(c, &g.E, plane)) ||
+    if ((rc = dalloc(c, &g.yp, plane * planes)) || (rc = dalloc(c, &g.E, plane)) ||
         (rc = dalloc(c, &g.B, plane)) || (rc = dalloc(c, &g.rv, plane)) ||
         (rc = dalloc(c, &g.out, (size_t)R * c->npatch)) || (rc = dalloc(c, &g.xhi, plane)) ||
         (rc = dalloc(c, &g.xlo, plane)) || (rc = dalloc(c, &g.parts, g.nblk)))
@@ -1549,8 +1555,9 @@ int rad_dense_apply(rad_ctx *c, const float *x, uint32_t R, int split, float *y)
     size_t n = (size_t)R * c->npatch;
     HIPCHK(c, hipMemcpyAsync(g.out, x, n * 4, hipMemcpyHostToDevice, c->st));
     launch_pad_columns(R, c->npad, c->npatch, g.out, nullptr, g.xhi, split ? g.xlo : nullptr, c->st);
-    launch_gemm(g.rt, split != 0, c->fhi, c->flo, g.xhi, g.xlo, c->npad, g.S, g.yp, c->st);
-    launch_sum_partials(R, c->npad, c->npatch, g.S, g.rp, g.yp, g.out, c->st);
+    const GemmSplit ks = split ? g.ks2 : g.ks1;
+    launch_gemm(g.rt, split != 0, c->fhi, c->flo, g.xhi, g.xlo, c->npad, ks, g.yp, c->st);
+    launch_sum_partials(R, c->npad, c->npatch, ks, g.rp, g.yp, g.out, c->st);
     if ((rc = check_launch(c, "dense_apply")))
         return rc;
     HIPCHK(c, hipMemcpyAsync(y, g.out, n * 4, hipMemcpyDeviceToHost, c->st));
@@ -1590,7 +1597,7 @@ int rad_gather_solve(rad_ctx *c, const float *rhs, uint32_t R, const rad_gather_
     }
     launch_pad_columns(R, P, N, g.out, g.E, nullptr, nullptr, c->st);
     // B = 0, r = E; denominator sum_i A_i sum_r |E_ir|
-    launch_gather_update(0, g.yp, g.S, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
+    launch_gather_update(0, g.yp, g.ks1, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
                          g.xlo, g.parts, g.nblk, c->st);
     if ((rc = check_launch(c, "gather init")))
         return rc;
@@ -1613,20 +1620,20 @@ int rad_gather_solve(rad_ctx *c, const float *rhs, uint32_t R, const rad_gather_
         for (uint32_t s = 0; s < inner; ++s) {
             if (s == 0 && have_ev)
                 (void)hipEventRecord(ev[0], c->st);
-            launch_gemm(g.rt, false, c->fhi, nullptr, g.xhi, nullptr, P, g.S, g.yp, c->st);
+            launch_gemm(g.rt, false, c->fhi, nullptr, g.xhi, nullptr, P, g.ks1, g.yp, c->st);
             if (s == 0 && have_ev)
                 (void)hipEventRecord(ev[1], c->st);
-            launch_gather_update(s + 1 == inner ? 2 : 1, g.yp, g.S, g.rp, R, P, N, g.E, c->refl,
+            launch_gather_update(s + 1 == inner ? 2 : 1, g.yp, g.ks1, g.rp, R, P, N, g.E, c->refl,
                                  c->area, g.rv, g.B, g.xhi, g.xlo, g.parts, g.nblk, c->st);
         }
         sweeps += inner;
         // residual r = E + diag(rho) M B - B from both halves, FP64 sums
         if (have_ev)
             (void)hipEventRecord(ev[2], c->st);
-        launch_gemm(g.rt, true, c->fhi, c->flo, g.xhi, g.xlo, P, g.S, g.yp, c->st);
+        launch_gemm(g.rt, true, c->fhi, c->flo, g.xhi, g.xlo, P, g.ks2, g.yp, c->st);
         if (have_ev)
             (void)hipEventRecord(ev[3], c->st);
-        launch_gather_update(3, g.yp, g.S, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
+        launch_gather_update(3, g.yp, g.ks2, g.rp, R, P, N, g.E, c->refl, c->area, g.rv, g.B, g.xhi,
                              g.xlo, g.parts, g.nblk, c->st);
         if ((rc = check_launch(c, "gather step")) || (rc = sum_parts(c, g, &num)))
             break;

@@ -225,8 +225,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
                                               const uint16_t *__restrict__ flo,
                                               const uint16_t *__restrict__ xhi,
                                               const uint16_t *__restrict__ xlo, uint32_t npad,
-                                              uint32_t kspan, uint32_t nslice,
-                                              float *__restrict__ yp)
+                                              GemmSplit ks, float *__restrict__ yp)
 {
     static_assert(PF == 1 || PF == 2, "one or two chunks of A in flight");
     constexpr int NB = SPLIT ? 2 : 1;
@@ -246,14 +245,20 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     const int m = lane & 15, kg = lane >> 4;
     // 1-D grid, k slice fastest: consecutive workgroups go to different XCDs
     // (round robin), so with S dividing 8 every XCD works on one k slice and
-    // keeps that slice of X in its own L2
-    const uint32_t slice = blockIdx.x % nslice, rtile = blockIdx.x / nslice;
+    // keeps that slice of X in its own L2.  The row tiles from ks.rsplit on
+    // come last, in ks.S2 slices.
+    const uint32_t head = ks.rsplit / (WAVES * 32) * ks.S;
+    const bool tail = blockIdx.x >= head;
+    const uint32_t ns = tail ? ks.S2 : ks.S, b = tail ? blockIdx.x - head : blockIdx.x;
+    const uint32_t slice = b % ns;
+    const uint32_t rtile = (tail ? ks.rsplit / (WAVES * 32) : 0u) + b / ns;
     const uint32_t i0 = rtile * (WAVES * 32) + wave * 32;
     // the last row tile may be partial (npad is a multiple of 512, a tile
-    // of 12 wavefronts holds 384 rows): its waves beyond npad load row 0,
-    // stage their share of X and store nothing
+    // of 12 wavefronts holds 384 rows): its waves beyond npad only stage
+    // their share of X
     const bool live = i0 < npad;
-    const uint32_t kbeg = slice * kspan, kend = kbeg + kspan;
+    const uint32_t nch = npad >> 6;
+    const uint32_t kbeg = slice * nch / ns * 64u, kend = (slice + 1u) * nch / ns * 64u;
 
     f32x4 acc[2][RT];
 #pragma unroll
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
 
     // the wave's row tile: chunk kc is the 4 KB block kc / 64 of it, the
     // lane's 16-byte unit of fragment (f, ks) at (f * 2 + ks) * 512 + 8 lane
-    const size_t abase = (size_t)((live ? i0 : 0u) >> 5) * (npad >> 6) * 2048u + (size_t)lane * 8u;
+    const size_t abase = (size_t)(i0 >> 5) * (npad >> 6) * 2048u + (size_t)lane * 8u;
 
     static_assert(UPT <= 3, "three staging units per thread at most");
     // staging registers of the X chunk units tid, tid + NT, tid + 2 NT (hi,
@@ -296,6 +301,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
         }                                                                              \
     }
 #define RAD_GEMM_LOAD_A(dst_, kc_)                                                     \
+    if (live)                                                                          \
     _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                                   \
     {                                                                                  \
         const size_t a0_ = abase + (size_t)((kc_) >> 6) * 2048u + ks * 512;            \
@@ -331,6 +337,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
             _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                           \
                 ac[b][f][ks] = src_[b][f][ks];
 #define RAD_GEMM_MFMA(cb_)                                                             \
+    if (live)                                                                          \
     _Pragma("unroll") for (int ks = 0; ks < 2; ++ks)                                   \
     {                                                                                  \
         _Pragma("unroll") for (int t = 0; t < RT; ++t)                                 \
@@ -433,7 +440,7 @@ __device__ __forceinline__ void block_sum_store(double v, double *parts)
 enum { kInit = 0, kInner = 1, kLast = 2, kResid = 3 };
 
 __global__ __launch_bounds__(256) void k_gather_update(
-    int kind, const float *__restrict__ yp, uint32_t S, uint32_t rp, uint32_t R, uint32_t npad,
+    int kind, const float *__restrict__ yp, GemmSplit ks, uint32_t rp, uint32_t R, uint32_t npad,
     uint32_t n, const float *__restrict__ E, const float *__restrict__ refl,
     const float *__restrict__ area, float *__restrict__ rv, float *__restrict__ B,
     uint16_t *__restrict__ xhi, uint16_t *__restrict__ xlo, double *__restrict__ parts)
@@ -455,6 +462,7 @@ __global__ __launch_bounds__(256) void k_gather_update(
             continue;
         }
         float y = 0.0f;
+        const uint32_t S = i < ks.rsplit ? ks.S : ks.S2;
         for (uint32_t s = 0; s < S; ++s)
             y += yp[s * plane + e];
         float rho = refl[(size_t)i * 3 + r % 3];
@@ -516,7 +524,7 @@ __global__ void k_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float
 }
 
 // Sum of the S partial products into host column order (R x n).
-__global__ void k_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
+__global__ void k_sum_partials(uint32_t R, uint32_t npad, uint32_t n, GemmSplit ks, uint32_t rp,
                                const float *__restrict__ yp, float *__restrict__ out)
 {
     size_t total = (size_t)R * n;
@@ -525,6 +533,7 @@ __global__ void k_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S
          e += (size_t)gridDim.x * blockDim.x) {
         size_t r = e / n, i = e - r * n;
         float y = 0.0f;
+        const uint32_t S = i < ks.rsplit ? ks.S : ks.S2;
         for (uint32_t s = 0; s < S; ++s)
             y += yp[s * plane + r * npad + i];
         out[e] = y;
@@ -716,47 +725,69 @@ uint32_t gemm_tiles(uint32_t R)
 #ifndef RAD_GEMM_WVS
 #define RAD_GEMM_WVS 12
 #endif
+// Rows of one workgroup of a product.
+static uint32_t gemm_wg_rows(uint32_t rt, bool split)
+{
+    const uint32_t wv = rt >= 4 ? 8u : 4u;
+    return (rt >= 12 && !split ? (uint32_t)RAD_GEMM_WVS : wv) * 32u;
+}
+
+GemmSplit gemm_split(uint32_t rt, bool split, uint32_t npad, uint32_t S, uint32_t ncu)
+{
+    const uint32_t rows = gemm_wg_rows(rt, split);
+    const uint32_t tiles = (npad + rows - 1) / rows, nch = npad >> 6;
+    // row tiles whose tiles x S workgroups make whole rounds over the CUs
+    // keep S slices; the rest, fewer than ncu / S tiles, get S2 slices
+    const uint32_t full = tiles * S / ncu * ncu / S;
+    if (full >= tiles || ncu % S != 0)
+        return GemmSplit{S, S, tiles * rows};
+    uint32_t S2 = ncu / (tiles - full);
+    if (S2 > nch)
+        S2 = nch;
+    if (S2 < S)
+        S2 = S;
+    return GemmSplit{S, S2, full * rows};
+}
+
 template <int RT>
 static void gemm_rt(bool split, const uint16_t *fhi, const uint16_t *flo, const uint16_t *xhi,
-                    const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp, hipStream_t st)
+                    const uint16_t *xlo, uint32_t npad, GemmSplit ks, float *yp, hipStream_t st)
 {
     constexpr int WV = RT >= 4 ? 8 : 4;
-    // the widest single-half sweep: 12 wavefronts (3 per SIMD at 162
+    // the widest single-half sweep: 12 wavefronts (3 per SIMD at 154
     // VGPRs), so each CU has 12 chunks of A in flight instead of 8
     constexpr int WVS = RT >= 12 ? RAD_GEMM_WVS : WV;
     // two chunks of A in flight for the single-half sweep; the split product
     // already holds both halves of A and X (PF = 2 would cost occupancy)
     constexpr int PFS = RT >= 4 ? RAD_GEMM_PF : 1;
-    uint32_t kspan = npad / S;
-    if (split) {
-        const uint32_t nwg = (npad / (WV * 32)) * S;
-        k_gemm<RT, true, WV, 1><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan, S, yp);
-    } else {
-        const uint32_t nwg = ((npad + WVS * 32 - 1) / (WVS * 32)) * S;
-        k_gemm<RT, false, WVS, PFS><<<nwg, WVS * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, kspan,
-                                                              S, yp);
-    }
+    const uint32_t rows = gemm_wg_rows(RT, split);
+    const uint32_t tiles = (npad + rows - 1) / rows, head = ks.rsplit / rows;
+    const uint32_t nwg = head * ks.S + (tiles - head) * ks.S2;
+    if (split)
+        k_gemm<RT, true, WV, 1><<<nwg, WV * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, ks, yp);
+    else
+        k_gemm<RT, false, WVS, PFS><<<nwg, WVS * 64, 0, st>>>(fhi, flo, xhi, xlo, npad, ks, yp);
 }
 
 void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
-                 const uint16_t *xhi, const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp,
-                 hipStream_t st)
+                 const uint16_t *xhi, const uint16_t *xlo, uint32_t npad, GemmSplit ks,
+                 float *yp, hipStream_t st)
 {
     switch (rt) {
-    case 1: gemm_rt<1>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
-    case 2: gemm_rt<2>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
-    case 4: gemm_rt<4>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
-    case 8: gemm_rt<8>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
-    default: gemm_rt<12>(split, fhi, flo, xhi, xlo, npad, S, yp, st); break;
+    case 1: gemm_rt<1>(split, fhi, flo, xhi, xlo, npad, ks, yp, st); break;
+    case 2: gemm_rt<2>(split, fhi, flo, xhi, xlo, npad, ks, yp, st); break;
+    case 4: gemm_rt<4>(split, fhi, flo, xhi, xlo, npad, ks, yp, st); break;
+    case 8: gemm_rt<8>(split, fhi, flo, xhi, xlo, npad, ks, yp, st); break;
+    default: gemm_rt<12>(split, fhi, flo, xhi, xlo, npad, ks, yp, st); break;
     }
 }
 
-void launch_gather_update(int kind, const float *yp, uint32_t S, uint32_t rp, uint32_t R,
+void launch_gather_update(int kind, const float *yp, GemmSplit ks, uint32_t rp, uint32_t R,
                           uint32_t npad, uint32_t n, const float *E, const float *refl,
                           const float *area, float *rv, float *B, uint16_t *xhi, uint16_t *xlo,
                           double *parts, uint32_t nblocks, hipStream_t st)
 {
-    k_gather_update<<<nblocks, 256, 0, st>>>(kind, yp, S, rp, R, npad, n, E, refl, area, rv, B,
+    k_gather_update<<<nblocks, 256, 0, st>>>(kind, yp, ks, rp, R, npad, n, E, refl, area, rv, B,
                                              xhi, xlo, parts);
 }
 
@@ -772,10 +803,10 @@ void launch_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *src,
     k_pad_columns<<<1024, 256, 0, st>>>(R, npad, n, src, dst, xhi, xlo);
 }
 
-void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
+void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, GemmSplit ks, uint32_t rp,
                          const float *yp, float *out, hipStream_t st)
 {
-    k_sum_partials<<<1024, 256, 0, st>>>(R, npad, n, S, rp, yp, out);
+    k_sum_partials<<<1024, 256, 0, st>>>(R, npad, n, ks, rp, yp, out);
 }
 
 } // namespace rad

@@ -208,10 +208,19 @@ void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint3
 void launch_dense_to_f32(const uint16_t *fhi, const uint16_t *flo, uint32_t rows, uint32_t row0,
                          uint32_t npatch, uint32_t npad, float *dst, hipStream_t st);
 uint32_t gemm_tiles(uint32_t R);
+// k slices of a gather product: rows below rsplit are summed over S slices
+// of npad / S columns, the rows from rsplit on (the row tiles of the last,
+// partial round of workgroups) over S2 slices of about npad / S2 columns,
+// so that round also fills the CUs.  The partial products hold max(S, S2)
+// planes.
+struct GemmSplit {
+    uint32_t S, S2, rsplit;
+};
+GemmSplit gemm_split(uint32_t rt, bool split, uint32_t npad, uint32_t S, uint32_t ncu);
 void launch_gemm(uint32_t rt, bool split, const uint16_t *fhi, const uint16_t *flo,
-                 const uint16_t *xhi, const uint16_t *xlo, uint32_t npad, uint32_t S, float *yp,
-                 hipStream_t st);
-void launch_gather_update(int kind, const float *yp, uint32_t S, uint32_t rp, uint32_t R,
+                 const uint16_t *xhi, const uint16_t *xlo, uint32_t npad, GemmSplit ks,
+                 float *yp, hipStream_t st);
+void launch_gather_update(int kind, const float *yp, GemmSplit ks, uint32_t rp, uint32_t R,
                           uint32_t npad, uint32_t n, const float *E, const float *refl,
                           const float *area, float *rv, float *B, uint16_t *xhi, uint16_t *xlo,
                           double *parts, uint32_t nblocks, hipStream_t st);
@@ -229,7 +238,7 @@ void launch_gather_out(uint32_t R, uint32_t npad, uint32_t n, const float *B, co
                        float *out, hipStream_t st);
 void launch_pad_columns(uint32_t R, uint32_t npad, uint32_t n, const float *src, float *dst,
                         uint16_t *xhi, uint16_t *xlo, hipStream_t st);
-void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, uint32_t S, uint32_t rp,
+void launch_sum_partials(uint32_t R, uint32_t npad, uint32_t n, GemmSplit ks, uint32_t rp,
                          const float *yp, float *out, hipStream_t st);
 
 // Launchers implemented in rad_ingest.hip.
