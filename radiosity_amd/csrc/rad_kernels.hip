@@ -2656,29 +2656,84 @@ __global__ __launch_bounds__(256) void k_topk_pass(const float *__restrict__ uns
 // is skipped), each workgroup merges its wave lists in LDS, then one
 // workgroup merges the workgroup lists.  The keys are distinct (they hold
 // the patch index), so the result does not depend on the merge order.
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
+// Lane exchanges of the top-K networks without ds_bpermute: lane i gets the
+// value of lane i ^ M, from DPP lane moves (strides 1, 2, 4, 8 inside a row
+// of 16 lanes) and the gfx950 row and half-wave swaps (16, 32).
+template <int M> __device__ __forceinline__ uint32_t lane_xor(uint32_t x)
 {
-    uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    const int lane = threadIdx.x & 63;
+    const int xi = (int)x;
+    if constexpr (M == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0xB1, 0xF, 0xF, false); // quad_perm 1,0,3,2
+    } else if constexpr (M == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0x4E, 0xF, 0xF, false); // quad_perm 2,3,0,1
+    } else if constexpr (M == 4) {
+        const int up = __builtin_amdgcn_update_dpp(xi, xi, 0x104, 0xF, 0xF, false); // row_shl:4, lane i + 4
+        const int dn = __builtin_amdgcn_update_dpp(xi, xi, 0x114, 0xF, 0xF, false); // row_shr:4, lane i - 4
+        return (uint32_t)((lane & 4) ? dn : up);
+    } else if constexpr (M == 8) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0x128, 0xF, 0xF, false); // row_ror:8
+    } else if constexpr (M == 16) {
+        // odd rows of the first operand swap with even rows of the second:
+        // r[0] holds rows (0, 0, 2, 2) of x, r[1] rows (1, 1, 3, 3)
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (lane & 16) ? r[0] : r[1];
+    } else {
+        static_assert(M == 32, "lane_xor strides 1 .. 32");
+        // the upper half of the first operand swaps with the lower half of
+        // the second: r[0] holds x's lower half twice, r[1] its upper half
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (lane & 32) ? r[0] : r[1];
+    }
+}
+
+template <int M> __device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v)
+{
+    return ((uint64_t)lane_xor<M>((uint32_t)(v >> 32)) << 32) | lane_xor<M>((uint32_t)v);
+}
+
+// The value of lane 63 - i: each row mirrored (row_mirror), then rows
+// r and 3 - r exchanged.
+__device__ __forceinline__ uint64_t lane_reverse_u64(uint64_t v)
+{
+    const int lo = __builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), 0x140, 0xF, 0xF, false);
+    const uint64_t m = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    return lane_xor_u64<32>(lane_xor_u64<16>(m));
+}
+
+// A wave-uniform lane's value (lane src, a constant).
+__device__ __forceinline__ uint64_t lane_read_u64(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
     return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
+// One compare-exchange step of a bitonic network at lane stride M.
+template <int M> __device__ __forceinline__ uint64_t bitonic_step(uint64_t v, bool take_max)
 {
-    uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
-    return ((uint64_t)hi << 32) | lo;
+    const uint64_t o = lane_xor_u64<M>(v);
+    return take_max ? (v > o ? v : o) : (v < o ? v : o);
 }
 
 __device__ __forceinline__ uint64_t wave_sort_desc(uint64_t v)
 {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int size = 2; size <= 64; size <<= 1)
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            uint64_t o = shfl_xor_u64(v, stride);
-            bool take_max = ((lane & stride) == 0) == ((lane & size) == 0);
-            v = take_max ? (v > o ? v : o) : (v < o ? v : o);
-        }
+    // stage `size`: lanes in blocks of size sort descending where
+    // lane & size == 0; a step at stride m keeps the max in the lower lane
+    // of a pair when that block is descending
+#define RAD_SORT_STEP(size_, m_) \
+    v = bitonic_step<m_>(v, ((lane & (m_)) == 0) == ((lane & (size_)) == 0));
+    RAD_SORT_STEP(2, 1)
+    RAD_SORT_STEP(4, 2) RAD_SORT_STEP(4, 1)
+    RAD_SORT_STEP(8, 4) RAD_SORT_STEP(8, 2) RAD_SORT_STEP(8, 1)
+    RAD_SORT_STEP(16, 8) RAD_SORT_STEP(16, 4) RAD_SORT_STEP(16, 2) RAD_SORT_STEP(16, 1)
+    RAD_SORT_STEP(32, 16) RAD_SORT_STEP(32, 8) RAD_SORT_STEP(32, 4) RAD_SORT_STEP(32, 2)
+    RAD_SORT_STEP(32, 1)
+    RAD_SORT_STEP(64, 32) RAD_SORT_STEP(64, 16) RAD_SORT_STEP(64, 8) RAD_SORT_STEP(64, 4)
+    RAD_SORT_STEP(64, 2) RAD_SORT_STEP(64, 1)
+#undef RAD_SORT_STEP
     return v;
 }
 
@@ -2686,13 +2741,14 @@ __device__ __forceinline__ uint64_t wave_sort_desc(uint64_t v)
 __device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
 {
     const int lane = threadIdx.x & 63;
-    uint64_t r = shfl_u64(b, 63 - lane);
+    const uint64_t r = lane_reverse_u64(b);
     uint64_t v = top > r ? top : r;
-#pragma unroll
-    for (int stride = 32; stride > 0; stride >>= 1) {
-        uint64_t o = shfl_xor_u64(v, stride);
-        v = (lane & stride) == 0 ? (v > o ? v : o) : (v < o ? v : o);
-    }
+    v = bitonic_step<32>(v, (lane & 32) == 0);
+    v = bitonic_step<16>(v, (lane & 16) == 0);
+    v = bitonic_step<8>(v, (lane & 8) == 0);
+    v = bitonic_step<4>(v, (lane & 4) == 0);
+    v = bitonic_step<2>(v, (lane & 2) == 0);
+    v = bitonic_step<1>(v, (lane & 1) == 0);
     return v;
 }
 
@@ -2713,7 +2769,7 @@ __global__ __launch_bounds__(kTopkScanWaves * 64) void k_topk64_scan(const float
         if (__ballot(key > thr) == 0)
             continue;
         top = wave_merge_desc(top, wave_sort_desc(key));
-        thr = shfl_u64(top, 63);
+        thr = lane_read_u64(top, 63);
     }
     // the workgroup's wave lists: pairwise merge levels in LDS
     const int w = threadIdx.x >> 6;
@@ -2747,10 +2803,10 @@ __global__ __launch_bounds__(1024) void k_topk64_merge(const uint64_t *__restric
     // cannot contribute
     for (uint32_t l = l0 + w; l < l1; l += nw) {
         const uint64_t v = in[(size_t)l * 64 + lane];
-        if (shfl_u64(v, 0) <= thr)
+        if (lane_read_u64(v, 0) <= thr)
             continue;
         top = wave_merge_desc(top, v);
-        thr = shfl_u64(top, 63);
+        thr = lane_read_u64(top, 63);
     }
     wl[w][lane] = top;
     // the 16 wave lists: pairwise merge levels in LDS
