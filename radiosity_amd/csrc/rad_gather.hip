@@ -362,12 +362,14 @@ __global__ __launch_bounds__(WAVES * 64) void k_gemm(const uint16_t *__restrict_
     if constexpr (PF == 1) {
         for (uint32_t kc = kbeg, ci = 0; kc < kend; kc += 64, ++ci) {
             const int cb = DB == 2 ? (int)(ci & 1u) : 0;
-            RAD_GEMM_STAGE_X(cb)
+            // the next chunk's A loads go out before the barrier, so they
+            // are in flight while the wave waits for the others
             RAD_GEMM_TAKE(ar)
-            if (kc + 64 < kend) {
-                RAD_GEMM_LOAD_X(kc + 64)
+            if (kc + 64 < kend)
                 RAD_GEMM_LOAD_A(ar, kc + 64)
-            }
+            RAD_GEMM_STAGE_X(cb)
+            if (kc + 64 < kend)
+                RAD_GEMM_LOAD_X(kc + 64)
             RAD_GEMM_MFMA(cb)
         }
     } else {
