@@ -1,9 +1,10 @@
 // rad_api.cpp - C ABI of libradhip: context, device memory, pass driver.
 //
 // The pass driver restates the OnIdle shoot loop (Main.cpp:1137-1309) as a
-// sequence of device kernels on one stream with no host synchronisation
-// between passes: select -> prepare -> clear -> raster setup -> raster tiles
-// -> reduce -> shoot.
+// sequence of device kernels on the context stream with no host
+// synchronisation between passes: select -> prepare -> clear -> raster setup
+// -> raster tiles -> reduce -> shoot; the raster stages of a pass of 16 or
+// more slots run as two halves on the context's two streams.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -210,11 +210,12 @@ template <int RT> __device__ __forceinline__ uint4 gemm_aload(const uint16_t *p)
     }
 }
 
-// YP[s] = F[:, kspan slice s] X[slice s]  for 32 rows per wave.
+// YP[s] = F[:, k slice s] X[k slice s]  for 32 rows per wave.
 // Wave w owns rows i0 .. i0+31 (two 16-row A fragments) and all RT column
 // tiles of 16.  Per 64-wide k chunk the X slice (RT*16 rows x 64 bf16) is
 // staged in LDS and shared by the waves; the A fragments go straight
-// from HBM to registers (each F element is used by one wave only).  PF
+// from HBM to registers (each F element is used by one wave only), one
+// contiguous 4 KB tile of the matrix per wave and chunk (dense_off).  PF
 // chunks of A fragments are in flight ahead of the one being multiplied
 // (PF = 2: two register sets, the loop unrolled by two chunks), so a load
 // has PF chunks of MFMA work to arrive in.
