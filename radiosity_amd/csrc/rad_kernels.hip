@@ -525,73 +525,15 @@ __device__ static void emit_tri(const SV &a, const SV &b, const SV &c, uint32_t 
     }
 }
 
-// Sutherland-Hodgman against the near plane and the guard band; an
-// intersection is interpolated from the inside vertex towards the outside.
-// The polygons live in LDS (lp: [2][kClipMax][64], one column per lane of
-// the 64-thread block) rather than in dynamically indexed private arrays,
-// which went to scratch memory.  A convex polygon gains at most one vertex
-// per plane (3 + 5 = 8); more sign changes along the outline than two can
-// only come from vertices within rounding of a plane, and a polygon that
-// would outgrow kClipMax is dropped and flagged as an overflow.
+// Clipping against the near plane and the guard band is Sutherland-Hodgman
+// (clip_wave, below): planes in clip_dist order, each polygon walked from
+// vertex 0, an intersection interpolated from the inside vertex towards the
+// outside one.  A convex polygon gains at most one vertex per plane (3 + 5 =
+// 8); more sign changes along the outline than two can only come from
+// vertices within rounding of a plane, and a polygon that would outgrow
+// kClipMax (fewer than two free places before a vertex) is dropped and
+// flagged as an overflow.
 constexpr int kClipMax = 12;
-__device__ static void emit_clipped(const CV4 *in3, float halfW, uint32_t id, const FaceView &fv,
-                                   uint32_t s, int face, const BinCtx &bc, CV4 (*lp)[kClipMax][64],
-                                   uint32_t ln)
-{
-    lp[0][0][ln] = in3[0];
-    lp[0][1][ln] = in3[1];
-    lp[0][2][ln] = in3[2];
-    int n = 3, cur = 0;
-    for (int plane = 0; plane < 5; ++plane) {
-        int nin = 0;
-        for (int i = 0; i < n; ++i)
-            nin += clip_dist(lp[cur][i][ln], plane) >= 0.0f;
-        if (nin == n)
-            continue;
-        if (nin == 0)
-            return;
-        int m = 0;
-        CV4 P = lp[cur][0][ln];
-        float dP = clip_dist(P, plane);
-        const CV4 P0 = P;
-        const float d0 = dP;
-        for (int i = 0; i < n; ++i) {
-            const bool last = i + 1 == n;
-            CV4 Q = last ? P0 : lp[cur][i + 1][ln];
-            float dQ = last ? d0 : clip_dist(Q, plane);
-            bool ini = dP >= 0.0f, inj = dQ >= 0.0f;
-            if (m + 2 > kClipMax) {
-                atomicOr(&bc.counters[kCntOverflow], 1u);
-                return;
-            }
-            if (ini)
-                lp[cur ^ 1][m++][ln] = P;
-            if (ini != inj) {
-                const CV4 &A = ini ? P : Q;
-                const CV4 &B = ini ? Q : P;
-                float da = ini ? dP : dQ, db = ini ? dQ : dP;
-                float t = da / (da - db);
-                CV4 r;
-                for (int c = 0; c < 4; ++c)
-                    r.c[c] = A.c[c] + t * (B.c[c] - A.c[c]);
-                lp[cur ^ 1][m++][ln] = r;
-            }
-            P = Q;
-            dP = dQ;
-        }
-        n = m;
-        cur ^= 1;
-    }
-    if (n < 3)
-        return;
-    const SV a = project(lp[cur][0][ln], halfW);
-    SV b = project(lp[cur][1][ln], halfW);
-    for (int k = 1; k + 1 < n; ++k) {
-        const SV c = project(lp[cur][k + 1][ln], halfW);
-        emit_tri(a, b, c, id, fv, s, face, bc);
-        b = c;
-    }
-}
 
 // Clip-space transform of a triangle: MVP (column-major) times (x, y, z, 1),
 // each row summed left to right.
@@ -667,30 +609,115 @@ __device__ static inline void setup_face(const float *mm, float4 scis, const flo
     emit_tri(a, b, c, id, fv, s, face, bc);
 }
 
+// Sutherland-Hodgman of one queued job by one wavefront: lane i holds
+// vertex i of the polygon (at most kClipMax).  Per plane every lane finds
+// its edge to the next vertex (the last one wraps to vertex 0), emits its
+// vertex when inside and the intersection when the edge crosses, at offsets
+// from two ballots, and the new polygon is gathered back through the
+// wavefront's LDS buffer.  The arithmetic and the vertex order are those of
+// the sequential walk (the oracle's clipper), vertex by vertex, and the
+// overflow test is its test before the last vertex, where the count is
+// largest.  The fan triangles (0, k, k + 1) are emitted by lanes
+// k = 1 .. n - 2 at once.
+__device__ static void clip_wave(CV4 v, float halfW, uint32_t id, const FaceView &fv, uint32_t s,
+                                 int face, const BinCtx &bc, CV4 *lp)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int n = 3;
+    for (int plane = 0; plane < 5; ++plane) {
+        const bool act = lane < n;
+        const float d = act ? clip_dist(v, plane) : 0.0f;
+        const bool in = act && d >= 0.0f;
+        const int nin = __popcll(__ballot(in));
+        if (nin == n)
+            continue;
+        if (nin == 0)
+            return;
+        const int nx = lane + 1 >= n ? 0 : lane + 1;
+        CV4 q;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            q.c[c] = __shfl(v.c[c], nx, 64);
+        const float dq = __shfl(d, nx, 64);
+        const bool inq = dq >= 0.0f;
+        const bool cross = act && in != inq;
+        const uint64_t b1 = __ballot(in), b2 = __ballot(cross);
+        const int off = __popcll(b1 & lt) + __popcll(b2 & lt);
+        const int m = __popcll(b1) + __popcll(b2);
+        const int last = (int)((b1 >> (n - 1)) & 1ull) + (int)((b2 >> (n - 1)) & 1ull);
+        if (m - last + 2 > kClipMax) {
+            if (lane == 0)
+                atomicOr(&bc.counters[kCntOverflow], 1u);
+            return;
+        }
+        if (in)
+            lp[off] = v;
+        if (cross) {
+            const CV4 &A = in ? v : q;
+            const CV4 &B = in ? q : v;
+            float da = in ? d : dq, db = in ? dq : d;
+            float t = da / (da - db);
+            CV4 r;
+            for (int c = 0; c < 4; ++c)
+                r.c[c] = A.c[c] + t * (B.c[c] - A.c[c]);
+            lp[off + (in ? 1 : 0)] = r;
+        }
+        __syncthreads();
+        n = m;
+        if (lane < n)
+            v = lp[lane];
+        __syncthreads();
+    }
+    if (n < 3)
+        return;
+    const SV pv = project(v, halfW);
+    SV a, c;
+    a.x = __shfl(pv.x, 0, 64);
+    a.y = __shfl(pv.y, 0, 64);
+    a.z = __shfl(pv.z, 0, 64);
+    const int nx = lane + 1 < 64 ? lane + 1 : lane;
+    c.x = __shfl(pv.x, nx, 64);
+    c.y = __shfl(pv.y, nx, 64);
+    c.z = __shfl(pv.z, nx, 64);
+    if (lane >= 1 && lane + 1 < n)
+        emit_tri(a, pv, c, id, fv, s, face, bc);
+}
+
 // Queued (triangle, shooter, face) jobs that cross the near plane or the
-// guard band: same transform, then Sutherland-Hodgman and a fan; records go
-// to the spill region.
+// guard band, one per wavefront (a 64-thread workgroup): lanes 0 .. 2
+// transform their vertex as transform3 does, then clip_wave; records go to
+// the spill region.
 __global__ __launch_bounds__(64) void k_setup_clipped(uint32_t W, const float *__restrict__ tri,
                                                        const uint32_t *__restrict__ tri_patch,
                                                        uint32_t ntri, const float *__restrict__ mvp,
                                                        BinCtx bc,
                                                        const uint32_t *__restrict__ slot_w)
 {
-    __shared__ CV4 lp[2][kClipMax][64];
-    uint32_t n = min(bc.counters[kCntClip], bc.clip_cap);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        uint2 job = bc.clipq[q];
-        uint32_t t = job.x, s = job.y >> 3;
-        int face = (int)(job.y & 7u);
-        float vx[3][3];
-        for (int k = 0; k < 3; ++k)
-            for (int c = 0; c < 3; ++c)
-                vx[k][c] = tri[(size_t)(3 * k + c) * ntri + t];
-        CV4 poly[3];
-        transform3(mvp + (size_t)s * kShooterStride + face * 16, vx, poly);
+    __shared__ CV4 lp[kClipMax];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n = min(bc.counters[kCntClip], bc.clip_cap);
+    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const uint2 job = bc.clipq[q];
+        const uint32_t t = job.x, s = job.y >> 3;
+        const int face = (int)(job.y & 7u);
+        CV4 v;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            v.c[c] = 0.0f;
+        if (lane < 3) {
+            const float x = tri[(size_t)(3 * lane + 0) * ntri + t];
+            const float y = tri[(size_t)(3 * lane + 1) * ntri + t];
+            const float z = tri[(size_t)(3 * lane + 2) * ntri + t];
+            const float *mm = mvp + (size_t)s * kShooterStride + face * 16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                v.c[r] = mm[0 * 4 + r] * x + mm[1 * 4 + r] * y + mm[2 * 4 + r] * z + mm[3 * 4 + r];
+        }
         const uint32_t Ws = slot_w ? slot_w[s] : W;
         FaceView fv = face_view((int32_t)Ws, face);
-        emit_clipped(poly, 0.5f * (float)Ws, tri_patch[t] + 1u, fv, s, face, bc, lp, threadIdx.x);
+        clip_wave(v, 0.5f * (float)Ws, tri_patch[t] + 1u, fv, s, face, bc, lp);
+        __syncthreads(); // lp is the next job's
     }
 }
 
