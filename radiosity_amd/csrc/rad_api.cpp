@@ -134,7 +134,7 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
             launch_tile(c->g, s0, s1, h, c->nshoot, c->bb, c->small_px, ids, depth,
                         c->pk.slot_w ? c->pk.dffs : c->dff, c->rowlen, c->pk.slot_pbeg,
                         ids || fq ? nullptr : c->ff, fq ? c->ffq : nullptr, hits, c->counters,
-                        c->pk.slot_w, c->pk.slot_dffo, st);
+                        c->pk.slot_w, c->pk.slot_dffo, c->fblk, c->K, st);
     };
     // A pass of 16 or more slots renders as two halves side by side: the
     // halves share no bins, records, list entries or counters, so the item
@@ -223,7 +223,8 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
         launch_pack_shoot(c->pk, c->cfg.mode, c->snap, c->ff, c->rowlen, c->refl, c->area,
                           c->npatch, c->unshot, c->shot, c->st);
     else
-        launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->refl,
+        launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff,
+                     c->det ? nullptr : c->fblk, c->refl,
                      c->area, c->npatch, c->unshot, c->shot, c->ctl, c->st);
     c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
@@ -375,7 +376,7 @@ void rad_destroy(rad_ctx *c)
         (void)hipStreamSynchronize(c->st2);
     drop_graph(c);
     void *ptrs[] = {c->tri,  c->tri_patch, c->tsph, c->frames, c->refl,     c->area,   c->emis,
-                    c->unshot, c->shot,   c->ff,     c->hits,     c->dff,    c->item_ids,
+                    c->unshot, c->shot,   c->ff,  c->fblk,   c->hits,     c->dff,    c->item_ids,
                     c->item_depth, c->ffq, c->runs_slots, c->runs_offs, c->trip_h, c->trip_i,
                     c->trip_e,
                     c->mvp,  c->snap,     c->shooters, c->nshoot, c->counters, c->ctl, c->sel,
@@ -483,7 +484,8 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
         (rc = dalloc(c, &c->frames, (size_t)npatch * 9)) || (rc = dalloc(c, &c->refl, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->area, npatch)) || (rc = dalloc(c, &c->emis, (size_t)npatch * 3)) ||
         (rc = dalloc(c, &c->unshot, (size_t)npatch * 3)) || (rc = dalloc(c, &c->shot, (size_t)npatch * 3)) ||
-        (rc = dalloc(c, &c->ff, K * rowlen)) || (rc = dalloc(c, &c->hits, K * rowlen)))
+        (rc = dalloc(c, &c->ff, K * rowlen)) || (rc = dalloc(c, &c->hits, K * rowlen)) ||
+        (rc = dalloc(c, &c->fblk, (size_t)(rowlen + 63) / 64 * K)))
         return rc;
     if (c->det) {
         if ((rc = dalloc(c, &c->ffq, K * rowlen)))
@@ -514,6 +516,7 @@ int scene_alloc(rad_ctx *c, uint32_t ntri, uint32_t npatch, uint32_t nblk, uint3
         (rc = dalloc(c, &c->cand, 2 * topk_cand_half(npatch, c->K))))
         return rc;
     HIPCHK(c, hipMemset(c->ff, 0, K * rowlen * 4));
+    HIPCHK(c, hipMemset(c->fblk, 0, (size_t)(rowlen + 63) / 64 * K * 4));
     c->ff_dirty = 0;
     HIPCHK(c, hipMemset(c->hits, 0, K * rowlen * 4));
     c->ntri = ntri;

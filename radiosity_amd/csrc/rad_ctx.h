@@ -39,6 +39,10 @@ struct rad_ctx {
     float *unshot = nullptr;       // npatch*3
     float *shot = nullptr;         // npatch*3
     float *ff = nullptr;           // K*npatch
+    // one flag per (64 F entries of a row, slot), [row / 64][slot]: set by
+    // the tile kernel when it adds to that block of the slot's row, cleared
+    // by k_shoot when it has read it; a block whose flag is clear is zero
+    uint32_t *fblk = nullptr;      // ceil(rowlen / 64) * K
     uint32_t *hits = nullptr;      // K*npatch
     // F rows [0, ff_dirty) may hold nonzero entries at this point of the
     // stream, rows from ff_dirty on are zero: a clear or a shoot (the shoot

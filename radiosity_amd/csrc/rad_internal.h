@@ -165,7 +165,8 @@ void launch_tile(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h, const 
                  const BinBuffers &bb, uint32_t small_px, uint32_t *out_ids, float *out_depth,
                  const float *dff, uint32_t rowlen, const uint32_t *slot_pbeg, float *ff,
                  unsigned long long *ffq, uint32_t *hits, uint32_t *counters,
-                 const uint32_t *slot_w, const uint32_t *slot_dffo, hipStream_t st);
+                 const uint32_t *slot_w, const uint32_t *slot_dffo, uint32_t *fblk,
+                 uint32_t fstride, hipStream_t st);
 // Deterministic mode: fixed-point F entries (2^-52 units) into the float rows.
 void launch_fold(unsigned long long *ffq, float *ff, size_t n, hipStream_t st);
 // ProcessHemicube over an id plane (K x RES u32): ff (or ffq) and hits +=.
@@ -180,8 +181,10 @@ void launch_runs_count(const Geometry &g, uint32_t k, uint32_t nx, const uint32_
 void launch_runs_write(const Geometry &g, uint32_t k, uint32_t nx, const uint32_t *item,
                        const float *dff, const uint32_t *offs, uint32_t cap, uint32_t *th,
                        uint32_t *tid, float *te, hipStream_t st);
+// fblk (nullable): the F-block flags, [row / 64][slot] with K slots per
+// block; null reads every F entry.
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
-                  const uint32_t *nshoot, const float *snap, float *ff,
+                  const uint32_t *nshoot, const float *snap, float *ff, uint32_t *fblk,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
                   float *shot, uint32_t *ctl, hipStream_t st);
 void launch_residual(const float *unshot, const float *area, const float *emis,

@@ -1332,7 +1332,8 @@ static_assert(kTileTable == kTileThreads, "the table is cleared and flushed one 
 // per run.  A run that finds no free slot goes straight to the global row.
 __device__ static inline void table_add(uint32_t p, float f, uint32_t c, bool with_c,
                                         uint32_t *hk, float *hs, uint32_t *hc,
-                                        float *__restrict__ frow, uint32_t *__restrict__ hrow)
+                                        float *__restrict__ frow, uint32_t *__restrict__ hrow,
+                                        uint32_t *__restrict__ fflag, uint32_t fstride)
 {
     const uint32_t key = p + 1;
     uint32_t h = (key * 2654435761u) >> 24; // kTileTable = 2^8
@@ -1350,6 +1351,8 @@ __device__ static inline void table_add(uint32_t p, float f, uint32_t c, bool wi
     atomicAdd(&frow[p], f);
     if (with_c)
         atomicAdd(&hrow[p], c);
+    if (fflag)
+        fflag[(p >> 6) * fstride] = 1u;
 }
 
 // One run (id, sums f and c) of a shooter's hemicube: id 0 (background) adds
@@ -1359,7 +1362,8 @@ __device__ static inline void table_add(uint32_t p, float f, uint32_t c, bool wi
 __device__ static inline void run_add(uint32_t id, float f, uint32_t c, uint32_t pbase,
                                       uint32_t rowlen, bool with_c, uint32_t *hk, float *hs,
                                       uint32_t *hc, float *__restrict__ frow,
-                                      uint32_t *__restrict__ hrow, uint32_t *__restrict__ counters)
+                                      uint32_t *__restrict__ hrow, uint32_t *__restrict__ counters,
+                                      uint32_t *__restrict__ fflag, uint32_t fstride)
 {
     if (!id)
         return;
@@ -1368,7 +1372,7 @@ __device__ static inline void run_add(uint32_t id, float f, uint32_t c, uint32_t
         atomicAdd(&counters[kCntUnknown], 1u);
         return;
     }
-    table_add(p, f, c, with_c, hk, hs, hc, frow, hrow);
+    table_add(p, f, c, with_c, hk, hs, hc, frow, hrow, fflag, fstride);
 }
 
 // ProcessHemicube on 256 consecutive texels of one shooter's hemicube, 4
@@ -1635,7 +1639,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     uint32_t rowlen, const uint32_t *__restrict__ slot_pbeg, float *__restrict__ ff,
     unsigned long long *__restrict__ ffq, uint32_t *__restrict__ hits,
     uint32_t *__restrict__ counters, const uint32_t *__restrict__ order,
-    const uint32_t *__restrict__ slot_w, const uint32_t *__restrict__ slot_dffo)
+    const uint32_t *__restrict__ slot_w, const uint32_t *__restrict__ slot_dffo,
+    uint32_t *__restrict__ fblk, uint32_t fstride)
 {
     __shared__ uint64_t tk[kMaxTile * kMaxTile];
     // the chunk queues, then (resolve) the table of F entries
@@ -1774,6 +1779,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     if (table) {
         float *const frow = ff + (size_t)s * rowlen;
         uint32_t *const hrow = hits ? hits + (size_t)s * rowlen : nullptr;
+        // this slot's column of the F-block flags
+        uint32_t *const fflag = fblk ? fblk + s : nullptr;
         if ((threadIdx.x >> 6) * 256u < TT) { // whole wavefronts
             uint32_t q4[4] = {0u, 0u, 0u, 0u};
             float f4[4] = {0.f, 0.f, 0.f, 0.f};
@@ -1789,7 +1796,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             const bool with_c = hrow != nullptr;
             reduce_quad(q4, f4, with_c, [&](uint32_t id, float f, uint32_t c) {
                 run_add(id, f, c, pbase, rowlen, with_c, sh.t.key, sh.t.sum, sh.t.cnt, frow, hrow,
-                        counters);
+                        counters, fflag, fstride);
             });
         }
         __syncthreads();
@@ -1798,6 +1805,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             atomicAdd(&frow[key - 1], sh.t.sum[threadIdx.x]);
             if (hrow)
                 atomicAdd(&hrow[key - 1], sh.t.cnt[threadIdx.x]);
+            if (fflag)
+                fflag[((key - 1) >> 6) * fstride] = 1u;
         }
         return;
     }
@@ -2019,7 +2028,8 @@ void launch_tile(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h, const 
                  const BinBuffers &bb, uint32_t small_px, uint32_t *out_ids, float *out_depth,
                  const float *dff, uint32_t rowlen, const uint32_t *slot_pbeg, float *ff,
                  unsigned long long *ffq, uint32_t *hits, uint32_t *counters,
-                 const uint32_t *slot_w, const uint32_t *slot_dffo, hipStream_t st)
+                 const uint32_t *slot_w, const uint32_t *slot_dffo, uint32_t *fblk,
+                 uint32_t fstride, hipStream_t st)
 {
     const uint32_t nb = (s1 - s0) * bb.ntiles;
     const uint32_t *order = bb.order + (size_t)s0 * bb.ntiles;
@@ -2029,12 +2039,13 @@ void launch_tile(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h, const 
         k_tile_raster<true><<<nb, kTileThreads, 0, st>>>(
             g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, list,
             bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
-            slot_pbeg, ff, ffq, hits, ctr, order, slot_w, slot_dffo);
+            slot_pbeg, ff, ffq, hits, ctr, order, slot_w, slot_dffo, nullptr, 0u);
     else
         k_tile_raster<false><<<nb, kTileThreads, 0, st>>>(
             g.W, g.TW, g.TH, g.RES, nshoot, bb.recs, bb.bbox, bb.count, bb.start, list,
             bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, small_px, out_ids, out_depth, dff, rowlen,
-            slot_pbeg, ff, ffq, hits, ctr, order, slot_w, slot_dffo);
+            slot_pbeg, ff, ffq, hits, ctr, order, slot_w, slot_dffo, ff ? fblk : nullptr,
+            fstride);
 }
 
 // Deterministic mode: n fixed-point entries (units of 2^-52) into the float
@@ -2361,6 +2372,7 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
                                                const uint32_t *__restrict__ nshoot,
                                                const float *__restrict__ snap,
                                                float *__restrict__ ff,
+                                               uint32_t *__restrict__ fblk, uint32_t fstride,
                                                const float *__restrict__ refl,
                                                const float *__restrict__ area, uint32_t npatch,
                                                float *__restrict__ unshot,
@@ -2382,36 +2394,63 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
     __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0)
         count_pass(ctl, n);
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= npatch)
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    // the wavefront's 64 receivers are one block of the F-block flags
+    const uint32_t blk = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) >> 6;
+    if (blk * 64u >= npatch)
         return;
-    float u0 = unshot[3 * (size_t)j + 0], u1 = unshot[3 * (size_t)j + 1],
-          u2 = unshot[3 * (size_t)j + 2];
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f, inv_aj = 0.f;
-    if (mode != RAD_MODE_REFERENCE) {
-        r0 = refl[3 * (size_t)j + 0];
-        r1 = refl[3 * (size_t)j + 1];
-        r2 = refl[3 * (size_t)j + 2];
-        inv_aj = 1.0f / area[j];
+    const bool live = j < npatch;
+    float u0 = 0.f, u1 = 0.f, u2 = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, inv_aj = 0.f;
+    if (live) {
+        u0 = unshot[3 * (size_t)j + 0];
+        u1 = unshot[3 * (size_t)j + 1];
+        u2 = unshot[3 * (size_t)j + 2];
+        if (mode != RAD_MODE_REFERENCE) {
+            r0 = refl[3 * (size_t)j + 0];
+            r1 = refl[3 * (size_t)j + 1];
+            r2 = refl[3 * (size_t)j + 2];
+            inv_aj = 1.0f / area[j];
+        }
     }
-    // F entries are read 8 shooters at a time (independent loads in flight),
-    // then added in shooter order.  The nonzero entries of a group of 64
-    // shooters are set back to zero after the group's last read (a load
-    // issued after a store waits for it), which leaves the F rows zero for
-    // the next pass (its clear skips them).
+    // Shooters are taken in groups of 64 and, within a group, in shooter
+    // order; only those whose F block of these receivers is flagged are
+    // read (an unflagged block is zero), 8 at a time (independent loads in
+    // flight).  Entries are added in shooter order, skipping zeros, so the
+    // sums are those of reading every entry.  The nonzero entries of a group
+    // are set back to zero after the group's last read (a load issued after
+    // a store waits for it), and the group's flags are cleared, which leaves
+    // the F rows zero for the next pass (its clear skips them).
     for (uint32_t g0 = 0; g0 < n; g0 += 64) {
         const uint32_t g1 = min(n, g0 + 64);
+        uint32_t *const fl = fblk ? fblk + (size_t)blk * fstride + g0 : nullptr;
+        const bool mine = g0 + lane < g1;
+        uint64_t todo;
+        if (fl) {
+            const bool set = mine && fl[lane] != 0u;
+            todo = __ballot(set);
+            if (set)
+                fl[lane] = 0u;
+        } else {
+            todo = __ballot(mine);
+        }
         uint64_t nz = 0;
-        for (uint32_t s0 = g0; s0 < g1; s0 += 8) {
+        while (todo) {
+            uint32_t ss[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                ss[q] = todo ? g0 + (uint32_t)__ffsll((long long)todo) - 1u : ~0u;
+                todo &= todo - 1;
+            }
             float fb[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q)
-                fb[q] = s0 + q < g1 ? ff[(size_t)(s0 + q) * npatch + j] : 0.0f;
+                fb[q] = live && ss[q] != ~0u ? ff[(size_t)ss[q] * npatch + j] : 0.0f;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const uint32_t s = s0 + q;
+                const uint32_t s = ss[q];
                 const float f = fb[q];
-                if (f == 0.0f)
+                if (s == ~0u || f == 0.0f)
                     continue;
                 nz |= 1ull << (s - g0);
                 float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
@@ -2431,6 +2470,8 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
         for (uint64_t m = nz; m; m &= m - 1)
             ff[(size_t)(g0 + (uint32_t)__ffsll((long long)m) - 1) * npatch + j] = 0.0f;
     }
+    if (!live)
+        return;
     // Main.cpp:1287-1295
     if (n && sid[n - 1] == j)
         stop_test(ctl, u0, u1, u2);
@@ -2451,12 +2492,12 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
 }
 
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
-                  const float *snap, float *ff, const float *refl, const float *area,
-                  uint32_t npatch, float *unshot, float *shot, uint32_t *ctl, hipStream_t st)
+                  const float *snap, float *ff, uint32_t *fblk, const float *refl,
+                  const float *area, uint32_t npatch, float *unshot, float *shot, uint32_t *ctl,
+                  hipStream_t st)
 {
-    (void)K;
-    k_shoot<<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, refl, area,
-                                                  npatch, unshot, shot, ctl);
+    k_shoot<<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk, K,
+                                                  refl, area, npatch, unshot, shot, ctl);
 }
 
 // ----------------------------------------------------------------- selection
