@@ -100,6 +100,10 @@ void record(rad_ctx *c, size_t idx)
         (void)hipEventRecord(c->ev[idx], c->st);
 }
 
+// A pass of 16 or more slots renders as two halves side by side, except a
+// timed one (stage events bracket whole stages).
+static bool split_render(const rad_ctx *c, uint32_t k) { return k >= kSplitMin && !c->timing; }
+
 // Renders the hemicubes of the shooters currently in c->shooters/nshoot
 // (k slots): ids != null writes the item buffer (and its depths when depth
 // != null), otherwise the tiles are reduced into F rows (and hit counts when
@@ -109,7 +113,8 @@ void record(rad_ctx *c, size_t idx)
 // writes: a render of fewer rows must not leave an earlier writer's rows
 // behind for a later pass that skips its clear).
 int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *depth,
-                uint32_t *hits, const uint32_t *shooters = nullptr, const float *frames = nullptr)
+                uint32_t *hits, const uint32_t *shooters = nullptr, const float *frames = nullptr,
+                bool shoot_half = false)
 {
     uint32_t rows = c->ff_dirty;
     if (hits && rows < k)
@@ -140,13 +145,20 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
     // halves share no bins, records, list entries or counters, so the item
     // buffers and F rows are those of one launch over all slots.  Timed
     // passes stay whole so that the stage events bracket every kernel.
-    if (k >= kSplitMin && !c->timing) {
+    // With shoot_half, the first half's F rows are shot on the context
+    // stream as soon as its tiles are done, while the second half still
+    // renders; the caller shoots the second half.
+    if (split_render(c, k)) {
         HIPCHK(c, hipEventRecord(c->ev_fork, c->st));
         HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
         for (int j = 0; j < 3; ++j) {
             stage(j, 0, k / 2, 0, c->st);
             stage(j, k / 2, k, 1, c->st2);
         }
+        if (shoot_half)
+            launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->fblk,
+                         c->refl, c->area, c->npatch, c->unshot, c->shot, c->ctl, 0, k / 2,
+                         false, c->st);
         HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
         HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
     } else {
@@ -217,15 +229,20 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
         return rc;
     }
     record(c, evbase + 1);
-    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr, nullptr)) != RAD_OK)
+    // the first half of a split pass is shot before the second finishes
+    // rendering (not for packs, nor with the fixed-point rows, whose fold
+    // comes after both halves)
+    const bool half = !c->pk.nscene && !c->det && split_render(c, c->K);
+    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr, nullptr, nullptr, nullptr, half)) !=
+        RAD_OK)
         return rc;
     if (c->pk.nscene)
         launch_pack_shoot(c->pk, c->cfg.mode, c->snap, c->ff, c->rowlen, c->refl, c->area,
                           c->npatch, c->unshot, c->shot, c->st);
     else
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff,
-                     c->det ? nullptr : c->fblk, c->refl,
-                     c->area, c->npatch, c->unshot, c->shot, c->ctl, c->st);
+                     c->det ? nullptr : c->fblk, c->refl, c->area, c->npatch, c->unshot,
+                     c->shot, c->ctl, half ? c->K / 2 : 0u, c->K, true, c->st);
     c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
     c->rendered = false;

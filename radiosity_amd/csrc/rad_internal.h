@@ -182,11 +182,14 @@ void launch_runs_write(const Geometry &g, uint32_t k, uint32_t nx, const uint32_
                        const float *dff, const uint32_t *offs, uint32_t cap, uint32_t *th,
                        uint32_t *tid, float *te, hipStream_t st);
 // fblk (nullable): the F-block flags, [row / 64][slot] with K slots per
-// block; null reads every F entry.
+// block; null reads every F entry.  Adds the F rows of shooters
+// [sbeg, send); the last launch of a pass also moves the shooters' energy,
+// runs the stop test and counts the pass.
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
                   const uint32_t *nshoot, const float *snap, float *ff, uint32_t *fblk,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
-                  float *shot, uint32_t *ctl, hipStream_t st);
+                  float *shot, uint32_t *ctl, uint32_t sbeg, uint32_t send, bool last,
+                  hipStream_t st);
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);

@@ -2376,7 +2376,8 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
                                                const float *__restrict__ refl,
                                                const float *__restrict__ area, uint32_t npatch,
                                                float *__restrict__ unshot,
-                                               float *__restrict__ shot, uint32_t *__restrict__ ctl)
+                                               float *__restrict__ shot, uint32_t *__restrict__ ctl,
+                                               uint32_t sbeg, uint32_t send, uint32_t last)
 {
     __shared__ uint32_t sid[RAD_MAX_K];
     __shared__ float sc[RAD_MAX_K * 4];
@@ -2392,7 +2393,7 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
         sc[4 * s + 3] = area[i];
     }
     __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (last && blockIdx.x == 0 && threadIdx.x == 0)
         count_pass(ctl, n);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2413,6 +2414,10 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
             inv_aj = 1.0f / area[j];
         }
     }
+    // Shooters [sbeg, send) are added here: a pass may be shot in two
+    // launches, the first half's as soon as its tiles are done, the second
+    // (last) one adding the rest and moving the shooters' energy; the sums
+    // are the same float additions in the same order either way.
     // Shooters are taken in groups of 64 and, within a group, in shooter
     // order; only those whose F block of these receivers is flagged are
     // read (an unflagged block is zero), 8 at a time (independent loads in
@@ -2421,8 +2426,9 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
     // are set back to zero after the group's last read (a load issued after
     // a store waits for it), and the group's flags are cleared, which leaves
     // the F rows zero for the next pass (its clear skips them).
-    for (uint32_t g0 = 0; g0 < n; g0 += 64) {
-        const uint32_t g1 = min(n, g0 + 64);
+    const uint32_t se = min(n, send);
+    for (uint32_t g0 = sbeg; g0 < se; g0 += 64) {
+        const uint32_t g1 = min(se, g0 + 64);
         uint32_t *const fl = fblk ? fblk + (size_t)blk * fstride + g0 : nullptr;
         const bool mine = g0 + lane < g1;
         uint64_t todo;
@@ -2472,6 +2478,12 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
     }
     if (!live)
         return;
+    if (!last) {
+        unshot[3 * (size_t)j + 0] = u0;
+        unshot[3 * (size_t)j + 1] = u1;
+        unshot[3 * (size_t)j + 2] = u2;
+        return;
+    }
     // Main.cpp:1287-1295
     if (n && sid[n - 1] == j)
         stop_test(ctl, u0, u1, u2);
@@ -2494,10 +2506,11 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
                   const float *snap, float *ff, uint32_t *fblk, const float *refl,
                   const float *area, uint32_t npatch, float *unshot, float *shot, uint32_t *ctl,
-                  hipStream_t st)
+                  uint32_t sbeg, uint32_t send, bool last, hipStream_t st)
 {
     k_shoot<<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk, K,
-                                                  refl, area, npatch, unshot, shot, ctl);
+                                                  refl, area, npatch, unshot, shot, ctl, sbeg,
+                                                  send, last ? 1u : 0u);
 }
 
 // ----------------------------------------------------------------- selection
