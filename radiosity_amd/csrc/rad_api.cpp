@@ -104,6 +104,9 @@ void record(rad_ctx *c, size_t idx)
 // timed one (stage events bracket whole stages).
 static bool split_render(const rad_ctx *c, uint32_t k) { return k >= kSplitMin && !c->timing; }
 
+// First slot of the second half of a split pass.
+static uint32_t split_at(uint32_t k) { return k / 2; }
+
 // Renders the hemicubes of the shooters currently in c->shooters/nshoot
 // (k slots): ids != null writes the item buffer (and its depths when depth
 // != null), otherwise the tiles are reduced into F rows (and hit counts when
@@ -152,13 +155,13 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
         HIPCHK(c, hipEventRecord(c->ev_fork, c->st));
         HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
         for (int j = 0; j < 3; ++j) {
-            stage(j, 0, k / 2, 0, c->st);
-            stage(j, k / 2, k, 1, c->st2);
+            stage(j, 0, split_at(k), 0, c->st);
+            stage(j, split_at(k), k, 1, c->st2);
         }
         if (shoot_half)
             launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->fblk,
-                         c->refl, c->area, c->npatch, c->unshot, c->shot, c->ctl, 0, k / 2,
-                         false, c->st);
+                         c->refl, c->area, c->npatch, c->unshot, c->shot, c->ctl, 0, split_at(k),
+                         false, nullptr, c->st);
         HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
         HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
     } else {
@@ -217,13 +220,28 @@ int select_dev(rad_ctx *c, uint32_t k)
     return check_launch(c, "select");
 }
 
-int one_pass(rad_ctx *c, bool do_select, size_t evbase)
+// Whether a pass's last shoot can leave the next POWER selection's
+// per-workgroup lists (top-K of 2 .. 64, one merge level for the lists).
+static bool fused_select(const rad_ctx *c)
+{
+    return !c->pk.nscene && c->cfg.select == RAD_SELECT_POWER && c->K > 1 && c->K <= 64 &&
+           shoot_lists(c->npatch) <= 256;
+}
+
+// One pass: select (lists_in: only merge the lists the previous pass's shoot
+// left), render, shoot (lists_out: leave the lists for the next pass).
+int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool lists_in = false,
+             bool lists_out = false)
 {
     int rc;
     record(c, evbase + 0);
     if (c->pk.nscene) {
         launch_pack_select(c->pk, c->unshot, c->area, c->st);
         if ((rc = check_launch(c, "pack select")))
+            return rc;
+    } else if (do_select && lists_in) {
+        launch_select_merge(c->cand, shoot_lists(c->npatch), c->K, c->shooters, c->nshoot, c->st);
+        if ((rc = check_launch(c, "select merge")))
             return rc;
     } else if (do_select && (rc = select_dev(c, c->K)) != RAD_OK) {
         return rc;
@@ -242,7 +260,8 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase)
     else
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff,
                      c->det ? nullptr : c->fblk, c->refl, c->area, c->npatch, c->unshot,
-                     c->shot, c->ctl, half ? c->K / 2 : 0u, c->K, true, c->st);
+                     c->shot, c->ctl, half ? split_at(c->K) : 0u, c->K, true,
+                     lists_out ? c->cand : nullptr, c->st);
     c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
     c->rendered = false;
@@ -908,8 +927,12 @@ static int capture_graph(rad_ctx *c, uint32_t n)
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     int rc = RAD_OK;
+    // inside a replay nothing else touches the state between passes, so
+    // every pass after the first merges the lists its predecessor's shoot
+    // left
+    const bool fuse = fused_select(c);
     for (uint32_t p = 0; p < n && rc == RAD_OK; ++p)
-        rc = one_pass(c, true, 0);
+        rc = one_pass(c, true, 0, fuse && p > 0, fuse && p + 1 < n);
     hipError_t e = hipStreamEndCapture(c->st, &graph);
     if (rc)
         return rc;

@@ -184,12 +184,18 @@ void launch_runs_write(const Geometry &g, uint32_t k, uint32_t nx, const uint32_
 // fblk (nullable): the F-block flags, [row / 64][slot] with K slots per
 // block; null reads every F entry.  Adds the F rows of shooters
 // [sbeg, send); the last launch of a pass also moves the shooters' energy,
-// runs the stop test and counts the pass.
+// runs the stop test and counts the pass, and with lists != null leaves the
+// next POWER selection's per-workgroup lists of the new state.
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
                   const uint32_t *nshoot, const float *snap, float *ff, uint32_t *fblk,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
                   float *shot, uint32_t *ctl, uint32_t sbeg, uint32_t send, bool last,
-                  hipStream_t st);
+                  uint64_t *lists, hipStream_t st);
+// Per-workgroup selection lists a last shoot leaves (lists != null): their
+// count for npatch receivers, and the next selection's merge of them.
+uint32_t shoot_lists(uint32_t npatch);
+void launch_select_merge(const uint64_t *lists, uint32_t nlists, uint32_t k, uint32_t *shooters,
+                         uint32_t *nshoot, hipStream_t st);
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);

@@ -2364,11 +2364,116 @@ __device__ static inline void count_pass(uint32_t *__restrict__ ctl, uint32_t n)
     *(unsigned long long *)(ctl + kCtlShotLo) += n;
 }
 
+// Lane exchanges of the top-K networks without ds_bpermute: lane i gets the
+// value of lane i ^ M, from DPP lane moves (strides 1, 2, 4, 8 inside a row
+// of 16 lanes) and the gfx950 row and half-wave swaps (16, 32).
+template <int M> __device__ __forceinline__ uint32_t lane_xor(uint32_t x)
+{
+    const int lane = threadIdx.x & 63;
+    const int xi = (int)x;
+    if constexpr (M == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0xB1, 0xF, 0xF, false); // quad_perm 1,0,3,2
+    } else if constexpr (M == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0x4E, 0xF, 0xF, false); // quad_perm 2,3,0,1
+    } else if constexpr (M == 4) {
+        const int up = __builtin_amdgcn_update_dpp(xi, xi, 0x104, 0xF, 0xF, false); // row_shl:4, lane i + 4
+        const int dn = __builtin_amdgcn_update_dpp(xi, xi, 0x114, 0xF, 0xF, false); // row_shr:4, lane i - 4
+        return (uint32_t)((lane & 4) ? dn : up);
+    } else if constexpr (M == 8) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0x128, 0xF, 0xF, false); // row_ror:8
+    } else if constexpr (M == 16) {
+        // odd rows of the first operand swap with even rows of the second:
+        // r[0] holds rows (0, 0, 2, 2) of x, r[1] rows (1, 1, 3, 3)
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (lane & 16) ? r[0] : r[1];
+    } else {
+        static_assert(M == 32, "lane_xor strides 1 .. 32");
+        // the upper half of the first operand swaps with the lower half of
+        // the second: r[0] holds x's lower half twice, r[1] its upper half
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return (lane & 32) ? r[0] : r[1];
+    }
+}
+
+template <int M> __device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v)
+{
+    return ((uint64_t)lane_xor<M>((uint32_t)(v >> 32)) << 32) | lane_xor<M>((uint32_t)v);
+}
+
+// The value of lane 63 - i: each row mirrored (row_mirror), then rows
+// r and 3 - r exchanged.
+__device__ __forceinline__ uint64_t lane_reverse_u64(uint64_t v)
+{
+    const int lo = __builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), 0x140, 0xF, 0xF, false);
+    const uint64_t m = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    return lane_xor_u64<32>(lane_xor_u64<16>(m));
+}
+
+// A wave-uniform lane's value (lane src, a constant).
+__device__ __forceinline__ uint64_t lane_read_u64(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One compare-exchange step of a bitonic network at lane stride M.
+template <int M> __device__ __forceinline__ uint64_t bitonic_step(uint64_t v, bool take_max)
+{
+    const uint64_t o = lane_xor_u64<M>(v);
+    return take_max ? (v > o ? v : o) : (v < o ? v : o);
+}
+
+__device__ __forceinline__ uint64_t wave_sort_desc(uint64_t v)
+{
+    const int lane = threadIdx.x & 63;
+    // stage `size`: lanes in blocks of size sort descending where
+    // lane & size == 0; a step at stride m keeps the max in the lower lane
+    // of a pair when that block is descending
+#define RAD_SORT_STEP(size_, m_) \
+    v = bitonic_step<m_>(v, ((lane & (m_)) == 0) == ((lane & (size_)) == 0));
+    RAD_SORT_STEP(2, 1)
+    RAD_SORT_STEP(4, 2) RAD_SORT_STEP(4, 1)
+    RAD_SORT_STEP(8, 4) RAD_SORT_STEP(8, 2) RAD_SORT_STEP(8, 1)
+    RAD_SORT_STEP(16, 8) RAD_SORT_STEP(16, 4) RAD_SORT_STEP(16, 2) RAD_SORT_STEP(16, 1)
+    RAD_SORT_STEP(32, 16) RAD_SORT_STEP(32, 8) RAD_SORT_STEP(32, 4) RAD_SORT_STEP(32, 2)
+    RAD_SORT_STEP(32, 1)
+    RAD_SORT_STEP(64, 32) RAD_SORT_STEP(64, 16) RAD_SORT_STEP(64, 8) RAD_SORT_STEP(64, 4)
+    RAD_SORT_STEP(64, 2) RAD_SORT_STEP(64, 1)
+#undef RAD_SORT_STEP
+    return v;
+}
+
+// top and b sorted descending over the lanes: the top 64 of both, sorted.
+__device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t r = lane_reverse_u64(b);
+    uint64_t v = top > r ? top : r;
+    v = bitonic_step<32>(v, (lane & 32) == 0);
+    v = bitonic_step<16>(v, (lane & 16) == 0);
+    v = bitonic_step<8>(v, (lane & 8) == 0);
+    v = bitonic_step<4>(v, (lane & 4) == 0);
+    v = bitonic_step<2>(v, (lane & 2) == 0);
+    v = bitonic_step<1>(v, (lane & 1) == 0);
+    return v;
+}
+
 // Main.cpp:1251-1295 for all receivers at once: one thread per receiver j
 // accumulates the contributions of the pass's shooters, then the shooters
 // move their snapshot from unshot to shot.  Every nonzero F entry it reads
 // is set back to zero, so the next pass finds the F rows cleared.
-__global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__restrict__ shooters,
+// With lists != null (the last launch of a pass whose next pass selects by
+// POWER), each workgroup also leaves the sorted 64 largest selection keys
+// of its 1,024 receivers' new state in lists[blockIdx.x * 64 + lane], the
+// per-workgroup lists k_topk64_scan would make from the stored state, so
+// the next pass's selection only merges them.  NT threads per workgroup:
+// 1,024 with lists (one merge level for up to 262,144 receivers), else 256.
+constexpr uint32_t kShootListThreads = 1024;
+
+template <uint32_t NT>
+__global__ __launch_bounds__(NT) void k_shoot(uint32_t mode, const uint32_t *__restrict__ shooters,
                                                const uint32_t *__restrict__ nshoot,
                                                const float *__restrict__ snap,
                                                float *__restrict__ ff,
@@ -2377,10 +2482,12 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
                                                const float *__restrict__ area, uint32_t npatch,
                                                float *__restrict__ unshot,
                                                float *__restrict__ shot, uint32_t *__restrict__ ctl,
-                                               uint32_t sbeg, uint32_t send, uint32_t last)
+                                               uint32_t sbeg, uint32_t send, uint32_t last,
+                                               uint64_t *__restrict__ lists)
 {
     __shared__ uint32_t sid[RAD_MAX_K];
     __shared__ float sc[RAD_MAX_K * 4];
+    __shared__ uint64_t wl[NT / 64][64];
     uint32_t n = *nshoot;
     for (uint32_t s = threadIdx.x; s < n; s += blockDim.x) {
         uint32_t i = shooters[s];
@@ -2395,12 +2502,10 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
     __syncthreads();
     if (last && blockIdx.x == 0 && threadIdx.x == 0)
         count_pass(ctl, n);
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     // the wavefront's 64 receivers are one block of the F-block flags
     const uint32_t blk = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) >> 6;
-    if (blk * 64u >= npatch)
-        return;
     const bool live = j < npatch;
     float u0 = 0.f, u1 = 0.f, u2 = 0.f, r0 = 0.f, r1 = 0.f, r2 = 0.f, inv_aj = 0.f;
     if (live) {
@@ -2427,91 +2532,122 @@ __global__ __launch_bounds__(256) void k_shoot(uint32_t mode, const uint32_t *__
     // a store waits for it), and the group's flags are cleared, which leaves
     // the F rows zero for the next pass (its clear skips them).
     const uint32_t se = min(n, send);
-    for (uint32_t g0 = sbeg; g0 < se; g0 += 64) {
-        const uint32_t g1 = min(se, g0 + 64);
-        uint32_t *const fl = fblk ? fblk + (size_t)blk * fstride + g0 : nullptr;
-        const bool mine = g0 + lane < g1;
-        uint64_t todo;
-        if (fl) {
-            const bool set = mine && fl[lane] != 0u;
-            todo = __ballot(set);
-            if (set)
-                fl[lane] = 0u;
-        } else {
-            todo = __ballot(mine);
-        }
-        uint64_t nz = 0;
-        while (todo) {
-            uint32_t ss[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                ss[q] = todo ? g0 + (uint32_t)__ffsll((long long)todo) - 1u : ~0u;
-                todo &= todo - 1;
+    if (blk * 64u < npatch) {
+        for (uint32_t g0 = sbeg; g0 < se; g0 += 64) {
+            const uint32_t g1 = min(se, g0 + 64);
+            uint32_t *const fl = fblk ? fblk + (size_t)blk * fstride + g0 : nullptr;
+            const bool mine = g0 + lane < g1;
+            uint64_t todo;
+            if (fl) {
+                const bool set = mine && fl[lane] != 0u;
+                todo = __ballot(set);
+                if (set)
+                    fl[lane] = 0u;
+            } else {
+                todo = __ballot(mine);
             }
-            float fb[8];
+            uint64_t nz = 0;
+            while (todo) {
+                uint32_t ss[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                fb[q] = live && ss[q] != ~0u ? ff[(size_t)ss[q] * npatch + j] : 0.0f;
+                for (int q = 0; q < 8; ++q) {
+                    ss[q] = todo ? g0 + (uint32_t)__ffsll((long long)todo) - 1u : ~0u;
+                    todo &= todo - 1;
+                }
+                float fb[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const uint32_t s = ss[q];
-                const float f = fb[q];
-                if (s == ~0u || f == 0.0f)
-                    continue;
-                nz |= 1ull << (s - g0);
-                float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
-                if (mode == RAD_MODE_REFERENCE) {
-                    // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
-                    u0 += d0 * f * 0.3f * sc[4 * s + 0];
-                    u1 += d1 * f * 0.3f * sc[4 * s + 1];
-                    u2 += d2 * f * 0.3f * sc[4 * s + 2];
-                } else {
-                    float g = f * (sc[4 * s + 3] * inv_aj);
-                    u0 += r0 * d0 * g;
-                    u1 += r1 * d1 * g;
-                    u2 += r2 * d2 * g;
+                for (int q = 0; q < 8; ++q)
+                    fb[q] = live && ss[q] != ~0u ? ff[(size_t)ss[q] * npatch + j] : 0.0f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t s = ss[q];
+                    const float f = fb[q];
+                    if (s == ~0u || f == 0.0f)
+                        continue;
+                    nz |= 1ull << (s - g0);
+                    float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
+                    if (mode == RAD_MODE_REFERENCE) {
+                        // Main.cpp:1274: radiosity += dB * F * 0.3 * colour_emitter
+                        u0 += d0 * f * 0.3f * sc[4 * s + 0];
+                        u1 += d1 * f * 0.3f * sc[4 * s + 1];
+                        u2 += d2 * f * 0.3f * sc[4 * s + 2];
+                    } else {
+                        float g = f * (sc[4 * s + 3] * inv_aj);
+                        u0 += r0 * d0 * g;
+                        u1 += r1 * d1 * g;
+                        u2 += r2 * d2 * g;
+                    }
+                }
+            }
+            for (uint64_t m = nz; m; m &= m - 1)
+                ff[(size_t)(g0 + (uint32_t)__ffsll((long long)m) - 1) * npatch + j] = 0.0f;
+        }
+    }
+    if (live) {
+        if (last) {
+            // Main.cpp:1287-1295
+            if (n && sid[n - 1] == j)
+                stop_test(ctl, u0, u1, u2);
+            for (uint32_t s = 0; s < n; ++s) {
+                if (sid[s] == j) {
+                    float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
+                    shot[3 * (size_t)j + 0] += d0;
+                    shot[3 * (size_t)j + 1] += d1;
+                    shot[3 * (size_t)j + 2] += d2;
+                    u0 -= d0;
+                    u1 -= d1;
+                    u2 -= d2;
                 }
             }
         }
-        for (uint64_t m = nz; m; m &= m - 1)
-            ff[(size_t)(g0 + (uint32_t)__ffsll((long long)m) - 1) * npatch + j] = 0.0f;
-    }
-    if (!live)
-        return;
-    if (!last) {
         unshot[3 * (size_t)j + 0] = u0;
         unshot[3 * (size_t)j + 1] = u1;
         unshot[3 * (size_t)j + 2] = u2;
+    }
+    if (!lists)
         return;
+    // sel_key(RAD_SELECT_POWER) of the state just stored
+    uint64_t key = 0ull;
+    if (live) {
+        float e = area[j] * (u0 + u1 + u2);
+        if (!(e > 0.0f))
+            e = 0.0f;
+        key = ((uint64_t)__float_as_uint(e) << 32) | (0xFFFFFFFFu - j);
     }
-    // Main.cpp:1287-1295
-    if (n && sid[n - 1] == j)
-        stop_test(ctl, u0, u1, u2);
-    for (uint32_t s = 0; s < n; ++s) {
-        if (sid[s] == j) {
-            float d0 = snap[3 * s + 0], d1 = snap[3 * s + 1], d2 = snap[3 * s + 2];
-            shot[3 * (size_t)j + 0] += d0;
-            shot[3 * (size_t)j + 1] += d1;
-            shot[3 * (size_t)j + 2] += d2;
-            u0 -= d0;
-            u1 -= d1;
-            u2 -= d2;
-        }
+    uint64_t top = wave_sort_desc(key);
+    wl[w][lane] = top;
+    for (uint32_t h = NT / 128; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (w < h)
+            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
+        __syncthreads();
+        if (w < h)
+            wl[w][lane] = top;
     }
-    unshot[3 * (size_t)j + 0] = u0;
-    unshot[3 * (size_t)j + 1] = u1;
-    unshot[3 * (size_t)j + 2] = u2;
+    if (w == 0)
+        lists[(size_t)blockIdx.x * 64 + lane] = top;
+}
+
+uint32_t shoot_lists(uint32_t npatch)
+{
+    return (npatch + kShootListThreads - 1) / kShootListThreads;
 }
 
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
                   const float *snap, float *ff, uint32_t *fblk, const float *refl,
                   const float *area, uint32_t npatch, float *unshot, float *shot, uint32_t *ctl,
-                  uint32_t sbeg, uint32_t send, bool last, hipStream_t st)
+                  uint32_t sbeg, uint32_t send, bool last, uint64_t *lists, hipStream_t st)
 {
-    k_shoot<<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk, K,
-                                                  refl, area, npatch, unshot, shot, ctl, sbeg,
-                                                  send, last ? 1u : 0u);
+    if (lists)
+        k_shoot<kShootListThreads><<<shoot_lists(npatch), kShootListThreads, 0, st>>>(
+            mode, shooters, nshoot, snap, ff, fblk, K, refl, area, npatch, unshot, shot, ctl,
+            sbeg, send, last ? 1u : 0u, lists);
+    else
+        k_shoot<256><<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk,
+                                                          K, refl, area, npatch, unshot, shot,
+                                                          ctl, sbeg, send, last ? 1u : 0u, lists);
 }
+
 
 // ----------------------------------------------------------------- selection
 // Single-shooter selection, one packed 64-bit key per patch reduced with
@@ -2737,102 +2873,6 @@ __global__ __launch_bounds__(256) void k_topk_pass(const float *__restrict__ uns
 // is skipped), each workgroup merges its wave lists in LDS, then one
 // workgroup merges the workgroup lists.  The keys are distinct (they hold
 // the patch index), so the result does not depend on the merge order.
-// Lane exchanges of the top-K networks without ds_bpermute: lane i gets the
-// value of lane i ^ M, from DPP lane moves (strides 1, 2, 4, 8 inside a row
-// of 16 lanes) and the gfx950 row and half-wave swaps (16, 32).
-template <int M> __device__ __forceinline__ uint32_t lane_xor(uint32_t x)
-{
-    const int lane = threadIdx.x & 63;
-    const int xi = (int)x;
-    if constexpr (M == 1) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0xB1, 0xF, 0xF, false); // quad_perm 1,0,3,2
-    } else if constexpr (M == 2) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0x4E, 0xF, 0xF, false); // quad_perm 2,3,0,1
-    } else if constexpr (M == 4) {
-        const int up = __builtin_amdgcn_update_dpp(xi, xi, 0x104, 0xF, 0xF, false); // row_shl:4, lane i + 4
-        const int dn = __builtin_amdgcn_update_dpp(xi, xi, 0x114, 0xF, 0xF, false); // row_shr:4, lane i - 4
-        return (uint32_t)((lane & 4) ? dn : up);
-    } else if constexpr (M == 8) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(xi, xi, 0x128, 0xF, 0xF, false); // row_ror:8
-    } else if constexpr (M == 16) {
-        // odd rows of the first operand swap with even rows of the second:
-        // r[0] holds rows (0, 0, 2, 2) of x, r[1] rows (1, 1, 3, 3)
-        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
-        return (lane & 16) ? r[0] : r[1];
-    } else {
-        static_assert(M == 32, "lane_xor strides 1 .. 32");
-        // the upper half of the first operand swaps with the lower half of
-        // the second: r[0] holds x's lower half twice, r[1] its upper half
-        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
-        return (lane & 32) ? r[0] : r[1];
-    }
-}
-
-template <int M> __device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v)
-{
-    return ((uint64_t)lane_xor<M>((uint32_t)(v >> 32)) << 32) | lane_xor<M>((uint32_t)v);
-}
-
-// The value of lane 63 - i: each row mirrored (row_mirror), then rows
-// r and 3 - r exchanged.
-__device__ __forceinline__ uint64_t lane_reverse_u64(uint64_t v)
-{
-    const int lo = __builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), 0x140, 0xF, 0xF, false);
-    const uint64_t m = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-    return lane_xor_u64<32>(lane_xor_u64<16>(m));
-}
-
-// A wave-uniform lane's value (lane src, a constant).
-__device__ __forceinline__ uint64_t lane_read_u64(uint64_t v, int src)
-{
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// One compare-exchange step of a bitonic network at lane stride M.
-template <int M> __device__ __forceinline__ uint64_t bitonic_step(uint64_t v, bool take_max)
-{
-    const uint64_t o = lane_xor_u64<M>(v);
-    return take_max ? (v > o ? v : o) : (v < o ? v : o);
-}
-
-__device__ __forceinline__ uint64_t wave_sort_desc(uint64_t v)
-{
-    const int lane = threadIdx.x & 63;
-    // stage `size`: lanes in blocks of size sort descending where
-    // lane & size == 0; a step at stride m keeps the max in the lower lane
-    // of a pair when that block is descending
-#define RAD_SORT_STEP(size_, m_) \
-    v = bitonic_step<m_>(v, ((lane & (m_)) == 0) == ((lane & (size_)) == 0));
-    RAD_SORT_STEP(2, 1)
-    RAD_SORT_STEP(4, 2) RAD_SORT_STEP(4, 1)
-    RAD_SORT_STEP(8, 4) RAD_SORT_STEP(8, 2) RAD_SORT_STEP(8, 1)
-    RAD_SORT_STEP(16, 8) RAD_SORT_STEP(16, 4) RAD_SORT_STEP(16, 2) RAD_SORT_STEP(16, 1)
-    RAD_SORT_STEP(32, 16) RAD_SORT_STEP(32, 8) RAD_SORT_STEP(32, 4) RAD_SORT_STEP(32, 2)
-    RAD_SORT_STEP(32, 1)
-    RAD_SORT_STEP(64, 32) RAD_SORT_STEP(64, 16) RAD_SORT_STEP(64, 8) RAD_SORT_STEP(64, 4)
-    RAD_SORT_STEP(64, 2) RAD_SORT_STEP(64, 1)
-#undef RAD_SORT_STEP
-    return v;
-}
-
-// top and b sorted descending over the lanes: the top 64 of both, sorted.
-__device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
-{
-    const int lane = threadIdx.x & 63;
-    const uint64_t r = lane_reverse_u64(b);
-    uint64_t v = top > r ? top : r;
-    v = bitonic_step<32>(v, (lane & 32) == 0);
-    v = bitonic_step<16>(v, (lane & 16) == 0);
-    v = bitonic_step<8>(v, (lane & 8) == 0);
-    v = bitonic_step<4>(v, (lane & 4) == 0);
-    v = bitonic_step<2>(v, (lane & 2) == 0);
-    v = bitonic_step<1>(v, (lane & 1) == 0);
-    return v;
-}
-
 constexpr uint32_t kTopkScanWaves = 8;
 
 __global__ __launch_bounds__(kTopkScanWaves * 64) void k_topk64_scan(const float *__restrict__ unshot,
@@ -2962,6 +3002,14 @@ void launch_select_topk(uint32_t rule, const float *unshot, const float *area, c
         n = nb * k;
         which ^= 1;
     }
+}
+
+// The selection of a pass whose previous shoot left per-workgroup lists
+// (k_shoot with lists): their final merge only.
+void launch_select_merge(const uint64_t *lists, uint32_t nlists, uint32_t k, uint32_t *shooters,
+                         uint32_t *nshoot, hipStream_t st)
+{
+    k_topk64_merge<<<1, 1024, 0, st>>>(lists, nlists, nlists, k, nullptr, shooters, nshoot);
 }
 
 size_t topk_cand_half(uint32_t npatch, uint32_t k)
