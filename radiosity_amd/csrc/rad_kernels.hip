@@ -1563,19 +1563,27 @@ __device__ static inline void draw_wave32(const RasterRec &r, int32_t vpx, int32
 // integer below 2^24.  Every bbox pixel issues its ds_min_u64 (a pixel
 // outside the triangle or the depth range with the neutral key ~0), which
 // ran faster than branching around the atomic.
+// Rows ay0 + ph, ay0 + ph + nph, ... of the bbox (row phase ph of nph; the
+// caller's other lanes draw the other phases).
 __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t vpy,
                                         int32_t ax0, int32_t ax1, int32_t ay0, int32_t ay1,
-                                        int32_t ox, int32_t oy, uint32_t tshift, uint64_t *tk)
+                                        int32_t ox, int32_t oy, uint32_t tshift, uint64_t *tk,
+                                        uint32_t ph, uint32_t nph)
 {
     int32_t xmin = min(r.x[0], min(r.x[1], r.x[2])), xmax = max(r.x[0], max(r.x[1], r.x[2]));
     int32_t ymin = min(r.y[0], min(r.y[1], r.y[2])), ymax = max(r.y[0], max(r.y[1], r.y[2]));
     if (xmax - xmin >= (64 << 8) || ymax - ymin >= (64 << 8)) {
+        if (ph != 0)
+            return; // phase 0 draws these (rare) records whole
         Edges e;
         make_edges(r.x, r.y, e);
         uint32_t w = (uint32_t)(ax1 - ax0 + 1), area = w * (uint32_t)(ay1 - ay0 + 1);
         draw_span(r, e, vpx, vpy, ax0, ay0, w, area, 0, 1, ox, oy, tshift, tk);
         return;
     }
+    ay0 += (int32_t)ph;
+    if (ay0 > ay1)
+        return;
     const int32_t px0 = (ax0 - vpx) * kSub + kSub / 2, py0 = (ay0 - vpy) * kSub + kSub / 2;
     int32_t v[3], dA[3], dB[3];
 #pragma unroll
@@ -1586,13 +1594,14 @@ __device__ static inline void draw_lane(const RasterRec &r, int32_t vpx, int32_t
         // E(p) = A (p.x - x_k) + B (p.y - y_k); inside iff E + bias - 1 >= 0
         v[k] = A * (px0 - r.x[k]) + B * (py0 - r.y[k]) + bias - 1;
         dA[k] = A * kSub;
-        dB[k] = B * kSub;
+        dB[k] = B * kSub * (int32_t)nph;
     }
     int32_t py = py0;
-    const uint32_t w = (uint32_t)(ax1 - ax0 + 1), rstride = 1u << tshift;
+    const uint32_t w = (uint32_t)(ax1 - ax0 + 1), rstride = nph << tshift;
     const float fx0 = (float)(px0 - r.x[0]);
     uint32_t prow = (uint32_t)((ay0 - oy) << tshift) + (uint32_t)(ax0 - ox);
-    for (int32_t ty = ay0; ty <= ay1; ++ty, py += kSub, prow += rstride) {
+    for (int32_t ty = ay0; ty <= ay1;
+         ty += (int32_t)nph, py += kSub * (int32_t)nph, prow += rstride) {
         float zr = fmaf(r.dzdy, (float)(py - r.y[0]), r.z0);
         int32_t e0 = v[0], e1 = v[1], e2 = v[2];
         float fx = fx0;
@@ -1679,12 +1688,26 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const int32_t ox = (int32_t)((tile % ntx) << tshift), oy = (int32_t)((tile / ntx) << tshift);
     const int32_t Wi = (int32_t)W;
     const uint32_t me = threadIdx.x;
-    // A lane's entry of the first chunk (list index, bbox and record, the
-    // record loaded with the bbox, not after the size test) is fetched while
-    // the tile clears; each later chunk's list entries are fetched during the
-    // chunk before it.
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(me >> 6);
     constexpr uint32_t kNoRec = ~0u;
-    uint32_t ri = me < n && st0 + me < ent_cap ? list[st0 + me] : kNoRec;
+    // Lane me's record of the chunk starting at entry cb.  A chunk of m <
+    // 256 records would leave waves idle: its records are dealt to
+    // ceil(m / 64) wave sets (one per lane), and the nph = 4 / sets waves of a
+    // set each draw row phase ph of its in-lane records (every nph-th bbox
+    // row); a full chunk keeps one record per lane, nph = 1.
+    auto deal = [&](uint32_t cb, uint32_t &ph, uint32_t &nph) -> uint32_t {
+        const uint32_t m = min(kTileThreads, n - cb);
+        const uint32_t sets = (m + 63u) >> 6;
+        nph = sets >= 3u ? 1u : 4u / sets;
+        ph = wv / sets;
+        const uint32_t k = (wv - ph * sets) * 64u + (me & 63u);
+        return ph < nph && k < m && st0 + cb + k < ent_cap ? list[st0 + cb + k] : kNoRec;
+    };
+    // A lane's entry of the first chunk (list index and bbox) is fetched
+    // while the tile clears; each later chunk's list entries are fetched
+    // during the chunk before it.
+    uint32_t ph = 0, nph = 1;
+    uint32_t ri = n ? deal(0u, ph, nph) : kNoRec;
     uint64_t bv = 0;
     if (ri != kNoRec)
         bv = bbox[ri];
@@ -1693,8 +1716,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     for (uint32_t c0 = 0; c0 < n; c0 += blockDim.x) {
         if (me == 0)
             nbig = 0u;
-        const uint32_t inext = c0 + blockDim.x + me;
-        const uint32_t rnext = inext < n && st0 + inext < ent_cap ? list[st0 + inext] : kNoRec;
+        uint32_t ph_n = 0, nph_n = 1;
+        const uint32_t rnext = c0 + blockDim.x < n ? deal(c0 + blockDim.x, ph_n, nph_n) : kNoRec;
         __syncthreads();
         if (ri != kNoRec) {
             const int32_t ax0 = max((int32_t)(bv & 0xFFFFu), ox);
@@ -1705,8 +1728,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             if (area <= small_px) {
                 const RasterRec r = recs[ri];
                 FaceView fv = face_view(Wi, (int)(r.sf & 7u));
-                draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk);
-            } else {
+                draw_lane(r, fv.vpx, fv.vpy, ax0, ax1, ay0, ay1, ox, oy, tshift, tk, ph, nph);
+            } else if (ph == 0) {
                 const uint32_t q = atomicAdd(&nbig, 1u);
                 perm[q] = ri;
                 pbb[q] = bv;
@@ -1735,6 +1758,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             }
         }
         ri = rnext;
+        ph = ph_n;
+        nph = nph_n;
         if (ri != kNoRec)
             bv = bbox[ri];
         __syncthreads();
