@@ -1987,6 +1987,141 @@ __global__ __launch_bounds__(256) void k_bin_fill_pipe(const RasterRec *__restri
     }
 }
 
+// k_bin_fill for up to 256 tiles per hemicube, all of a workgroup's shooter
+// slots (at most 8) at once: the region records of every slot are counted
+// per (slot, tile) in LDS, one returning global atomic per touched (slot,
+// tile) reserves the list space (all of a thread's issued before any is
+// used), and the records are placed.  Three barriers and one atomic round
+// trip per workgroup, where walking the slots one by one takes one round
+// trip per slot.  A thread keeps the bboxes of its first kFillKeep records
+// in registers for the placement and loads the others again.
+constexpr uint32_t kFillSlots = 8, kFillTiles = 256, kFillKeep = 4;
+
+__global__ __launch_bounds__(256) void k_bin_fill_all(const RasterRec *__restrict__ recs,
+                                                      const uint64_t *__restrict__ bbox,
+                                                      const uint32_t *__restrict__ blk_nrec,
+                                                      const uint32_t *__restrict__ counters,
+                                                      uint32_t nblk, uint32_t spill_base,
+                                                      uint32_t spill_cap,
+                                                      const uint32_t *__restrict__ start,
+                                                      uint32_t *__restrict__ fill,
+                                                      uint32_t *__restrict__ list,
+                                                      uint32_t ent_cap, uint32_t ntx,
+                                                      uint32_t ntiles, uint32_t tshift,
+                                                      const uint32_t *__restrict__ nshoot,
+                                                      uint32_t group, uint32_t slot0,
+                                                      uint32_t nslot)
+{
+    __shared__ uint32_t lc[kFillSlots * kFillTiles], lb[kFillSlots * kFillTiles];
+    __shared__ uint32_t off[kFillSlots + 1];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ns = *nshoot < nslot ? *nshoot : nslot;
+    const uint32_t sbeg = slot0 + blockIdx.y * group, send = min(ns, sbeg + group);
+    if (blockIdx.x == nblk) { // the last column: the spill records
+        if (blockIdx.z == 0)
+            bin_spill(recs, bbox, counters, spill_base, spill_cap, start, fill, list, ent_cap, ntx,
+                      ntiles, tshift, sbeg, send);
+        return;
+    }
+    if (sbeg >= send)
+        return;
+    const uint32_t nsl = send - sbeg;
+    auto region = [&](uint32_t sl) {
+        return ((sbeg + sl) * gridDim.z + blockIdx.z) * nblk + blockIdx.x;
+    };
+    if (tid < 64) {
+        // the slots' record counts and their running offsets
+        const uint32_t c = tid < nsl ? blk_nrec[region(tid)] : 0u;
+        uint32_t inc = c;
+#pragma unroll
+        for (int d = 1; d < (int)kFillSlots; d <<= 1) {
+            const uint32_t o = __shfl_up(inc, d, 64);
+            if ((int)tid >= d)
+                inc += o;
+        }
+        if (tid < kFillSlots)
+            off[tid + 1] = inc;
+        if (tid == 0)
+            off[0] = 0u;
+    }
+    for (uint32_t i = tid; i < nsl * ntiles; i += 256u)
+        lc[i] = 0u;
+    __syncthreads();
+    const uint32_t total = off[nsl];
+    // record f of the workgroup: slot sl (off[sl] <= f < off[sl + 1]) and
+    // its index q in the slot's region
+    auto locate = [&](uint32_t f, uint32_t &sl, uint32_t &q) {
+        sl = 0;
+        while (sl + 1 < nsl && f >= off[sl + 1])
+            ++sl;
+        q = f - off[sl];
+    };
+    uint64_t keep[kFillKeep];
+#pragma unroll
+    for (uint32_t k = 0; k < kFillKeep; ++k)
+        keep[k] = 0ull;
+    uint32_t k = 0;
+    for (uint32_t f = tid; f < total; f += 256u, ++k) {
+        uint32_t sl, q;
+        locate(f, sl, q);
+        const uint64_t v = bbox[(size_t)region(sl) * kRecsPerBlock + q];
+#pragma unroll
+        for (uint32_t j = 0; j < kFillKeep; ++j)
+            if (j == k)
+                keep[j] = v;
+        uint32_t g0, g1, h0, h1;
+        unpack_bbox(v, tshift, g0, g1, h0, h1);
+        for (uint32_t gy = h0; gy <= h1; ++gy)
+            for (uint32_t gx = g0; gx <= g1; ++gx)
+                atomicAdd(&lc[sl * ntiles + gy * ntx + gx], 1u);
+    }
+    __syncthreads();
+    // reserve: every count of this thread goes out before any result is used
+    constexpr uint32_t kPer = kFillSlots * kFillTiles / 256u;
+    uint32_t rb[kPer], st[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = tid + 256u * r;
+        const uint32_t c = i < nsl * ntiles ? lc[i] : 0u;
+        const uint32_t bin = sbeg * ntiles + i;
+        rb[r] = c ? atomicAdd(&fill[bin], c) : 0u;
+        st[r] = c ? start[bin] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t i = tid + 256u * r;
+        if (i < nsl * ntiles) {
+            lb[i] = st[r] + rb[r];
+            lc[i] = 0u;
+        }
+    }
+    __syncthreads();
+    k = 0;
+    for (uint32_t f = tid; f < total; f += 256u, ++k) {
+        uint32_t sl, q;
+        locate(f, sl, q);
+        const uint32_t rr = region(sl) * kRecsPerBlock + q;
+        uint64_t v = 0ull;
+        if (k < kFillKeep) {
+#pragma unroll
+            for (uint32_t j = 0; j < kFillKeep; ++j)
+                if (j == k)
+                    v = keep[j];
+        } else {
+            v = bbox[rr];
+        }
+        uint32_t g0, g1, h0, h1;
+        unpack_bbox(v, tshift, g0, g1, h0, h1);
+        for (uint32_t gy = h0; gy <= h1; ++gy)
+            for (uint32_t gx = g0; gx <= g1; ++gx) {
+                const uint32_t b = sl * ntiles + gy * ntx + gx;
+                const uint32_t pos = lb[b] + atomicAdd(&lc[b], 1u);
+                if (pos < ent_cap)
+                    list[pos] = rr;
+            }
+    }
+}
+
 void launch_raster_setup(const Geometry &g, uint32_t s0, uint32_t s1, uint32_t h,
                          const float *tri, const float4 *tsph, const uint32_t *tri_patch,
                          uint32_t ntri, const float *mvp, const uint32_t *nshoot,
@@ -2039,7 +2174,11 @@ void launch_bin(uint32_t s0, uint32_t s1, uint32_t h, const uint32_t *nshoot,
         return;
     }
     const size_t lds = 4 * bb.ntiles * sizeof(uint32_t);
-    if (bb.ntiles <= 256)
+    if (bb.ntiles <= kFillTiles && group <= kFillSlots)
+        k_bin_fill_all<<<grid, 256, 0, st>>>(
+            bb.recs, bb.bbox, bb.blk_nrec, ctr, bb.nblk, spill_base, bb.spill_cap, bb.start,
+            bb.fill, list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, s0, s1);
+    else if (bb.ntiles <= 256)
         k_bin_fill_pipe<1><<<grid, 256, lds, st>>>(
             bb.recs, bb.bbox, bb.blk_nrec, ctr, bb.nblk, spill_base, bb.spill_cap, bb.start,
             bb.fill, list, bb.ent_cap, bb.ntx, bb.ntiles, bb.tshift, nshoot, group, s0, s1);
