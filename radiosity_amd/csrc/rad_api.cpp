@@ -117,15 +117,18 @@ static uint32_t split_at(uint32_t k) { return k / 2; }
 // behind for a later pass that skips its clear).
 int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *depth,
                 uint32_t *hits, const uint32_t *shooters = nullptr, const float *frames = nullptr,
-                bool shoot_half = false)
+                bool shoot_half = false, bool prepared = false)
 {
     uint32_t rows = c->ff_dirty;
     if (hits && rows < k)
         rows = k;
-    launch_prepare_clear(k, shooters ? shooters : c->shooters, c->nshoot,
-                         frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->bb.count,
-                         c->bb.ntiles, c->ff_dirty ? c->ff : nullptr, hits, rows, c->rowlen,
-                         c->counters, c->bb.ent_cap, c->ctl, c->st);
+    // prepared: the previous pass's last shoot already prepared this pass
+    // and cleared its bins and counters (no F rows or hits to clear)
+    if (!prepared)
+        launch_prepare_clear(k, shooters ? shooters : c->shooters, c->nshoot,
+                             frames ? frames : c->frames, c->unshot, c->snap, c->mvp, c->bb.count,
+                             c->bb.ntiles, c->ff_dirty ? c->ff : nullptr, hits, rows, c->rowlen,
+                             c->counters, c->bb.ent_cap, c->ctl, c->st);
     c->ff_dirty = 0;
     record(c, evbase + 2);
     const bool fq = !ids && c->det;
@@ -228,10 +231,13 @@ static bool fused_select(const rad_ctx *c)
            shoot_lists(c->npatch) <= 256;
 }
 
-// One pass: select (lists_in: only merge the lists the previous pass's shoot
-// left), render, shoot (lists_out: leave the lists for the next pass).
-int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool lists_in = false,
-             bool lists_out = false)
+// One pass: select, render, shoot.  chain_in: the previous pass already
+// selected and prepared this one (no selection or prepare launch);
+// chain_out: this pass's last shoot leaves per-workgroup lists of the POWER
+// keys of the state it stores, and k_next_pass merges them into the next
+// pass's shooters and prepares that pass.
+int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool chain_in = false,
+             bool chain_out = false)
 {
     int rc;
     record(c, evbase + 0);
@@ -239,11 +245,7 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool lists_in = false,
         launch_pack_select(c->pk, c->unshot, c->area, c->st);
         if ((rc = check_launch(c, "pack select")))
             return rc;
-    } else if (do_select && lists_in) {
-        launch_select_merge(c->cand, shoot_lists(c->npatch), c->K, c->shooters, c->nshoot, c->st);
-        if ((rc = check_launch(c, "select merge")))
-            return rc;
-    } else if (do_select && (rc = select_dev(c, c->K)) != RAD_OK) {
+    } else if (do_select && !chain_in && (rc = select_dev(c, c->K)) != RAD_OK) {
         return rc;
     }
     record(c, evbase + 1);
@@ -251,8 +253,8 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool lists_in = false,
     // rendering (not for packs, nor with the fixed-point rows, whose fold
     // comes after both halves)
     const bool half = !c->pk.nscene && !c->det && split_render(c, c->K);
-    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr, nullptr, nullptr, nullptr, half)) !=
-        RAD_OK)
+    if ((rc = render_pass(c, c->K, evbase, nullptr, nullptr, nullptr, nullptr, nullptr, half,
+                          chain_in)) != RAD_OK)
         return rc;
     if (c->pk.nscene)
         launch_pack_shoot(c->pk, c->cfg.mode, c->snap, c->ff, c->rowlen, c->refl, c->area,
@@ -261,7 +263,13 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool lists_in = false,
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff,
                      c->det ? nullptr : c->fblk, c->refl, c->area, c->npatch, c->unshot,
                      c->shot, c->ctl, half ? split_at(c->K) : 0u, c->K, true,
-                     lists_out ? c->cand : nullptr, c->st);
+                     chain_out ? c->cand : nullptr, c->st);
+    if (chain_out) {
+        const NextPass np{c->K,    c->shooters, c->nshoot,   c->frames,
+                          c->snap, c->mvp,      c->bb.count, (size_t)c->K * c->bb.ntiles,
+                          c->counters, c->bb.ent_cap};
+        launch_next_pass(c->cand, shoot_lists(c->npatch), np, c->unshot, c->ctl, c->st);
+    }
     c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
     c->rendered = false;
@@ -928,8 +936,7 @@ static int capture_graph(rad_ctx *c, uint32_t n)
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     int rc = RAD_OK;
     // inside a replay nothing else touches the state between passes, so
-    // every pass after the first merges the lists its predecessor's shoot
-    // left
+    // every pass but the last selects and prepares its successor
     const bool fuse = fused_select(c);
     for (uint32_t p = 0; p < n && rc == RAD_OK; ++p)
         rc = one_pass(c, true, 0, fuse && p > 0, fuse && p + 1 < n);

@@ -181,6 +181,18 @@ void launch_runs_count(const Geometry &g, uint32_t k, uint32_t nx, const uint32_
 void launch_runs_write(const Geometry &g, uint32_t k, uint32_t nx, const uint32_t *item,
                        const float *dff, const uint32_t *offs, uint32_t cap, uint32_t *th,
                        uint32_t *tid, float *te, hipStream_t st);
+// What k_next_pass selects into and prepares for the pass after one whose
+// last shoot left per-workgroup lists.
+struct NextPass {
+    uint32_t K;
+    uint32_t *shooters, *nshoot;
+    const float *frames;
+    float *snap, *mvp;
+    uint32_t *bin_count;
+    size_t nbins;
+    uint32_t *counters;
+    uint32_t ent_cap;
+};
 // fblk (nullable): the F-block flags, [row / 64][slot] with K slots per
 // block; null reads every F entry.  Adds the F rows of shooters
 // [sbeg, send); the last launch of a pass also moves the shooters' energy,
@@ -191,11 +203,11 @@ void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
                   float *shot, uint32_t *ctl, uint32_t sbeg, uint32_t send, bool last,
                   uint64_t *lists, hipStream_t st);
+void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np,
+                      const float *unshot, const uint32_t *ctl, hipStream_t st);
 // Per-workgroup selection lists a last shoot leaves (lists != null): their
-// count for npatch receivers, and the next selection's merge of them.
+// count for npatch receivers.
 uint32_t shoot_lists(uint32_t npatch);
-void launch_select_merge(const uint64_t *lists, uint32_t nlists, uint32_t k, uint32_t *shooters,
-                         uint32_t *nshoot, hipStream_t st);
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);

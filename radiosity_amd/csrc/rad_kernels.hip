@@ -2624,6 +2624,59 @@ __device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
     return v;
 }
 
+// The 64 largest keys of the sorted 64-key lists [l0, l1) by the
+// workgroup's waves: each wave merges every nw-th list, loading 16 of them
+// at a time (a list whose largest key is not above the 64th of the wave's
+// running list cannot contribute),
+// then the wave lists are merged pairwise through wl; wave 0 returns the
+// result.  Every thread of the workgroup calls it.
+__device__ static uint64_t merge_lists_block(const uint64_t *__restrict__ in, uint32_t l0,
+                                             uint32_t l1, uint64_t (*wl)[64])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr int kLoads = 16; // lists a wave loads before merging them
+    uint64_t top = 0, thr = 0;
+    for (uint32_t lb = l0 + w; lb < l1; lb += (uint32_t)nw * kLoads) {
+        uint64_t v[kLoads];
+#pragma unroll
+        for (int q = 0; q < kLoads; ++q) {
+            const uint32_t l = lb + (uint32_t)(q * nw);
+            v[q] = l < l1 ? in[(size_t)l * 64 + lane] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < kLoads; ++q) {
+            if (lane_read_u64(v[q], 0) <= thr)
+                continue;
+            top = wave_merge_desc(top, v[q]);
+            thr = lane_read_u64(top, 63);
+        }
+    }
+    wl[w][lane] = top;
+    for (int h = nw >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (w < h)
+            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
+        __syncthreads();
+        if (w < h)
+            wl[w][lane] = top;
+    }
+    return top;
+}
+
+// Wave 0 of a merge: the patches of the k largest POWER keys (a zero key
+// fills no slot) and their count.
+__device__ static void write_shooters(uint64_t top, uint32_t k, uint32_t *__restrict__ shooters,
+                                      uint32_t *__restrict__ nshoot)
+{
+    const int lane = threadIdx.x & 63;
+    const bool ok = (uint32_t)lane < k && (top >> 32) != 0;
+    if ((uint32_t)lane < k)
+        shooters[lane] = ok ? 0xFFFFFFFFu - (uint32_t)top : 0u;
+    const uint64_t m = __ballot(ok);
+    if (lane == 0)
+        *nshoot = (uint32_t)__popcll(m);
+}
+
 // Main.cpp:1251-1295 for all receivers at once: one thread per receiver j
 // accumulates the contributions of the pass's shooters, then the shooters
 // move their snapshot from unshot to shot.  Every nonzero F entry it reads
@@ -2810,6 +2863,36 @@ void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uin
         k_shoot<256><<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk,
                                                           K, refl, area, npatch, unshot, shot,
                                                           ctl, sbeg, send, last ? 1u : 0u, lists);
+}
+
+// The start of the pass after one whose last shoot left per-workgroup lists,
+// in one workgroup: the selection (the merge of the lists into the K
+// largest POWER keys, as the last level of k_topk64_merge) and what k_prepare_clear does (the stop gate, the shooters' snapshots and
+// MVPs, bin counts and pass counters cleared; no F rows or hits, which the
+// shoot leaves zero).
+__global__ __launch_bounds__(1024) void k_next_pass(const uint64_t *__restrict__ lists,
+                                                    uint32_t nlists, NextPass np,
+                                                    const float *__restrict__ unshot,
+                                                    const uint32_t *__restrict__ ctl)
+{
+    __shared__ uint64_t wl[16][64];
+    const uint64_t best = merge_lists_block(lists, 0u, nlists, wl);
+    if (threadIdx.x < 64)
+        write_shooters(best, np.K, np.shooters, np.nshoot);
+    __syncthreads();
+    if (threadIdx.x == 0 && ctl[kCtlStop])
+        *np.nshoot = 0u;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < np.K * 5; t += blockDim.x)
+        prepare_item(t, np.K, np.shooters, np.nshoot, np.frames, unshot, np.snap, np.mvp, ctl);
+    clear_range(threadIdx.x, blockDim.x, np.bin_count, np.nbins, nullptr, nullptr, 0, np.counters,
+                np.ent_cap);
+}
+
+void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np,
+                      const float *unshot, const uint32_t *ctl, hipStream_t st)
+{
+    k_next_pass<<<1, 1024, 0, st>>>(lists, nlists, np, unshot, ctl);
 }
 
 
@@ -3080,41 +3163,15 @@ __global__ __launch_bounds__(1024) void k_topk64_merge(const uint64_t *__restric
                                                        uint32_t *__restrict__ nshoot)
 {
     __shared__ uint64_t wl[16][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint32_t l0 = blockIdx.x * per_block;
     const uint32_t l1 = l0 + per_block < nlists ? l0 + per_block : nlists;
-    uint64_t top = 0, thr = 0;
-    // a list whose largest key is not above the 64th of the running list
-    // cannot contribute
-    for (uint32_t l = l0 + w; l < l1; l += nw) {
-        const uint64_t v = in[(size_t)l * 64 + lane];
-        if (lane_read_u64(v, 0) <= thr)
-            continue;
-        top = wave_merge_desc(top, v);
-        thr = lane_read_u64(top, 63);
-    }
-    wl[w][lane] = top;
-    // the 16 wave lists: pairwise merge levels in LDS
-    for (int h = nw >> 1; h >= 1; h >>= 1) {
-        __syncthreads();
-        if (w < h)
-            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
-        __syncthreads();
-        if (w < h)
-            wl[w][lane] = top;
-    }
-    if (w != 0)
+    const uint64_t top = merge_lists_block(in, l0, l1, wl);
+    if ((threadIdx.x >> 6) != 0)
         return;
-    if (shooters) {
-        bool ok = (uint32_t)lane < k && (top >> 32) != 0;
-        if ((uint32_t)lane < k)
-            shooters[lane] = ok ? 0xFFFFFFFFu - (uint32_t)top : 0u;
-        uint64_t m = __ballot(ok);
-        if (lane == 0)
-            *nshoot = (uint32_t)__popcll(m);
-    } else {
-        out[(size_t)blockIdx.x * 64 + lane] = top;
-    }
+    if (shooters)
+        write_shooters(top, k, shooters, nshoot);
+    else
+        out[(size_t)blockIdx.x * 64 + (threadIdx.x & 63)] = top;
 }
 
 void launch_select_topk(uint32_t rule, const float *unshot, const float *area, const float *emis,
@@ -3166,14 +3223,6 @@ void launch_select_topk(uint32_t rule, const float *unshot, const float *area, c
         n = nb * k;
         which ^= 1;
     }
-}
-
-// The selection of a pass whose previous shoot left per-workgroup lists
-// (k_shoot with lists): their final merge only.
-void launch_select_merge(const uint64_t *lists, uint32_t nlists, uint32_t k, uint32_t *shooters,
-                         uint32_t *nshoot, hipStream_t st)
-{
-    k_topk64_merge<<<1, 1024, 0, st>>>(lists, nlists, nlists, k, nullptr, shooters, nshoot);
 }
 
 size_t topk_cand_half(uint32_t npatch, uint32_t k)
