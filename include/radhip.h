@@ -252,6 +252,17 @@ int rad_debug_bins(rad_ctx *ctx, uint32_t *out_or_null, uint32_t *ntiles, uint32
  * capacities: [3], [5] and [6] are then the fuller half's, [4] the sum. */
 int rad_debug_counters(rad_ctx *ctx, uint32_t *out8, uint32_t *out_caps_or_null);
 
+/* Diagnostics (no counterpart in the reference): the candidate pool of the
+ * POWER selection in captured passes.  Inside a graph, a pass's selection
+ * may take the K largest keys from a pool of every receiver whose key is at
+ * least a bound, which the previous pass's shoot collected; it does so only
+ * when the pool holds K keys (then they are exact) and at most `cap`, else
+ * it scans every patch.  cap != 0 sets that most (1 .. 2048, default 2048)
+ * and drops the captured graph.  out4 (optional, synchronises) gets the
+ * selections taken from the pool, the full scans, the times the bound was
+ * set, and whether it is set now, counted since the context was created. */
+int rad_debug_pool(rad_ctx *ctx, uint32_t cap, uint32_t *out4_or_null);
+
 /* Autotuner of the rasterizer: times `passes` render passes of K shooters
  * spread evenly over the patches (render + reduce, no shooting, state
  * unchanged; the device selection is overwritten) for each screen tile

@@ -126,6 +126,7 @@ SIGNATURES = {
                                   C.POINTER(C.POINTER(rad_scene))]),
     "rad_debug_bins": (C.c_int, [_P, _PU, _PU, _PU]),
     "rad_debug_counters": (C.c_int, [_P, _PU, _PU]),
+    "rad_debug_pool": (C.c_int, [_P, _u32, _PU]),
     "rad_autotune": (C.c_int, [_P, _u32, C.POINTER(rad_tune_result)]),
     "rad_pack_upload": (C.c_int, [_P, _u32, _PU, _PU, _PU, _PF, _PU, _u32, _u32, _PF, _PF, _PF]),
     "rad_pack_begin": (C.c_int, [_P, C.c_double]),
@@ -505,6 +506,15 @@ class Solver:
         d = {k: int(v) for k, v in zip(names, out)}
         d.update(spill_cap=int(caps[0]), entry_cap=int(caps[1]), clip_cap=int(caps[2]))
         return d
+
+    def debug_pool(self, cap=0):
+        """Candidate pool of the POWER selection in captured passes: cap != 0
+        sets the most keys a selection takes from it (else it scans every
+        patch); returns the selections taken from the pool, the full scans,
+        the times its bound was set and whether it is set now."""
+        out = np.zeros(4, np.uint32)
+        self._chk(lib().rad_debug_pool(self._ctx, int(cap), uptr(out)), "rad_debug_pool")
+        return dict(hits=int(out[0]), scans=int(out[1]), arms=int(out[2]), armed=int(out[3]))
 
     def kernel_times(self, enable=-1):
         out = (C.c_double * 6)()

@@ -5,6 +5,7 @@
 // synchronisation between passes: select -> prepare -> clear -> raster setup
 // -> raster tiles -> reduce -> shoot; the raster stages of a pass of 16 or
 // more slots run as two halves on the context's two streams.
+#include <cstddef>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -164,7 +165,7 @@ int render_pass(rad_ctx *c, uint32_t k, size_t evbase, uint32_t *ids, float *dep
         if (shoot_half)
             launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff, c->fblk,
                          c->refl, c->area, c->npatch, c->unshot, c->shot, c->ctl, 0, split_at(k),
-                         false, nullptr, c->st);
+                         false, nullptr, nullptr, c->st);
         HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
         HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
     } else {
@@ -234,10 +235,13 @@ static bool fused_select(const rad_ctx *c)
 // One pass: select, render, shoot.  chain_in: the previous pass already
 // selected and prepared this one (no selection or prepare launch);
 // chain_out: this pass's last shoot leaves per-workgroup lists of the POWER
-// keys of the state it stores, and k_next_pass merges them into the next
-// pass's shooters and prepares that pass.
+// keys of the state it stores, or, in a chain (chain_in too) whose pool is
+// armed, pools the keys at or above the pool's bound; k_next_pass then
+// selects the next pass's shooters and prepares that pass.  target: the
+// keys of the state that the pool's bound must leave above it when
+// k_next_pass arms it from lists (0: no arming).
 int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool chain_in = false,
-             bool chain_out = false)
+             bool chain_out = false, uint32_t target = 0)
 {
     int rc;
     record(c, evbase + 0);
@@ -263,12 +267,14 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool chain_in = false,
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff,
                      c->det ? nullptr : c->fblk, c->refl, c->area, c->npatch, c->unshot,
                      c->shot, c->ctl, half ? split_at(c->K) : 0u, c->K, true,
-                     chain_out ? c->cand : nullptr, c->st);
+                     chain_out ? c->cand : nullptr, chain_out && chain_in ? c->pool : nullptr,
+                     c->st);
     if (chain_out) {
         const NextPass np{c->K,    c->shooters, c->nshoot,   c->frames,
                           c->snap, c->mvp,      c->bb.count, (size_t)c->K * c->bb.ntiles,
                           c->counters, c->bb.ent_cap};
-        launch_next_pass(c->cand, shoot_lists(c->npatch), np, c->unshot, c->ctl, c->st);
+        launch_next_pass(c->cand, shoot_lists(c->npatch), np, c->unshot, c->area, c->npatch,
+                         c->ctl, c->pool, !chain_in, target, c->pool_cap, c->st);
     }
     c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
@@ -371,12 +377,14 @@ int rad_create(const rad_config *cfg, rad_ctx **out)
         (rc = dalloc(c, &c->mvp, (size_t)c->K * kShooterStride)) || (rc = dalloc(c, &c->snap, (size_t)c->K * 3)) ||
         (rc = dalloc(c, &c->shooters, c->K)) || (rc = dalloc(c, &c->nshoot, 1)) ||
         (rc = dalloc(c, &c->counters, kCntN * kCntBlocks)) || (rc = dalloc(c, &c->sel, 4)) ||
-        (rc = dalloc(c, &c->resid, 8)) || (rc = dalloc(c, &c->ctl, kCtlN))) {
+        (rc = dalloc(c, &c->resid, 8)) || (rc = dalloc(c, &c->ctl, kCtlN)) ||
+        (rc = dalloc(c, &c->pool, 1))) {
         rad_destroy(c);
         return rc;
     }
     if (hipMemsetAsync(c->counters, 0, kCntN * kCntBlocks * sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->ctl, 0, kCtlN * sizeof(uint32_t), c->st) != hipSuccess ||
+        hipMemsetAsync(c->pool, 0, sizeof(SelPool), c->st) != hipSuccess ||
         hipMemsetAsync(c->nshoot, 0, sizeof(uint32_t), c->st) != hipSuccess ||
         hipMemsetAsync(c->shooters, 0, c->K * sizeof(uint32_t), c->st) != hipSuccess) {
         rad_destroy(c);
@@ -432,7 +440,7 @@ void rad_destroy(rad_ctx *c)
                     c->pk.cand, c->pk.patch_scene, c->pk.pass, c->pk.slot_tbeg, c->pk.slot_tcnt,
                     c->pk.slot_pbeg, c->pk.sc_w, c->pk.sc_dffo, c->pk.slot_w, c->pk.slot_dffo,
                     c->pk.dffs, c->patch_quad, c->quad_pbeg, c->quad_nbr, c->qval,
-                    c->viewfr, c->view_sh};
+                    c->viewfr, c->view_sh, c->pool};
     for (void *p : ptrs)
         dfree(p);
     for (hipEvent_t e : c->ev)
@@ -936,10 +944,18 @@ static int capture_graph(rad_ctx *c, uint32_t n)
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     int rc = RAD_OK;
     // inside a replay nothing else touches the state between passes, so
-    // every pass but the last selects and prepares its successor
+    // every pass but the last selects and prepares its successor.  When the
+    // selection of pass p + 1 (after pass p's shoot) arms the pool from
+    // lists, the shoots of passes p + 1 .. n - 2 pool for passes
+    // p + 2 .. n - 1; each of those passes shoots
+    // K of the keys at or above the bound while the others only grow (all
+    // additions are of non-negative terms): (n - 1 - p) K keys at or above
+    // it keep every pool at K or more.  Whatever the bound, a pool is used
+    // only when it holds K keys.
     const bool fuse = fused_select(c);
     for (uint32_t p = 0; p < n && rc == RAD_OK; ++p)
-        rc = one_pass(c, true, 0, fuse && p > 0, fuse && p + 1 < n);
+        rc = one_pass(c, true, 0, fuse && p > 0, fuse && p + 1 < n,
+                      fuse && p + 2 < n ? (n - 1 - p) * c->K : 0u);
     hipError_t e = hipStreamEndCapture(c->st, &graph);
     if (rc)
         return rc;
@@ -1343,6 +1359,29 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
         return rc2;
     if (out)
         *out = r;
+    return RAD_OK;
+}
+
+int rad_debug_pool(rad_ctx *c, uint32_t cap, uint32_t *out4)
+{
+    if (!c || cap > kPoolLds)
+        return RAD_E_ARG;
+    if (cap) {
+        c->pool_cap = cap;
+        drop_graph(c); // the cap is an argument of the captured kernels
+    }
+    if (out4) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        // the header: tau (two words), armed, hits, falls, arms
+        static_assert(offsetof(SelPool, armed) == 8 && offsetof(SelPool, arms) == 20,
+                      "pool header layout");
+        uint32_t head[6];
+        HIPCHK(c, hipMemcpy(head, c->pool, sizeof(head), hipMemcpyDeviceToHost));
+        out4[0] = head[3];
+        out4[1] = head[4];
+        out4[2] = head[5];
+        out4[3] = head[2];
+    }
     return RAD_OK;
 }
 

@@ -193,6 +193,20 @@ struct NextPass {
     uint32_t *counters;
     uint32_t ent_cap;
 };
+// Candidate pool of a chained POWER selection (device memory).  When armed,
+// a pass's last shoot keeps every receiver whose key is at least tau:
+// count keys, the first kPoolLds of them in keys (one global atomic per
+// shoot workgroup places its keys).  The pool holds every key >= tau, so
+// when it has at least K keys its K largest are the state's; the next pass
+// then selects from it, else it scans every patch.
+constexpr uint32_t kPoolLds = 2048; // most pool keys a selection takes
+struct SelPool {
+    uint64_t tau;
+    uint32_t armed;             // the next last shoot pools (else leaves lists)
+    uint32_t hits, falls, arms; // selections from the pool, full scans, armings
+    uint32_t count;             // keys the last shoot pooled (reset by k_next_pass)
+    uint64_t keys[kPoolLds];
+};
 // fblk (nullable): the F-block flags, [row / 64][slot] with K slots per
 // block; null reads every F entry.  Adds the F rows of shooters
 // [sbeg, send); the last launch of a pass also moves the shooters' energy,
@@ -202,9 +216,16 @@ void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
                   const uint32_t *nshoot, const float *snap, float *ff, uint32_t *fblk,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
                   float *shot, uint32_t *ctl, uint32_t sbeg, uint32_t send, bool last,
-                  uint64_t *lists, hipStream_t st);
+                  uint64_t *lists, SelPool *pool, hipStream_t st);
+// The selection and preparation of a pass after one whose last shoot left
+// lists or pooled (pool->armed, unless lists_only).  From lists, with
+// target != 0, it arms the pool for the shoots that follow: tau is set so
+// that at least `target` keys of the state are >= tau.  pool_cap (at most
+// kPoolLds): the most pool keys taken before scanning every patch instead.
 void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np,
-                      const float *unshot, const uint32_t *ctl, hipStream_t st);
+                      const float *unshot, const float *area, uint32_t npatch,
+                      const uint32_t *ctl, SelPool *pool, bool lists_only, uint32_t target,
+                      uint32_t pool_cap, hipStream_t st);
 // Per-workgroup selection lists a last shoot leaves (lists != null): their
 // count for npatch receivers.
 uint32_t shoot_lists(uint32_t npatch);

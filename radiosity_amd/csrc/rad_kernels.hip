@@ -2624,6 +2624,24 @@ __device__ __forceinline__ uint64_t wave_merge_desc(uint64_t top, uint64_t b)
     return v;
 }
 
+// The sorted 64-key lists of the workgroup's waves (top) merged pairwise
+// through wl; wave 0 returns the 64 largest keys of all of them.  Every
+// thread of the workgroup calls it.
+__device__ static uint64_t merge_wave_lists(uint64_t top, uint64_t (*wl)[64])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    wl[w][lane] = top;
+    for (int h = nw >> 1; h >= 1; h >>= 1) {
+        __syncthreads();
+        if (w < h)
+            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
+        __syncthreads();
+        if (w < h)
+            wl[w][lane] = top;
+    }
+    return top;
+}
+
 // The 64 largest keys of the sorted 64-key lists [l0, l1) by the
 // workgroup's waves: each wave merges every nw-th list, loading 16 of them
 // at a time (a list whose largest key is not above the 64th of the wave's
@@ -2651,16 +2669,7 @@ __device__ static uint64_t merge_lists_block(const uint64_t *__restrict__ in, ui
             thr = lane_read_u64(top, 63);
         }
     }
-    wl[w][lane] = top;
-    for (int h = nw >> 1; h >= 1; h >>= 1) {
-        __syncthreads();
-        if (w < h)
-            top = wave_merge_desc(wl[w][lane], wl[w + h][lane]);
-        __syncthreads();
-        if (w < h)
-            wl[w][lane] = top;
-    }
-    return top;
+    return merge_wave_lists(top, wl);
 }
 
 // Wave 0 of a merge: the patches of the k largest POWER keys (a zero key
@@ -2685,8 +2694,10 @@ __device__ static void write_shooters(uint64_t top, uint32_t k, uint32_t *__rest
 // POWER), each workgroup also leaves the sorted 64 largest selection keys
 // of its 1,024 receivers' new state in lists[blockIdx.x * 64 + lane], the
 // per-workgroup lists k_topk64_scan would make from the stored state, so
-// the next pass's selection only merges them.  NT threads per workgroup:
-// 1,024 with lists (one merge level for up to 262,144 receivers), else 256.
+// the next pass's selection only merges them; or, with the pool armed, it
+// adds every receiver whose key is at least the pool's tau to the pool
+// instead.  NT threads per workgroup: 1,024 with lists (one merge
+// level for up to 262,144 receivers), else 256.
 constexpr uint32_t kShootListThreads = 1024;
 
 template <uint32_t NT>
@@ -2700,11 +2711,16 @@ __global__ __launch_bounds__(NT) void k_shoot(uint32_t mode, const uint32_t *__r
                                                float *__restrict__ unshot,
                                                float *__restrict__ shot, uint32_t *__restrict__ ctl,
                                                uint32_t sbeg, uint32_t send, uint32_t last,
-                                               uint64_t *__restrict__ lists)
+                                               uint64_t *__restrict__ lists,
+                                               SelPool *__restrict__ pool)
 {
     __shared__ uint32_t sid[RAD_MAX_K];
     __shared__ float sc[RAD_MAX_K * 4];
     __shared__ uint64_t wl[NT / 64][64];
+    __shared__ uint32_t npool, gbase;
+    const uint32_t pooled = lists && pool ? pool->armed : 0u;
+    if (threadIdx.x == 0)
+        npool = 0u;
     uint32_t n = *nshoot;
     for (uint32_t s = threadIdx.x; s < n; s += blockDim.x) {
         uint32_t i = shooters[s];
@@ -2831,6 +2847,29 @@ __global__ __launch_bounds__(NT) void k_shoot(uint32_t mode, const uint32_t *__r
             e = 0.0f;
         key = ((uint64_t)__float_as_uint(e) << 32) | (0xFFFFFFFFu - j);
     }
+    if (pooled) {
+        // every receiver whose key is at least tau (a positive power) goes
+        // to this workgroup's pool region
+        const uint64_t tau = pool->tau;
+        const bool c = live && key >= tau;
+        const uint64_t m = __ballot(c);
+        uint32_t local = 0u;
+        if (m) {
+            uint32_t base = 0u;
+            if (lane == 0)
+                base = atomicAdd(&npool, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, 0);
+            local = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            gbase = npool ? atomicAdd(&pool->count, npool) : 0u;
+        __syncthreads();
+        const uint32_t idx = gbase + local;
+        if (c && idx < kPoolLds)
+            pool->keys[idx] = key;
+        return;
+    }
     uint64_t top = wave_sort_desc(key);
     wl[w][lane] = top;
     for (uint32_t h = NT / 128; h >= 1; h >>= 1) {
@@ -2853,46 +2892,217 @@ uint32_t shoot_lists(uint32_t npatch)
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
                   const float *snap, float *ff, uint32_t *fblk, const float *refl,
                   const float *area, uint32_t npatch, float *unshot, float *shot, uint32_t *ctl,
-                  uint32_t sbeg, uint32_t send, bool last, uint64_t *lists, hipStream_t st)
+                  uint32_t sbeg, uint32_t send, bool last, uint64_t *lists, SelPool *pool,
+                  hipStream_t st)
 {
     if (lists)
         k_shoot<kShootListThreads><<<shoot_lists(npatch), kShootListThreads, 0, st>>>(
             mode, shooters, nshoot, snap, ff, fblk, K, refl, area, npatch, unshot, shot, ctl,
-            sbeg, send, last ? 1u : 0u, lists);
+            sbeg, send, last ? 1u : 0u, lists, pool);
     else
         k_shoot<256><<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk,
                                                           K, refl, area, npatch, unshot, shot,
-                                                          ctl, sbeg, send, last ? 1u : 0u, lists);
+                                                          ctl, sbeg, send, last ? 1u : 0u, lists,
+                                                          nullptr);
 }
 
-// The start of the pass after one whose last shoot left per-workgroup lists,
-// in one workgroup: the selection (the merge of the lists into the K
-// largest POWER keys, as the last level of k_topk64_merge) and what k_prepare_clear does (the stop gate, the shooters' snapshots and
-// MVPs, bin counts and pass counters cleared; no F rows or hits, which the
-// shoot leaves zero).
+__device__ static inline uint64_t sel_key(uint32_t rule, const float *unshot, const float *area,
+                                          uint32_t j);
+
+// The 64 largest of n unsorted keys in LDS: each wave sorts every nw-th
+// batch of 64 that holds a key above its running 64th and merges it in;
+// wave 0 returns the result.  Every thread of the workgroup calls it.
+__device__ static uint64_t select_keys_block(const uint64_t *pk, uint32_t n, uint64_t (*wl)[64])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint64_t top = 0, thr = 0;
+    for (uint32_t b = (uint32_t)w * 64; b < n; b += (uint32_t)nw * 64) {
+        const uint64_t key = b + lane < n ? pk[b + lane] : 0ull;
+        if (__ballot(key > thr) == 0)
+            continue;
+        top = wave_merge_desc(top, wave_sort_desc(key));
+        thr = lane_read_u64(top, 63);
+    }
+    return merge_wave_lists(top, wl);
+}
+
+// The 64 largest POWER keys of all npatch patches, by one workgroup (the
+// pool's fallback): as k_topk64_scan, four batches loaded at a time.
+__device__ static uint64_t select_all_block(const float *__restrict__ unshot,
+                                            const float *__restrict__ area, uint32_t npatch,
+                                            uint64_t (*wl)[64])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t step = (uint32_t)nw * 64;
+    uint64_t top = 0, thr = 0;
+    for (uint32_t base = (uint32_t)w * 64; base < npatch; base += 4 * step) {
+        uint64_t kk[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t j = base + (uint32_t)q * step + lane;
+            kk[q] = j < npatch ? sel_key(RAD_SELECT_POWER, unshot, area, j) : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (__ballot(kk[q] > thr) == 0)
+                continue;
+            top = wave_merge_desc(top, wave_sort_desc(kk[q]));
+            thr = lane_read_u64(top, 63);
+        }
+    }
+    return merge_wave_lists(top, wl);
+}
+
+// Arms the pool after the merge of nlists sorted lists whose 64th largest
+// key is t64.  The lists hold a subset of the state's keys, so at least as
+// many keys of the state as of the lists are >= any tau.  tau steps down
+// from t64 by 1/8 octave of power (2^20 in the float bits of the key's
+// upper word), at most 8 octaves, until the lists hold `target` keys >= tau
+// (t64 itself when target <= 64).  A tau of zero power is never armed.
+// hist: 64 words of LDS.  Every thread of the workgroup calls it.
+__device__ static void arm_pool(const uint64_t *__restrict__ in, uint32_t nlists, uint64_t t64,
+                                uint32_t target, SelPool *__restrict__ pool, uint32_t *hist)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t th = (uint32_t)(t64 >> 32);
+    if (threadIdx.x < 64)
+        hist[threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t l = (uint32_t)w; l < nlists; l += (uint32_t)nw) {
+        const uint64_t e = in[(size_t)l * 64 + lane];
+        const uint32_t eh = (uint32_t)(e >> 32);
+        // bucket (th - eh) >> 20 of the keys below t64 within 8 octaves
+        // (64: none)
+        uint32_t b = 64u;
+        if (e < t64 && eh != 0u && th - eh < (64u << 20))
+            b = (th - eh) >> 20;
+        // a list is sorted, so a bucket's keys are a run of lanes: the run's
+        // first lane adds its length
+        const uint32_t prev = (uint32_t)__shfl_up((int)b, 1);
+        const bool start = lane == 0 || prev != b;
+        const uint64_t starts = __ballot(start);
+        if (start && b < 64u) {
+            const uint64_t after = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+            const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+            atomicAdd(&hist[b], next - (uint32_t)lane);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t armed = 0u;
+        uint64_t tau = 0ull;
+        if (th != 0u) {
+            uint32_t c = 64u; // the merged 64 are >= t64
+            if (c >= target) {
+                tau = t64;
+                armed = 1u;
+            }
+            for (uint32_t b = 0; b < 64u && !armed; ++b) {
+                c += hist[b];
+                if (c >= target) {
+                    const uint32_t drop = (b + 1u) << 20;
+                    if (th > drop) {
+                        tau = (uint64_t)(th - drop + 1u) << 32;
+                        armed = 1u;
+                    }
+                    break;
+                }
+            }
+        }
+        if (armed) {
+            pool->tau = tau;
+            pool->arms += 1u;
+        }
+        pool->armed = armed;
+    }
+}
+
+// The start of the pass after one whose last shoot left per-workgroup lists
+// or pooled, in one workgroup: the selection and what k_prepare_clear does
+// (the stop gate, the shooters' snapshots and MVPs, bin counts and pass
+// counters cleared; no F rows or hits, which the shoot leaves zero).
+// The selection takes the K largest POWER keys (as k_topk64_merge):
+//  - pooled: from the pool when it holds K .. pool_cap keys, which are then
+//    exactly the keys >= tau, so their K largest are the state's; otherwise
+//    from a scan of every patch, and the pool is disarmed (the next shoot
+//    leaves lists);
+//  - else: the merge of the lists, then the pool armed for `target` keys
+//    (target 0: disarmed).
 __global__ __launch_bounds__(1024) void k_next_pass(const uint64_t *__restrict__ lists,
                                                     uint32_t nlists, NextPass np,
                                                     const float *__restrict__ unshot,
-                                                    const uint32_t *__restrict__ ctl)
+                                                    const float *__restrict__ area,
+                                                    uint32_t npatch,
+                                                    const uint32_t *__restrict__ ctl,
+                                                    SelPool *__restrict__ pool,
+                                                    uint32_t lists_only, uint32_t target,
+                                                    uint32_t pool_cap)
 {
     __shared__ uint64_t wl[16][64];
-    const uint64_t best = merge_lists_block(lists, 0u, nlists, wl);
-    if (threadIdx.x < 64)
-        write_shooters(best, np.K, np.shooters, np.nshoot);
+    __shared__ uint32_t hist[64], ssel[64], sn;
+    __shared__ uint64_t t64s;
+    const int lane = threadIdx.x & 63;
+    const uint32_t t = threadIdx.x;
+    const uint32_t stop = ctl[kCtlStop];
+    const uint32_t pooled = lists_only ? 0u : pool->armed;
+    uint64_t best;
+    if (pooled) {
+        const uint32_t n = pool->count;
+        if (n >= np.K && n <= pool_cap) {
+            best = select_keys_block(pool->keys, n, wl);
+            if (t == 0)
+                pool->hits += 1u;
+        } else {
+            best = select_all_block(unshot, area, npatch, wl);
+            if (t == 0) {
+                pool->falls += 1u;
+                pool->armed = 0u;
+            }
+        }
+    } else {
+        best = merge_lists_block(lists, 0u, nlists, wl);
+        if (target) {
+            if (t == 0)
+                t64s = lane_read_u64(best, 63);
+            __syncthreads();
+            arm_pool(lists, nlists, t64s, target, pool, hist);
+        } else if (t == 0) {
+            pool->armed = 0u;
+        }
+    }
+    // wave 0: the shooters (as write_shooters) into memory and LDS; the
+    // stop gate of k_prepare_clear sets the count to 0
+    if (t < 64) {
+        const bool ok = (uint32_t)lane < np.K && (best >> 32) != 0;
+        const uint32_t id = ok ? 0xFFFFFFFFu - (uint32_t)best : 0u;
+        if ((uint32_t)lane < np.K) {
+            np.shooters[lane] = id;
+            ssel[lane] = id;
+        }
+        const uint32_t m = stop ? 0u : (uint32_t)__popcll(__ballot(ok));
+        if (lane == 0) {
+            *np.nshoot = m;
+            sn = m;
+        }
+    }
     __syncthreads();
-    if (threadIdx.x == 0 && ctl[kCtlStop])
-        *np.nshoot = 0u;
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < np.K * 5; t += blockDim.x)
-        prepare_item(t, np.K, np.shooters, np.nshoot, np.frames, unshot, np.snap, np.mvp, ctl);
-    clear_range(threadIdx.x, blockDim.x, np.bin_count, np.nbins, nullptr, nullptr, 0, np.counters,
+    // every read of the pool count is done: the next shoot pools from zero
+    if (t == 0)
+        pool->count = 0u;
+    for (uint32_t i = t; i < np.K * 5; i += blockDim.x)
+        prepare_item(i, np.K, ssel, &sn, np.frames, unshot, np.snap, np.mvp, nullptr);
+    clear_range(t, blockDim.x, np.bin_count, np.nbins, nullptr, nullptr, 0, np.counters,
                 np.ent_cap);
 }
 
 void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np,
-                      const float *unshot, const uint32_t *ctl, hipStream_t st)
+                      const float *unshot, const float *area, uint32_t npatch,
+                      const uint32_t *ctl, SelPool *pool, bool lists_only, uint32_t target,
+                      uint32_t pool_cap, hipStream_t st)
 {
-    k_next_pass<<<1, 1024, 0, st>>>(lists, nlists, np, unshot, ctl);
+    k_next_pass<<<1, 1024, 0, st>>>(lists, nlists, np, unshot, area, npatch, ctl, pool,
+                                    lists_only ? 1u : 0u, target,
+                                    pool_cap < kPoolLds ? pool_cap : kPoolLds);
 }
 
 

@@ -1,0 +1,67 @@
+"""The chained POWER selection of captured passes against eager passes.
+
+Inside a graph, a pass's selection either merges the per-workgroup lists
+the previous shoot left or takes the K largest keys from the pool of every
+receiver at or above a bound (when the pool holds K .. cap keys), else it
+scans every patch.  Each must select exactly what the eager selection (a
+scan of the whole state) selects: with the deterministic F sums, the state
+after the same passes is the same bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def boxes_mid(rad):
+    return rad.Scene.boxes(nboxes=120, room=(12.0, 4.0, 12.0), target_tris=60000, nlights=8,
+                           seed=5)
+
+
+def _run(rad, sc, graph, passes, cap=0):
+    s = rad.Solver(hemicube=32, hemicubes=64, graph_passes=graph)
+    s.set_deterministic(True)
+    s.upload(sc)
+    if cap:
+        s.debug_pool(cap)
+    s.step(passes)
+    s.finish()
+    u, sh = s.state()
+    return u.view(np.uint32).copy(), sh.view(np.uint32).copy(), s.debug_pool()
+
+
+@pytest.mark.parametrize("cap", [0, 64])
+def test_chained_selection_bit_exact(rad, boxes_mid, cap):
+    """40 passes replayed from graphs of 8 against 40 eager passes, K = 64,
+    on 60k patches.  cap 0 keeps the default pool (most selections after
+    the first two of a graph come from it); cap 64 sends every pooled
+    selection that finds more than 64 keys to the scan of all patches."""
+    sc = boxes_mid
+    say(f"chained selection: {sc.npatch} patches, K=64, cap={cap}")
+    ue, se, _ = _run(rad, sc, 0, 40)
+    ug, sg, st = _run(rad, sc, 8, 40, cap)
+    say(f"  pool: {st}")
+    assert np.array_equal(ue, ug)
+    assert np.array_equal(se, sg)
+    if cap:
+        assert st["scans"] > 0
+    else:
+        assert st["hits"] > 0
+    assert st["arms"] > 0
+
+
+def test_pool_cap_bounds(rad, boxes_mid):
+    """rad_debug_pool rejects a cap beyond 2,048 and reads the counters."""
+    s = rad.Solver(hemicube=32, hemicubes=64, graph_passes=8)
+    s.upload(boxes_mid)
+    with pytest.raises(Exception):
+        s.debug_pool(4096)
+    st = s.debug_pool()
+    assert st == dict(hits=0, scans=0, arms=0, armed=0)
