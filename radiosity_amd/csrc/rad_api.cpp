@@ -239,9 +239,10 @@ static bool fused_select(const rad_ctx *c)
 // armed, pools the keys at or above the pool's bound; k_next_pass then
 // selects the next pass's shooters and prepares that pass.  target: the
 // keys of the state that the pool's bound must leave above it when
-// k_next_pass arms it from lists (0: no arming).
+// k_next_pass arms it from lists (0: no arming).  relist: this pass's shoot
+// leaves lists whatever the pool's state, and the next selection re-arms.
 int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool chain_in = false,
-             bool chain_out = false, uint32_t target = 0)
+             bool chain_out = false, uint32_t target = 0, bool relist = false)
 {
     int rc;
     record(c, evbase + 0);
@@ -267,14 +268,14 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool chain_in = false,
         launch_shoot(c->cfg.mode, c->K, c->shooters, c->nshoot, c->snap, c->ff,
                      c->det ? nullptr : c->fblk, c->refl, c->area, c->npatch, c->unshot,
                      c->shot, c->ctl, half ? split_at(c->K) : 0u, c->K, true,
-                     chain_out ? c->cand : nullptr, chain_out && chain_in ? c->pool : nullptr,
-                     c->st);
+                     chain_out ? c->cand : nullptr,
+                     chain_out && chain_in && !relist ? c->pool : nullptr, c->st);
     if (chain_out) {
         const NextPass np{c->K,    c->shooters, c->nshoot,   c->frames,
                           c->snap, c->mvp,      c->bb.count, (size_t)c->K * c->bb.ntiles,
                           c->counters, c->bb.ent_cap};
         launch_next_pass(c->cand, shoot_lists(c->npatch), np, c->unshot, c->area, c->npatch,
-                         c->ctl, c->pool, !chain_in, target, c->pool_cap, c->st);
+                         c->ctl, c->pool, !chain_in || relist, target, c->pool_cap, c->st);
     }
     c->ff_dirty = 0; // the shoot kernels zero the entries they read
     record(c, evbase + 6);
@@ -944,18 +945,21 @@ static int capture_graph(rad_ctx *c, uint32_t n)
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     int rc = RAD_OK;
     // inside a replay nothing else touches the state between passes, so
-    // every pass but the last selects and prepares its successor.  When the
-    // selection of pass p + 1 (after pass p's shoot) arms the pool from
-    // lists, the shoots of passes p + 1 .. n - 2 pool for passes
-    // p + 2 .. n - 1; each of those passes shoots
-    // K of the keys at or above the bound while the others only grow (all
-    // additions are of non-negative terms): (n - 1 - p) K keys at or above
+    // every pass but the last selects and prepares its successor.  Every
+    // kPoolRearm passes (from the first) a pass's shoot leaves lists and the
+    // next selection arms the pool from them.  Armed after pass p's shoot,
+    // the pool serves the selections of passes p + 2 .. q (q: the pass
+    // before the next such shoot's successor, at most n - 1); each pass
+    // shoots K keys at or above the bound while the others only grow (all
+    // additions are of non-negative terms), so (q - p) K keys at or above
     // it keep every pool at K or more.  Whatever the bound, a pool is used
     // only when it holds K keys.
     const bool fuse = fused_select(c);
-    for (uint32_t p = 0; p < n && rc == RAD_OK; ++p)
+    for (uint32_t p = 0; p < n && rc == RAD_OK; ++p) {
+        const uint32_t q = std::min(kPoolRearm * (p / kPoolRearm + 1), n - 1);
         rc = one_pass(c, true, 0, fuse && p > 0, fuse && p + 1 < n,
-                      fuse && p + 2 < n ? (n - 1 - p) * c->K : 0u);
+                      fuse && q >= p + 2 ? (q - p) * c->K : 0u, p % kPoolRearm == 0);
+    }
     hipError_t e = hipStreamEndCapture(c->st, &graph);
     if (rc)
         return rc;

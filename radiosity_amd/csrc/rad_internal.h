@@ -200,6 +200,7 @@ struct NextPass {
 // when it has at least K keys its K largest are the state's; the next pass
 // then selects from it, else it scans every patch.
 constexpr uint32_t kPoolLds = 2048; // most pool keys a selection takes
+constexpr uint32_t kPoolRearm = 8;  // passes of a graph between armings
 struct SelPool {
     uint64_t tau;
     uint32_t armed;             // the next last shoot pools (else leaves lists)
