@@ -274,7 +274,8 @@ int one_pass(rad_ctx *c, bool do_select, size_t evbase, bool chain_in = false,
         const NextPass np{c->K,    c->shooters, c->nshoot,   c->frames,
                           c->snap, c->mvp,      c->bb.count, (size_t)c->K * c->bb.ntiles,
                           c->counters, c->bb.ent_cap};
-        launch_next_pass(c->cand, shoot_lists(c->npatch), np, c->unshot, c->area, c->npatch,
+        launch_next_pass(c->cand, shoot_lists(c->npatch, chain_in && !relist), np, c->unshot,
+                         c->area, c->npatch,
                          c->ctl, c->pool, !chain_in || relist, target, c->pool_cap, c->st);
     }
     c->ff_dirty = 0; // the shoot kernels zero the entries they read

@@ -212,7 +212,8 @@ struct SelPool {
 // block; null reads every F entry.  Adds the F rows of shooters
 // [sbeg, send); the last launch of a pass also moves the shooters' energy,
 // runs the stop test and counts the pass, and with lists != null leaves the
-// next POWER selection's per-workgroup lists of the new state.
+// next POWER selection's per-workgroup lists of the new state, or, given
+// the pool and when it is armed, adds the keys at or above its bound to it.
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
                   const uint32_t *nshoot, const float *snap, float *ff, uint32_t *fblk,
                   const float *refl, const float *area, uint32_t npatch, float *unshot,
@@ -228,8 +229,8 @@ void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np
                       const uint32_t *ctl, SelPool *pool, bool lists_only, uint32_t target,
                       uint32_t pool_cap, hipStream_t st);
 // Per-workgroup selection lists a last shoot leaves (lists != null): their
-// count for npatch receivers.
-uint32_t shoot_lists(uint32_t npatch);
+// count for npatch receivers, when the shoot is given the pool or not.
+uint32_t shoot_lists(uint32_t npatch, bool pooled = false);
 void launch_residual(const float *unshot, const float *area, const float *emis,
                      uint32_t npatch, double *partials, uint32_t *counter, double *resid,
                      int nblocks, hipStream_t st);

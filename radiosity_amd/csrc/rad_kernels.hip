@@ -2697,7 +2697,9 @@ __device__ static void write_shooters(uint64_t top, uint32_t k, uint32_t *__rest
 // the next pass's selection only merges them; or, with the pool armed, it
 // adds every receiver whose key is at least the pool's tau to the pool
 // instead.  NT threads per workgroup: 1,024 with lists (one merge
-// level for up to 262,144 receivers), else 256.
+// level for up to 262,144 receivers); 256 without lists, and with lists
+// when the pool is expected (then lists, left only when the pool is not
+// armed, cover 256 receivers each).
 constexpr uint32_t kShootListThreads = 1024;
 
 template <uint32_t NT>
@@ -2884,9 +2886,10 @@ __global__ __launch_bounds__(NT) void k_shoot(uint32_t mode, const uint32_t *__r
         lists[(size_t)blockIdx.x * 64 + lane] = top;
 }
 
-uint32_t shoot_lists(uint32_t npatch)
+uint32_t shoot_lists(uint32_t npatch, bool pooled)
 {
-    return (npatch + kShootListThreads - 1) / kShootListThreads;
+    const uint32_t nt = pooled ? 256u : kShootListThreads;
+    return (npatch + nt - 1) / nt;
 }
 
 void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uint32_t *nshoot,
@@ -2895,15 +2898,15 @@ void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uin
                   uint32_t sbeg, uint32_t send, bool last, uint64_t *lists, SelPool *pool,
                   hipStream_t st)
 {
-    if (lists)
-        k_shoot<kShootListThreads><<<shoot_lists(npatch), kShootListThreads, 0, st>>>(
+    if (lists && !pool)
+        k_shoot<kShootListThreads><<<shoot_lists(npatch, false), kShootListThreads, 0, st>>>(
             mode, shooters, nshoot, snap, ff, fblk, K, refl, area, npatch, unshot, shot, ctl,
-            sbeg, send, last ? 1u : 0u, lists, pool);
+            sbeg, send, last ? 1u : 0u, lists, nullptr);
     else
         k_shoot<256><<<(npatch + 255) / 256, 256, 0, st>>>(mode, shooters, nshoot, snap, ff, fblk,
                                                           K, refl, area, npatch, unshot, shot,
                                                           ctl, sbeg, send, last ? 1u : 0u, lists,
-                                                          nullptr);
+                                                          pool);
 }
 
 __device__ static inline uint64_t sel_key(uint32_t rule, const float *unshot, const float *area,
