@@ -257,7 +257,7 @@ int rad_debug_counters(rad_ctx *ctx, uint32_t *out8, uint32_t *out_caps_or_null)
  * may take the K largest keys from a pool of every receiver whose key is at
  * least a bound, which the previous pass's shoot collected; it does so only
  * when the pool holds K keys (then they are exact) and at most `cap`, else
- * it scans every patch.  cap != 0 sets that most (1 .. 2048, default 2048)
+ * it scans every patch.  cap != 0 sets that most (1 .. 16384, default 16384)
  * and drops the captured graph.  out4 (optional, synchronises) gets the
  * selections taken from the pool, the full scans, the times the bound was
  * set, and whether it is set now, counted since the context was created. */

@@ -1369,7 +1369,7 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
 
 int rad_debug_pool(rad_ctx *c, uint32_t cap, uint32_t *out4)
 {
-    if (!c || cap > kPoolLds)
+    if (!c || cap > kPoolCap)
         return RAD_E_ARG;
     if (cap) {
         c->pool_cap = cap;

@@ -75,7 +75,7 @@ struct rad_ctx {
     BinBuffers bb{};               // sort-middle rasterizer bins
     uint64_t *cand = nullptr;      // top-K candidates (two halves)
     rad::SelPool *pool = nullptr;  // candidate pool of chained selections
-    uint32_t pool_cap = rad::kPoolLds;
+    uint32_t pool_cap = rad::kPoolCap;
     uint32_t small_px = 16;        // in-lane rasterization limit (pixels of a tile)
     int sel_blocks = 1024;
     bool rendered = false;        // item_ids hold the item buffer of the last render

@@ -195,18 +195,18 @@ struct NextPass {
 };
 // Candidate pool of a chained POWER selection (device memory).  When armed,
 // a pass's last shoot keeps every receiver whose key is at least tau:
-// count keys, the first kPoolLds of them in keys (one global atomic per
+// count keys, the first kPoolCap of them in keys (one global atomic per
 // shoot workgroup places its keys).  The pool holds every key >= tau, so
 // when it has at least K keys its K largest are the state's; the next pass
 // then selects from it, else it scans every patch.
-constexpr uint32_t kPoolLds = 2048; // most pool keys a selection takes
+constexpr uint32_t kPoolCap = 16384; // most pool keys a selection takes
 constexpr uint32_t kPoolRearm = 8;  // passes of a graph between armings
 struct SelPool {
     uint64_t tau;
     uint32_t armed;             // the next last shoot pools (else leaves lists)
     uint32_t hits, falls, arms; // selections from the pool, full scans, armings
     uint32_t count;             // keys the last shoot pooled (reset by k_next_pass)
-    uint64_t keys[kPoolLds];
+    uint64_t keys[kPoolCap];
 };
 // fblk (nullable): the F-block flags, [row / 64][slot] with K slots per
 // block; null reads every F entry.  Adds the F rows of shooters
@@ -223,7 +223,7 @@ void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters,
 // lists or pooled (pool->armed, unless lists_only).  From lists, with
 // target != 0, it arms the pool for the shoots that follow: tau is set so
 // that at least `target` keys of the state are >= tau.  pool_cap (at most
-// kPoolLds): the most pool keys taken before scanning every patch instead.
+// kPoolCap): the most pool keys taken before scanning every patch instead.
 void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np,
                       const float *unshot, const float *area, uint32_t npatch,
                       const uint32_t *ctl, SelPool *pool, bool lists_only, uint32_t target,

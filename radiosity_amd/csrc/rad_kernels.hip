@@ -2868,7 +2868,7 @@ __global__ __launch_bounds__(NT) void k_shoot(uint32_t mode, const uint32_t *__r
             gbase = npool ? atomicAdd(&pool->count, npool) : 0u;
         __syncthreads();
         const uint32_t idx = gbase + local;
-        if (c && idx < kPoolLds)
+        if (c && idx < kPoolCap)
             pool->keys[idx] = key;
         return;
     }
@@ -2912,19 +2912,31 @@ void launch_shoot(uint32_t mode, uint32_t K, const uint32_t *shooters, const uin
 __device__ static inline uint64_t sel_key(uint32_t rule, const float *unshot, const float *area,
                                           uint32_t j);
 
-// The 64 largest of n unsorted keys in LDS: each wave sorts every nw-th
-// batch of 64 that holds a key above its running 64th and merges it in;
-// wave 0 returns the result.  Every thread of the workgroup calls it.
-__device__ static uint64_t select_keys_block(const uint64_t *pk, uint32_t n, uint64_t (*wl)[64])
+// The 64 largest of n unsorted keys: each wave takes every nw-th batch of
+// 64, loading 8 batches at a time, and sorts and merges in those that hold
+// a key above its running 64th; wave 0 returns the result.  Every thread of
+// the workgroup calls it.
+__device__ static uint64_t select_keys_block(const uint64_t *__restrict__ pk, uint32_t n,
+                                             uint64_t (*wl)[64])
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    constexpr int kLoads = 8;
+    const uint32_t step = (uint32_t)nw * 64;
     uint64_t top = 0, thr = 0;
-    for (uint32_t b = (uint32_t)w * 64; b < n; b += (uint32_t)nw * 64) {
-        const uint64_t key = b + lane < n ? pk[b + lane] : 0ull;
-        if (__ballot(key > thr) == 0)
-            continue;
-        top = wave_merge_desc(top, wave_sort_desc(key));
-        thr = lane_read_u64(top, 63);
+    for (uint32_t b0 = (uint32_t)w * 64; b0 < n; b0 += step * kLoads) {
+        uint64_t v[kLoads];
+#pragma unroll
+        for (int q = 0; q < kLoads; ++q) {
+            const uint32_t i = b0 + (uint32_t)q * step + lane;
+            v[q] = i < n ? pk[i] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < kLoads; ++q) {
+            if (__ballot(v[q] > thr) == 0)
+                continue;
+            top = wave_merge_desc(top, wave_sort_desc(v[q]));
+            thr = lane_read_u64(top, 63);
+        }
     }
     return merge_wave_lists(top, wl);
 }
@@ -3105,7 +3117,7 @@ void launch_next_pass(const uint64_t *lists, uint32_t nlists, const NextPass &np
 {
     k_next_pass<<<1, 1024, 0, st>>>(lists, nlists, np, unshot, area, npatch, ctl, pool,
                                     lists_only ? 1u : 0u, target,
-                                    pool_cap < kPoolLds ? pool_cap : kPoolLds);
+                                    pool_cap < kPoolCap ? pool_cap : kPoolCap);
 }
 
 

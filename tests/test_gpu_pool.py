@@ -59,10 +59,10 @@ def test_chained_selection_bit_exact(rad, boxes_mid, graph, cap):
 
 
 def test_pool_cap_bounds(rad, boxes_mid):
-    """rad_debug_pool rejects a cap beyond 2,048 and reads the counters."""
+    """rad_debug_pool rejects a cap beyond 16,384 and reads the counters."""
     s = rad.Solver(hemicube=32, hemicubes=64, graph_passes=8)
     s.upload(boxes_mid)
     with pytest.raises(Exception):
-        s.debug_pool(4096)
+        s.debug_pool(16385)
     st = s.debug_pool()
     assert st == dict(hits=0, scans=0, arms=0, armed=0)
