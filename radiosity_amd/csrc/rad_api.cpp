@@ -1558,10 +1558,12 @@ int rad_dense_build(rad_ctx *c, uint32_t layout)
         launch_iota_shooters(base, k, c->shooters, c->nshoot, c->st);
         if ((rc = render_pass(c, k, 0, nullptr, nullptr, nullptr)))
             break;
-        if (layout == RAD_DENSE_ROWS)
+        if (layout == RAD_DENSE_ROWS) {
             launch_dense_store(c->ff, k, base, c->npatch, c->npad, c->fhi, c->flo, c->st);
-        else
+        } else {
             launch_dense_store_t(c->ff, k, base, c->npatch, c->npad, c->area, c->fhi, c->flo, c->st);
+            c->ff_dirty = 0; // the transposed store zeroes the F entries it reads
+        }
         if ((rc = check_launch(c, "dense store")))
             break;
     }

@@ -103,8 +103,10 @@ __global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t
 // takes 64 receivers; for each chunk of 64 shooters it reads the F entries
 // along the rows (coalesced), scales and splits them, transposes them
 // through LDS and writes 128-byte runs of each G row (8 lanes x 16 B of hi
-// and of lo).  Only the k columns of this pass are written.
-__global__ __launch_bounds__(256) void k_dense_store_t(const float *__restrict__ ff, uint32_t k,
+// and of lo).  Only the k columns of this pass are written.  Each thread
+// issues its 16 loads of a chunk at once, and sets every nonzero F entry it
+// read back to zero, so the pass's F rows are left zero for the next one.
+__global__ __launch_bounds__(256) void k_dense_store_t(float *__restrict__ ff, uint32_t k,
                                                        uint32_t row0, uint32_t npatch,
                                                        uint32_t npad,
                                                        const float *__restrict__ area,
@@ -118,11 +120,20 @@ __global__ __launch_bounds__(256) void k_dense_store_t(const float *__restrict__
     const float inv_aj = jok ? 1.0f / area[j] : 0.0f;
     for (uint32_t c0 = 0; c0 < k; c0 += 64) {
         const uint32_t nc = min(64u, k - c0);
-        for (uint32_t sl = t >> 6; sl < 64; sl += 4) {
-            const uint32_t sh = c0 + sl;
-            float f = 0.0f;
-            if (jok && sl < nc)
-                f = ff[(size_t)sh * npatch + j] * area[row0 + sh] * inv_aj;
+        float fv[16], as[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t sl = (t >> 6) + 4u * (uint32_t)q;
+            const bool ok = jok && sl < nc;
+            fv[q] = ok ? ff[(size_t)(c0 + sl) * npatch + j] : 0.0f;
+            as[q] = sl < nc ? area[row0 + c0 + sl] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t sl = (t >> 6) + 4u * (uint32_t)q;
+            if (fv[q] != 0.0f)
+                ff[(size_t)(c0 + sl) * npatch + j] = 0.0f;
+            const float f = fv[q] * as[q] * inv_aj;
             const uint16_t h = bf16_rne(f), l = bf16_rne(f - bf16_val(h));
             tile[jl][sl] = (uint32_t)h | ((uint32_t)l << 16);
         }
@@ -686,7 +697,7 @@ void launch_dense_store(const float *ff, uint32_t k, uint32_t row0, uint32_t npa
     k_dense_store<<<dim3(bx, k), 256, 0, st>>>(ff, k, row0, npatch, npad, fhi, flo);
 }
 
-void launch_dense_store_t(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
+void launch_dense_store_t(float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
                           uint32_t npad, const float *area, uint16_t *fhi, uint16_t *flo,
                           hipStream_t st)
 {

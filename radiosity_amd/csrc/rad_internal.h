@@ -247,7 +247,7 @@ void launch_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, uint32_
                           hipStream_t st);
 void launch_dense_store(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch, uint32_t npad,
                         uint16_t *fhi, uint16_t *flo, hipStream_t st);
-void launch_dense_store_t(const float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
+void launch_dense_store_t(float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
                           uint32_t npad, const float *area, uint16_t *fhi, uint16_t *flo,
                           hipStream_t st);
 void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint32_t npatch,
