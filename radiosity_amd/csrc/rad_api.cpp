@@ -1542,6 +1542,57 @@ int sum_parts(rad_ctx *c, GatherBufs &g, double *out)
 
 extern "C" {
 
+// The shooting-layout build as replays of a captured graph of
+// kDenseGraphPasses passes: each pass takes the next K rows from a device
+// counter (k_iota_next), renders them and stores them transposed; passes
+// beyond the last patch select nobody and store nothing.  The F rows are
+// zero at the start of every captured pass (the transposed store zeroes
+// what it reads).
+static int dense_build_graph(rad_ctx *c, uint32_t npass)
+{
+    if (c->ff_dirty) {
+        launch_clear(c->K, c->bb.count, c->bb.ntiles, c->ff, nullptr, c->ff_dirty, c->rowlen,
+                     c->counters, c->bb.ent_cap, c->st);
+        c->ff_dirty = 0;
+    }
+    const uint32_t start = 0u - c->K; // the first pass advances it to row 0
+    HIPCHK(c, hipMemcpyAsync(c->ctl + kCtlDenseBase, &start, 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    int rc = RAD_OK;
+    for (uint32_t p = 0; p < kDenseGraphPasses && rc == RAD_OK; ++p) {
+        launch_iota_next(c->K, c->npatch, c->ctl, c->shooters, c->nshoot, c->st);
+        if ((rc = render_pass(c, c->K, 0, nullptr, nullptr, nullptr)))
+            break;
+        launch_dense_store_t_dev(c->ff, c->K, c->ctl, c->nshoot, c->npatch, c->npad, c->area,
+                                 c->fhi, c->flo, c->st);
+        c->ff_dirty = 0;
+        rc = check_launch(c, "dense store");
+    }
+    hipError_t e = hipStreamEndCapture(c->st, &graph);
+    if (rc) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    for (uint32_t r = 0; r < npass && e == hipSuccess; r += kDenseGraphPasses)
+        e = hipGraphLaunch(exec, c->st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->st);
+    (void)hipGraphExecDestroy(exec);
+    if (e != hipSuccess)
+        return fail(c, RAD_E_HIP, std::string("dense build graph: ") + hipGetErrorString(e));
+    return RAD_OK;
+}
+
 int rad_dense_build(rad_ctx *c, uint32_t layout)
 {
     if (!c)
@@ -1553,19 +1604,25 @@ int rad_dense_build(rad_ctx *c, uint32_t layout)
         return rc;
     bool timing = c->timing;
     c->timing = false;
-    for (uint32_t base = 0; base < c->npatch; base += c->K) {
-        uint32_t k = c->npatch - base < c->K ? c->npatch - base : c->K;
-        launch_iota_shooters(base, k, c->shooters, c->nshoot, c->st);
-        if ((rc = render_pass(c, k, 0, nullptr, nullptr, nullptr)))
-            break;
-        if (layout == RAD_DENSE_ROWS) {
-            launch_dense_store(c->ff, k, base, c->npatch, c->npad, c->fhi, c->flo, c->st);
-        } else {
-            launch_dense_store_t(c->ff, k, base, c->npatch, c->npad, c->area, c->fhi, c->flo, c->st);
-            c->ff_dirty = 0; // the transposed store zeroes the F entries it reads
+    const uint32_t npass = (c->npatch + c->K - 1) / c->K;
+    if (layout == RAD_DENSE_SHOOT && npass >= 2 * kDenseGraphPasses) {
+        rc = dense_build_graph(c, npass);
+    } else {
+        for (uint32_t base = 0; base < c->npatch; base += c->K) {
+            uint32_t k = c->npatch - base < c->K ? c->npatch - base : c->K;
+            launch_iota_shooters(base, k, c->shooters, c->nshoot, c->st);
+            if ((rc = render_pass(c, k, 0, nullptr, nullptr, nullptr)))
+                break;
+            if (layout == RAD_DENSE_ROWS) {
+                launch_dense_store(c->ff, k, base, c->npatch, c->npad, c->fhi, c->flo, c->st);
+            } else {
+                launch_dense_store_t(c->ff, k, base, c->npatch, c->npad, c->area, c->fhi, c->flo,
+                                     c->st);
+                c->ff_dirty = 0; // the transposed store zeroes the F entries it reads
+            }
+            if ((rc = check_launch(c, "dense store")))
+                break;
         }
-        if ((rc = check_launch(c, "dense store")))
-            break;
     }
     c->timing = timing;
     c->rendered = false;

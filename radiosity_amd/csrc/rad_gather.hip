@@ -59,6 +59,24 @@ __global__ void k_iota_shooters(uint32_t row0, uint32_t k, uint32_t *shooters, u
         *nshoot = k;
 }
 
+// The next pass of a captured dense build: ctl[kCtlDenseBase] advances by K
+// (one workgroup of at least K threads) and its rows become the shooters.
+__global__ void k_iota_next(uint32_t K, uint32_t npatch, uint32_t *__restrict__ ctl,
+                            uint32_t *__restrict__ shooters, uint32_t *__restrict__ nshoot)
+{
+    __shared__ uint32_t base;
+    if (threadIdx.x == 0) {
+        base = ctl[kCtlDenseBase] + K;
+        ctl[kCtlDenseBase] = base;
+    }
+    __syncthreads();
+    const uint32_t k = base < npatch ? min(K, npatch - base) : 0u;
+    if (threadIdx.x < k)
+        shooters[threadIdx.x] = base + threadIdx.x;
+    if (threadIdx.x == 0)
+        *nshoot = k;
+}
+
 // Rows row0 .. row0+k-1 of F from the F rows of one pass (k x npatch f32).
 // One lane writes 8 consecutive columns (16 B of hi, 16 B of lo).
 __global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t row0,
@@ -106,13 +124,21 @@ __global__ void k_dense_store(const float *__restrict__ ff, uint32_t k, uint32_t
 // and of lo).  Only the k columns of this pass are written.  Each thread
 // issues its 16 loads of a chunk at once, and sets every nonzero F entry it
 // read back to zero, so the pass's F rows are left zero for the next one.
+// With dctl != null (a captured build), row0 is dctl[kCtlDenseBase] and k
+// is *dk.
 __global__ __launch_bounds__(256) void k_dense_store_t(float *__restrict__ ff, uint32_t k,
                                                        uint32_t row0, uint32_t npatch,
                                                        uint32_t npad,
                                                        const float *__restrict__ area,
                                                        uint16_t *__restrict__ fhi,
-                                                       uint16_t *__restrict__ flo)
+                                                       uint16_t *__restrict__ flo,
+                                                       const uint32_t *__restrict__ dctl,
+                                                       const uint32_t *__restrict__ dk)
 {
+    if (dctl) {
+        row0 = dctl[kCtlDenseBase];
+        k = *dk;
+    }
     __shared__ uint32_t tile[64][65]; // hi | lo << 16 per (receiver, shooter)
     const uint32_t t = threadIdx.x, jl = t & 63;
     const uint32_t j0 = blockIdx.x * 64, j = j0 + jl;
@@ -702,7 +728,21 @@ void launch_dense_store_t(float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
                           hipStream_t st)
 {
     k_dense_store_t<<<(npatch + 63) / 64, 256, 0, st>>>(ff, k, row0, npatch, npad, area, fhi,
-                                                        flo);
+                                                        flo, nullptr, nullptr);
+}
+
+void launch_iota_next(uint32_t K, uint32_t npatch, uint32_t *ctl, uint32_t *shooters,
+                      uint32_t *nshoot, hipStream_t st)
+{
+    k_iota_next<<<1, (K + 63) / 64 * 64, 0, st>>>(K, npatch, ctl, shooters, nshoot);
+}
+
+void launch_dense_store_t_dev(float *ff, uint32_t K, const uint32_t *ctl, const uint32_t *nshoot,
+                              uint32_t npatch, uint32_t npad, const float *area, uint16_t *fhi,
+                              uint16_t *flo, hipStream_t st)
+{
+    k_dense_store_t<<<(npatch + 63) / 64, 256, 0, st>>>(ff, K, 0u, npatch, npad, area, fhi, flo,
+                                                        ctl, nshoot);
 }
 
 void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint32_t npatch,

@@ -65,6 +65,7 @@ enum {
                        // emitter update (lastEnergy, Main.cpp:1290-1292), float bits
     kCtlShotLo = 8,    // [8..9] shooters shot (u64, low word first)
     kCtlPassLo = 10,   // [10..11] passes that shot at least one shooter (u64)
+    kCtlDenseBase = 12, // first row of the current pass of a captured dense build
     kCtlN = 16
 };
 
@@ -201,6 +202,7 @@ struct NextPass {
 // then selects from it, else it scans every patch.
 constexpr uint32_t kPoolCap = 16384; // most pool keys a selection takes
 constexpr uint32_t kPoolRearm = 8;  // passes of a graph between armings
+constexpr uint32_t kDenseGraphPasses = 8; // passes per captured dense-build graph
 struct SelPool {
     uint64_t tau;
     uint32_t armed;             // the next last shoot pools (else leaves lists)
@@ -250,6 +252,15 @@ void launch_dense_store(const float *ff, uint32_t k, uint32_t row0, uint32_t npa
 void launch_dense_store_t(float *ff, uint32_t k, uint32_t row0, uint32_t npatch,
                           uint32_t npad, const float *area, uint16_t *fhi, uint16_t *flo,
                           hipStream_t st);
+// A captured dense build: the next pass's rows (ctl[kCtlDenseBase] advanced
+// by K, from 0 - K before the first; K rows or the rest of the patches) as
+// its shooters, and the transposed store of the pass's rows taken from the
+// device (ctl[kCtlDenseBase], nshoot).
+void launch_iota_next(uint32_t K, uint32_t npatch, uint32_t *ctl, uint32_t *shooters,
+                      uint32_t *nshoot, hipStream_t st);
+void launch_dense_store_t_dev(float *ff, uint32_t K, const uint32_t *ctl, const uint32_t *nshoot,
+                              uint32_t npatch, uint32_t npad, const float *area, uint16_t *fhi,
+                              uint16_t *flo, hipStream_t st);
 void launch_dense_from_f32(const float *src, uint32_t rows, uint32_t row0, uint32_t npatch,
                            uint32_t npad, uint16_t *fhi, uint16_t *flo, hipStream_t st);
 void launch_dense_to_f32(const uint16_t *fhi, const uint16_t *flo, uint32_t rows, uint32_t row0,

@@ -78,6 +78,25 @@ def test_dense_shoot_layout_is_area_weighted_transpose(rad, cornell, built):
     assert np.allclose(G, want, rtol=1e-4, atol=1e-7)
 
 
+def test_dense_shoot_layout_graph_tail(rad, cornell):
+    """K = 20: the shooting-layout build replays captured graphs of 8 passes
+    over 77 passes (the last one partial, three passes of the last replay
+    past the last patch); the matrix is still the area-weighted transpose of
+    the rows layout, which builds eagerly."""
+    s = rad.Solver(hemicube=32, hemicubes=20)
+    s.upload(cornell)
+    say(f"dense build K=20 N={cornell.npatch}")
+    s.dense_build(rad.RAD_DENSE_ROWS)
+    F = s.dense_download()
+    s.dense_build(rad.RAD_DENSE_SHOOT)
+    G = s.dense_download()
+    a = cornell.area.astype(np.float64)
+    want = F.T.astype(np.float64) * a[None, :] / a[:, None]
+    assert np.allclose(G, want, rtol=1e-4, atol=1e-7)
+    c = s.debug_counters()
+    assert c["overflow"] == 0 and c["sticky"] == 0
+
+
 @pytest.mark.parametrize("layout", ["rows", "shoot"])
 def test_gather_matches_fp64_oracle(rad, cornell, built, layout):
     s, F, G = built
