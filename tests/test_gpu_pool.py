@@ -66,3 +66,37 @@ def test_pool_cap_bounds(rad, boxes_mid):
         s.debug_pool(16385)
     st = s.debug_pool()
     assert st == dict(hits=0, scans=0, arms=0, armed=0)
+
+
+def test_chained_passes_stop_like_eager(rad, boxes_mid):
+    """The reference's per-pass stop test (the last shooter's |lastEnergy|
+    below a threshold) inside captured chains, K = 64, graphs of 8 (lists,
+    then the pool): a threshold that first holds at pass 4 or later stops the
+    eager and the replayed runs on the same pass, with the same state bit for
+    bit (deterministic F sums)."""
+    sc = boxes_mid
+    s = rad.Solver(hemicube=32, hemicubes=64)
+    s.set_deterministic(True)
+    s.upload(sc)
+    e = []
+    for _ in range(12):
+        st = s.run_cycle(1, 0.0)
+        e.append(float(np.linalg.norm(np.array(st.last_energy[:], np.float64))))
+    say(f"last energies: {[round(x, 5) for x in e]}")
+    # the first pass (index p) whose energy lies below all earlier ones, with
+    # a gap wide enough for the device's float length
+    p = next(i for i in range(3, 12) if e[i] < 0.999 * min(e[:i]))
+    thr = 0.5 * (e[p] + min(e[:p]))
+    outs = []
+    for graph in (0, 8):
+        s = rad.Solver(hemicube=32, hemicubes=64, graph_passes=graph)
+        s.set_deterministic(True)
+        s.upload(sc)
+        st = s.run_cycle(40, thr)
+        u, sh = s.state()
+        say(f"graph {graph}: stopped {st.stopped} after {st.passes} passes")
+        outs.append((st.stopped, st.passes, u.view(np.uint32).copy(), sh.view(np.uint32).copy()))
+    assert outs[0][0] == 1 and outs[1][0] == 1
+    assert outs[0][1] == outs[1][1] == p + 1
+    assert np.array_equal(outs[0][2], outs[1][2])
+    assert np.array_equal(outs[0][3], outs[1][3])
