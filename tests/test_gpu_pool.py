@@ -25,8 +25,8 @@ def boxes_mid(rad):
                            seed=5)
 
 
-def _run(rad, sc, graph, passes, cap=0):
-    s = rad.Solver(hemicube=32, hemicubes=64, graph_passes=graph)
+def _run(rad, sc, graph, passes, cap=0, K=64):
+    s = rad.Solver(hemicube=32, hemicubes=K, graph_passes=graph)
     s.set_deterministic(True)
     s.upload(sc)
     if cap:
@@ -37,17 +37,17 @@ def _run(rad, sc, graph, passes, cap=0):
     return u.view(np.uint32).copy(), sh.view(np.uint32).copy(), s.debug_pool()
 
 
-@pytest.mark.parametrize("graph,cap", [(8, 0), (8, 64), (20, 0)])
-def test_chained_selection_bit_exact(rad, boxes_mid, graph, cap):
+@pytest.mark.parametrize("graph,cap,K", [(8, 0, 64), (8, 64, 64), (20, 0, 64), (8, 0, 24)])
+def test_chained_selection_bit_exact(rad, boxes_mid, graph, cap, K):
     """40 passes replayed from graphs of 8 or 20 against 40 eager passes,
-    K = 64, on 60k patches.  cap 0 keeps the default pool (most selections
-    after the first two of every 8 passes of a graph come from it); cap 64
-    sends every pooled selection that finds more than 64 keys to the scan
-    of all patches."""
+    K = 64 or 24, on 60k patches.  cap 0 keeps the default pool (most
+    selections after the first two of every 8 passes of a graph come from
+    it); cap 64 sends every pooled selection that finds more than 64 keys
+    to the scan of all patches."""
     sc = boxes_mid
-    say(f"chained selection: {sc.npatch} patches, K=64, graph={graph} cap={cap}")
-    ue, se, _ = _run(rad, sc, 0, 40)
-    ug, sg, st = _run(rad, sc, graph, 40, cap)
+    say(f"chained selection: {sc.npatch} patches, K={K}, graph={graph} cap={cap}")
+    ue, se, _ = _run(rad, sc, 0, 40, K=K)
+    ug, sg, st = _run(rad, sc, graph, 40, cap, K)
     say(f"  pool: {st}")
     assert np.array_equal(ue, ug)
     assert np.array_equal(se, sg)
