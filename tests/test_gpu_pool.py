@@ -25,8 +25,9 @@ def boxes_mid(rad):
                            seed=5)
 
 
-def _run(rad, sc, graph, passes, cap=0, K=64):
-    s = rad.Solver(hemicube=32, hemicubes=K, graph_passes=graph)
+def _run(rad, sc, graph, passes, cap=0, K=64, mode=None):
+    mode = rad.RAD_MODE_PHYSICAL if mode is None else mode
+    s = rad.Solver(hemicube=32, hemicubes=K, graph_passes=graph, mode=mode)
     s.set_deterministic(True)
     s.upload(sc)
     if cap:
@@ -37,17 +38,21 @@ def _run(rad, sc, graph, passes, cap=0, K=64):
     return u.view(np.uint32).copy(), sh.view(np.uint32).copy(), s.debug_pool()
 
 
-@pytest.mark.parametrize("graph,cap,K", [(8, 0, 64), (8, 64, 64), (20, 0, 64), (8, 0, 24)])
-def test_chained_selection_bit_exact(rad, boxes_mid, graph, cap, K):
+@pytest.mark.parametrize("graph,cap,K,ref", [(8, 0, 64, False), (8, 64, 64, False),
+                                             (20, 0, 64, False), (8, 0, 24, False),
+                                             (8, 0, 64, True)])
+def test_chained_selection_bit_exact(rad, boxes_mid, graph, cap, K, ref):
     """40 passes replayed from graphs of 8 or 20 against 40 eager passes,
     K = 64 or 24, on 60k patches.  cap 0 keeps the default pool (most
     selections after the first two of every 8 passes of a graph come from
     it); cap 64 sends every pooled selection that finds more than 64 keys
-    to the scan of all patches."""
+    to the scan of all patches; ref: REFERENCE shooting (Main.cpp:1274)
+    instead of PHYSICAL."""
     sc = boxes_mid
+    mode = rad.RAD_MODE_REFERENCE if ref else rad.RAD_MODE_PHYSICAL
     say(f"chained selection: {sc.npatch} patches, K={K}, graph={graph} cap={cap}")
-    ue, se, _ = _run(rad, sc, 0, 40, K=K)
-    ug, sg, st = _run(rad, sc, graph, 40, cap, K)
+    ue, se, _ = _run(rad, sc, 0, 40, K=K, mode=mode)
+    ug, sg, st = _run(rad, sc, graph, 40, cap, K, mode)
     say(f"  pool: {st}")
     assert np.array_equal(ue, ug)
     assert np.array_equal(se, sg)
