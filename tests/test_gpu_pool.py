@@ -105,3 +105,27 @@ def test_chained_passes_stop_like_eager(rad, boxes_mid):
     assert outs[0][1] == outs[1][1] == p + 1
     assert np.array_equal(outs[0][2], outs[1][2])
     assert np.array_equal(outs[0][3], outs[1][3])
+
+
+def test_chained_selection_headline_workload(rad):
+    """The bench's workload: room200k (400 boxes, seed 3, 199,936 patches),
+    W = 256, K = 64, tile 32, in-lane limit 64, graphs of 10.  30 replayed
+    passes (lists, then the pool) equal 30 eager passes bit for bit in the
+    deterministic mode."""
+    sc = rad.Scene.boxes(nboxes=400, room=(20.0, 5.0, 20.0), target_tris=200000, nlights=16,
+                         seed=3)
+    outs = []
+    for graph in (0, 10):
+        say(f"room200k: {sc.npatch} patches, graph={graph}")
+        s = rad.Solver(hemicube=256, hemicubes=64, tile_px=32, small_tri_px=64,
+                       graph_passes=graph)
+        s.set_deterministic(True)
+        s.upload(sc)
+        s.step(30)
+        s.finish()
+        u, sh = s.state()
+        outs.append((u.view(np.uint32).copy(), sh.view(np.uint32).copy(), s.debug_pool()))
+    say(f"  pool: {outs[1][2]}")
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert outs[1][2]["hits"] > 0 and outs[1][2]["scans"] == 0
