@@ -267,9 +267,9 @@ int rad_debug_pool(rad_ctx *ctx, uint32_t cap, uint32_t *out4_or_null);
  * spread evenly over the patches (render + reduce, no shooting, state
  * unchanged; the device selection is overwritten) for each screen tile
  * side {16, 32} (and 8 for K <= 2) and in-lane triangle limit
- * {16, 32, 64, 128, 256}, and keeps the fastest pair.  No counterpart in the
+ * {16, 32, 48, 64, 128, 256}, and keeps the fastest pair.  No counterpart in the
  * reference (its GL driver rasterizes). */
-#define RAD_TUNE_MAX 16
+#define RAD_TUNE_MAX 18
 typedef struct rad_tune_result {
     uint32_t tile_px, small_tri_px; /* chosen */
     uint32_t ncand, reserved;

@@ -68,8 +68,8 @@ class rad_gather_stats(C.Structure):
 class rad_tune_result(C.Structure):
     _fields_ = [("tile_px", C.c_uint32), ("small_tri_px", C.c_uint32), ("ncand", C.c_uint32),
                 ("reserved", C.c_uint32), ("best_ms", C.c_double),
-                ("cand_tile", C.c_uint32 * 16), ("cand_small", C.c_uint32 * 16),
-                ("cand_ms", C.c_double * 16)]
+                ("cand_tile", C.c_uint32 * 18), ("cand_small", C.c_uint32 * 18),
+                ("cand_ms", C.c_double * 18)]
 
 
 _lib = None

@@ -1295,7 +1295,7 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
     if (passes == 0)
         passes = 3;
     rad_tune_result r{};
-    const uint32_t tiles[3] = {8, 16, 32}, smalls[5] = {16, 32, 64, 128, 256};
+    const uint32_t tiles[3] = {8, 16, 32}, smalls[6] = {16, 32, 48, 64, 128, 256};
     const uint32_t T0 = c->cfg.tile_px, S0 = c->small_px;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(c, hipEventCreate(&e0));
@@ -1321,7 +1321,7 @@ int rad_autotune(rad_ctx *c, uint32_t passes, rad_tune_result *out)
         for (uint32_t ti = 0; ti < 3 && rc == RAD_OK; ++ti) {
             if (tiles[ti] == 8 && c->K > 2)
                 continue;
-            for (uint32_t si = 0; si < 5 && rc == RAD_OK && r.ncand < RAD_TUNE_MAX; ++si) {
+            for (uint32_t si = 0; si < 6 && rc == RAD_OK && r.ncand < RAD_TUNE_MAX; ++si) {
                 if ((rc = set_tiles(c, tiles[ti], smalls[si])))
                     break;
                 rc = render_pass(c, c->K, 0, nullptr, nullptr, nullptr); // warm
